@@ -1,0 +1,156 @@
+"""Batched, device-driven particle filters: many independent filters advanced by
+one hipGraph-captured step.
+
+Each filter has its own particle count, satellite count, seed and anchor; all
+of them share one structure-of-arrays arena in HBM and every stage of the step
+is one launch over all filters (tile -> filter tables).  Satellite data for a
+whole trajectory can be preloaded into epoch slots; the step reads slot
+``epoch % n_slots`` from a device-side epoch counter, so graph replays advance
+the filters with no host work.
+
+There is no reference counterpart for the batch (the reference advances one
+filter per ParticleFilterDevice); a batch of one reproduces
+ParticleFilterDevice's step exactly.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, dptr, f64, i32, iptr
+
+STAGE_WEIGHT_PREDICT = 0
+STAGE_WEIGHT = 1
+STAGE_CDF = 2
+STAGE_GATHER_SYSTEMATIC = 3
+STAGE_MEGOPOLIS = 4
+STAGE_GATHER_ANCESTORS = 5
+
+RESAMPLERS = {"systematic": 0, "megopolis": 1, "megopolis_ref": 2}
+
+
+class BatchedParticleFilter:
+    def __init__(self, n_particles, n_isb=0, sat_cap=64, n_slots=1, with_velocity=False):
+        n = np.atleast_1d(np.asarray(n_particles, dtype=np.int32))
+        self.n_filters = int(n.size)
+        self.n = n
+        cap = np.broadcast_to(np.asarray(sat_cap, dtype=np.int32), n.shape).copy()
+        self.sat_cap = cap
+        self.sat_off = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+        self.sat_rows = int(cap.sum())
+        self.n_isb = int(n_isb)
+        self.D = 4 + self.n_isb
+        self.n_slots = int(n_slots)
+        h = C.c_void_p()
+        nn = i32(n)
+        cc = i32(cap)
+        call("gg_pf_create", self.n_filters, iptr(nn), iptr(cc), self.n_isb, self.n_slots,
+             1 if with_velocity else 0, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.lib.gg_pf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -----------------------------------------------------------------
+    def initialize(self, init, seeds, spread_pos=10.0, spread_cb=10.0, spread_isb=0.0):
+        init = f64(init).reshape(self.n_filters, self.D)
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64).reshape(self.n_filters))
+        call("gg_pfb_initialize", self._h, dptr(init),
+             seeds.ctypes.data_as(C.POINTER(C.c_uint64)), float(spread_pos), float(spread_cb),
+             float(spread_isb))
+
+    def load_epochs(self, sats, prs, n_sat, weights=None, sys=None, vel=None):
+        """sats[t][f] -> (S_tf, 3), prs[t][f] -> (S_tf,), optional weights/sys
+        the same shape, vel[t][f] -> (3,) ECEF.  Lists (epoch-major) of
+        per-filter arrays, or dense arrays shaped [T, F, S, ...]."""
+        T = len(prs)
+        rows = self.sat_rows
+        xyz = np.zeros((T, rows, 3))
+        pr = np.zeros((T, rows))
+        w = np.ones((T, rows))
+        sy = np.zeros((T, rows), dtype=np.int32)
+        ns = np.zeros((T, self.n_filters), dtype=np.int32)
+        for t in range(T):
+            for f in range(self.n_filters):
+                k = int(n_sat[t][f])
+                o = int(self.sat_off[f])
+                if k > self.sat_cap[f]:
+                    raise ValueError("more satellites than the filter's sat_cap")
+                xyz[t, o:o + k] = np.asarray(sats[t][f], dtype=np.float64).reshape(-1, 3)[:k]
+                pr[t, o:o + k] = np.asarray(prs[t][f], dtype=np.float64)[:k]
+                if weights is not None:
+                    w[t, o:o + k] = np.asarray(weights[t][f], dtype=np.float64)[:k]
+                if sys is not None:
+                    sy[t, o:o + k] = np.asarray(sys[t][f], dtype=np.int32)[:k]
+                ns[t, f] = k
+        v = np.zeros((T, self.n_filters, 3)) if vel is None else f64(vel).reshape(T, self.n_filters, 3)
+        call("gg_pfb_load_epochs", self._h, T, dptr(xyz), dptr(pr), dptr(w), iptr(sy), iptr(ns),
+             dptr(v))
+        self.loaded_slots = T
+
+    # -- stepping --------------------------------------------------------------
+    def step_args(self, **kw):
+        kw.setdefault("n_slots", getattr(self, "loaded_slots", self.n_slots))
+        return _lib.default_step_args(**kw)
+
+    def step(self, args, resampler="systematic", n_iter=32):
+        call("gg_pfb_step", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter))
+
+    def capture(self, args, resampler="systematic", n_iter=32, steps=1):
+        call("gg_pfb_capture", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
+             int(steps))
+
+    def replay(self, n=1):
+        call("gg_pfb_replay", self._h, int(n))
+
+    def time_replays(self, n):
+        ms = C.c_float()
+        call("gg_pfb_time_replays", self._h, int(n), C.byref(ms))
+        return float(ms.value)
+
+    def time_stage(self, args, stage, reps, n_iter=32):
+        ms = C.c_float()
+        call("gg_pfb_time_stage", self._h, C.byref(args), int(stage), int(n_iter), int(reps),
+             C.byref(ms))
+        return float(ms.value)
+
+    def set_epoch(self, epoch):
+        call("gg_pfb_set_epoch", self._h, int(epoch))
+
+    def sync(self):
+        call("gg_pf_device_sync", self._h)
+
+    # -- results ---------------------------------------------------------------
+    def read_dyn(self):
+        arr = (_lib.Dyn * self.n_filters)()
+        call("gg_pfb_read_dyn", self._h, arr)
+        return arr
+
+    def estimates(self):
+        d = self.read_dyn()
+        return np.array([[d[f].est[k] for k in range(self.D)] for f in range(self.n_filters)])
+
+    def ess(self):
+        d = self.read_dyn()
+        return np.array([d[f].ess for f in range(self.n_filters)])
+
+    def get_particles(self, f):
+        out = np.empty((int(self.n[f]), self.D), dtype=np.float64)
+        call("gg_pfb_get_particles", self._h, int(f), dptr(out))
+        return out
+
+    def get_log_weights(self, f):
+        out = np.empty(int(self.n[f]), dtype=np.float64)
+        call("gg_pfb_get_log_weights", self._h, int(f), dptr(out))
+        return out

@@ -1,0 +1,271 @@
+// Shared device-side building blocks for the particle-filter and WLS kernels.
+//
+// Everything in this header that feeds the resampling decision (the f64 exp,
+// the fixed-order block scans, Philox) is written so that a NumPy twin using
+// only IEEE-754 +,-,*,/ reproduces it bit for bit: floating-point contraction
+// is switched off for these functions, and every reduction/scan uses one fixed
+// association order that does not depend on scheduling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gnss_gpu_amd_layout.h"
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel Random Numbers: As Easy
+// as 1, 2, 3", SC'11).  Counter-based: a draw is a pure function of
+// (counter, key), so any launch geometry gives the same numbers.
+// ---------------------------------------------------------------------------
+struct gg_u32x4 {
+  uint32_t v[4];
+};
+
+__host__ __device__ __forceinline__ gg_u32x4 gg_philox4x32_10(uint32_t c0, uint32_t c1,
+                                                              uint32_t c2, uint32_t c3,
+                                                              uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n3 = (uint32_t)p0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+  gg_u32x4 out;
+  out.v[0] = c0;
+  out.v[1] = c1;
+  out.v[2] = c2;
+  out.v[3] = c3;
+  return out;
+}
+
+// Draw `call` of stream (seed, purpose) for element `idx` at step `step`.
+__host__ __device__ __forceinline__ gg_u32x4 gg_draw(uint64_t seed, uint32_t idx, uint32_t step,
+                                                     uint32_t purpose, uint32_t call) {
+  return gg_philox4x32_10(idx, step, purpose, call, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// uint32 -> float uniform in the open interval (0, 1): the top 23 bits plus
+// one half, scaled by 2^-23, so every value is exact in binary32 and the
+// largest is 1 - 2^-24.
+__host__ __device__ __forceinline__ float gg_u01f(uint32_t x) {
+  return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f;  // 2^-23
+}
+
+// two uint32 -> double uniform in (0, 1]; exact in binary64.
+__host__ __device__ __forceinline__ double gg_u01d(uint32_t hi, uint32_t lo) {
+  const uint64_t m = ((uint64_t)hi << 21) | (uint64_t)(lo >> 11);
+  return (double)(m + 1) * 1.1102230246251565e-16;  // 2^-53
+}
+
+// Box-Muller pair from two (0,1) uniforms, binary32.
+__device__ __forceinline__ void gg_box_muller(float u1, float u2, float& z0, float& z1) {
+  const float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincospif(2.0f * u2, &s, &c);
+  z0 = r * c;
+  z1 = r * s;
+}
+
+// Four standard normals from one Philox block.
+__device__ __forceinline__ void gg_normal4(uint64_t seed, uint32_t idx, uint32_t step,
+                                           uint32_t purpose, uint32_t call, float z[4]) {
+  const gg_u32x4 r = gg_draw(seed, idx, step, purpose, call);
+  gg_box_muller(gg_u01f(r.v[0]), gg_u01f(r.v[1]), z[0], z[1]);
+  gg_box_muller(gg_u01f(r.v[2]), gg_u01f(r.v[3]), z[2], z[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic binary64 exp.  Cody-Waite reduction x = k ln2 + r with ln2
+// split into a 32-bit head and a tail, then the degree-13 Taylor polynomial of
+// exp(r) on |r| <= ln2/2 evaluated by Horner, then exact scaling by 2^k.
+// Only correctly rounded IEEE operations are used, in a fixed order, so the
+// NumPy twin in the oracle reproduces every bit.  Max error about 1 ulp.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double gg_exp64(double x) {
+#pragma clang fp contract(off)
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+  const double r = (x - k * 0x1.62e42fee00000p-1) - k * 0x1.a39ef35793c76p-33;
+  double p = 0x1.6124613a86d09p-33;
+  p = p * r + 0x1.1eed8eff8d898p-29;
+  p = p * r + 0x1.ae64567f544e4p-26;
+  p = p * r + 0x1.27e4fb7789f5cp-22;
+  p = p * r + 0x1.71de3a556c734p-19;
+  p = p * r + 0x1.a01a01a01a01ap-16;
+  p = p * r + 0x1.a01a01a01a01ap-13;
+  p = p * r + 0x1.6c16c16c16c17p-10;
+  p = p * r + 0x1.1111111111111p-7;
+  p = p * r + 0x1.5555555555555p-5;
+  p = p * r + 0x1.5555555555555p-3;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return __builtin_ldexp(p, (int)k);
+}
+
+// Deterministic binary32 exp with the same construction (16-bit ln2 head,
+// degree-7 Taylor polynomial).  Used by the Megopolis acceptance test.
+// gg_exp32_core is the branch-free body, valid for |x| <= 87.
+__host__ __device__ __forceinline__ float gg_exp32_core(float x) {
+#pragma clang fp contract(off)
+  const float k = __builtin_rintf(x * 0x1.715476p+0f);
+  const float r = (x - k * 0x1.62e4p-1f) - k * 0x1.7f7d1cp-20f;
+  float p = 0x1.a01a02p-13f;
+  p = p * r + 0x1.6c16c2p-10f;
+  p = p * r + 0x1.111112p-7f;
+  p = p * r + 0x1.555556p-5f;
+  p = p * r + 0x1.555556p-3f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return __builtin_ldexpf(p, (int)k);
+}
+
+__host__ __device__ __forceinline__ float gg_exp32(float x) {
+#pragma clang fp contract(off)
+  if (x != x) return x;
+  if (x > 88.72283f) return __builtin_huge_valf();
+  if (x < -103.97208f) return 0.0f;
+  const float k = __builtin_rintf(x * 0x1.715476p+0f);
+  const float r = (x - k * 0x1.62e4p-1f) - k * 0x1.7f7d1cp-20f;
+  float p = 0x1.a01a02p-13f;
+  p = p * r + 0x1.6c16c2p-10f;
+  p = p * r + 0x1.111112p-7f;
+  p = p * r + 0x1.555556p-5f;
+  p = p * r + 0x1.555556p-3f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return __builtin_ldexpf(p, (int)k);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-order block scan of 256 doubles (4 wavefronts of 64 lanes).
+//   1. Hillis-Steele inclusive scan inside each wavefront:
+//        for d in 1,2,4,8,16,32: x[l] = x[l-d] + x[l]   (l >= d)
+//   2. wavefront totals T0..T3 combined left to right: O1=T0, O2=T0+T1,
+//      O3=(T0+T1)+T2; lane value in wave w>0 becomes O_w + x.
+// Returns the inclusive value of this thread; `total` gets thread 255's value
+// and `excl` the inclusive value of thread t-1 (0 for t=0).
+// `lds` must hold at least 8 + 256 doubles and is reused on return.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double gg_block_scan256(double v, double* lds, double& excl,
+                                                   double& total) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  double x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x = y + x;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  if (t == 0) {
+    const double o1 = lds[0];
+    const double o2 = o1 + lds[1];
+    const double o3 = o2 + lds[2];
+    lds[4] = o1;
+    lds[5] = o2;
+    lds[6] = o3;
+    lds[7] = o3 + lds[3];
+  }
+  __syncthreads();
+  if (w > 0) x = lds[3 + w] + x;
+  lds[8 + t] = x;
+  __syncthreads();
+  excl = (t == 0) ? 0.0 : lds[8 + t - 1];
+  total = lds[7];
+  __syncthreads();
+  return x;
+}
+
+// Block-wide max of a double (exact; any order gives the same result).
+__device__ __forceinline__ double gg_block_max256(double v, double* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  if ((t & 63) == 0) lds[t >> 6] = v;
+  __syncthreads();
+  const double m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  __syncthreads();
+  return m;
+}
+
+__device__ __forceinline__ int gg_block_max256_i(int v, int* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+  if ((t & 63) == 0) lds[t >> 6] = v;
+  __syncthreads();
+  const int m = max(max(lds[0], lds[1]), max(lds[2], lds[3]));
+  __syncthreads();
+  return m;
+}
+
+__device__ __forceinline__ int gg_block_sum256_i(int v, int* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((t & 63) == 0) lds[t >> 6] = v;
+  __syncthreads();
+  const int m = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return m;
+}
+
+// Inclusive prefix max of ints over the block (exact, order-free).
+__device__ __forceinline__ int gg_block_prefix_max256_i(int v, int* lds) {
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d, 64);
+    if (lane >= d) v = max(v, y);
+  }
+  if (lane == 63) lds[w] = v;
+  __syncthreads();
+  int pre = INT_MIN;
+  for (int k = 0; k < w; ++k) pre = max(pre, lds[k]);
+  v = max(v, pre);
+  __syncthreads();
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// "Last block" handshake: every block of a group publishes its partials, then
+// bumps the group's counter; the block that observes count == n-1 is the last
+// one and may read every partial.  It resets the counter for the next launch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool gg_last_block(unsigned int* counter, unsigned int n_blocks,
+                                              int* lds_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int prev = atomicAdd(counter, 1u);
+    *lds_flag = (prev == n_blocks - 1u) ? 1 : 0;
+  }
+  __syncthreads();
+  const bool last = *lds_flag != 0;
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  }
+  return last;
+}

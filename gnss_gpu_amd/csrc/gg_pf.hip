@@ -1,0 +1,1072 @@
+// Host side of the particle-filter C ABI: handle, allocation, launches,
+// hipGraph capture.  Kernels live in gg_pf_kernels_impl.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gg_common_host.h"
+#include "gg_pf_kernels_impl.h"
+#include "gnss_gpu_amd.h"
+
+struct gg_pf {
+  int32_t n_filters = 0;
+  int32_t n_isb = 0;
+  int32_t D = 4;
+  int32_t ncomp = 4;
+  int32_t has_vel = 0;
+  int32_t n_slots = 1;
+  int32_t sat_rows = 0;
+  int64_t P = 0;
+  int32_t n_tiles = 0;
+  int32_t n_chunks = 0;
+  std::vector<gg_filter> filt_h;
+  std::vector<int32_t> tile_filter_h, chunk_filter_h;
+  // device memory
+  float* state = nullptr;
+  float* lw = nullptr;
+  int32_t* E = nullptr;
+  int32_t* anc = nullptr;
+  double* part = nullptr;
+  double* g = nullptr;
+  double* Pb = nullptr;
+  int32_t* emax = nullptr;
+  int32_t* pmax = nullptr;
+  double* csum = nullptr;
+  double* scratch_f = nullptr;
+  gg_filter* filt = nullptr;
+  gg_dyn* dyn = nullptr;
+  int32_t* tile_filter = nullptr;
+  int32_t* chunk_filter = nullptr;
+  double* satd = nullptr;
+  int32_t* sati = nullptr;
+  double* vel_guide = nullptr;
+  double* io_tmp = nullptr;
+  size_t io_tmp_bytes = 0;
+  // pinned host staging
+  char* pin = nullptr;
+  size_t pin_bytes = 0;
+  hipEvent_t pin_done = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  // host-side validity of device results (single-filter API)
+  bool decide_valid = false;  // M, W, g, Pb consistent with lw
+  bool ess_valid = false;
+  bool est_valid = false;
+  bool initialized = false;
+  std::vector<double> init_isb;
+  uint64_t seed0 = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  GG_HIP(hipMalloc((void**)p, count * sizeof(T)));
+  GG_HIP(hipMemset(*p, 0, count * sizeof(T)));
+  return 0;
+}
+
+gg_arena arena_of(const gg_pf* h) {
+  gg_arena a;
+  a.state = h->state;
+  a.lw = h->lw;
+  a.E = h->E;
+  a.anc = h->anc;
+  a.part = h->part;
+  a.g = h->g;
+  a.Pb = h->Pb;
+  a.emax = h->emax;
+  a.pmax = h->pmax;
+  a.csum = h->csum;
+  a.scratch_f = h->scratch_f;
+  a.filt = h->filt;
+  a.dyn = h->dyn;
+  a.tile_filter = h->tile_filter;
+  a.chunk_filter = h->chunk_filter;
+  a.satd = h->satd;
+  a.sati = h->sati;
+  a.vel_guide = h->vel_guide;
+  a.P = h->P;
+  a.ncomp = h->ncomp;
+  a.n_isb = h->n_isb;
+  a.has_vel = h->has_vel;
+  a.sat_rows = h->sat_rows;
+  a.n_filters = h->n_filters;
+  a.pad = 0;
+  return a;
+}
+
+size_t sat_smem(const gg_pf* h) {
+  int cap = 0;
+  for (const auto& f : h->filt_h) cap = std::max(cap, f.sat_cap);
+  return (size_t)std::max(cap, 1) * 7 * sizeof(float);
+}
+
+int ensure_pinned(gg_pf* h, size_t bytes) {
+  if (h->pin_bytes >= bytes) {
+    GG_HIP(hipEventSynchronize(h->pin_done));
+    return 0;
+  }
+  if (h->pin) {
+    GG_HIP(hipEventSynchronize(h->pin_done));
+    GG_HIP(hipHostFree(h->pin));
+    h->pin = nullptr;
+  }
+  GG_HIP(hipHostMalloc((void**)&h->pin, bytes, hipHostMallocDefault));
+  h->pin_bytes = bytes;
+  return 0;
+}
+
+int ensure_io(gg_pf* h, size_t bytes) {
+  if (h->io_tmp_bytes >= bytes) return 0;
+  if (h->io_tmp) {
+    GG_HIP(hipStreamSynchronize(h->stream));
+    GG_HIP(hipFree(h->io_tmp));
+    h->io_tmp = nullptr;
+  }
+  GG_HIP(hipMalloc((void**)&h->io_tmp, bytes));
+  h->io_tmp_bytes = bytes;
+  return 0;
+}
+
+int read_dyn(gg_pf* h, int f, gg_dyn* out) {
+  if (ensure_pinned(h, std::max(h->pin_bytes, sizeof(gg_dyn)))) return -1;
+  GG_HIP(hipMemcpyAsync(h->pin, h->dyn + f, sizeof(gg_dyn), hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  std::memcpy(out, h->pin, sizeof(gg_dyn));
+  return 0;
+}
+
+int write_dyn(gg_pf* h, int f, const gg_dyn& d) {
+  if (ensure_pinned(h, std::max(h->pin_bytes, sizeof(gg_dyn)))) return -1;
+  std::memcpy(h->pin, &d, sizeof(gg_dyn));
+  GG_HIP(hipMemcpyAsync(h->dyn + f, h->pin, sizeof(gg_dyn), hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  return 0;
+}
+
+// ENU rotation at the anchor (geocentric latitude); identity near the origin.
+void enu_rotation(const double* x, double* R) {
+  const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (!(r > 1.0)) {
+    for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  const double lon = std::atan2(x[1], x[0]);
+  const double lat = std::atan2(x[2], std::hypot(x[0], x[1]));
+  const double sl = std::sin(lon), cl = std::cos(lon), sp = std::sin(lat), cp = std::cos(lat);
+  R[0] = -sl;
+  R[1] = cl;
+  R[2] = 0.0;
+  R[3] = -sp * cl;
+  R[4] = -sp * sl;
+  R[5] = cp;
+  R[6] = cp * cl;
+  R[7] = cp * sl;
+  R[8] = sp;
+}
+
+#define GG_DISPATCH_ISB(nisb, ...) \
+  switch (nisb) {                     \
+    case 0: {                         \
+      constexpr int NI = 0;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    case 1: {                         \
+      constexpr int NI = 1;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    case 2: {                         \
+      constexpr int NI = 2;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+    default: {                        \
+      constexpr int NI = 3;           \
+      __VA_ARGS__;                    \
+    } break;                          \
+  }
+
+int launch_weight(gg_pf* h, const gg_step_args& s, bool pred) {
+  const gg_arena a = arena_of(h);
+  const size_t smem = sat_smem(h);
+  if (pred) {
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, true>), dim3(h->n_tiles),
+                                                  dim3(GG_BLOCK), smem, h->stream, a, s));
+  } else {
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, false>), dim3(h->n_tiles),
+                                                  dim3(GG_BLOCK), smem, h->stream, a, s));
+  }
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_reduce(gg_pf* h, const gg_step_args& s) {
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_reduce<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, h->stream, a, s));
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_cdf(gg_pf* h, const gg_step_args& s, int force) {
+  const gg_arena a = arena_of(h);
+  hipLaunchKernelGGL(gg_k_cdf, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, s, force);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf) {
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_gather<NI>), dim3(h->n_chunks),
+                                                dim3(GG_BLOCK), 0, h->stream, a, s, force,
+                                                anc_mode, src_buf));
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
+                    int* final_buf) {
+  const gg_arena a = arena_of(h);
+  for (int it = 0; it < n_iter; ++it) {
+    hipLaunchKernelGGL(gg_k_mego_ref, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, it,
+                       force, seed, use_ovr);
+    GG_HIP(hipGetLastError());
+  }
+  *final_buf = n_iter & 1;
+  return 0;
+}
+
+int launch_mego(gg_pf* h, int n_iter, int force) {
+  const gg_arena a = arena_of(h);
+  hipLaunchKernelGGL(gg_k_mego, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+                     force);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+// One full device-driven step (no host synchronisation).
+int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, bool pred) {
+  if (launch_weight(h, s, pred)) return -1;
+  if (resampler == 0) {
+    if (launch_cdf(h, s, 0)) return -1;
+    return launch_gather(h, s, 0, 0, 0);
+  }
+  if (resampler == 1) {
+    if (launch_mego(h, n_iter, 0)) return -1;
+    return launch_gather(h, s, 0, 1, 1);
+  }
+  int fb = 0;
+  if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0);
+  if (launch_mego_ref(h, n_iter, 0, 0, 0, &fb)) return -1;
+  return launch_gather(h, s, 0, 1, fb);
+}
+
+gg_step_args default_args() {
+  gg_step_args s;
+  std::memset(&s, 0, sizeof(s));
+  s.dt = 1.0;
+  s.sigma_pos = 1.0f;
+  s.sigma_cb = 300.0f;
+  s.sigma_pr = 5.0f;
+  s.alpha = 1.0f;
+  s.huber_k = 1.5f;
+  s.n_slots = 1;
+  s.ess_frac = 0.5;
+  s.recenter = 1;
+  return s;
+}
+
+int check_single(gg_pf* h) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  return 0;
+}
+
+int ensure_decide(gg_pf* h) {
+  if (h->decide_valid) return 0;
+  gg_step_args s = default_args();
+  s.auto_flag = 0;
+  s.recenter = 1;
+  if (launch_reduce(h, s)) return -1;
+  h->decide_valid = true;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int gg_struct_sizes(int32_t* step_args, int32_t* dyn, int32_t* filter) {
+  if (step_args) *step_args = (int32_t)sizeof(gg_step_args);
+  if (dyn) *dyn = (int32_t)sizeof(gg_dyn);
+  if (filter) *filter = (int32_t)sizeof(gg_filter);
+  return 0;
+}
+
+int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* sat_cap,
+                 int32_t n_isb, int32_t n_slots, int32_t with_velocity, gg_pf** out) {
+  if (!out) return gg_fail(-2, "out is null");
+  *out = nullptr;
+  if (n_filters <= 0) return gg_fail(-2, "n_filters must be positive");
+  if (n_isb < 0 || n_isb > GG_MAX_ISB) return gg_fail(-2, "n_isb must be in [0, 3]");
+  if (n_slots <= 0) return gg_fail(-2, "n_slots must be positive");
+  gg_pf* h = new gg_pf();
+  h->n_filters = n_filters;
+  h->n_isb = n_isb;
+  h->D = 4 + n_isb;
+  h->has_vel = with_velocity ? 1 : 0;
+  h->ncomp = h->D + (h->has_vel ? GG_NVEL : 0);
+  h->n_slots = n_slots;
+  int64_t p = 0;
+  int32_t tiles = 0, chunks = 0, rows = 0;
+  for (int f = 0; f < n_filters; ++f) {
+    const int n = n_particles[f];
+    if (n <= 0 || n > (1 << 30)) {
+      delete h;
+      return gg_fail(-2, "n_particles must be in [1, 2^30]");
+    }
+    gg_filter F;
+    std::memset(&F, 0, sizeof(F));
+    F.p_off = p;
+    F.n = n;
+    F.n_tiles = (n + GG_TILE - 1) / GG_TILE;
+    F.tile_off = tiles;
+    F.n_chunks = (int32_t)((2 * (int64_t)n + GG_QCHUNK - 1) / GG_QCHUNK);
+    F.chunk_off = chunks;
+    F.sat_off = rows;
+    F.sat_cap = std::max(sat_cap ? sat_cap[f] : 64, 1);
+    F.seed = 0;
+    h->filt_h.push_back(F);
+    for (int t = 0; t < F.n_tiles; ++t) h->tile_filter_h.push_back(f);
+    for (int c = 0; c < F.n_chunks; ++c) h->chunk_filter_h.push_back(f);
+    p += (int64_t)F.n_tiles * GG_TILE;
+    tiles += F.n_tiles;
+    chunks += F.n_chunks;
+    rows += F.sat_cap;
+  }
+  h->P = p;
+  h->n_tiles = tiles;
+  h->n_chunks = chunks;
+  h->sat_rows = rows;
+  int rc = 0;
+  if (!rc) rc = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess ? 0 : -1;
+  h->own_stream = true;
+  if (!rc) rc = dev_alloc(&h->state, (size_t)2 * h->ncomp * h->P);
+  if (!rc) rc = dev_alloc(&h->lw, (size_t)h->P);
+  if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
+  if (!rc) rc = dev_alloc(&h->anc, (size_t)2 * h->P);
+  if (!rc) rc = dev_alloc(&h->part, (size_t)tiles * GG_NPART);
+  if (!rc) rc = dev_alloc(&h->g, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->Pb, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->pmax, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
+  if (!rc) rc = dev_alloc(&h->scratch_f, (size_t)n_filters * 16);
+  if (!rc) rc = dev_alloc(&h->filt, (size_t)n_filters);
+  if (!rc) rc = dev_alloc(&h->dyn, (size_t)n_filters);
+  if (!rc) rc = dev_alloc(&h->tile_filter, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->chunk_filter, (size_t)chunks);
+  if (!rc) rc = dev_alloc(&h->satd, (size_t)n_slots * 5 * rows);
+  if (!rc) rc = dev_alloc(&h->sati, (size_t)n_slots * (rows + n_filters));
+  if (!rc) rc = dev_alloc(&h->vel_guide, (size_t)n_slots * n_filters * 3);
+  if (!rc) rc = hipEventCreateWithFlags(&h->pin_done, hipEventDisableTiming) == hipSuccess ? 0 : -1;
+  if (!rc) rc = hipEventCreate(&h->ev0) == hipSuccess ? 0 : -1;
+  if (!rc) rc = hipEventCreate(&h->ev1) == hipSuccess ? 0 : -1;
+  if (!rc) rc = ensure_pinned(h, std::max<size_t>(64 * 1024, sizeof(gg_dyn) * n_filters));
+  if (!rc) {
+    rc = hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter) * n_filters,
+                   hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+  }
+  if (!rc) {
+    rc = hipMemcpy(h->tile_filter, h->tile_filter_h.data(), sizeof(int32_t) * tiles,
+                   hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+  }
+  if (!rc) {
+    rc = hipMemcpy(h->chunk_filter, h->chunk_filter_h.data(), sizeof(int32_t) * chunks,
+                   hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+  }
+  if (rc) {
+    std::string msg = gg_last_error_str();
+    gg_pf_destroy(h);
+    return gg_fail(-1, msg.empty() ? "gg_pf_create: HIP allocation failed" : msg.c_str());
+  }
+  *out = h;
+  return 0;
+}
+
+int gg_pf_device_create(int32_t n_particles, int32_t n_isb, gg_pf** out) {
+  const int32_t cap = 64;
+  return gg_pf_create(1, &n_particles, &cap, n_isb, 1, 1, out);
+}
+
+int gg_pf_destroy(gg_pf* h) {
+  if (!h) return 0;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,         h->part,         h->g,
+                  h->Pb,    h->emax,  h->pmax,      h->csum,        h->scratch_f,    h->filt,
+                  h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
+                  h->io_tmp};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (h->pin) hipHostFree(h->pin);
+  if (h->pin_done) hipEventDestroy(h->pin_done);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int gg_pf_set_stream(gg_pf* h, void* stream) {
+  if (!h) return gg_fail(-2, "null handle");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  if (stream) {
+    if (h->own_stream) GG_HIP(hipStreamDestroy(h->stream));
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+  } else if (!h->own_stream) {
+    GG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+  }
+  return 0;
+}
+
+int gg_pf_device_sync(gg_pf* h) {
+  if (!h) return 0;
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pf_info(gg_pf* h, int32_t* n_filters, int32_t* n_isb, int64_t* arena_slots) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (n_filters) *n_filters = h->n_filters;
+  if (n_isb) *n_isb = h->n_isb;
+  if (arena_slots) *arena_slots = h->P;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// single-filter API
+// ---------------------------------------------------------------------------
+int gg_pf_device_initialize(gg_pf* h, double init_x, double init_y, double init_z,
+                            double init_cb, double spread_pos, double spread_cb, uint64_t seed,
+                            double init_vx, double init_vy, double init_vz, double spread_vel,
+                            double init_vel_sigma, const double* init_isb, double spread_isb) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (spread_vel > 0.0 && !h->has_vel)
+    return gg_fail(-2, "spread_vel > 0 needs velocity storage (with_velocity)");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  for (int f = 0; f < h->n_filters; ++f) h->filt_h[f].seed = seed;
+  GG_HIP(hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter) * h->n_filters,
+                   hipMemcpyHostToDevice));
+  h->seed0 = seed;
+  for (int f = 0; f < h->n_filters; ++f) {
+    gg_dyn d;
+    std::memset(&d, 0, sizeof(d));
+    d.anchor[0] = init_x;
+    d.anchor[1] = init_y;
+    d.anchor[2] = init_z;
+    d.cb_anchor = init_cb;
+    for (int k = 0; k < h->n_isb; ++k) d.isb_anchor[k] = init_isb ? init_isb[k] : 0.0;
+    enu_rotation(d.anchor, d.R);
+    const double v[3] = {init_vx, init_vy, init_vz};
+    d.vel_u[0] = d.R[0] * v[0] + d.R[1] * v[1] + d.R[2] * v[2];
+    d.vel_u[1] = d.R[3] * v[0] + d.R[4] * v[1] + d.R[5] * v[2];
+    d.vel_u[2] = d.R[6] * v[0] + d.R[7] * v[1] + d.R[8] * v[2];
+    const double vv = init_vel_sigma * init_vel_sigma;
+    d.vcov_u[0] = vv;
+    d.vcov_u[3] = vv;
+    d.vcov_u[5] = vv;
+    d.vel_mode = (spread_vel > 0.0) ? 1 : 0;
+    d.lw_zero = 1;
+    d.ess = 0.0;
+    GG_HIP(hipMemcpy(h->dyn + f, &d, sizeof(d), hipMemcpyHostToDevice));
+  }
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_init<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, h->stream, a,
+                                                (float)spread_pos, (float)spread_cb,
+                                                (float)spread_isb, (float)spread_vel));
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(h->stream));
+  h->initialized = true;
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
+int gg_pf_device_predict(gg_pf* h, double vx, double vy, double vz, double dt, double sigma_pos,
+                         double sigma_cb, uint64_t seed, int32_t step, double sigma_vel,
+                         double velocity_guide_alpha, int32_t velocity_kf,
+                         double velocity_process_noise, double sigma_isb) {
+  if (int rc = check_single(h)) return rc;
+  if (sigma_vel > 0.0 && !velocity_kf && !h->has_vel)
+    return gg_fail(-2, "sigma_vel > 0 needs velocity storage (with_velocity)");
+  if (seed != h->filt_h[0].seed) {
+    GG_HIP(hipStreamSynchronize(h->stream));
+    for (auto& F : h->filt_h) F.seed = seed;
+    GG_HIP(hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter) * h->n_filters,
+                     hipMemcpyHostToDevice));
+  }
+  if (ensure_pinned(h, h->pin_bytes)) return -1;
+  double* pv = (double*)h->pin;
+  for (int f = 0; f < h->n_filters; ++f) {
+    pv[f * 3 + 0] = vx;
+    pv[f * 3 + 1] = vy;
+    pv[f * 3 + 2] = vz;
+  }
+  GG_HIP(hipMemcpyAsync(h->vel_guide, pv, sizeof(double) * 3 * h->n_filters,
+                        hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  gg_step_args s = default_args();
+  s.dt = dt;
+  s.sigma_pos = (float)sigma_pos;
+  s.sigma_cb = (float)sigma_cb;
+  s.sigma_isb = (float)sigma_isb;
+  s.sigma_vel = (float)sigma_vel;
+  s.alpha = (float)velocity_guide_alpha;
+  s.velocity_kf = velocity_kf ? 1 : 0;
+  s.qv = (float)velocity_process_noise;
+  s.step = step;
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_predict<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, h->stream, a, s));
+  GG_HIP(hipGetLastError());
+  h->est_valid = false;
+  return 0;
+}
+
+int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                        const double* weights_sat, const int32_t* sys, int32_t n_sat,
+                        double sigma_pr, double nu, double per_particle_nlos_threshold_m,
+                        int32_t per_particle_huber, double per_particle_huber_k) {
+  if (int rc = check_single(h)) return rc;
+  if (n_sat < 0) return gg_fail(-2, "n_sat must be >= 0");
+  if (h->n_filters != 1) return gg_fail(-2, "single-filter call on a batch handle");
+  if (n_sat > h->filt_h[0].sat_cap) {
+    // grow the satellite table (rare path; the reference grows past 64 too)
+    GG_HIP(hipStreamSynchronize(h->stream));
+    const int cap = std::max(n_sat, 2 * h->filt_h[0].sat_cap);
+    h->filt_h[0].sat_cap = cap;
+    h->sat_rows = cap;
+    GG_HIP(hipFree(h->satd));
+    GG_HIP(hipFree(h->sati));
+    h->satd = nullptr;
+    h->sati = nullptr;
+    if (dev_alloc(&h->satd, (size_t)h->n_slots * 5 * cap)) return -1;
+    if (dev_alloc(&h->sati, (size_t)h->n_slots * (cap + 1))) return -1;
+    GG_HIP(hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter), hipMemcpyHostToDevice));
+  }
+  const int rows = h->sat_rows;
+  const size_t dbytes = sizeof(double) * 5 * rows;
+  const size_t ibytes = sizeof(int32_t) * (rows + 1);
+  if (ensure_pinned(h, std::max(h->pin_bytes, dbytes + ibytes + 64))) return -1;
+  double* pd = (double*)h->pin;
+  std::memset(pd, 0, dbytes);
+  for (int k = 0; k < n_sat; ++k) {
+    pd[k * 3 + 0] = sat_ecef[k * 3 + 0];
+    pd[k * 3 + 1] = sat_ecef[k * 3 + 1];
+    pd[k * 3 + 2] = sat_ecef[k * 3 + 2];
+    pd[3 * rows + k] = pseudoranges[k];
+    pd[4 * rows + k] = weights_sat ? weights_sat[k] : 1.0;
+  }
+  int32_t* pi = (int32_t*)(h->pin + dbytes);
+  for (int k = 0; k < rows; ++k) pi[k] = (sys && k < n_sat) ? sys[k] : 0;
+  pi[rows] = n_sat;
+  GG_HIP(hipMemcpyAsync(h->satd, pd, dbytes, hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipMemcpyAsync(h->sati, pi, ibytes, hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  gg_step_args s = default_args();
+  s.sigma_pr = (float)sigma_pr;
+  s.nu = (float)nu;
+  s.gate_thr = (float)per_particle_nlos_threshold_m;
+  s.huber_k = (float)per_particle_huber_k;
+  const bool student = nu > 0.0;
+  s.lik = student ? GG_LIK_STUDENT
+                  : ((per_particle_huber && per_particle_huber_k > 0.0) ? GG_LIK_HUBER
+                                                                        : GG_LIK_GAUSS);
+  s.auto_flag = 0;
+  if (launch_weight(h, s, false)) return -1;
+  h->decide_valid = true;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_ess(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  if (!h->ess_valid && ensure_decide(h)) return -1;
+  gg_dyn d;
+  if (read_dyn(h, 0, &d)) return -1;
+  *out = d.ess;
+  return 0;
+}
+
+int gg_pf_device_position_spread(gg_pf* h, double cx, double cy, double cz, double* out) {
+  if (int rc = check_single(h)) return rc;
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_spread<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, h->stream, a, cx, cy, cz));
+  GG_HIP(hipGetLastError());
+  // the spread kernel reuses the tile-partial buffer
+  h->decide_valid = false;
+  GG_HIP(hipMemcpyAsync(h->pin, h->scratch_f, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  *out = *(double*)h->pin;
+  return 0;
+}
+
+int gg_pf_device_resample_systematic(gg_pf* h, uint64_t seed, int32_t u0_mode) {
+  if (int rc = check_single(h)) return rc;
+  if (ensure_decide(h)) return -1;
+  gg_step_args s = default_args();
+  if (u0_mode == 0) {
+    s.u0_mode = 2;
+    const int n = h->filt_h[0].n;
+    const int32_t den = (int32_t)(uint32_t)((uint64_t)1000000u * (uint64_t)(uint32_t)n);
+    s.u0_value = (double)(seed % 1000000ull) / (double)den;
+  } else {
+    s.u0_mode = 1;
+  }
+  if (s.u0_mode == 1) {
+    // fresh draw from the filter's stream, evaluated on the host with the same
+    // Philox as the device
+    gg_dyn d;
+    if (read_dyn(h, 0, &d)) return -1;
+    const gg_u32x4 r = gg_draw(seed, 0u, (uint32_t)d.resample_count, GG_PURPOSE_U0, 0u);
+    s.u0_mode = 2;
+    s.u0_value = gg_u01d(r.v[0], r.v[1]) - 1.1102230246251565e-16;
+  }
+  if (launch_cdf(h, s, 1)) return -1;
+  if (launch_gather(h, s, 1, 0, 0)) return -1;
+  h->decide_valid = false;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_resample_megopolis(gg_pf* h, int32_t n_iterations, uint64_t seed) {
+  if (int rc = check_single(h)) return rc;
+  if (n_iterations < 0) return gg_fail(-2, "n_iterations must be >= 0");
+  if (ensure_decide(h)) return -1;
+  gg_step_args s = default_args();
+  int fb = 0;
+  if (n_iterations == 0) {
+    if (launch_gather(h, s, 1, 2, 0)) return -1;
+  } else {
+    if (launch_mego_ref(h, n_iterations, 1, seed, 1, &fb)) return -1;
+    if (launch_gather(h, s, 1, 1, fb)) return -1;
+  }
+  h->decide_valid = false;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_resample_megopolis_chesser(gg_pf* h, int32_t n_iterations) {
+  if (int rc = check_single(h)) return rc;
+  if (n_iterations < 0) return gg_fail(-2, "n_iterations must be >= 0");
+  if (ensure_decide(h)) return -1;
+  gg_step_args s = default_args();
+  if (launch_mego(h, n_iterations, 1)) return -1;
+  if (launch_gather(h, s, 1, 1, 1)) return -1;
+  h->decide_valid = false;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_estimate_full(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  if (!h->est_valid && ensure_decide(h)) return -1;
+  gg_dyn d;
+  if (read_dyn(h, 0, &d)) return -1;
+  for (int k = 0; k < h->D; ++k) out[k] = d.est[k];
+  return 0;
+}
+
+int gg_pf_device_estimate(gg_pf* h, double* out) {
+  double tmp[GG_MAX_D];
+  if (int rc = gg_pf_device_estimate_full(h, tmp)) return rc;
+  for (int k = 0; k < 4; ++k) out[k] = tmp[k];
+  return 0;
+}
+
+static int export_rows(gg_pf* h, int f, int layout, double* out) {
+  const int n = h->filt_h[f].n;
+  const int ncol = (layout == 0) ? 4 : (layout == 1 ? 16 : h->D);
+  const size_t bytes = sizeof(double) * (size_t)n * ncol;
+  if (ensure_io(h, bytes)) return -1;
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_export<NI>), dim3(grid), dim3(GG_BLOCK),
+                                                0, h->stream, a, f, layout, h->io_tmp));
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipMemcpyAsync(out, h->io_tmp, bytes, hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pf_device_get_particles(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  return export_rows(h, 0, 0, out);
+}
+
+int gg_pf_device_get_particles_full(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  return export_rows(h, 0, 2, out);
+}
+
+int gg_pf_device_get_particle_states(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  return export_rows(h, 0, 1, out);
+}
+
+int gg_pf_device_set_particle_states(gg_pf* h, const double* in) {
+  if (int rc = check_single(h)) return rc;
+  const int n = h->filt_h[0].n;
+  gg_dyn d;
+  if (read_dyn(h, 0, &d)) return -1;
+  // new anchor: mean position and clock bias (binary64, host)
+  double m[4] = {0, 0, 0, 0};
+  bool vel_uniform = true, cov_uniform = true;
+  for (int i = 0; i < n; ++i) {
+    const double* r = in + (size_t)i * 16;
+    for (int k = 0; k < 4; ++k) m[k] += r[k];
+    for (int k = 4; k < 7; ++k) vel_uniform = vel_uniform && (r[k] == in[k]);
+    for (int k = 7; k < 16; ++k) cov_uniform = cov_uniform && (r[k] == in[k]);
+  }
+  for (int k = 0; k < 4; ++k) m[k] /= (double)n;
+  if ((!vel_uniform || !cov_uniform) && !h->has_vel)
+    return gg_fail(-2, "non-uniform velocity states need velocity storage (with_velocity)");
+  d.anchor[0] = m[0];
+  d.anchor[1] = m[1];
+  d.anchor[2] = m[2];
+  d.cb_anchor = m[3];
+  for (int k = 0; k < GG_MAX_ISB; ++k) d.isb_anchor[k] = d.isb_anchor[k];
+  const double* R = d.R;
+  d.vel_u[0] = R[0] * in[4] + R[1] * in[5] + R[2] * in[6];
+  d.vel_u[1] = R[3] * in[4] + R[4] * in[5] + R[5] * in[6];
+  d.vel_u[2] = R[6] * in[4] + R[7] * in[5] + R[8] * in[6];
+  {
+    double Sl[9];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) {
+        double acc = 0.0;
+        for (int u = 0; u < 3; ++u)
+          for (int v = 0; v < 3; ++v) acc += R[p * 3 + u] * in[7 + u * 3 + v] * R[q * 3 + v];
+        Sl[p * 3 + q] = acc;
+      }
+    d.vcov_u[0] = Sl[0];
+    d.vcov_u[1] = Sl[1];
+    d.vcov_u[2] = Sl[2];
+    d.vcov_u[3] = Sl[4];
+    d.vcov_u[4] = Sl[5];
+    d.vcov_u[5] = Sl[8];
+  }
+  d.vel_mode = !cov_uniform ? 2 : (!vel_uniform ? 1 : 0);
+  d.lw_zero = 1;
+  d.flag = 0;
+  d.est_valid = 0;
+  d.cnt_a = d.cnt_b = d.cnt_c = d.cnt_d = 0;
+  if (write_dyn(h, 0, d)) return -1;
+  const size_t bytes = sizeof(double) * (size_t)n * 16;
+  if (ensure_io(h, bytes)) return -1;
+  GG_HIP(hipMemcpyAsync(h->io_tmp, in, bytes, hipMemcpyHostToDevice, h->stream));
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_import<NI>), dim3(grid), dim3(GG_BLOCK),
+                                                0, h->stream, a, 0, h->io_tmp));
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(h->stream));
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
+int gg_pf_device_get_log_weights(gg_pf* h, double* out) {
+  if (int rc = check_single(h)) return rc;
+  return gg_pfb_get_log_weights(h, 0, out);
+}
+
+int gg_pf_device_set_log_weights(gg_pf* h, const double* in) {
+  if (int rc = check_single(h)) return rc;
+  const int n = h->filt_h[0].n;
+  const size_t bytes = sizeof(float) * (size_t)n;
+  if (ensure_pinned(h, std::max(h->pin_bytes, bytes))) return -1;
+  float* pf = (float*)h->pin;
+  for (int i = 0; i < n; ++i) pf[i] = (float)in[i];
+  GG_HIP(hipMemcpyAsync(h->lw + h->filt_h[0].p_off, pf, bytes, hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  hipLaunchKernelGGL(gg_k_clear_lw_zero, dim3(1), dim3(64), 0, h->stream, h->dyn, 0);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(h->stream));
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
+int gg_pf_device_get_resample_ancestors(gg_pf* h, int32_t* out) {
+  if (int rc = check_single(h)) return rc;
+  GG_HIP(hipMemcpyAsync(out, h->anc + h->filt_h[0].p_off, sizeof(int32_t) * h->filt_h[0].n,
+                        hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pf_device_shift_clock_bias(gg_pf* h, double shift) {
+  if (int rc = check_single(h)) return rc;
+  hipLaunchKernelGGL(gg_k_frame_shift, dim3(1), dim3(64), 0, h->stream, h->dyn, 0, 0.0, 0.0, 0.0,
+                     shift);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int gg_pf_device_shift_position(gg_pf* h, double dx, double dy, double dz) {
+  if (int rc = check_single(h)) return rc;
+  hipLaunchKernelGGL(gg_k_frame_shift, dim3(1), dim3(64), 0, h->stream, h->dyn, 0, dx, dy, dz,
+                     0.0);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int gg_pf_device_position_update(gg_pf* h, double ref_x, double ref_y, double ref_z,
+                                 double sigma_pos) {
+  if (int rc = check_single(h)) return rc;
+  const int n = h->filt_h[0].n;
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  const double inv_s2 = 1.0 / (sigma_pos * sigma_pos);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_position_update<NI>), dim3(grid),
+                                                dim3(GG_BLOCK), 0, h->stream, a, 0, ref_x, ref_y,
+                                                ref_z, inv_s2));
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_clear_lw_zero, dim3(1), dim3(64), 0, h->stream, h->dyn, 0);
+  GG_HIP(hipGetLastError());
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched API
+// ---------------------------------------------------------------------------
+int gg_pfb_initialize(gg_pf* h, const double* init, const uint64_t* seeds, double spread_pos,
+                      double spread_cb, double spread_isb) {
+  if (!h) return gg_fail(-2, "null handle");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  for (int f = 0; f < h->n_filters; ++f) h->filt_h[f].seed = seeds[f];
+  GG_HIP(hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter) * h->n_filters,
+                   hipMemcpyHostToDevice));
+  std::vector<gg_dyn> dv(h->n_filters);
+  for (int f = 0; f < h->n_filters; ++f) {
+    gg_dyn& d = dv[f];
+    std::memset(&d, 0, sizeof(d));
+    const double* x = init + (size_t)f * h->D;
+    d.anchor[0] = x[0];
+    d.anchor[1] = x[1];
+    d.anchor[2] = x[2];
+    d.cb_anchor = x[3];
+    for (int k = 0; k < h->n_isb; ++k) d.isb_anchor[k] = x[4 + k];
+    enu_rotation(d.anchor, d.R);
+    d.lw_zero = 1;
+  }
+  GG_HIP(hipMemcpy(h->dyn, dv.data(), sizeof(gg_dyn) * h->n_filters, hipMemcpyHostToDevice));
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_init<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, h->stream, a,
+                                                (float)spread_pos, (float)spread_cb,
+                                                (float)spread_isb, 0.0f));
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(h->stream));
+  h->initialized = true;
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_load_epochs(gg_pf* h, int32_t n_slots, const double* sat_xyz, const double* pr,
+                       const double* w, const int32_t* sys, const int32_t* n_sat,
+                       const double* vel) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (n_slots <= 0 || n_slots > h->n_slots) return gg_fail(-2, "n_slots exceeds the handle's slots");
+  const int rows = h->sat_rows, F = h->n_filters;
+  for (int s = 0; s < n_slots; ++s)
+    for (int f = 0; f < F; ++f) {
+      const int ns = n_sat[(size_t)s * F + f];
+      if (ns < 0 || ns > h->filt_h[f].sat_cap)
+        return gg_fail(-2, "n_sat exceeds the filter's sat_cap");
+    }
+  std::vector<double> hd((size_t)n_slots * 5 * rows, 0.0);
+  std::vector<int32_t> hi((size_t)n_slots * (rows + F), 0);
+  for (int s = 0; s < n_slots; ++s) {
+    double* d = hd.data() + (size_t)s * 5 * rows;
+    int32_t* ii = hi.data() + (size_t)s * (rows + F);
+    for (int r = 0; r < rows; ++r) {
+      const size_t src = (size_t)s * rows + r;
+      d[r * 3 + 0] = sat_xyz[src * 3 + 0];
+      d[r * 3 + 1] = sat_xyz[src * 3 + 1];
+      d[r * 3 + 2] = sat_xyz[src * 3 + 2];
+      d[3 * rows + r] = pr[src];
+      d[4 * rows + r] = w ? w[src] : 1.0;
+      ii[r] = sys ? sys[src] : 0;
+    }
+    for (int f = 0; f < F; ++f) ii[rows + f] = n_sat[(size_t)s * F + f];
+  }
+  GG_HIP(hipStreamSynchronize(h->stream));
+  GG_HIP(hipMemcpy(h->satd, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice));
+  GG_HIP(hipMemcpy(h->sati, hi.data(), hi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (vel) {
+    GG_HIP(hipMemcpy(h->vel_guide, vel, sizeof(double) * 3 * F * n_slots, hipMemcpyHostToDevice));
+  } else {
+    GG_HIP(hipMemset(h->vel_guide, 0, sizeof(double) * 3 * F * n_slots));
+  }
+  return 0;
+}
+
+int gg_pfb_step(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter) {
+  if (!h || !args) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  if (enqueue_step(h, *args, resampler, n_iter, true)) return -1;
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                   int32_t steps) {
+  if (!h || !args) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (!h->own_stream) return gg_fail(-2, "graph capture needs the handle's own stream");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  if (h->graph_exec) {
+    GG_HIP(hipGraphExecDestroy(h->graph_exec));
+    h->graph_exec = nullptr;
+  }
+  if (h->graph) {
+    GG_HIP(hipGraphDestroy(h->graph));
+    h->graph = nullptr;
+  }
+  GG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter, true);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return -1;
+  }
+  GG_HIP(e);
+  h->graph = g;
+  GG_HIP(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int gg_pfb_replay(gg_pf* h, int32_t n_replays) {
+  if (!h || !h->graph_exec) return gg_fail(-2, "no captured graph");
+  for (int k = 0; k < n_replays; ++k) GG_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total) {
+  if (!h || !h->graph_exec) return gg_fail(-2, "no captured graph");
+  GG_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int k = 0; k < n_replays; ++k) GG_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+  GG_HIP(hipEventRecord(h->ev1, h->stream));
+  GG_HIP(hipEventSynchronize(h->ev1));
+  GG_HIP(hipEventElapsedTime(ms_total, h->ev0, h->ev1));
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
+                      int32_t reps, float* ms_total) {
+  if (!h || !args) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  GG_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int k = 0; k < reps; ++k) {
+    int rc = 0;
+    switch (stage) {
+      case 0: rc = launch_weight(h, *args, true); break;
+      case 1: rc = launch_weight(h, *args, false); break;
+      case 2: rc = launch_cdf(h, *args, 1); break;
+      case 3: rc = launch_gather(h, *args, 1, 0, 0); break;
+      case 4: rc = launch_mego(h, n_iter, 1); break;
+      case 5: rc = launch_gather(h, *args, 1, 1, 1); break;
+      default: return gg_fail(-2, "unknown stage");
+    }
+    if (rc) return rc;
+  }
+  GG_HIP(hipEventRecord(h->ev1, h->stream));
+  GG_HIP(hipEventSynchronize(h->ev1));
+  GG_HIP(hipEventElapsedTime(ms_total, h->ev0, h->ev1));
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out) {
+  if (!h) return gg_fail(-2, "null handle");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  GG_HIP(hipMemcpy(out, h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int gg_pfb_set_epoch(gg_pf* h, int32_t epoch) {
+  if (!h) return gg_fail(-2, "null handle");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  std::vector<gg_dyn> dv(h->n_filters);
+  GG_HIP(hipMemcpy(dv.data(), h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost));
+  for (auto& d : dv) d.epoch = epoch;
+  GG_HIP(hipMemcpy(h->dyn, dv.data(), sizeof(gg_dyn) * h->n_filters, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int gg_pfb_get_particles(gg_pf* h, int32_t f, double* out) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (f < 0 || f >= h->n_filters) return gg_fail(-2, "filter index out of range");
+  return export_rows(h, f, 2, out);
+}
+
+int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (f < 0 || f >= h->n_filters) return gg_fail(-2, "filter index out of range");
+  gg_dyn d;
+  if (read_dyn(h, f, &d)) return -1;
+  const int n = h->filt_h[f].n;
+  if (d.lw_zero) {
+    for (int i = 0; i < n; ++i) out[i] = 0.0;
+    return 0;
+  }
+  const size_t bytes = sizeof(float) * (size_t)n;
+  if (ensure_pinned(h, std::max(h->pin_bytes, bytes))) return -1;
+  GG_HIP(hipMemcpyAsync(h->pin, h->lw + h->filt_h[f].p_off, bytes, hipMemcpyDeviceToHost,
+                        h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  const float* pf = (const float*)h->pin;
+  for (int i = 0; i < n; ++i) out[i] = (double)pf[i];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,527 @@
+// Particle-filter kernels for gfx950 (CDNA4, 64-lane wavefronts).
+//
+// One workgroup = 256 threads = 4 wavefronts; each thread owns GG_ITEMS (4)
+// consecutive particles, so a workgroup covers one tile of GG_TILE (1024)
+// particles and every SoA load/store is a 16-byte vector access that the
+// wavefront coalesces into one contiguous 1 KiB transaction.
+//
+// The step for a batch of filters is three launches:
+//   gg_k_weight   predict (optional) + pseudorange likelihood + per-tile
+//                 partials; the last tile of each filter combines the partials
+//                 (max, sum, ESS, weighted mean, CDF tile offsets) and takes
+//                 the resampling decision on the device.
+//   gg_k_cdf      systematic resampling, part 1: per-tile f64 CDF in a fixed
+//                 order and the output "end" index of every particle.
+//   gg_k_gather   merge-path partition of (ends, outputs), ancestor indices,
+//                 state gather with re-centring, post-resample mean.
+// Megopolis replaces gg_k_cdf by gg_k_mego (or B launches of gg_k_mego_ref).
+#pragma once
+
+#include "gg_device.h"
+
+struct gg_arena {
+  float* state;           // [2][ncomp][P]
+  float* lw;              // [P]
+  int32_t* E;             // [P]
+  int32_t* anc;           // [2][P]
+  double* part;           // [n_tiles_total][GG_NPART]
+  double* g;              // [n_tiles_total]
+  double* Pb;             // [n_tiles_total]
+  int32_t* emax;          // [n_tiles_total]
+  int32_t* pmax;          // [n_tiles_total]
+  double* csum;           // [n_chunks_total][GG_MAX_D]
+  double* scratch_f;      // [n_filters][16] misc per-filter results
+  const gg_filter* filt;  // [n_filters]
+  gg_dyn* dyn;            // [n_filters]
+  const int32_t* tile_filter;   // [n_tiles_total]
+  const int32_t* chunk_filter;  // [n_chunks_total]
+  // epoch slot s of the satellite table:
+  //   satd + s*5*sat_rows: xyz[sat_rows][3] | pr[sat_rows] | w[sat_rows]   (binary64)
+  //   sati + s*(sat_rows+n_filters): sys[sat_rows] | n_sat[n_filters]      (int32)
+  const double* satd;
+  const int32_t* sati;
+  const double* vel_guide;  // [n_slots][n_filters][3] (ECEF)
+  int64_t P;
+  int32_t ncomp;
+  int32_t n_isb;
+  int32_t has_vel;
+  int32_t sat_rows;
+  int32_t n_filters;
+  int32_t pad;
+};
+
+__device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) {
+  return a.state + ((int64_t)buf * a.ncomp + comp) * a.P;
+}
+
+// Satellite geometry staged in LDS as binary32 values relative to the anchor.
+struct gg_sat_view {
+  float* ux;
+  float* uy;
+  float* uz;
+  float* irho;  // 1 / |sat - anchor|
+  float* c;     // pr - |sat - anchor| - cb_anchor - isb_anchor
+  float* w;
+  int* isb;     // index into the ISB offsets, -1 for the reference system
+};
+
+// ---------------------------------------------------------------------------
+// Host-style helpers that run per block (binary64, identical in all blocks).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gg_rot_local(const double* R, const double* v, double* out) {
+  out[0] = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
+  out[1] = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
+  out[2] = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+}
+
+__device__ __forceinline__ void gg_rot_ecef(const double* R, const double* v, double* out) {
+  out[0] = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
+  out[1] = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
+  out[2] = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+}
+
+// Velocity bookkeeping of one predict call, computed identically by every
+// block of a filter.  v_ref is the part of the motion that moves the anchor;
+// particles only integrate (v_i - v_ref) * dt.
+struct gg_pred_plan {
+  double v_new_u[3];   // new uniform velocity (mode 0 staying uniform)
+  double v_ref[3];     // local-frame velocity carried by the anchor
+  float vg[3];         // guide, binary32 local frame
+  float vref_f[3];
+  float alpha;
+  int mode_in;
+  int mode_out;
+  int per_particle_noise;  // sample velocity noise per particle
+  // uniform-covariance Cholesky factor of sigma_pos^2 I + dt^2 Sigma_v (kf)
+  float L[6];  // l00 l10 l11 l20 l21 l22
+  double vcov_new[6];
+};
+
+__device__ __forceinline__ void gg_chol3(const double* a /*a00 a01 a02 a11 a12 a22*/, double* L) {
+  const double l00 = sqrt(fmax(a[0], 0.0));
+  double l10 = 0.0, l20 = 0.0;
+  if (l00 > 1e-12) {
+    l10 = a[1] / l00;
+    l20 = a[2] / l00;
+  }
+  const double l11 = sqrt(fmax(a[3] - l10 * l10, 0.0));
+  double l21 = 0.0;
+  if (l11 > 1e-12) l21 = (a[4] - l20 * l10) / l11;
+  const double l22 = sqrt(fmax(a[5] - l20 * l20 - l21 * l21, 0.0));
+  L[0] = l00;
+  L[1] = l10;
+  L[2] = l11;
+  L[3] = l20;
+  L[4] = l21;
+  L[5] = l22;
+}
+
+__device__ __forceinline__ gg_pred_plan gg_make_plan(const gg_dyn* dy, const gg_step_args& s,
+                                                     const double* vg_ecef) {
+  gg_pred_plan pl;
+  double vg[3];
+  gg_rot_local(dy->R, vg_ecef, vg);
+  const double alpha = fmin(1.0, fmax(0.0, (double)s.alpha));
+  pl.alpha = (float)alpha;
+  pl.mode_in = dy->vel_mode;
+  pl.per_particle_noise = (!s.velocity_kf && s.sigma_vel > 0.0f) ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    pl.vg[k] = (float)vg[k];
+    pl.v_new_u[k] = (1.0 - alpha) * dy->vel_u[k] + alpha * vg[k];
+  }
+  if (pl.mode_in == 0 && !pl.per_particle_noise) {
+    pl.mode_out = 0;
+    for (int k = 0; k < 3; ++k) {
+      pl.v_ref[k] = pl.v_new_u[k];
+      pl.vref_f[k] = 0.0f;
+    }
+  } else if (pl.mode_in == 0) {
+    pl.mode_out = 1;
+    for (int k = 0; k < 3; ++k) {
+      pl.vref_f[k] = (float)pl.v_new_u[k];
+      pl.v_ref[k] = (double)pl.vref_f[k];
+    }
+  } else {
+    pl.mode_out = pl.mode_in;
+    for (int k = 0; k < 3; ++k) {
+      pl.vref_f[k] = pl.vg[k];
+      pl.v_ref[k] = (double)pl.vg[k];
+    }
+  }
+  // uniform covariance path
+  const double dt = s.dt;
+  const double qdt = fmax((double)s.qv, 0.0) * fmax(dt, 0.0);
+  for (int k = 0; k < 6; ++k) pl.vcov_new[k] = dy->vcov_u[k];
+  for (int k = 0; k < 6; ++k) pl.L[k] = 0.0f;
+  if (s.velocity_kf) {
+    const double sp2 = (double)s.sigma_pos * (double)s.sigma_pos;
+    const double dt2 = dt * dt;
+    const double* c = dy->vcov_u;
+    double a6[6] = {sp2 + dt2 * c[0], dt2 * c[1], dt2 * c[2], sp2 + dt2 * c[3], dt2 * c[4],
+                    sp2 + dt2 * c[5]};
+    double L[6];
+    gg_chol3(a6, L);
+    for (int k = 0; k < 6; ++k) pl.L[k] = (float)L[k];
+    pl.vcov_new[0] += qdt;
+    pl.vcov_new[3] += qdt;
+    pl.vcov_new[5] += qdt;
+  }
+  return pl;
+}
+
+// Predict one particle in place (binary32 offsets).
+template <int NISB>
+__device__ __forceinline__ void gg_predict_one(const gg_arena& a, int cur, int64_t slot,
+                                               uint64_t seed, uint32_t lidx, uint32_t step,
+                                               const gg_step_args& s, const gg_pred_plan& pl,
+                                               float* p /*[4+NISB]*/) {
+  float z0[4];
+  gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 0u, z0);
+  const float dt = (float)s.dt;
+  float dv[3] = {0.0f, 0.0f, 0.0f};
+  if (pl.mode_out >= 1) {
+    float v[3];
+    if (pl.mode_in == 0) {
+      for (int k = 0; k < 3; ++k) v[k] = pl.vref_f[k];
+    } else {
+      for (int k = 0; k < 3; ++k) {
+        const float vi = gg_comp(a, cur, 4 + NISB + k)[slot];
+        v[k] = (1.0f - pl.alpha) * vi + pl.alpha * pl.vg[k];
+      }
+    }
+    if (pl.per_particle_noise) {
+      float zv[4];
+      gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 1u, zv);
+      for (int k = 0; k < 3; ++k) v[k] += zv[k] * s.sigma_vel;
+    }
+    for (int k = 0; k < 3; ++k) {
+      gg_comp(a, cur, 4 + NISB + k)[slot] = v[k];
+      dv[k] = (v[k] - pl.vref_f[k]) * dt;
+    }
+  }
+  float n0, n1, n2;
+  if (s.velocity_kf) {
+    float L[6];
+    if (pl.mode_out == 2) {
+      float* cv[6];
+      for (int k = 0; k < 6; ++k) cv[k] = gg_comp(a, cur, 7 + NISB + k);
+      const double sp2 = (double)s.sigma_pos * (double)s.sigma_pos;
+      const double dt2 = s.dt * s.dt;
+      double a6[6] = {sp2 + dt2 * cv[0][slot], dt2 * cv[1][slot], dt2 * cv[2][slot],
+                      sp2 + dt2 * cv[3][slot], dt2 * cv[4][slot], sp2 + dt2 * cv[5][slot]};
+      double Ld[6];
+      gg_chol3(a6, Ld);
+      for (int k = 0; k < 6; ++k) L[k] = (float)Ld[k];
+      const float qdt = fmaxf(s.qv, 0.0f) * fmaxf(dt, 0.0f);
+      cv[0][slot] += qdt;
+      cv[3][slot] += qdt;
+      cv[5][slot] += qdt;
+    } else {
+      for (int k = 0; k < 6; ++k) L[k] = pl.L[k];
+    }
+    n0 = L[0] * z0[0];
+    n1 = L[1] * z0[0] + L[2] * z0[1];
+    n2 = L[3] * z0[0] + L[4] * z0[1] + L[5] * z0[2];
+  } else {
+    n0 = z0[0] * s.sigma_pos;
+    n1 = z0[1] * s.sigma_pos;
+    n2 = z0[2] * s.sigma_pos;
+  }
+  p[0] += dv[0] + n0;
+  p[1] += dv[1] + n1;
+  p[2] += dv[2] + n2;
+  p[3] += z0[3] * s.sigma_cb;
+  if (NISB > 0) {
+    float zi[4];
+    gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 2u, zi);
+#pragma unroll
+    for (int k = 0; k < NISB; ++k) p[4 + k] += zi[k] * s.sigma_isb;
+  }
+}
+
+// Stage the filter's satellites for this epoch slot in LDS.
+__device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_filter& F,
+                                             const gg_dyn* dy, const double* anchor, int slot,
+                                             gg_sat_view& sv, char* smem) {
+  const double* sd = a.satd + (int64_t)slot * 5 * a.sat_rows;
+  const int32_t* si = a.sati + (int64_t)slot * (a.sat_rows + a.n_filters);
+  const int ns = si[a.sat_rows + f];
+  const int cap = F.sat_cap;
+  sv.ux = (float*)smem;
+  sv.uy = sv.ux + cap;
+  sv.uz = sv.uy + cap;
+  sv.irho = sv.uz + cap;
+  sv.c = sv.irho + cap;
+  sv.w = sv.c + cap;
+  sv.isb = (int*)(sv.w + cap);
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const int64_t r = F.sat_off + k;
+    double dx[3] = {sd[r * 3 + 0] - anchor[0], sd[r * 3 + 1] - anchor[1], sd[r * 3 + 2] - anchor[2]};
+    double d[3];
+    gg_rot_local(dy->R, dx, d);
+    const double rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const int sys = si[r];
+    const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
+    double c = sd[3 * a.sat_rows + r] - rho - dy->cb_anchor;
+    if (ib >= 0) c -= dy->isb_anchor[ib];
+    sv.ux[k] = (float)(d[0] / rho);
+    sv.uy[k] = (float)(d[1] / rho);
+    sv.uz[k] = (float)(d[2] / rho);
+    sv.irho[k] = (float)(1.0 / rho);
+    sv.c[k] = (float)c;
+    sv.w[k] = (float)sd[4 * a.sat_rows + r];
+    sv.isb[k] = ib;
+  }
+  __syncthreads();
+  return ns;
+}
+
+// Pseudorange residual pr - (|sat - x| + cb) for a binary32 offset particle,
+// using |sat-x| - rho = q / (1 + sqrt(1 + q/rho)), q = |p|^2/rho - 2 u.p,
+// which never forms the 2e7 m range in binary32.
+template <int NISB>
+__device__ __forceinline__ float gg_residual(const gg_sat_view& sv, int k, const float* p,
+                                             float pp) {
+  const float ud = sv.ux[k] * p[0] + sv.uy[k] * p[1] + sv.uz[k] * p[2];
+  const float ir = sv.irho[k];
+  const float q = pp * ir - 2.0f * ud;
+  const float dr = q / (1.0f + sqrtf(1.0f + q * ir));
+  float e = sv.c[k] - dr - p[3];
+  if (NISB > 0) {
+    const int ib = sv.isb[k];
+#pragma unroll
+    for (int j = 0; j < NISB; ++j)
+      if (ib == j) e -= p[4 + j];
+  }
+  return e;
+}
+
+template <int NISB>
+__device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const float* p,
+                                           const gg_step_args& s) {
+  const float pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+  const float sig = s.sigma_pr;
+  const float inv_s2 = 1.0f / (sig * sig);
+  bool gate = s.gate_thr > 0.0f;
+  if (gate) {
+    int kept = 0;
+    for (int k = 0; k < ns; ++k) {
+      const float e = gg_residual<NISB>(sv, k, p, pp);
+      kept += (fabsf(e) <= s.gate_thr) ? 1 : 0;
+    }
+    const int min_kept = ns < 4 ? ns : 4;
+    if (kept < min_kept) gate = false;
+  }
+  float lw = 0.0f;
+  if (s.lik == GG_LIK_STUDENT) {
+    const float half_nup1 = 0.5f * (s.nu + 1.0f);
+    const float inv_nu_s2 = 1.0f / (s.nu * sig * sig);
+    // sum of logs = log of the product; the product is taken over groups of
+    // three factors so that it stays far from binary32 overflow.
+    float acc = 0.0f;
+    float prod = 1.0f;
+    int cnt = 0;
+    for (int k = 0; k < ns; ++k) {
+      const float e = gg_residual<NISB>(sv, k, p, pp);
+      if (gate && fabsf(e) > s.gate_thr) continue;
+      prod *= 1.0f + sv.w[k] * e * e * inv_nu_s2;
+      if (++cnt == 3) {
+        acc += logf(prod);
+        prod = 1.0f;
+        cnt = 0;
+      }
+    }
+    acc += logf(prod);
+    lw = -half_nup1 * acc;
+  } else if (s.lik == GG_LIK_HUBER) {
+    const float hk = s.huber_k;
+    for (int k = 0; k < ns; ++k) {
+      const float e = gg_residual<NISB>(sv, k, p, pp);
+      if (gate && fabsf(e) > s.gate_thr) continue;
+      const float z = fabsf(e) / sig;
+      const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
+      lw -= sv.w[k] * rho;
+    }
+  } else {
+    for (int k = 0; k < ns; ++k) {
+      const float e = gg_residual<NISB>(sv, k, p, pp);
+      if (gate && fabsf(e) > s.gate_thr) continue;
+      lw += -0.5f * sv.w[k] * e * e * inv_s2;
+    }
+  }
+  return lw;
+}
+
+// ---------------------------------------------------------------------------
+// Tile partials: m_b = max lw, S_b = sum e, S2_b = sum e^2, sum_d = sum e p_d
+// with e = exp64(lw - m_b), each sum in the fixed order: per-thread left to
+// right over its 4 particles, then gg_block_scan256's total.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void gg_block_totals(double (&v)[D + 2], double* lds, double* out) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+#pragma unroll
+  for (int q = 0; q < D + 2; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double y = __shfl_up(x, d, 64);
+      if (lane >= d) x = y + x;
+    }
+    if (lane == 63) lds[q * 4 + w] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < D + 2; ++q)
+    out[q] = ((lds[q * 4 + 0] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
+  __syncthreads();
+}
+
+template <int D>
+__device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
+                                                 const bool (&valid)[GG_ITEMS],
+                                                 const float (&p)[GG_ITEMS][D], double* lds,
+                                                 double* part_out) {
+#pragma clang fp contract(off)
+  double m = -__builtin_huge_val();
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    if (valid[q]) m = fmax(m, (double)lw[q]);
+  const double mb = gg_block_max256(m, lds);
+  double e[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    e[q] = (valid[q] && mb > -__builtin_huge_val()) ? gg_exp64((double)lw[q] - mb) : 0.0;
+  double v[D + 2];
+  v[0] = e[0];
+  v[1] = e[0] * e[0];
+#pragma unroll
+  for (int d = 0; d < D; ++d) v[2 + d] = e[0] * (double)p[0][d];
+#pragma unroll
+  for (int q = 1; q < GG_ITEMS; ++q) {
+    v[0] = v[0] + e[q];
+    v[1] = v[1] + e[q] * e[q];
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + e[q] * (double)p[q][d];
+  }
+  double tot[D + 2];
+  gg_block_totals<D>(v, lds, tot);
+  if (threadIdx.x == 0) {
+    part_out[0] = mb;
+#pragma unroll
+    for (int q = 0; q < D + 2; ++q) part_out[1 + q] = tot[q];
+  }
+}
+
+// systematic offset u0 (reference formula, int32 product included).
+__device__ __forceinline__ double gg_u0(const gg_step_args& s, uint64_t seed, int n,
+                                        int resample_count) {
+  if (s.u0_mode == 2) return s.u0_value;
+  if (s.u0_mode == 1) {
+    const gg_u32x4 r = gg_draw(seed, 0u, (uint32_t)resample_count, GG_PURPOSE_U0, 0u);
+    return gg_u01d(r.v[0], r.v[1]) - 1.1102230246251565e-16;  // [0, 1)
+  }
+  const int32_t den = (int32_t)(uint32_t)((uint64_t)1000000u * (uint64_t)(uint32_t)n);
+  return (double)(seed % 1000000ull) / (double)den;
+}
+
+// Combine a filter's tile partials (run by the last block of the filter).
+// Chunked fixed order: thread t owns tiles [t*c, t*c+c), sums them left to
+// right from 0.0, and the per-thread values go through gg_block_scan256.
+template <int D>
+__device__ void gg_decide(const gg_arena& a, const gg_step_args& s, const gg_filter& F,
+                          gg_dyn* dy, const double* anchor, double* lds) {
+#pragma clang fp contract(off)
+  const int nt = F.n_tiles;
+  const int t = threadIdx.x;
+  const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
+  double m = -__builtin_huge_val();
+  for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
+  const double M = gg_block_max256(m, lds);
+  const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+  const int b0 = min(t * c, nt);
+  const int b1 = min(b0 + c, nt);
+  double T = 0.0, T2 = 0.0, Q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) Q[d] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const double* pb = part + (int64_t)b * GG_NPART;
+    const double mb = pb[0];
+    const double gb = (mb > -__builtin_huge_val()) ? gg_exp64(mb - M) : 0.0;
+    a.g[F.tile_off + b] = gb;
+    T = T + pb[1] * gb;
+    T2 = T2 + pb[2] * (gb * gb);
+#pragma unroll
+    for (int d = 0; d < D; ++d) Q[d] = Q[d] + pb[3 + d] * gb;
+  }
+  double excl, W;
+  gg_block_scan256(T, lds, excl, W);
+  // per-tile exclusive CDF offsets
+  double run = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const double* pb = part + (int64_t)b * GG_NPART;
+    a.Pb[F.tile_off + b] = excl + run;
+    run = run + pb[1] * a.g[F.tile_off + b];
+  }
+  double vv[D + 2];
+  vv[0] = T2;
+  vv[1] = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) vv[2 + d] = Q[d];
+  double tot[D + 2];
+  gg_block_totals<D>(vv, lds, tot);
+  if (t == 0) {
+    const double W2 = tot[0];
+    dy->M = M;
+    dy->W = W;
+    dy->W2 = W2;
+    const double ess = (W2 > 0.0) ? (W * W) / W2 : 0.0;
+    dy->ess = ess;
+    for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / W;
+    double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
+    double me[3];
+    gg_rot_ecef(dy->R, mp, me);
+    dy->est[0] = anchor[0] + me[0];
+    dy->est[1] = anchor[1] + me[1];
+    dy->est[2] = anchor[2] + me[2];
+    dy->est[3] = dy->cb_anchor + dy->mean[3];
+    for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
+    for (int d = 0; d < D; ++d) dy->shift[d] = s.recenter ? (float)dy->mean[d] : 0.0f;
+    if (s.auto_flag) dy->flag = (ess < s.ess_frac * (double)F.n) ? 1 : 0;
+    dy->u0 = gg_u0(s, F.seed, F.n, dy->resample_count);
+    dy->est_valid = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Load/store helpers for the 4 consecutive particles of one thread.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void gg_load4(const gg_arena& a, int buf, int64_t base,
+                                         float (&p)[GG_ITEMS][D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float4 v = *reinterpret_cast<const float4*>(gg_comp(a, buf, d) + base);
+    p[0][d] = v.x;
+    p[1][d] = v.y;
+    p[2][d] = v.z;
+    p[3][d] = v.w;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void gg_store4(const gg_arena& a, int buf, int64_t base,
+                                          const float (&p)[GG_ITEMS][D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float4 v;
+    v.x = p[0][d];
+    v.y = p[1][d];
+    v.z = p[2][d];
+    v.w = p[3][d];
+    *reinterpret_cast<float4*>(gg_comp(a, buf, d) + base) = v;
+  }
+}

@@ -1,0 +1,776 @@
+// Particle-filter kernels (see gg_pf_kernels.h for the launch plan).
+#include "gg_pf_kernels.h"
+
+#include <climits>
+
+// ---------------------------------------------------------------------------
+// initialise: offsets = spread * N(0,1) around the anchor the host wrote.
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_init(gg_arena a, float spread_pos,
+                                                      float spread_cb, float spread_isb,
+                                                      float spread_vel) {
+  constexpr int D = 4 + NISB;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) p[q][d] = 0.0f;
+    const int i = i0 + q;
+    if (i >= F.n) continue;
+    float z[4];
+    gg_normal4(F.seed, (uint32_t)i, 0u, GG_PURPOSE_INIT, 0u, z);
+    p[q][0] = z[0] * spread_pos;
+    p[q][1] = z[1] * spread_pos;
+    p[q][2] = z[2] * spread_pos;
+    p[q][3] = z[3] * spread_cb;
+    if (NISB > 0) {
+      gg_normal4(F.seed, (uint32_t)i, 0u, GG_PURPOSE_INIT, 1u, z);
+#pragma unroll
+      for (int k = 0; k < NISB; ++k) p[q][4 + k] = z[k] * spread_isb;
+    }
+    if (dy->vel_mode >= 1) {
+      gg_normal4(F.seed, (uint32_t)i, 0u, GG_PURPOSE_INIT, 2u, z);
+      for (int k = 0; k < 3; ++k)
+        gg_comp(a, cur, D + k)[base + q] = (float)dy->vel_u[k] + z[k] * spread_vel;
+    }
+  }
+  gg_store4<D>(a, cur, base, p);
+}
+
+// ---------------------------------------------------------------------------
+// predict only (drop-in path): in place on the current buffer.
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_predict(gg_arena a, gg_step_args s) {
+  constexpr int D = 4 + NISB;
+  __shared__ int sflag;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  const int slot = s.slot_from_device ? (dy->epoch % s.n_slots) : 0;
+  const uint32_t step = s.step_from_device ? (uint32_t)(dy->epoch + 1) : (uint32_t)s.step;
+  const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+  double vge[3] = {vg[0], vg[1], vg[2]};
+  const gg_pred_plan pl = gg_make_plan(dy, s, vge);
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, cur, base, p);
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    if (i0 + q < F.n) gg_predict_one<NISB>(a, cur, base + q, F.seed, i0 + q, step, s, pl, p[q]);
+  gg_store4<D>(a, cur, base, p);
+  if (gg_last_block(&dy->cnt_d, (unsigned)F.n_tiles, &sflag) && threadIdx.x == 0) {
+    double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
+    double ve[3];
+    gg_rot_ecef(dy->R, vr, ve);
+    for (int k = 0; k < 3; ++k) {
+      dy->anchor[k] += ve[k];
+      dy->vel_u[k] = pl.v_new_u[k];
+    }
+    dy->vel_mode = pl.mode_out;
+    if (s.velocity_kf && pl.mode_out < 2)
+      for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
+    dy->est_valid = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// predict (optional) + likelihood + tile partials + on-device decision.
+// ---------------------------------------------------------------------------
+template <int NISB, bool PRED>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args s) {
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ int sflag;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  const int epoch = dy->epoch;
+  const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
+  const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
+  double anchor[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
+  gg_pred_plan pl;
+  if (PRED) {
+    const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+    double vge[3] = {vg[0], vg[1], vg[2]};
+    pl = gg_make_plan(dy, s, vge);
+    double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
+    double ve[3];
+    gg_rot_ecef(dy->R, vr, ve);
+    for (int k = 0; k < 3; ++k) anchor[k] += ve[k];
+  }
+  gg_sat_view sv;
+  const int ns = gg_stage_sats(a, f, F, dy, anchor, slot, sv, smem);
+
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, cur, base, p);
+  bool valid[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
+  if (PRED) {
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q)
+      if (valid[q]) gg_predict_one<NISB>(a, cur, base + q, F.seed, i0 + q, step, s, pl, p[q]);
+    gg_store4<D>(a, cur, base, p);
+  }
+  float lw[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? gg_loglik<NISB>(sv, ns, p[q], s) : 0.0f;
+  *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
+  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
+
+  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
+    if (threadIdx.x == 0) {
+      if (PRED) {
+        for (int k = 0; k < 3; ++k) {
+          dy->anchor[k] = anchor[k];
+          dy->vel_u[k] = pl.v_new_u[k];
+        }
+        dy->vel_mode = pl.mode_out;
+        if (s.velocity_kf && pl.mode_out < 2)
+          for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
+      }
+      dy->lw_zero = 0;
+      dy->did_resample = 0;
+    }
+    gg_decide<D>(a, s, F, dy, anchor, lds);
+    if (threadIdx.x == 0 && (s.slot_from_device || s.step_from_device)) dy->epoch = epoch + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Re-run the reduction and decision over the stored log-weights.
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a, gg_step_args s) {
+  constexpr int D = 4 + NISB;
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ int sflag;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, cur, base, p);
+  bool valid[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
+  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!dy->lw_zero) {
+    const float4 v = *reinterpret_cast<const float4*>(a.lw + base);
+    lw[0] = v.x;
+    lw[1] = v.y;
+    lw[2] = v.z;
+    lw[3] = v.w;
+  }
+  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
+  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
+    double anchor[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
+    gg_decide<D>(a, s, F, dy, anchor, lds);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Systematic resampling, part 1.
+// cdf_i = (P_b + g_b * L_i) / W with L the fixed-order inclusive scan of
+// exp64(lw - m_b) inside the tile.  The particle's output end is
+// k(cdf_i) = #{ j : (u0 + j) / n <= cdf_i } (n for the last particle, which
+// reproduces the hi = n-1 clamp of a binary search).  Ancestor of output j is
+// min{ i : cdf_i >= t_j } = #{ i : E_i <= j } with E the prefix max of k, so
+// no monotonicity of the floating-point CDF is assumed.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
+#pragma clang fp contract(off)
+  const double dn = (double)n;
+  if (!(c >= (u0 + 0.0) / dn)) return 0;
+  if (c >= (u0 + (double)(n - 1)) / dn) return n;
+  double est = floor(c * dn - u0);
+  int j = (est < 0.0) ? 0 : (est > (double)(n - 1) ? n - 1 : (int)est);
+  while (j + 1 < n && (u0 + (double)(j + 1)) / dn <= c) ++j;
+  while (j > 0 && (u0 + (double)j) / dn > c) --j;
+  return j + 1;
+}
+
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s, int force) {
+#pragma clang fp contract(off)
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ int ilds[8];
+  __shared__ int sflag;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int lt = tile - F.tile_off;
+  const int n = F.n;
+  const double mb = a.part[(int64_t)tile * GG_NPART];
+  const double gb = a.g[tile];
+  const double Pb = a.Pb[tile];
+  const double W = dy->W;
+  const double u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!dy->lw_zero) {
+    const float4 v = *reinterpret_cast<const float4*>(a.lw + base);
+    lw[0] = v.x;
+    lw[1] = v.y;
+    lw[2] = v.z;
+    lw[3] = v.w;
+  }
+  double l[GG_ITEMS];
+  double run = 0.0;
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    const bool valid = (i0 + q) < n;
+    const double e = (valid && mb > -__builtin_huge_val()) ? gg_exp64((double)lw[q] - mb) : 0.0;
+    run = (q == 0) ? e : run + e;
+    l[q] = run;
+  }
+  double excl, total;
+  gg_block_scan256(l[GG_ITEMS - 1], lds, excl, total);
+  int E[GG_ITEMS];
+  int rmax = 0;
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    const int i = i0 + q;
+    int k;
+    if (i >= n) {
+      k = n;
+    } else if (i == n - 1) {
+      k = n;
+    } else {
+      const double L = excl + l[q];
+      const double cdf = (Pb + gb * L) / W;
+      k = gg_kcount(cdf, u0, n);
+    }
+    rmax = max(rmax, k);
+    E[q] = rmax;
+  }
+  const int pre_incl = gg_block_prefix_max256_i(rmax, ilds);
+  // exclusive prefix max of the thread maxima: shift by one thread
+  __shared__ int tmax[GG_BLOCK];
+  tmax[threadIdx.x] = pre_incl;
+  __syncthreads();
+  const int pre = (threadIdx.x == 0) ? 0 : tmax[threadIdx.x - 1];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) E[q] = max(E[q], pre);
+  *reinterpret_cast<int4*>(a.E + base) = make_int4(E[0], E[1], E[2], E[3]);
+  if (threadIdx.x == GG_BLOCK - 1) a.emax[tile] = E[GG_ITEMS - 1];
+  if (gg_last_block(&dy->cnt_b, (unsigned)F.n_tiles, &sflag)) {
+    // inclusive prefix max over the filter's tiles
+    int carry = 0;
+    for (int b0 = 0; b0 < F.n_tiles; b0 += GG_BLOCK) {
+      const int b = b0 + threadIdx.x;
+      const int v = (b < F.n_tiles) ? a.emax[F.tile_off + b] : 0;
+      int pm = gg_block_prefix_max256_i(v, ilds);
+      pm = max(pm, carry);
+      if (b < F.n_tiles) a.pmax[F.tile_off + b] = pm;
+      tmax[threadIdx.x] = pm;
+      __syncthreads();
+      carry = tmax[GG_BLOCK - 1];
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Systematic resampling, part 2 / common gather.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int gg_Eglob(const gg_arena& a, const gg_filter& F, int i) {
+  const int b = i / GG_TILE;
+  const int pm = (b > 0) ? a.pmax[F.tile_off + b - 1] : 0;
+  return max(pm, a.E[F.p_off + i]);
+}
+
+// #{ i in [0,n) : i + E_i < d }  (i + E_i is strictly increasing), found by
+// one full wavefront with a 64-ary search.
+__device__ __forceinline__ int gg_merge_search(const gg_arena& a, const gg_filter& F, int64_t d) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = F.n;
+  while (lo < hi) {
+    const int span = hi - lo;
+    const int step = (span + 63) >> 6;
+    const int64_t pidx = (int64_t)lo + (int64_t)lane * step;
+    bool pred = false;
+    if (pidx < hi) {
+      const int i = (int)pidx;
+      pred = ((int64_t)i + (int64_t)gg_Eglob(a, F, i)) < d;
+    }
+    const uint64_t m = __ballot(pred);
+    const int cnt = __popcll(m);
+    if (cnt == 0) {
+      hi = lo;
+    } else {
+      const int pl = lo + (cnt - 1) * step;
+      lo = pl + 1;
+      hi = min(pl + step, hi);
+    }
+  }
+  return lo;
+}
+
+// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args s, int force,
+                                                        int anc_mode, int src_buf) {
+  constexpr int D = 4 + NISB;
+  __shared__ int sE[GG_QCHUNK];
+  __shared__ int sA[GG_QCHUNK];
+  __shared__ int sI[2];
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ int sflag;
+  const int chunk = blockIdx.x;
+  const int f = a.chunk_filter[chunk];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int cl = chunk - F.chunk_off;
+  const int n = F.n;
+  const int cur = dy->cur;
+  const int nxt = 1 - cur;
+  int jbase, nout;
+  if (anc_mode == 0) {
+    const int64_t d0 = (int64_t)cl * GG_QCHUNK;
+    const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
+    if (threadIdx.x < 128) {
+      const int w = threadIdx.x >> 6;
+      const int i = gg_merge_search(a, F, w ? d1 : d0);
+      if ((threadIdx.x & 63) == 0) sI[w] = i;
+    }
+    __syncthreads();
+    const int ia = sI[0], ib = sI[1];
+    const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
+    for (int k = threadIdx.x; k < ib - ia; k += GG_BLOCK) sE[k] = gg_Eglob(a, F, ia + k);
+    __syncthreads();
+    const int na = ib - ia;
+    for (int k = threadIdx.x; k < j1 - j0; k += GG_BLOCK) {
+      const int j = j0 + k;
+      int lo = 0, hi = na;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sE[mid] <= j)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      sA[k] = ia + lo;
+    }
+    __syncthreads();
+    jbase = j0;
+    nout = j1 - j0;
+  } else {
+    jbase = cl * GG_QCHUNK;
+    nout = min(max(n - jbase, 0), GG_QCHUNK);
+  }
+  float sh[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) sh[d] = dy->shift[d];
+  const int vm = dy->vel_mode;
+  const int32_t* asrc = a.anc + (int64_t)src_buf * a.P + F.p_off;
+  double acc[D + 2];
+#pragma unroll
+  for (int d = 0; d < D + 2; ++d) acc[d] = 0.0;
+  for (int k = threadIdx.x; k < nout; k += GG_BLOCK) {
+    const int j = jbase + k;
+    const int src = (anc_mode == 0) ? sA[k] : (anc_mode == 1 ? asrc[j] : j);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float v = gg_comp(a, cur, d)[F.p_off + src] - sh[d];
+      gg_comp(a, nxt, d)[F.p_off + j] = v;
+      acc[2 + d] += (double)v;
+    }
+    if (vm >= 1) {
+      const int nv = (vm == 2) ? 9 : 3;
+      for (int k2 = 0; k2 < nv; ++k2)
+        gg_comp(a, nxt, D + k2)[F.p_off + j] = gg_comp(a, cur, D + k2)[F.p_off + src];
+    }
+    a.anc[F.p_off + j] = src;
+  }
+  double tot[D + 2];
+  gg_block_totals<D>(acc, lds, tot);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < D; ++d) a.csum[(int64_t)chunk * GG_MAX_D + d] = tot[2 + d];
+  if (gg_last_block(&dy->cnt_c, (unsigned)F.n_chunks, &sflag)) {
+    const int nc = F.n_chunks;
+    const int c = (nc + GG_BLOCK - 1) / GG_BLOCK;
+    const int c0 = min((int)threadIdx.x * c, nc);
+    const int c1 = min(c0 + c, nc);
+    double v[D + 2];
+#pragma unroll
+    for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
+    for (int cc = c0; cc < c1; ++cc)
+#pragma unroll
+      for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + a.csum[(int64_t)(F.chunk_off + cc) * GG_MAX_D + d];
+    gg_block_totals<D>(v, lds, tot);
+    if (threadIdx.x == 0) {
+      double shp[3] = {(double)sh[0], (double)sh[1], (double)sh[2]};
+      double she[3];
+      gg_rot_ecef(dy->R, shp, she);
+      for (int k = 0; k < 3; ++k) dy->anchor[k] += she[k];
+      dy->cb_anchor += (double)sh[3];
+      for (int d = 4; d < D; ++d) dy->isb_anchor[d - 4] += (double)sh[d];
+      for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / (double)n;
+      double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
+      double me[3];
+      gg_rot_ecef(dy->R, mp, me);
+      for (int k = 0; k < 3; ++k) dy->est[k] = dy->anchor[k] + me[k];
+      dy->est[3] = dy->cb_anchor + dy->mean[3];
+      for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
+      dy->cur = nxt;
+      dy->lw_zero = 1;
+      dy->flag = 0;
+      dy->did_resample = 1;
+      dy->resample_count += 1;
+      dy->ess = (double)n;
+      dy->M = 0.0;
+      dy->W = (double)n;
+      dy->W2 = (double)n;
+      dy->est_valid = 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Megopolis with the reference's semantics: slot weights lw[j] vs lw[i], a
+// fresh proposal per particle and iteration, state copied through ancestors.
+// Iteration `it` reads anc[it & 1] (identity for it == 0), writes anc[(it+1) & 1].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, int force,
+                                                          uint64_t seed_ovr, int use_ovr) {
+#pragma clang fp contract(off)
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int lt = tile - F.tile_off;
+  const int n = F.n;
+  const uint64_t seed = use_ovr ? seed_ovr : F.seed;
+  const bool lwz = dy->lw_zero != 0;
+  const float* lw = a.lw + F.p_off;
+  const int32_t* ar = a.anc + (int64_t)(it & 1) * a.P + F.p_off;
+  int32_t* aw = a.anc + (int64_t)((it + 1) & 1) * a.P + F.p_off;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    const int i = i0 + q;
+    if (i >= n) continue;
+    const gg_u32x4 r = gg_draw(seed, (uint32_t)i, (uint32_t)it, GG_PURPOSE_MEGO_REF, 0u);
+    const double u1 = gg_u01d(r.v[0], r.v[1]);
+    const double u2 = gg_u01d(r.v[2], r.v[3]);
+    const int off = (int)(u1 * (double)(n - 1)) + 1;
+    const int j = (int)(((int64_t)i + off) % n);
+    const double lwi = lwz ? 0.0 : (double)lw[i];
+    const double lwj = lwz ? 0.0 : (double)lw[j];
+    const double alpha = fmin(1.0, gg_exp64(lwj - lwi));
+    int src;
+    if (u2 < alpha)
+      src = (it == 0) ? j : ar[j];
+    else
+      src = (it == 0) ? i : ar[i];
+    aw[i] = src;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Megopolis (Chesser et al. 2021): Metropolis chains on the original weights
+// with one proposal offset per 256-particle segment and iteration, so the
+// wavefront's reads of lw[(i + O) mod n] are contiguous.  Writes anc[1].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int force) {
+#pragma clang fp contract(off)
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int lt = tile - F.tile_off;
+  const int n = F.n;
+  const uint32_t call = (uint32_t)dy->resample_count;
+  const bool lwz = dy->lw_zero != 0;
+  const float* lw = a.lw + F.p_off;
+  const int lane = threadIdx.x & 63;
+  const uint32_t seg = (uint32_t)(lt * (GG_TILE / 256) + (threadIdx.x >> 6));
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  int k[GG_ITEMS];
+  float lk[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    k[q] = i0 + q;
+    lk[q] = (lwz || i0 + q >= n) ? 0.0f : lw[i0 + q];
+  }
+  for (int b0 = 0; b0 < B; b0 += 64) {
+    uint32_t myO = 0;
+    if (b0 + lane < B) {
+      const gg_u32x4 r = gg_draw(F.seed, seg, call, GG_PURPOSE_MEGO, (uint32_t)(b0 + lane));
+      myO = (uint32_t)(((uint64_t)r.v[0] * (uint64_t)(uint32_t)n) >> 32);
+    }
+    const int bend = min(B - b0, 64);
+    for (int bb = 0; bb < bend; ++bb) {
+      const int b = b0 + bb;
+      const uint32_t O = (uint32_t)__shfl((int)myO, bb, 64);
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const int i = i0 + q;
+        if (i >= n) continue;
+        const gg_u32x4 r = gg_draw(F.seed, (uint32_t)i, call, GG_PURPOSE_MEGO_U, (uint32_t)(b >> 2));
+        const float u = gg_u01f(r.v[b & 3]);
+        int j = i + (int)O;
+        if (j >= n) j -= n;
+        const float lj = lwz ? 0.0f : lw[j];
+        const float dl = lj - lk[q];
+        // accept iff dl >= 0, or -18 <= dl < 0 and u < exp(dl); u >= 2^-24
+        // so exp(dl) < u for every dl < -18.  Evaluated without branches:
+        // the exp core runs on the clamped argument and the move is a select.
+        const float e = gg_exp32_core(fminf(fmaxf(dl, -18.0f), 0.0f));
+        const bool acc = (dl >= 0.0f) | ((dl >= -18.0f) & (u < e));
+        k[q] = acc ? j : k[q];
+        lk[q] = acc ? lj : lk[q];
+      }
+    }
+  }
+  *reinterpret_cast<int4*>(a.anc + a.P + F.p_off + i0) = make_int4(k[0], k[1], k[2], k[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Weighted RMS distance of the positions to a centre (local frame, binary32).
+// Writes scratch_f[f*16 + 0].
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_spread(gg_arena a, double cx, double cy,
+                                                        double cz) {
+#pragma clang fp contract(off)
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ int sflag;
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  double cl[3];
+  double ce[3] = {cx - dy->anchor[0], cy - dy->anchor[1], cz - dy->anchor[2]};
+  gg_rot_local(dy->R, ce, cl);
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][3];
+  gg_load4<3>(a, cur, base, p);
+  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!dy->lw_zero) {
+    const float4 v = *reinterpret_cast<const float4*>(a.lw + base);
+    lw[0] = v.x;
+    lw[1] = v.y;
+    lw[2] = v.z;
+    lw[3] = v.w;
+  }
+  bool valid[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
+  // partial: m_b, S_b, sum e*d^2 (binary64 distances)
+  double m = -__builtin_huge_val();
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    if (valid[q]) m = fmax(m, (double)lw[q]);
+  const double mb = gg_block_max256(m, lds);
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    if (!valid[q]) continue;
+    const double e = (mb > -__builtin_huge_val()) ? gg_exp64((double)lw[q] - mb) : 0.0;
+    const double dx = (double)p[q][0] - cl[0];
+    const double dyy = (double)p[q][1] - cl[1];
+    const double dz = (double)p[q][2] - cl[2];
+    v[0] = v[0] + e;
+    v[2] = v[2] + e * (dx * dx + dyy * dyy + dz * dz);
+  }
+  double tot[3];
+  gg_block_totals<1>(v, lds, tot);
+  if (threadIdx.x == 0) {
+    double* pb = a.part + (int64_t)tile * GG_NPART;
+    pb[0] = mb;
+    pb[1] = tot[0];
+    pb[2] = tot[2];
+  }
+  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
+    const int nt = F.n_tiles;
+    const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
+    double mm = -__builtin_huge_val();
+    for (int b = threadIdx.x; b < nt; b += GG_BLOCK) mm = fmax(mm, part[(int64_t)b * GG_NPART]);
+    const double M = gg_block_max256(mm, lds);
+    const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+    const int b0 = min((int)threadIdx.x * c, nt);
+    const int b1 = min(b0 + c, nt);
+    double w[3] = {0.0, 0.0, 0.0};
+    for (int b = b0; b < b1; ++b) {
+      const double* pb = part + (int64_t)b * GG_NPART;
+      const double gb = (pb[0] > -__builtin_huge_val()) ? gg_exp64(pb[0] - M) : 0.0;
+      w[0] = w[0] + pb[1] * gb;
+      w[2] = w[2] + pb[2] * gb;
+    }
+    gg_block_totals<1>(w, lds, tot);
+    if (threadIdx.x == 0) {
+      double r = 0.0;
+      if (tot[0] > 0.0 && isfinite(tot[2])) r = sqrt(fmax(tot[2] / tot[0], 0.0));
+      a.scratch_f[(int64_t)f * 16 + 0] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// State export / import (binary64 absolute values, AoS rows).
+// layout 0: [x y z cb]; 1: [x y z cb vx vy vz S00..S22] (16); 2: [x y z cb isb..]
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_export(gg_arena a, int f, int layout,
+                                                        double* out) {
+  constexpr int D = 4 + NISB;
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  double pl[3] = {(double)gg_comp(a, cur, 0)[s], (double)gg_comp(a, cur, 1)[s],
+                  (double)gg_comp(a, cur, 2)[s]};
+  double pe[3];
+  gg_rot_ecef(dy->R, pl, pe);
+  const int ncol = (layout == 0) ? 4 : (layout == 1 ? 16 : D);
+  double* o = out + (int64_t)i * ncol;
+  o[0] = dy->anchor[0] + pe[0];
+  o[1] = dy->anchor[1] + pe[1];
+  o[2] = dy->anchor[2] + pe[2];
+  o[3] = dy->cb_anchor + (double)gg_comp(a, cur, 3)[s];
+  if (layout == 2) {
+    for (int k = 4; k < D; ++k) o[k] = dy->isb_anchor[k - 4] + (double)gg_comp(a, cur, k)[s];
+  } else if (layout == 1) {
+    double vl[3], ve[3];
+    if (dy->vel_mode >= 1) {
+      for (int k = 0; k < 3; ++k) vl[k] = (double)gg_comp(a, cur, D + k)[s];
+    } else {
+      for (int k = 0; k < 3; ++k) vl[k] = dy->vel_u[k];
+    }
+    gg_rot_ecef(dy->R, vl, ve);
+    o[4] = ve[0];
+    o[5] = ve[1];
+    o[6] = ve[2];
+    double c6[6];
+    if (dy->vel_mode == 2) {
+      for (int k = 0; k < 6; ++k) c6[k] = (double)gg_comp(a, cur, D + 3 + k)[s];
+    } else {
+      for (int k = 0; k < 6; ++k) c6[k] = dy->vcov_u[k];
+    }
+    const double Sl[9] = {c6[0], c6[1], c6[2], c6[1], c6[3], c6[4], c6[2], c6[4], c6[5]};
+    // S_ecef = R^T S_local R
+    const double* R = dy->R;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) acc += R[p * 3 + r] * Sl[p * 3 + q] * R[q * 3 + c];
+        o[7 + r * 3 + c] = acc;
+      }
+  }
+}
+
+// Import 16-column reference rows; the host has already set the anchor frame
+// and vel_mode.
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_import(gg_arena a, int f, const double* in) {
+  constexpr int D = 4 + NISB;
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  const double* r = in + (int64_t)i * 16;
+  double de[3] = {r[0] - dy->anchor[0], r[1] - dy->anchor[1], r[2] - dy->anchor[2]};
+  double dl[3];
+  gg_rot_local(dy->R, de, dl);
+  for (int k = 0; k < 3; ++k) gg_comp(a, cur, k)[s] = (float)dl[k];
+  gg_comp(a, cur, 3)[s] = (float)(r[3] - dy->cb_anchor);
+  for (int k = 4; k < D; ++k) gg_comp(a, cur, k)[s] = 0.0f;
+  if (dy->vel_mode >= 1) {
+    double vl[3];
+    gg_rot_local(dy->R, r + 4, vl);
+    for (int k = 0; k < 3; ++k) gg_comp(a, cur, D + k)[s] = (float)vl[k];
+  }
+  if (dy->vel_mode == 2) {
+    // S_local = R S_ecef R^T
+    const double* R = dy->R;
+    double Sl[9];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) {
+        double acc = 0.0;
+        for (int u = 0; u < 3; ++u)
+          for (int v = 0; v < 3; ++v) acc += R[p * 3 + u] * r[7 + u * 3 + v] * R[q * 3 + v];
+        Sl[p * 3 + q] = acc;
+      }
+    const double c6[6] = {Sl[0], Sl[1], Sl[2], Sl[4], Sl[5], Sl[8]};
+    for (int k = 0; k < 6; ++k) gg_comp(a, cur, D + 3 + k)[s] = (float)c6[k];
+  }
+}
+
+// O(1) frame edits: shift position (ECEF) and clock bias of every particle.
+__global__ void gg_k_frame_shift(gg_dyn* dyn, int f, double dx, double dy_, double dz,
+                                 double dcb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  gg_dyn* d = dyn + f;
+  d->anchor[0] += dx;
+  d->anchor[1] += dy_;
+  d->anchor[2] += dz;
+  d->cb_anchor += dcb;
+  d->est[0] += dx;
+  d->est[1] += dy_;
+  d->est[2] += dz;
+  d->est[3] += dcb;
+}
+
+// Gaussian position-domain likelihood added to the log-weights.
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_position_update(gg_arena a, int f, double rx,
+                                                                 double ry, double rz,
+                                                                 double inv_s2) {
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  double de[3] = {rx - dy->anchor[0], ry - dy->anchor[1], rz - dy->anchor[2]};
+  double rl[3];
+  gg_rot_local(dy->R, de, rl);
+  const double dx = (double)gg_comp(a, cur, 0)[s] - rl[0];
+  const double dyy = (double)gg_comp(a, cur, 1)[s] - rl[1];
+  const double dz = (double)gg_comp(a, cur, 2)[s] - rl[2];
+  const float base = dy->lw_zero ? 0.0f : a.lw[s];
+  a.lw[s] = base + (float)(-0.5 * (dx * dx + dyy * dyy + dz * dz) * inv_s2);
+}
+
+__global__ void gg_k_clear_lw_zero(gg_dyn* dyn, int f) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dyn[f].lw_zero = 0;
+}

@@ -1,0 +1,374 @@
+"""Device-resident particle filter for GNSS pseudorange positioning (MI355X).
+
+Drop-in for ``gnss_gpu.particle_filter_device.ParticleFilterDevice``
+(python/gnss_gpu/particle_filter_device.py:18-886): same constructor keywords
+and defaults, same method names and return types.  The state stays on the GPU
+between calls; only satellite rows (per update), the velocity guide (per
+predict) and the requested results cross PCIe.
+
+Additions (keyword-only, reference behaviour by default):
+  n_isb / sigma_isb      inter-system-bias states for multi-GNSS pseudoranges
+                         (``update(..., system_ids=...)``)
+  megopolis_mode         "reference" (the reference kernel's rule) or "chesser"
+                         (Megopolis of Chesser et al. 2021)
+  megopolis_iterations   iterations per Megopolis resample (reference: 15)
+  systematic_u0          "reference" (u0 = (seed % 1e6) / (1e6 N), identical
+                         every call) or "fresh" (a new Philox u0 in [0,1))
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _gnss_gpu_pf_device as _dev
+
+
+class ParticleFilterDevice:
+    """High-performance particle filter with persistent GPU memory.
+
+    Parameters
+    ----------
+    n_particles : int
+        Number of particles (e.g., 1_000_000).
+    sigma_pos : float
+        Position noise standard deviation [m] for prediction.
+    sigma_cb : float
+        Clock bias noise standard deviation [m] for prediction.
+    sigma_pr : float
+        Pseudorange observation standard deviation [m] for weighting.
+    nu : float
+        Student's t degrees of freedom (0 = Gaussian, 1 = Cauchy).
+    resampling : str
+        "megopolis" or "systematic".
+    ess_threshold : float
+        Resample when ESS < ess_threshold * n_particles.
+    seed : int
+        Random seed.
+    """
+
+    def __init__(self, n_particles=1_000_000, sigma_pos=1.0, sigma_cb=300.0,
+                 sigma_pr=5.0, nu=0.0, resampling="megopolis", ess_threshold=0.5,
+                 seed=42, per_particle_nlos_gate=False,
+                 per_particle_nlos_dd_pr_threshold_m=10.0,
+                 per_particle_nlos_dd_carrier_threshold_cycles=0.5,
+                 per_particle_nlos_undiff_pr_threshold_m=30.0,
+                 per_particle_huber=False,
+                 per_particle_huber_dd_pr_k=1.5,
+                 per_particle_huber_dd_carrier_k=1.5,
+                 per_particle_huber_undiff_pr_k=1.5,
+                 sigma_vel=0.0,
+                 velocity_guide_alpha=1.0,
+                 rbpf_velocity_kf=False,
+                 velocity_process_noise=0.0,
+                 *, n_isb=0, sigma_isb=0.0, megopolis_mode="reference",
+                 megopolis_iterations=15, systematic_u0="reference"):
+        if megopolis_mode not in ("reference", "chesser"):
+            raise ValueError("megopolis_mode must be 'reference' or 'chesser'")
+        if systematic_u0 not in ("reference", "fresh"):
+            raise ValueError("systematic_u0 must be 'reference' or 'fresh'")
+        self.n_particles = n_particles
+        self.sigma_pos = sigma_pos
+        self.sigma_cb = sigma_cb
+        self.sigma_pr = sigma_pr
+        self.nu = nu
+        self.resampling = resampling
+        self.ess_threshold = ess_threshold
+        self.seed = seed
+        self.per_particle_nlos_gate = bool(per_particle_nlos_gate)
+        self.per_particle_nlos_dd_pr_threshold_m = float(per_particle_nlos_dd_pr_threshold_m)
+        self.per_particle_nlos_dd_carrier_threshold_cycles = float(
+            per_particle_nlos_dd_carrier_threshold_cycles)
+        self.per_particle_nlos_undiff_pr_threshold_m = float(
+            per_particle_nlos_undiff_pr_threshold_m)
+        self.per_particle_huber = bool(per_particle_huber)
+        self.per_particle_huber_dd_pr_k = float(per_particle_huber_dd_pr_k)
+        self.per_particle_huber_dd_carrier_k = float(per_particle_huber_dd_carrier_k)
+        self.per_particle_huber_undiff_pr_k = float(per_particle_huber_undiff_pr_k)
+        self.sigma_vel = float(sigma_vel)
+        self.velocity_guide_alpha = float(velocity_guide_alpha)
+        self.rbpf_velocity_kf = bool(rbpf_velocity_kf)
+        self.velocity_process_noise = float(velocity_process_noise)
+        self.n_isb = int(n_isb)
+        self.sigma_isb = float(sigma_isb)
+        self.megopolis_mode = megopolis_mode
+        self.megopolis_iterations = int(megopolis_iterations)
+        self.systematic_u0 = systematic_u0
+
+        self._state = _dev.pf_device_create(n_particles, self.n_isb)
+        self._initialized = False
+        self._step = 0
+
+    # -- helpers (particle_filter_device.py:138-146) --------------------------
+    def _per_particle_threshold(self, value):
+        if not self.per_particle_nlos_gate:
+            return 0.0
+        value = float(value)
+        return value if np.isfinite(value) and value > 0.0 else 0.0
+
+    def _per_particle_huber_k(self, value):
+        value = float(value)
+        return value if np.isfinite(value) and value > 0.0 else 1.5
+
+    def _require_init(self):
+        if not self._initialized:
+            raise RuntimeError("ParticleFilterDevice not initialized. Call initialize() first.")
+
+    # -- lifecycle -------------------------------------------------------------
+    def initialize(self, position_ecef, clock_bias=0.0, spread_pos=100.0,
+                   spread_cb=1000.0, velocity=None, spread_vel=0.0,
+                   velocity_init_sigma=0.0, *, isb=None, spread_isb=0.0):
+        """Scatter particles around an initial estimate (runs on the GPU)."""
+        pos = np.asarray(position_ecef, dtype=np.float64).ravel()
+        vel = (np.zeros(3, dtype=np.float64) if velocity is None
+               else np.asarray(velocity, dtype=np.float64).ravel())
+        _dev.pf_device_initialize(
+            self._state, float(pos[0]), float(pos[1]), float(pos[2]), float(clock_bias),
+            float(spread_pos), float(spread_cb), self.seed,
+            float(vel[0]), float(vel[1]), float(vel[2]), float(spread_vel),
+            float(velocity_init_sigma), init_isb=isb, spread_isb=spread_isb)
+        self._initialized = True
+        self._step = 0
+        self._velocity_init_sigma = float(velocity_init_sigma)
+
+    def predict(self, velocity=None, dt=1.0, sigma_pos=None, sigma_vel=None,
+                velocity_guide_alpha=None, rbpf_velocity_kf=None,
+                velocity_process_noise=None):
+        """Predict step with optional velocity guide (24 bytes to the GPU)."""
+        self._require_init()
+        if velocity is None:
+            vx, vy, vz = 0.0, 0.0, 0.0
+        else:
+            vel = np.asarray(velocity, dtype=np.float64).ravel()
+            vx, vy, vz = float(vel[0]), float(vel[1]), float(vel[2])
+        sp = float(self.sigma_pos if sigma_pos is None else sigma_pos)
+        sv = float(self.sigma_vel if sigma_vel is None else sigma_vel)
+        alpha = float(self.velocity_guide_alpha if velocity_guide_alpha is None
+                      else velocity_guide_alpha)
+        use_kf = self.rbpf_velocity_kf if rbpf_velocity_kf is None else bool(rbpf_velocity_kf)
+        qv = float(self.velocity_process_noise if velocity_process_noise is None
+                   else velocity_process_noise)
+        self._step += 1
+        _dev.pf_device_predict(self._state, vx, vy, vz, float(dt), sp, float(self.sigma_cb),
+                               self.seed, self._step, sv, alpha, use_kf, qv,
+                               sigma_isb=self.sigma_isb)
+
+    def update(self, sat_ecef, pseudoranges, weights=None, sigma_pr=None, resample=True,
+               *, system_ids=None):
+        """Pseudorange weight update (overwrites the log-weights, like the reference)."""
+        self._require_init()
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
+        n_sat = len(pr)
+        if weights is None:
+            weights = np.ones(n_sat, dtype=np.float64)
+        else:
+            weights = np.asarray(weights, dtype=np.float64).ravel()
+        sp = float(self.sigma_pr if sigma_pr is None else sigma_pr)
+        _dev.pf_device_weight(
+            self._state, sat.ravel(), pr, weights, n_sat, sp, float(self.nu),
+            self._per_particle_threshold(self.per_particle_nlos_undiff_pr_threshold_m),
+            self.per_particle_huber,
+            self._per_particle_huber_k(self.per_particle_huber_undiff_pr_k),
+            system_ids=system_ids)
+        if resample:
+            _ = self.resample_if_needed()
+
+    def update_gmm(self, *args, **kwargs):
+        return _dev.pf_device_weight_gmm(self._state, *args, **kwargs)
+
+    def update_dd_pseudorange(self, *args, **kwargs):
+        return _dev.pf_device_weight_dd_pseudorange(self._state, *args, **kwargs)
+
+    def update_carrier_afv(self, *args, **kwargs):
+        return _dev.pf_device_weight_carrier_afv(self._state, *args, **kwargs)
+
+    def update_dd_carrier_afv(self, *args, **kwargs):
+        return _dev.pf_device_weight_dd_carrier_afv(self._state, *args, **kwargs)
+
+    def update_doppler(self, *args, **kwargs):
+        return _dev.pf_device_weight_doppler(self._state, *args, **kwargs)
+
+    def update_doppler_kf(self, *args, **kwargs):
+        return _dev.pf_device_doppler_kf_update(self._state, *args, **kwargs)
+
+    def position_update(self, ref_ecef, sigma_pos=30.0):
+        """Gaussian position-domain likelihood added to the log-weights."""
+        self._require_init()
+        ref = np.asarray(ref_ecef, dtype=np.float64).ravel()
+        _dev.pf_device_position_update(self._state, float(ref[0]), float(ref[1]),
+                                       float(ref[2]), float(sigma_pos))
+
+    def shift_clock_bias(self, shift: float):
+        """Shift every particle's clock bias by a constant (O(1): moves the anchor)."""
+        self._require_init()
+        _dev.pf_device_shift_clock_bias(self._state, float(shift))
+
+    def recenter_position(self, ref_ecef, max_shift_m=10000.0):
+        """Shift the particle cloud centre to ``ref_ecef`` when within a cap."""
+        self._require_init()
+        ref = np.asarray(ref_ecef, dtype=np.float64).ravel()
+        if ref.size != 3:
+            raise ValueError("ref_ecef must have shape (3,)")
+        if not np.all(np.isfinite(ref)):
+            return float("inf"), False
+        est = np.asarray(self.estimate(), dtype=np.float64)
+        delta = ref[:3] - est[:3]
+        shift_norm = float(np.linalg.norm(delta))
+        cap = float(max_shift_m)
+        if cap > 0.0 and shift_norm > cap:
+            return shift_norm, False
+        _dev.pf_device_shift_position(self._state, float(delta[0]), float(delta[1]),
+                                      float(delta[2]))
+        return shift_norm, True
+
+    def correct_clock_bias(self, sat_ecef, pseudoranges, quantile=0.5):
+        """Re-centre the clock bias on a quantile of pseudorange residuals."""
+        self._require_init()
+        est = np.asarray(self.estimate(), dtype=np.float64)
+        pos = est[:3]
+        current_cb = est[3]
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
+        ranges = np.linalg.norm(sat - pos, axis=1)
+        residuals = pr - ranges
+        q = float(np.clip(quantile, 0.0, 1.0))
+        expected_cb = float(np.quantile(residuals, q))
+        _dev.pf_device_shift_clock_bias(self._state, expected_cb - current_cb)
+
+    # -- resampling ------------------------------------------------------------
+    def _resample(self):
+        if self.resampling == "megopolis":
+            if self.megopolis_mode == "chesser":
+                _dev.pf_device_resample_megopolis_chesser(self._state, self.megopolis_iterations)
+            else:
+                _dev.pf_device_resample_megopolis(self._state, self.megopolis_iterations,
+                                                  self.seed)
+        else:
+            _dev.pf_device_resample_systematic(
+                self._state, self.seed, 0 if self.systematic_u0 == "reference" else 1)
+
+    def resample_if_needed(self):
+        """Resample when ESS < ess_threshold * n_particles; returns True if it ran."""
+        ess = self.get_ess()
+        if ess < self.ess_threshold * self.n_particles:
+            self._resample()
+            return True
+        return False
+
+    # -- results ---------------------------------------------------------------
+    def estimate(self):
+        """Weighted mean [x, y, z, clock_bias] (ECEF, metres)."""
+        self._require_init()
+        return _dev.pf_device_estimate(self._state)
+
+    def estimate_full(self):
+        """Weighted mean [x, y, z, clock_bias, isb...]."""
+        self._require_init()
+        return _dev.pf_device_estimate_full(self._state)
+
+    def get_particles(self):
+        """All particles as an (N, 4) array of [x, y, z, clock_bias]."""
+        self._require_init()
+        return _dev.pf_device_get_particles(self._state)
+
+    def get_particle_states(self):
+        """(N, 16) rows [x, y, z, cb, vx, vy, vz, Sigma_v (3x3)]."""
+        self._require_init()
+        return _dev.pf_device_get_particle_states(self._state)
+
+    def set_particle_states(self, states):
+        """Replace the (N, 16) states and reset the weights."""
+        self._require_init()
+        arr = np.asarray(states, dtype=np.float64)
+        if arr.shape != (self.n_particles, 16):
+            raise ValueError(f"states shape {arr.shape} != ({self.n_particles}, 16)")
+        _dev.pf_device_set_particle_states(self._state, np.ascontiguousarray(arr))
+
+    def get_log_weights(self):
+        """Per-particle log-weights (synchronises the stream)."""
+        self._require_init()
+        out = np.empty(self.n_particles, dtype=np.float64)
+        _dev.pf_device_get_log_weights(self._state, out)
+        return out
+
+    def set_log_weights(self, log_weights):
+        """Replace the per-particle log-weights on the device."""
+        self._require_init()
+        lw = np.asarray(log_weights, dtype=np.float64).reshape(-1)
+        if lw.size != self.n_particles:
+            raise ValueError(f"log_weights size {lw.size} != {self.n_particles}")
+        _dev.pf_device_set_log_weights(self._state, np.ascontiguousarray(lw))
+
+    def position_mixture_update(self, refs_ecef, sigma_pos=1.0, mixture_weights=None):
+        """Gaussian-mixture position likelihood (host-side, via get/set log-weights)."""
+        self._require_init()
+        refs = np.asarray(refs_ecef, dtype=np.float64).reshape(-1, 3)
+        if refs.shape[0] == 0:
+            return
+        refs = refs[np.all(np.isfinite(refs), axis=1)]
+        if refs.shape[0] == 0:
+            return
+        sigma = max(float(sigma_pos), 1.0e-6)
+        if mixture_weights is None:
+            mix = np.full(refs.shape[0], 1.0 / refs.shape[0], dtype=np.float64)
+        else:
+            mix = np.asarray(mixture_weights, dtype=np.float64).reshape(-1)
+            if mix.size != refs.shape[0]:
+                raise ValueError("mixture_weights length must match refs_ecef")
+            mix = np.where(np.isfinite(mix) & (mix > 0.0), mix, 0.0)
+            total = float(np.sum(mix))
+            mix = (np.full(refs.shape[0], 1.0 / refs.shape[0], dtype=np.float64)
+                   if total <= 0.0 else mix / total)
+        particles = np.asarray(self.get_particles(), dtype=np.float64)
+        log_weights = np.asarray(self.get_log_weights(), dtype=np.float64)
+        diff = particles[:, np.newaxis, :3] - refs[np.newaxis, :, :]
+        with np.errstate(divide="ignore"):
+            log_components = (np.log(mix[np.newaxis, :])
+                              - 0.5 * np.sum(diff * diff, axis=2) / (sigma * sigma))
+        max_log = np.max(log_components, axis=1)
+        finite = np.isfinite(max_log)
+        increment = np.full(log_weights.shape, -np.inf, dtype=np.float64)
+        increment[finite] = max_log[finite] + np.log(
+            np.sum(np.exp(log_components[finite] - max_log[finite, np.newaxis]), axis=1))
+        self.set_log_weights(log_weights + increment)
+
+    def get_resample_ancestors(self):
+        """Ancestors of the last systematic resample: out[j] = source slot of slot j."""
+        self._require_init()
+        if self.resampling != "systematic":
+            raise RuntimeError("get_resample_ancestors requires resampling='systematic'")
+        out = np.empty(self.n_particles, dtype=np.int32)
+        _dev.pf_device_get_resample_ancestors(self._state, out)
+        return out
+
+    def get_ess(self):
+        """Effective sample size (computed on the GPU; scalar to the host)."""
+        self._require_init()
+        return _dev.pf_device_ess(self._state)
+
+    def get_position_spread(self, center=None):
+        """Weighted RMS particle distance to ``center`` (default: the estimate)."""
+        self._require_init()
+        if center is None:
+            center_arr = np.asarray(self.estimate()[:3], dtype=np.float64)
+        else:
+            center_arr = np.asarray(center, dtype=np.float64).ravel()
+            if center_arr.size != 3:
+                raise ValueError("center must have shape (3,)")
+        return _dev.pf_device_position_spread(self._state, float(center_arr[0]),
+                                              float(center_arr[1]), float(center_arr[2]))
+
+    def sync(self):
+        """Wait for all queued GPU work of this filter."""
+        if getattr(self, "_state", None) is not None:
+            _dev.pf_device_sync(self._state)
+
+    # -- smoother (not part of the forward step on this backend) ----------------
+    def enable_smoothing(self):
+        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")
+
+    def store_epoch(self, *args, **kwargs):
+        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")
+
+    def smooth(self, *args, **kwargs):
+        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")

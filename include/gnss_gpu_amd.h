@@ -1,0 +1,165 @@
+/*
+ * gnss_gpu_amd — C ABI of the MI355X (gfx950) particle-filter step and
+ * batched WLS library (libgnss_gpu_amd.so).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on failure;
+ *     gg_last_error() returns the thread-local message of the last failure.
+ *   - Host pointers are plain C arrays (row-major, binary64 unless stated).
+ *     Functions named *_device take device pointers and a hipStream_t.
+ *   - A gg_pf handle owns its device memory and one HIP stream (or uses the
+ *     stream given to gg_pf_set_stream).  A handle is not thread-safe.
+ *
+ * Reference counterparts are cited per function (paths relative to the
+ * rsasaki0109/gnss_gpu tree).
+ */
+#ifndef GNSS_GPU_AMD_H
+#define GNSS_GPU_AMD_H
+
+#include <stdint.h>
+
+#include "gnss_gpu_amd_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gg_pf gg_pf;
+
+/* --- library ----------------------------------------------------------- */
+const char* gg_last_error(void);
+int gg_abi_version(void);
+/* sizeof(gg_step_args) and sizeof(gg_dyn), for binding layout checks */
+int gg_struct_sizes(int32_t* step_args, int32_t* dyn, int32_t* filter);
+int gg_device_count(int32_t* out);
+
+/* --- particle filter: handle lifecycle ----------------------------------- */
+/* Batch of n_filters independent filters sharing one arena.
+ * sat_cap[f]: satellite rows reserved for filter f (grown on demand by the
+ * single-filter weight call).  n_slots: epoch slots of the satellite table
+ * (1 for host-driven use, T to preload a T-epoch trajectory).
+ * with_velocity: reserve per-particle velocity and velocity-covariance
+ * storage (needed for sigma_vel > 0, non-uniform states, RBPF). */
+int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* sat_cap,
+                 int32_t n_isb, int32_t n_slots, int32_t with_velocity, gg_pf** out);
+/* reference: pf_device_create, src/particle_filter/pf_device.cu:1306 */
+int gg_pf_device_create(int32_t n_particles, int32_t n_isb, gg_pf** out);
+/* reference: pf_device_destroy, pf_device.cu:1445; idempotent on NULL */
+int gg_pf_destroy(gg_pf* h);
+int gg_pf_set_stream(gg_pf* h, void* hip_stream);
+/* reference: pf_device_sync, pf_device.cu:2274 */
+int gg_pf_device_sync(gg_pf* h);
+int gg_pf_info(gg_pf* h, int32_t* n_filters, int32_t* n_isb, int64_t* arena_slots);
+
+/* --- particle filter: reference-shaped single-filter API (filter 0) ------- */
+/* reference: pf_device_initialize, pf_device.cu:1455-1476 (init_isb may be NULL) */
+int gg_pf_device_initialize(gg_pf* h, double init_x, double init_y, double init_z,
+                            double init_cb, double spread_pos, double spread_cb, uint64_t seed,
+                            double init_vx, double init_vy, double init_vz, double spread_vel,
+                            double init_vel_sigma, const double* init_isb, double spread_isb);
+/* reference: pf_device_predict, pf_device.cu:1482-1510 */
+int gg_pf_device_predict(gg_pf* h, double vx, double vy, double vz, double dt, double sigma_pos,
+                         double sigma_cb, uint64_t seed, int32_t step, double sigma_vel,
+                         double velocity_guide_alpha, int32_t velocity_kf,
+                         double velocity_process_noise, double sigma_isb);
+/* reference: pf_device_weight, pf_device.cu:1516-1574.  sys (may be NULL):
+ * per-satellite system id 0..3 (0 = reference system, k>0 uses ISB state k). */
+int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                        const double* weights_sat, const int32_t* sys, int32_t n_sat,
+                        double sigma_pr, double nu, double per_particle_nlos_threshold_m,
+                        int32_t per_particle_huber, double per_particle_huber_k);
+/* reference: pf_device_ess, pf_device.cu:1722-1762 */
+int gg_pf_device_ess(gg_pf* h, double* out);
+/* reference: pf_device_position_spread, pf_device.cu:1764-1807 */
+int gg_pf_device_position_spread(gg_pf* h, double cx, double cy, double cz, double* out);
+/* reference: pf_device_resample_systematic, pf_device.cu:1813-1878.
+ * u0_mode 0 reproduces the reference offset (seed % 1e6) / (1e6 * N) with the
+ * reference's int32 product; 1 draws a fresh Philox u0 in [0,1). */
+int gg_pf_device_resample_systematic(gg_pf* h, uint64_t seed, int32_t u0_mode);
+/* reference: pf_device_resample_megopolis, pf_device.cu:1884-1920 (same
+ * proposal/acceptance rule, ancestor form, one gather) */
+int gg_pf_device_resample_megopolis(gg_pf* h, int32_t n_iterations, uint64_t seed);
+/* Megopolis of Chesser et al. (arXiv:2109.13504); no reference counterpart
+ * (the reference's kernel is the slot-weight variant above). */
+int gg_pf_device_resample_megopolis_chesser(gg_pf* h, int32_t n_iterations);
+/* reference: pf_device_estimate, pf_device.cu:1926-1969; out[4] */
+int gg_pf_device_estimate(gg_pf* h, double* out);
+/* out[4 + n_isb]: x y z cb isb...  (no reference counterpart: ISB states) */
+int gg_pf_device_estimate_full(gg_pf* h, double* out);
+/* reference: pf_device_get_particles, pf_device.cu:1975-1993; out[N*4] */
+int gg_pf_device_get_particles(gg_pf* h, double* out);
+/* out[N*(4+n_isb)] */
+int gg_pf_device_get_particles_full(gg_pf* h, double* out);
+/* reference: pf_device_get_particle_states / set_particle_states,
+ * pf_device.cu:1995-2031; rows [x y z cb vx vy vz S(3x3)] */
+int gg_pf_device_get_particle_states(gg_pf* h, double* out);
+int gg_pf_device_set_particle_states(gg_pf* h, const double* in);
+/* reference: pf_device_get_log_weights / set_log_weights, pf_device.cu:2033-2051 */
+int gg_pf_device_get_log_weights(gg_pf* h, double* out);
+int gg_pf_device_set_log_weights(gg_pf* h, const double* in);
+/* reference: pf_device_get_resample_ancestors, pf_device.cu:2053-2058 */
+int gg_pf_device_get_resample_ancestors(gg_pf* h, int32_t* out);
+/* reference: pf_device_shift_clock_bias / shift_position, pf_device.cu:1699-1716 */
+int gg_pf_device_shift_clock_bias(gg_pf* h, double shift);
+int gg_pf_device_shift_position(gg_pf* h, double dx, double dy, double dz);
+/* reference: pf_device_position_update, pf_device.cu:1682-1693 */
+int gg_pf_device_position_update(gg_pf* h, double ref_x, double ref_y, double ref_z,
+                                 double sigma_pos);
+
+/* --- particle filter: batched, device-driven step -------------------------- */
+/* init[f*(4+n_isb) + k]: x y z cb isb...; seeds[f]. */
+int gg_pfb_initialize(gg_pf* h, const double* init, const uint64_t* seeds, double spread_pos,
+                      double spread_cb, double spread_isb);
+/* Host arrays for n_slots epochs:
+ *   sat_xyz[s][row][3], pr[s][row], w[s][row], sys[s][row] with the rows of
+ *   filter f at [sum_{g<f} sat_cap[g], ...); n_sat[s][f]; vel[s][f][3] (ECEF). */
+int gg_pfb_load_epochs(gg_pf* h, int32_t n_slots, const double* sat_xyz, const double* pr,
+                       const double* w, const int32_t* sys, const int32_t* n_sat,
+                       const double* vel);
+/* resampler: 0 systematic, 1 Megopolis (Chesser), 2 Megopolis (reference rule) */
+int gg_pfb_step(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter);
+/* Capture `steps` consecutive steps in one hipGraph (replaced on each call). */
+int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                   int32_t steps);
+int gg_pfb_replay(gg_pf* h, int32_t n_replays);
+/* Device time of n_replays graph launches (hipEvents on the handle's stream). */
+int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total);
+/* Device time of `reps` launches of one stage: 0 weight+predict, 1 weight,
+ * 2 cdf, 3 gather (systematic), 4 Megopolis (Chesser), 5 gather (ancestors). */
+int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
+                      int32_t reps, float* ms_total);
+int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out);
+int gg_pfb_set_epoch(gg_pf* h, int32_t epoch);
+/* Export filter f's particles as binary64 rows [x y z cb isb...]. */
+int gg_pfb_get_particles(gg_pf* h, int32_t f, double* out);
+int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out);
+
+/* --- batched Gauss-Newton WLS ---------------------------------------------- */
+/* reference: wls_batch, src/positioning/wls.cu:172-278 (Sagnac range model).
+ * n_sat may be NULL (all epochs use n_sat_max rows) or give per-epoch counts;
+ * rows beyond an epoch's count are ignored.  Host arrays. */
+int gg_wls_batch(const double* sat_ecef, const double* pseudoranges, const double* weights,
+                 const int32_t* n_sat, int32_t n_epoch, int32_t n_sat_max, int32_t max_iter,
+                 double tol, double* results, int32_t* iters);
+/* Device-pointer variant; mode 0 = wls_batch semantics, 1 = wls_position semantics
+ * (no Sagnac, singular-matrix early return, -1 for < 4 satellites). */
+int gg_wls_batch_device(const double* sat_ecef, const double* pseudoranges,
+                        const double* weights, const int32_t* n_sat, int32_t n_epoch,
+                        int32_t n_sat_max, int32_t max_iter, double tol, double* results,
+                        int32_t* iters, int32_t mode, void* hip_stream);
+/* reference: wls_position, src/positioning/wls.cu:10-123 (no Sagnac) */
+int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const double* weights,
+                    int32_t n_sat, int32_t max_iter, double tol, double* result,
+                    int32_t* iters);
+
+/* --- test hook (no reference counterpart) ----------------------------------- */
+/* Evaluates a deterministic device building block for n inputs so tests can
+ * compare it bit for bit with the oracle: op 0 exp64, 1 exp32, 2 Philox words
+ * (out holds 4n values for op 2). */
+int gg_debug_math(int32_t op, const double* in, double* out, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GNSS_GPU_AMD_H */

@@ -1,0 +1,47 @@
+"""Build oracle/_ref-free CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP).
+
+Measurement infrastructure only; the output lands in oracle/_build/.
+"""
+
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT_DIR, "liboracle_cpu_pf.so")
+
+
+def build() -> str:
+    os.makedirs(OUT_DIR, exist_ok=True)
+    src = os.path.join(HERE, "cpu_pf.c")
+    tmp = LIB + ".tmp"
+    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11",
+           "-D_GNU_SOURCE", src, "-o", tmp, "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def load():
+    import ctypes as C
+
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    D, I, U64, P = C.c_double, C.c_int, C.c_uint64, C.c_void_p
+    DP = C.POINTER(C.c_double)
+    IP = C.POINTER(C.c_int)
+    lib.cpu_pf_create.restype = P
+    lib.cpu_pf_create.argtypes = [I, I, U64, I]
+    lib.cpu_pf_destroy.argtypes = [P]
+    lib.cpu_pf_threads.restype = I
+    lib.cpu_pf_initialize.argtypes = [P, DP, D, D, D]
+    lib.cpu_pf_step.restype = I
+    lib.cpu_pf_step.argtypes = [P, DP, D, D, D, D, I, DP, DP, IP, I, D, D, D, DP]
+    return lib
+
+
+if __name__ == "__main__":
+    print(build())

@@ -1,0 +1,273 @@
+"""Bit-exact NumPy twins of the deterministic pieces of the GPU resampler.
+
+Every function here uses only correctly rounded IEEE-754 operations in the
+same order as the HIP code (gnss_gpu_amd/csrc/gg_device.h and
+gg_pf_kernels*.h), so for identical inputs the outputs are identical bit for
+bit.  The reference algorithm they implement is the systematic resampler of
+src/particle_filter/pf_device.cu:1813-1878 (max, exp-shift, normalise,
+inclusive scan, a_j = min{i : cdf_i >= (u0 + j)/N} with hi = N-1) and the
+Megopolis kernel of pf_device.cu:1231-1298; the reference leaves the scan
+order to thrust, so the order here is the library's own fixed one.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import philox
+
+TILE = 1024
+BLOCK = 256
+ITEMS = 4
+
+_INV_LN2 = float.fromhex("0x1.71547652b82fep+0")
+_LN2_HI = float.fromhex("0x1.62e42fee00000p-1")
+_LN2_LO = float.fromhex("0x1.a39ef35793c76p-33")
+_EXP_C = [float.fromhex(h) for h in (
+    "0x1.6124613a86d09p-33", "0x1.1eed8eff8d898p-29", "0x1.ae64567f544e4p-26",
+    "0x1.27e4fb7789f5cp-22", "0x1.71de3a556c734p-19", "0x1.a01a01a01a01ap-16",
+    "0x1.a01a01a01a01ap-13", "0x1.6c16c16c16c17p-10", "0x1.1111111111111p-7",
+    "0x1.5555555555555p-5", "0x1.5555555555555p-3", "0x1.0p-1", "0x1.0p+0", "0x1.0p+0")]
+
+_F = np.float32
+_INV_LN2_F = _F(float.fromhex("0x1.715476p+0"))
+_LN2_HI_F = _F(float.fromhex("0x1.62e4p-1"))
+_LN2_LO_F = _F(float.fromhex("0x1.7f7d1cp-20"))
+_EXP_CF = [_F(float.fromhex(h)) for h in (
+    "0x1.a01a02p-13", "0x1.6c16c2p-10", "0x1.111112p-7", "0x1.555556p-5",
+    "0x1.555556p-3", "0x1.0p-1", "0x1.0p+0", "0x1.0p+0")]
+
+
+def exp64(x):
+    """Twin of gg_exp64: Cody-Waite reduction + degree-13 Taylor, binary64."""
+    x = np.asarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    nan = np.isnan(x)
+    hi = x > 709.782712893384
+    lo = x < -745.1332191019412
+    mid = ~(nan | hi | lo)
+    xm = x[mid]
+    k = np.rint(xm * _INV_LN2)
+    r = (xm - k * _LN2_HI) - k * _LN2_LO
+    p = np.full_like(r, _EXP_C[0])
+    for c in _EXP_C[1:]:
+        p = p * r + c
+    out[mid] = np.ldexp(p, k.astype(np.int32))
+    out[nan] = x[nan]
+    out[hi] = np.inf
+    return out
+
+
+def exp32(x):
+    """Twin of gg_exp32 (binary32)."""
+    x = np.asarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    nan = np.isnan(x)
+    hi = x > _F(88.72283)
+    lo = x < _F(-103.97208)
+    mid = ~(nan | hi | lo)
+    xm = x[mid]
+    k = np.rint(xm * _INV_LN2_F)
+    r = (xm - k * _LN2_HI_F) - k * _LN2_LO_F
+    p = np.full_like(r, _EXP_CF[0])
+    for c in _EXP_CF[1:]:
+        p = p * r + c
+    out[mid] = np.ldexp(p, k.astype(np.int32)).astype(np.float32)
+    out[nan] = x[nan]
+    out[hi] = np.inf
+    return out
+
+
+def _hs_wave(x):
+    """Hillis-Steele inclusive scan over the last axis (64 lanes)."""
+    x = np.array(x, dtype=np.float64, copy=True)
+    d = 1
+    while d < 64:
+        y = x.copy()
+        x[..., d:] = y[..., :-d] + y[..., d:]
+        d <<= 1
+    return x
+
+
+def block_scan256(v):
+    """Twin of gg_block_scan256 over the last axis (256 threads).
+    Returns (inclusive, exclusive, total)."""
+    v = np.asarray(v, dtype=np.float64)
+    lead = v.shape[:-1]
+    x = _hs_wave(v.reshape(*lead, 4, 64))
+    t = x[..., :, 63]
+    o1 = t[..., 0]
+    o2 = o1 + t[..., 1]
+    o3 = o2 + t[..., 2]
+    tot = o3 + t[..., 3]
+    incl = x.copy()
+    incl[..., 1, :] = o1[..., None] + x[..., 1, :]
+    incl[..., 2, :] = o2[..., None] + x[..., 2, :]
+    incl[..., 3, :] = o3[..., None] + x[..., 3, :]
+    incl = incl.reshape(*lead, 256)
+    excl = np.concatenate([np.zeros((*lead, 1)), incl[..., :-1]], axis=-1)
+    return incl, excl, tot
+
+
+def block_total256(v):
+    """Twin of gg_block_totals for one value per thread."""
+    return block_scan256(v)[2]
+
+
+def tile_view(lw, n):
+    """Pad a filter's log-weights to whole tiles: returns (lw[nt,256,4], valid)."""
+    nt = (n + TILE - 1) // TILE
+    pad = np.zeros(nt * TILE, dtype=np.float32)
+    pad[:n] = np.asarray(lw, dtype=np.float32)[:n]
+    valid = np.zeros(nt * TILE, dtype=bool)
+    valid[:n] = True
+    return pad.reshape(nt, BLOCK, ITEMS), valid.reshape(nt, BLOCK, ITEMS)
+
+
+def tile_partials(lw, n):
+    """Per-tile m_b and S_b = sum exp64(lw - m_b) (pf_device.cu:924-965 pattern,
+    library order).  Returns (m[nt], S[nt], e[nt,256,4])."""
+    lwt, val = tile_view(lw, n)
+    lw64 = lwt.astype(np.float64)
+    m = np.where(val, lw64, -np.inf).reshape(lw64.shape[0], -1).max(axis=1)
+    e = np.zeros_like(lw64)
+    ok = val & (m[:, None, None] > -np.inf)
+    e[ok] = exp64((lw64 - m[:, None, None])[ok])
+    s = e[..., 0].copy()
+    for q in range(1, ITEMS):
+        s = s + e[..., q]
+    S = block_total256(s)
+    return m, S, e
+
+
+def decide(m, S):
+    """Twin of gg_decide's CDF part: M, g_b, P_b, W."""
+    nt = m.shape[0]
+    M = float(np.max(m))
+    g = np.zeros(nt)
+    ok = m > -np.inf
+    g[ok] = exp64(m[ok] - M)
+    T = S * g
+    c = (nt + BLOCK - 1) // BLOCK
+    TT = np.zeros((BLOCK, c))
+    idx = np.arange(nt)
+    TT[idx // c, idx % c] = T
+    tot = np.zeros(BLOCK)
+    for k in range(c):
+        tot = tot + TT[:, k]
+    _, excl, W = block_scan256(tot)
+    P = np.zeros(nt)
+    run = np.zeros(BLOCK)
+    for k in range(c):
+        sel = np.arange(BLOCK) * c + k
+        okb = sel < nt
+        P[sel[okb]] = excl[okb] + run[okb]
+        run = run + TT[:, k]
+    return M, g, P, float(W)
+
+
+def systematic_targets(n, u0):
+    """t_j = (u0 + j) / N in binary64 (pf_device.cu:1197)."""
+    return (u0 + np.arange(n, dtype=np.float64)) / float(n)
+
+
+def reference_u0(seed, n):
+    """u0 = (seed % 1e6) / (1e6 * N) with the reference's int32 product
+    (pf_device.cu:1852)."""
+    den = (1000000 * int(n)) & 0xFFFFFFFF
+    if den >= 2 ** 31:
+        den -= 2 ** 32
+    return float(int(seed) % 1000000) / float(den)
+
+
+def fresh_u0(seed, resample_count):
+    r = philox.draw(seed, 0, resample_count, philox.PURPOSE_U0, 0)
+    return float(philox.u01d(r[0], r[1])) - 2.0 ** -53
+
+
+def systematic_ends(lw, n, u0):
+    """Per-particle output ends E_i (prefix max of #{j : t_j <= cdf_i})."""
+    m, S, e = tile_partials(lw, n)
+    M, g, P, W = decide(m, S)
+    nt = m.shape[0]
+    l = e.copy()
+    for q in range(1, ITEMS):
+        l[..., q] = l[..., q - 1] + e[..., q]
+    _, excl, _ = block_scan256(l[..., ITEMS - 1])
+    L = excl[..., None] + l
+    cdf = (P[:, None, None] + g[:, None, None] * L) / W
+    cdf = cdf.reshape(nt * TILE)[:n]
+    t = systematic_targets(n, u0)
+    k = np.searchsorted(t, cdf, side="right").astype(np.int64)
+    k[np.isnan(cdf)] = 0
+    k[n - 1] = n
+    return np.maximum.accumulate(k), cdf
+
+
+def systematic_ancestors(lw, n, u0):
+    """Ancestor indices of the library's systematic resampler."""
+    E, _ = systematic_ends(lw, n, u0)
+    return np.searchsorted(E, np.arange(n), side="right").astype(np.int32)
+
+
+def megopolis_ref_ancestors(lw, n, n_iter, seed):
+    """Reference-rule Megopolis in ancestor form (pf_device.cu:1231-1298):
+    per iteration, offset = floor(U1 (N-1)) + 1, j = (i + offset) mod N,
+    accept if U2 < min(1, exp(lw_j - lw_i)) with the slot weights."""
+    lw64 = np.asarray(lw, dtype=np.float32)[:n].astype(np.float64)
+    i = np.arange(n)
+    a = i.copy()
+    for it in range(n_iter):
+        r = philox.draw(seed, i, it, philox.PURPOSE_MEGO_REF, 0)
+        u1 = philox.u01d(r[0], r[1])
+        u2 = philox.u01d(r[2], r[3])
+        off = (u1 * float(n - 1)).astype(np.int64) + 1
+        j = (i + off) % n
+        alpha = np.minimum(1.0, exp64(lw64[j] - lw64[i]))
+        acc = u2 < alpha
+        a = np.where(acc, a[j], a[i])
+    return a.astype(np.int32)
+
+
+def megopolis_ancestors(lw, n, n_iter, seed, call):
+    """Megopolis of Chesser et al. (2021) as implemented by gg_k_mego: Metropolis
+    chains on the original weights, one offset per 256-particle segment."""
+    lw32 = np.asarray(lw, dtype=np.float32)[:n]
+    i = np.arange(n, dtype=np.int64)
+    seg = i // 256
+    k = i.copy()
+    lk = lw32.copy()
+    nseg = int(seg.max()) + 1 if n else 0
+    segs = np.arange(nseg)
+    for b in range(n_iter):
+        r = philox.draw(seed, segs, call, philox.PURPOSE_MEGO, b)
+        O = ((r[0] * np.uint64(n)) >> np.uint64(32)).astype(np.int64)
+        ru = philox.draw(seed, i, call, philox.PURPOSE_MEGO_U, b >> 2)
+        u = philox.u01f(ru[b & 3])
+        j = i + O[seg]
+        j = np.where(j >= n, j - n, j)
+        lj = lw32[j]
+        dl = (lj - lk).astype(np.float32)
+        acc = dl >= 0
+        mid = (~acc) & (dl >= _F(-18.0))
+        acc[mid] = u[mid] < exp32(dl[mid])
+        k = np.where(acc, j, k)
+        lk = np.where(acc, lj, lk)
+    return k.astype(np.int32)
+
+
+def sequential_systematic(w_norm, u0):
+    """Textbook systematic resampling on normalised weights (sequential cumsum,
+    first index with cdf >= target, clamped) — the reference semantics with a
+    sequential scan; used as an independent check."""
+    n = w_norm.shape[0]
+    cdf = np.cumsum(np.asarray(w_norm, dtype=np.float64))
+    t = systematic_targets(n, u0)
+    a = np.searchsorted(cdf, t, side="left")
+    return np.minimum(a, n - 1).astype(np.int32)
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]
+_ = math

@@ -1,0 +1,183 @@
+"""Binary64 restatement of the reference particle filter, driven by the same
+random numbers as the GPU library.
+
+Follows src/particle_filter/pf_device.cu:
+  init      :67-104    predict :106-187    weight :189-276 (Huber :19-26)
+  ESS       :924-965 + :1722-1762          estimate :968-1023 + :1926-1969
+  spread    :1025-1076 + :1764-1807        systematic :1813-1878
+  Megopolis :1231-1298 + :1884-1920
+and python/gnss_gpu/particle_filter_device.py:198-303,679-714 for the
+orchestration.  State is absolute ECEF binary64 ([x y z cb isb...]), like the
+reference.  Random draws: the reference's cuRAND streams cannot be reproduced
+(closed library), so the noise comes from the library's Philox streams, drawn
+in the library's local ENU frame and rotated to ECEF.  Resampling uses the
+bit-exact twins in detmath on binary32-rounded log-weights (the library stores
+log-weights in binary32).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import detmath, philox
+
+
+def enu_rotation(x):
+    x = np.asarray(x, dtype=np.float64).reshape(3)
+    r = float(np.linalg.norm(x))
+    if not r > 1.0:
+        return np.eye(3)
+    lon = np.arctan2(x[1], x[0])
+    lat = np.arctan2(x[2], np.hypot(x[0], x[1]))
+    sl, cl, sp, cp = np.sin(lon), np.cos(lon), np.sin(lat), np.cos(lat)
+    return np.array([[-sl, cl, 0.0], [-sp * cl, -sp * sl, cp], [cp * cl, cp * sl, sp]])
+
+
+def huber_cost(residual, sigma, w, k):
+    """pf_device.cu:19-26."""
+    z = np.abs(residual) / sigma
+    rho = np.where(z < k, 0.5 * z * z, k * z - 0.5 * k * k)
+    return w * rho
+
+
+def log_likelihood(pos, cb, isb, sat, pr, w, sys, sigma_pr, nu=0.0, gate_thr=0.0,
+                   huber=False, huber_k=1.5):
+    """Per-particle log-weight, pf_device.cu:189-276 (overwrites, not adds).
+    pos [N,3], cb [N], isb [N, n_isb] (may be empty), sat [S,3]."""
+    N = pos.shape[0]
+    S = sat.shape[0]
+    if S == 0:
+        return np.zeros(N)
+    r = np.linalg.norm(pos[:, None, :] - sat[None, :, :], axis=2)
+    isb_term = np.zeros((N, S))
+    for s in range(S):
+        if sys is not None and sys[s] >= 1 and isb.shape[1] >= sys[s]:
+            isb_term[:, s] = isb[:, sys[s] - 1]
+    resid = pr[None, :] - (r + cb[:, None] + isb_term)
+    use_gate = gate_thr > 0.0
+    keep = np.ones((N, S), dtype=bool)
+    if use_gate:
+        keep = np.abs(resid) <= gate_thr
+        min_kept = min(S, 4)
+        gate_on = keep.sum(axis=1) >= min_kept
+        keep = np.where(gate_on[:, None], keep, True)
+    if nu > 0.0:
+        term = -0.5 * (nu + 1.0) * np.log(1.0 + w[None, :] * resid * resid / (nu * sigma_pr ** 2))
+    elif huber and huber_k > 0.0:
+        term = -huber_cost(resid, sigma_pr, w[None, :], huber_k)
+    else:
+        term = -0.5 * w[None, :] * resid * resid / (sigma_pr * sigma_pr)
+    return np.sum(np.where(keep, term, 0.0), axis=1)
+
+
+def ess(lw):
+    """(sum w)^2 / sum w^2 with w = exp(lw - max), pf_device.cu:1722-1762."""
+    lw = np.asarray(lw, dtype=np.float64)
+    w = np.exp(lw - np.max(lw))
+    return float(np.sum(w) ** 2 / np.sum(w * w))
+
+
+def weighted_mean(state, lw):
+    """pf_device.cu:968-1023 + :1926-1969."""
+    lw = np.asarray(lw, dtype=np.float64)
+    w = np.exp(lw - np.max(lw))
+    return (w[:, None] * state).sum(axis=0) / w.sum()
+
+
+def position_spread(pos, lw, center):
+    """pf_device.cu:1025-1076 + :1764-1807."""
+    lw = np.asarray(lw, dtype=np.float64)
+    w = np.exp(lw - np.max(lw))
+    d2 = np.sum((pos - np.asarray(center)[None, :]) ** 2, axis=1)
+    tw = w.sum()
+    if not tw > 0.0:
+        return 0.0
+    return float(np.sqrt(max(np.sum(w * d2) / tw, 0.0)))
+
+
+class OraclePF:
+    """Binary64 particle filter mirroring ParticleFilterDevice call by call."""
+
+    def __init__(self, n_particles, seed=42, n_isb=0):
+        self.n = int(n_particles)
+        self.seed = int(seed)
+        self.n_isb = int(n_isb)
+        self.state = None          # [N, 4 + n_isb] absolute
+        self.vel = None            # [N, 3] ECEF velocity
+        self.lw = np.zeros(self.n)
+        self.R = np.eye(3)
+        self.resample_count = 0
+        self.ancestors = np.arange(self.n, dtype=np.int32)
+
+    # pf_device.cu:67-104
+    def initialize(self, pos, cb=0.0, spread_pos=100.0, spread_cb=1000.0, isb=None,
+                   spread_isb=0.0, velocity=None):
+        pos = np.asarray(pos, dtype=np.float64).reshape(3)
+        self.R = enu_rotation(pos)
+        i = np.arange(self.n)
+        z = philox.normal4(self.seed, i, 0, philox.PURPOSE_INIT, 0)
+        D = 4 + self.n_isb
+        st = np.zeros((self.n, D))
+        st[:, :3] = pos[None, :] + (spread_pos * z[:, :3]) @ self.R
+        st[:, 3] = cb + spread_cb * z[:, 3]
+        if self.n_isb:
+            zi = philox.normal4(self.seed, i, 0, philox.PURPOSE_INIT, 1)
+            isb0 = np.zeros(self.n_isb) if isb is None else np.asarray(isb, dtype=np.float64)
+            st[:, 4:] = isb0[None, :] + spread_isb * zi[:, : self.n_isb]
+        self.state = st
+        v = np.zeros(3) if velocity is None else np.asarray(velocity, dtype=np.float64)
+        self.vel = np.tile(v, (self.n, 1))
+        self.lw = np.zeros(self.n)
+        self.resample_count = 0
+
+    # pf_device.cu:106-187 (velocity guide with alpha = 1, no velocity noise)
+    def predict(self, velocity, dt, sigma_pos, sigma_cb, step, sigma_isb=0.0, alpha=1.0):
+        i = np.arange(self.n)
+        z = philox.normal4(self.seed, i, step, philox.PURPOSE_PREDICT, 0)
+        vg = np.asarray(velocity, dtype=np.float64).reshape(3)
+        a = min(1.0, max(0.0, alpha))
+        self.vel = (1.0 - a) * self.vel + a * vg[None, :]
+        self.state[:, :3] += self.vel * dt + (sigma_pos * z[:, :3]) @ self.R
+        self.state[:, 3] += sigma_cb * z[:, 3]
+        if self.n_isb:
+            zi = philox.normal4(self.seed, i, step, philox.PURPOSE_PREDICT, 2)
+            self.state[:, 4:] += sigma_isb * zi[:, : self.n_isb]
+
+    def weight(self, sat, pr, w=None, sys=None, sigma_pr=5.0, nu=0.0, gate_thr=0.0,
+               huber=False, huber_k=1.5):
+        sat = np.asarray(sat, dtype=np.float64).reshape(-1, 3)
+        pr = np.asarray(pr, dtype=np.float64).ravel()
+        w = np.ones(pr.size) if w is None else np.asarray(w, dtype=np.float64).ravel()
+        isb = self.state[:, 4:]
+        self.lw = log_likelihood(self.state[:, :3], self.state[:, 3], isb, sat, pr, w, sys,
+                                 sigma_pr, nu, gate_thr, huber, huber_k)
+
+    def ess(self):
+        return ess(self.lw)
+
+    def estimate(self):
+        return weighted_mean(self.state, self.lw)
+
+    def _apply(self, anc):
+        self.ancestors = anc
+        self.state = self.state[anc]
+        self.vel = self.vel[anc]
+        self.lw = np.zeros(self.n)
+        self.resample_count += 1
+
+    def resample_systematic(self, u0=None, lw32=None):
+        """pf_device.cu:1813-1878 with the library's fixed scan order."""
+        if u0 is None:
+            u0 = detmath.reference_u0(self.seed, self.n)
+        lw = self.lw.astype(np.float32) if lw32 is None else lw32
+        self._apply(detmath.systematic_ancestors(lw, self.n, u0))
+
+    def resample_megopolis(self, n_iter=15, lw32=None):
+        """pf_device.cu:1884-1920 (reference acceptance rule)."""
+        lw = self.lw.astype(np.float32) if lw32 is None else lw32
+        self._apply(detmath.megopolis_ref_ancestors(lw, self.n, n_iter, self.seed))
+
+    def resample_megopolis_chesser(self, n_iter, lw32=None):
+        lw = self.lw.astype(np.float32) if lw32 is None else lw32
+        self._apply(detmath.megopolis_ancestors(lw, self.n, n_iter, self.seed,
+                                                self.resample_count))

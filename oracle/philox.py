@@ -1,0 +1,73 @@
+"""NumPy twin of the Philox4x32-10 streams used by the GPU library.
+
+Philox4x32-10 as published by Salmon et al., "Parallel Random Numbers: As Easy
+as 1, 2, 3" (SC'11); pinned by the published known-answer vectors in
+tests/test_oracle_pinning.py.  Stream layout (shared with gg_device.h):
+key = (seed lo, seed hi), counter = (index, step, purpose, call).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+M32 = np.uint64(0xFFFFFFFF)
+MUL0 = np.uint64(0xD2511F53)
+MUL1 = np.uint64(0xCD9E8D57)
+WEYL0 = np.uint64(0x9E3779B9)
+WEYL1 = np.uint64(0xBB67AE85)
+
+PURPOSE_INIT = 1
+PURPOSE_PREDICT = 2
+PURPOSE_MEGO_REF = 3
+PURPOSE_MEGO = 4
+PURPOSE_U0 = 5
+PURPOSE_MEGO_U = 6
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Vectorised Philox4x32-10; inputs broadcast, returns 4 uint64 arrays (< 2^32)."""
+    c0, c1, c2, c3, k0, k1 = (np.asarray(v, dtype=np.uint64) & M32
+                              for v in (c0, c1, c2, c3, k0, k1))
+    c0, c1, c2, c3, k0, k1 = np.broadcast_arrays(c0, c1, c2, c3, k0, k1)
+    c0, c1, c2, c3, k0, k1 = (v.copy() for v in (c0, c1, c2, c3, k0, k1))
+    for r in range(10):
+        if r:
+            k0 = (k0 + WEYL0) & M32
+            k1 = (k1 + WEYL1) & M32
+        p0 = MUL0 * c0
+        p1 = MUL1 * c2
+        c0, c1, c2, c3 = ((p1 >> np.uint64(32)) ^ c1 ^ k0, p1 & M32,
+                          (p0 >> np.uint64(32)) ^ c3 ^ k1, p0 & M32)
+    return c0, c1, c2, c3
+
+
+def draw(seed, idx, step, purpose, call):
+    seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return philox4x32_10(idx, step, purpose, call, seed & 0xFFFFFFFF, seed >> 32)
+
+
+def u01f(x):
+    """uint32 -> binary32 uniform in (0, 1), exact."""
+    x = np.asarray(x, dtype=np.uint64)
+    return ((x >> np.uint64(9)).astype(np.float32) + np.float32(0.5)) * np.float32(2.0 ** -23)
+
+
+def u01d(hi, lo):
+    """two uint32 -> binary64 uniform in (0, 1], exact."""
+    hi = np.asarray(hi, dtype=np.uint64)
+    lo = np.asarray(lo, dtype=np.uint64)
+    m = (hi << np.uint64(21)) | (lo >> np.uint64(11))
+    return (m + np.uint64(1)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal4(seed, idx, step, purpose, call):
+    """Four standard normals per index (binary64 Box-Muller of the same uniforms
+    the GPU turns into binary32 normals).  Returns shape (len(idx), 4)."""
+    r = draw(seed, idx, step, purpose, call)
+    u = [u01f(v).astype(np.float64) for v in r]
+    out = []
+    for a, b in ((u[0], u[1]), (u[2], u[3])):
+        rad = np.sqrt(-2.0 * np.log(a))
+        out.append(rad * np.cos(2.0 * np.pi * b))
+        out.append(rad * np.sin(2.0 * np.pi * b))
+    return np.stack(out, axis=-1)

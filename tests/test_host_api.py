@@ -1,0 +1,75 @@
+"""Host-side behaviour that needs no GPU: argument checks that run before any
+device work, and the synthetic scenario generators."""
+
+import numpy as np
+import pytest
+
+from gnss_gpu_amd import sim
+
+
+def test_wls_rejects_fewer_than_four_satellites():
+    from gnss_gpu_amd import wls_batch, wls_position
+
+    with pytest.raises(RuntimeError, match="at least 4 satellites"):
+        wls_position(np.zeros(9), np.zeros(3), np.ones(3))
+    with pytest.raises(RuntimeError, match="at least 4 satellites"):
+        wls_batch(np.zeros((2, 3, 3)), np.zeros((2, 3)), np.ones((2, 3)))
+
+
+def test_wls_batch_shape_checks():
+    from gnss_gpu_amd import wls_batch
+
+    with pytest.raises(ValueError):
+        wls_batch(np.zeros((2, 5)), np.zeros((2, 5)), np.ones((2, 5)))
+    with pytest.raises(ValueError):
+        wls_batch(np.zeros((2, 5, 3)), np.zeros((2, 5)), np.ones((2, 5)), n_sat=[5])
+
+
+def test_particle_filter_device_validates_modes_before_allocating():
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    with pytest.raises(ValueError):
+        ParticleFilterDevice(n_particles=10, megopolis_mode="bogus")
+    with pytest.raises(ValueError):
+        ParticleFilterDevice(n_particles=10, systematic_u0="bogus")
+
+
+def test_legacy_filter_requires_initialize():
+    from gnss_gpu_amd import ParticleFilter
+
+    pf = ParticleFilter(n_particles=16)
+    for call in (lambda: pf.predict(), lambda: pf.estimate(), lambda: pf.get_ess(),
+                 lambda: pf.get_particles(), lambda: pf.update(np.zeros((4, 3)), np.zeros(4))):
+        with pytest.raises(RuntimeError, match="not initialized"):
+            call()
+
+
+def test_scenario_geometry():
+    sc = sim.scenario_plain(n_epochs=5, seed=0)
+    rho = np.linalg.norm(sc.sat, axis=1)
+    np.testing.assert_allclose(rho, sim.SAT_RADIUS, rtol=1e-12)
+    # elevations reproduce the requested ones
+    R = sim.enu_rotation(sc.truth[0])
+    enu = (sc.sat - sc.truth[0]) @ R.T
+    el = np.degrees(np.arcsin(enu[:, 2] / np.linalg.norm(enu, axis=1)))
+    np.testing.assert_allclose(el, sc.el, atol=1e-9)
+    assert sc.pr.shape == (5, 8)
+    assert np.linalg.norm(sc.truth[1] - sc.truth[0]) == pytest.approx(10.0)
+
+
+def test_multi_gnss_nlos_fraction_and_isb():
+    sc = sim.scenario_multi_gnss(n_epochs=50, seed=1)
+    assert sc.n_sat == 12
+    assert np.all(sc.nlos.sum(axis=1) == round(0.3 * 12))
+    resid = sc.pr - np.linalg.norm(sc.sat[None] - sc.truth[:, None], axis=2) - sc.cb
+    los = ~sc.nlos
+    for s_id in (1, 2, 3):
+        m = los & (sc.sys[None, :] == s_id)
+        assert abs(resid[m].mean() - sc.isb[s_id - 1]) < 1.5
+
+
+def test_wls_epochs_ragged():
+    sat, pr, w, counts, truth = sim.wls_epochs(n_epoch=50, seed=3)
+    assert counts.min() >= 6 and counts.max() <= 20
+    for e in range(50):
+        assert np.all(w[e, counts[e]:] == 0) and np.all(w[e, :counts[e]] == 1)

@@ -1,0 +1,330 @@
+"""Pins the CPU oracle before it is trusted as the GPU's checker.
+
+Each oracle function is checked against something that does not restate its
+own formula: published known-answer vectors, exact (rational) arithmetic,
+textbook closed forms, brute force on tiny inputs, the reference's own test
+constants (tests/golden/reference_cases.json) and invariants.
+"""
+
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import detmath, philox
+from oracle import pf as opf
+from oracle import wls as owls
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_cases.json")))
+
+
+# ---------------------------------------------------------------- Philox ----
+@pytest.mark.parametrize(
+    "ctr,key,expect",
+    [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ],
+)
+def test_philox_known_answers(ctr, key, expect):
+    """Random123 known-answer vectors for philox4x32-10."""
+    out = philox.philox4x32_10(*ctr, *key)
+    assert tuple(int(v) for v in out) == expect
+
+
+def test_uniform_ranges_and_exactness():
+    x = np.array([0, 1, 255, 256, 0xFFFFFFFF], dtype=np.uint64)
+    u = philox.u01f(x)
+    assert u.dtype == np.float32
+    assert np.all(u > 0) and np.all(u < 1)
+    assert u[0] == np.float32(2.0 ** -24) and u[-1] == np.float32(1 - 2.0 ** -24)
+    d = philox.u01d(np.array([0, 0xFFFFFFFF], dtype=np.uint64),
+                    np.array([0, 0xFFFFFFFF], dtype=np.uint64))
+    assert d[0] == 2.0 ** -53 and d[1] == 1.0
+
+
+def test_normals_are_standard():
+    z = philox.normal4(7, np.arange(200_000), 3, philox.PURPOSE_PREDICT, 0).ravel()
+    assert abs(z.mean()) < 0.01
+    assert abs(z.std() - 1.0) < 0.01
+
+
+# ------------------------------------------------------------- exp twins ----
+def test_exp64_against_libm():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-745, 709, 20000), rng.uniform(-5, 5, 20000),
+                        np.array([0.0, -0.0, 1.0, -1.0, 709.7, -708.0, -744.0])])
+    got = detmath.exp64(x)
+    ref = np.array([math.exp(v) for v in x])
+    normal = ref >= np.finfo(np.float64).tiny
+    ulp = np.abs(got[normal] - ref[normal]) / np.spacing(ref[normal])
+    assert ulp.max() <= 2.0
+    assert np.all(np.abs(got[~normal] - ref[~normal]) <= 2 * 5e-324)
+    assert detmath.exp64(np.array([-800.0]))[0] == 0.0
+    assert np.isinf(detmath.exp64(np.array([800.0]))[0])
+    assert detmath.exp64(np.array([0.0]))[0] == 1.0
+
+
+def test_exp32_against_float64_exp():
+    x = np.linspace(-87.0, 88.0, 50001, dtype=np.float32)
+    got = detmath.exp32(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    rel = np.abs(got - ref) / ref
+    assert rel.max() < 3.0 * 2.0 ** -24
+    assert detmath.exp32(np.array([0.0], dtype=np.float32))[0] == 1.0
+
+
+# ----------------------------------------------------------- block scans ----
+def test_block_scan_exact_on_integers():
+    """On small integers every association order is exact: the fixed-order
+    scan must equal the sequential prefix sums."""
+    v = np.random.default_rng(2).integers(0, 1000, (3, 256)).astype(np.float64)
+    incl, excl, tot = detmath.block_scan256(v)
+    np.testing.assert_array_equal(incl, np.cumsum(v, axis=1))
+    np.testing.assert_array_equal(excl[:, 1:], np.cumsum(v, axis=1)[:, :-1])
+    np.testing.assert_array_equal(tot, v.sum(axis=1))
+
+
+def test_block_scan_close_to_exact_rational():
+    v = np.random.default_rng(3).random(256)
+    incl, _, tot = detmath.block_scan256(v)
+    exact = [Fraction(0)]
+    for x in v:
+        exact.append(exact[-1] + Fraction(x))
+    err = max(abs(float(Fraction(incl[i]) - exact[i + 1])) for i in range(256))
+    assert err < 256 * 2.0 ** -52 * 256
+    assert abs(float(Fraction(tot) - exact[-1])) < 1e-12
+
+
+# ---------------------------------------------------- systematic resample ----
+def test_reference_u0_int32_product():
+    """pf_device.cu:1852 multiplies 1000000 * N in int: N=10000 wraps."""
+    assert detmath.reference_u0(42, 100) == 42 / 1e8
+    assert detmath.reference_u0(42, 10000) == 42 / 1410065408.0
+    assert detmath.reference_u0(42, 1_000_000) == 42 / -727379968.0
+
+
+def _brute_systematic(w, u0):
+    """Exact rational CDF, first index with cdf >= t, clamped to N-1."""
+    n = len(w)
+    tot = sum(Fraction(x) for x in w)
+    cdf, run = [], Fraction(0)
+    for x in w:
+        run += Fraction(x)
+        cdf.append(run / tot)
+    out = []
+    for j in range(n):
+        t = (Fraction(u0) + j) / n
+        i = next((k for k, c in enumerate(cdf) if c >= t), n - 1)
+        out.append(i)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 300, 1024, 1500])
+def test_systematic_matches_brute_force(n):
+    rng = np.random.default_rng(n)
+    lw = rng.normal(0.0, 2.0, n).astype(np.float32)
+    u0 = 0.37
+    got = detmath.systematic_ancestors(lw, n, u0)
+    e = np.exp(lw.astype(np.float64) - lw.max())
+    ref = _brute_systematic(e, u0)
+    # identical except where a target lands within rounding of a CDF value
+    assert np.sum(got != ref) <= max(1, n // 1000)
+
+
+def test_systematic_invariants_large():
+    n = 20_000
+    lw = np.random.default_rng(5).normal(0, 3, n).astype(np.float32)
+    u0 = 0.5
+    a = detmath.systematic_ancestors(lw, n, u0)
+    assert a.dtype == np.int32
+    assert np.all(np.diff(a) >= 0) and a.min() >= 0 and a.max() < n
+    w = np.exp(lw.astype(np.float64) - lw.max())
+    w /= w.sum()
+    counts = np.bincount(a, minlength=n)
+    assert np.all(counts >= np.floor(n * w) - 1) and np.all(counts <= np.ceil(n * w) + 1)
+    assert np.mean((counts == np.floor(n * w)) | (counts == np.ceil(n * w))) > 0.999
+    ref = detmath.sequential_systematic(w, u0)
+    assert np.mean(ref == a) > 0.999
+
+
+def test_systematic_uniform_weights_identity():
+    n = 5000
+    a = detmath.systematic_ancestors(np.zeros(n, np.float32), n, detmath.reference_u0(42, n))
+    np.testing.assert_array_equal(a, np.arange(n))
+
+
+def test_systematic_degenerate_single_particle_weight():
+    n = 3000
+    lw = np.full(n, -1e4, np.float32)
+    lw[1234] = 0.0
+    a = detmath.systematic_ancestors(lw, n, 0.1)
+    assert np.all(a == 1234)
+
+
+# ------------------------------------------------------------- Megopolis ----
+def test_megopolis_ref_zero_iterations_identity():
+    lw = np.random.default_rng(6).normal(size=100).astype(np.float32)
+    np.testing.assert_array_equal(detmath.megopolis_ref_ancestors(lw, 100, 0, 3),
+                                  np.arange(100))
+
+
+def test_megopolis_ref_uniform_weights_always_accept():
+    """alpha = 1 for equal weights: each iteration moves to the proposal."""
+    n = 97
+    a = detmath.megopolis_ref_ancestors(np.zeros(n, np.float32), n, 1, 11)
+    r = philox.draw(11, np.arange(n), 0, philox.PURPOSE_MEGO_REF, 0)
+    off = (philox.u01d(r[0], r[1]) * (n - 1)).astype(np.int64) + 1
+    np.testing.assert_array_equal(a, (np.arange(n) + off) % n)
+
+
+def test_megopolis_concentrates_on_heavy_particles():
+    """Metropolis chains on the original weights converge to the posterior."""
+    n = 4096
+    lw = np.full(n, -50.0, np.float32)
+    heavy = np.random.default_rng(12).choice(n, n // 4, replace=False)
+    lw[heavy] = 0.0
+    a = detmath.megopolis_ancestors(lw, n, 60, 5, 0)
+    assert np.mean(np.isin(a, heavy)) > 0.999
+
+
+def test_megopolis_ref_slot_weight_rule():
+    """The reference kernel compares the weights of the *slots* (pf_device.cu:
+    1265-1270), not of their current contents: a heavy slot only takes content
+    from heavy slots, so its content stays heavy; light slots accept anything."""
+    n = 4096
+    lw = np.full(n, -50.0, np.float32)
+    heavy = np.arange(0, n, 64)
+    lw[heavy] = 0.0
+    a = detmath.megopolis_ref_ancestors(lw, n, 30, 5)
+    assert np.all(np.isin(a[heavy], heavy))
+    assert np.mean(np.isin(a, heavy)) > len(heavy) / n
+
+
+def test_megopolis_chesser_matches_target_distribution():
+    n = 8192
+    rng = np.random.default_rng(8)
+    lw = rng.normal(0, 1, n).astype(np.float32)
+    a = detmath.megopolis_ancestors(lw, n, 64, 9, 0)
+    w = np.exp(lw.astype(np.float64))
+    w /= w.sum()
+    # weighted mean of a smooth test function is preserved in expectation
+    f = np.sin(np.arange(n) / 50.0)
+    assert abs(f[a].mean() - (w * f).sum()) < 0.05
+
+
+# -------------------------------------------------------- PF closed forms ----
+def _stream_scenario(n_sat=5):
+    rng = np.random.RandomState(123)
+    d = rng.randn(n_sat, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    sat = d * 20_000_000.0
+    pr = np.linalg.norm(sat, axis=1) + 100.0
+    return sat, pr
+
+
+def test_closed_form_gaussian_and_huber():
+    g = GOLD["stream_closed_forms"]
+    sat, pr = _stream_scenario()
+    pr = pr.copy()
+    pr[0] += 100.0
+    pos, cb, isb = np.zeros((1, 3)), np.array([100.0]), np.zeros((1, 0))
+    w = np.ones(5)
+    lg = opf.log_likelihood(pos, cb, isb, sat, pr, w, None, 5.0)
+    lh = opf.log_likelihood(pos, cb, isb, sat, pr, w, None, 5.0, huber=True, huber_k=1.5)
+    assert lg[0] == pytest.approx(g["gaussian_log_weight"])
+    assert lh[0] == pytest.approx(g["huber_log_weight"])
+
+
+def test_gate_semantics():
+    g = GOLD["stream_closed_forms"]
+    sat, pr = _stream_scenario()
+    pos, cb, isb = np.zeros((1, 3)), np.array([100.0]), np.zeros((1, 0))
+    out = pr.copy()
+    out[0] += 100.0
+    l1 = opf.log_likelihood(pos, cb, isb, sat, out, np.ones(5), None, 5.0,
+                            gate_thr=g["gate_threshold_m"])
+    assert abs(l1[0]) < 1e-6
+    l2 = opf.log_likelihood(pos, cb, isb, sat, pr + 100.0, np.ones(5), None, 5.0,
+                            gate_thr=g["gate_threshold_m"])
+    assert l2[0] < g["gate_reject_all_max_log_weight"]
+
+
+def test_student_t_cauchy_brute_force():
+    """nu = 1 (Cauchy): -log(1 + (r/s)^2) per satellite (pf_device.cu:264-266)."""
+    sat, pr = _stream_scenario(4)
+    pos, cb, isb = np.zeros((1, 3)), np.array([100.0]), np.zeros((1, 0))
+    pr = pr + np.array([1.0, -2.0, 30.0, 0.5])
+    got = opf.log_likelihood(pos, cb, isb, sat, pr, np.ones(4), None, 5.0, nu=1.0)
+    r = np.array([1.0, -2.0, 30.0, 0.5])
+    assert got[0] == pytest.approx(-np.sum(np.log1p((r / 5.0) ** 2)), rel=1e-12)
+
+
+def test_ess_and_estimate_closed_forms():
+    n = 1000
+    assert opf.ess(np.zeros(n)) == pytest.approx(n)
+    lw = np.full(n, -100.0)
+    lw[3] = 0.0
+    assert opf.ess(lw) < 2.0
+    st = np.random.default_rng(9).normal(size=(n, 4))
+    np.testing.assert_allclose(opf.weighted_mean(st, np.zeros(n)), st.mean(axis=0))
+    w = np.random.default_rng(10).random(n)
+    np.testing.assert_allclose(opf.weighted_mean(st, np.log(w)), (w[:, None] * st).sum(0) / w.sum())
+
+
+# -------------------------------------------------------------------- WLS ----
+def test_wls_position_no_sagnac_truth():
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    pr = np.linalg.norm(sat - tp, axis=1) + g["true_cb"]
+    res, it = owls.wls_position_ref(sat, pr, np.ones(8))
+    assert np.linalg.norm(res[:3] - tp) < GOLD["wls_bounds"]["single_epoch_error_m"]
+    assert abs(res[3] - g["true_cb"]) < 0.01
+    assert 1 <= it <= 10
+
+
+def test_wls_batch_sagnac_truth_and_noise_bound():
+    from gnss_gpu_amd.range_model import geometric_ranges_sagnac
+
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    pr = geometric_ranges_sagnac(tp, sat) + g["true_cb"]
+    res, it = owls.wls_batch_ref(sat[None], pr[None], np.ones((1, 8)))
+    assert np.linalg.norm(res[0, :3] - tp) < 1e-3
+    rng = np.random.default_rng(42)
+    E = 50
+    prn = np.tile(pr, (E, 1)) + rng.normal(0, 3.0, (E, 8))
+    res, it = owls.wls_batch_ref(np.tile(sat, (E, 1, 1)), prn, np.ones((E, 8)))
+    assert np.all(np.linalg.norm(res[:, :3] - tp, axis=1) < GOLD["wls_bounds"]["batch_error_m_with_3m_noise"])
+
+
+def test_wls_fewer_than_four_and_singular():
+    sat = np.array(GOLD["tokyo_constellation"]["sat_ecef"])[:3]
+    res, it = owls.wls_position_ref(sat, np.ones(3) * 2e7, np.ones(3))
+    assert it == -1 and np.all(res == 0)
+    # all satellites at one point: singular normal matrix -> early return
+    s1 = np.tile(np.array([[1.5e7, 1.0e7, 2.0e7]]), (5, 1))
+    res, it = owls.wls_position_ref(s1, np.full(5, 2.2e7), np.ones(5))
+    assert it == 0 and np.all(np.isfinite(res))
+
+
+def test_sagnac_rotation_closed_form():
+    """Rotation by theta about z preserves |sat| and the z component, and theta
+    equals omega_E * range / c (range_model.py:20-29)."""
+    from gnss_gpu_amd.range_model import OMEGA_E, C_LIGHT, rotate_satellites_sagnac
+
+    rx = np.array([-3957199.0, 3310205.0, 3737911.0])
+    sat = np.array(GOLD["tokyo_constellation"]["sat_ecef"])
+    rot = rotate_satellites_sagnac(rx, sat)
+    np.testing.assert_allclose(np.linalg.norm(rot, axis=1), np.linalg.norm(sat, axis=1), rtol=1e-15)
+    np.testing.assert_array_equal(rot[:, 2], sat[:, 2])
+    th = OMEGA_E * np.linalg.norm(rx - sat, axis=1) / C_LIGHT
+    ang = np.arctan2(sat[:, 1], sat[:, 0]) - np.arctan2(rot[:, 1], rot[:, 0])
+    np.testing.assert_allclose(ang, th, rtol=1e-9)

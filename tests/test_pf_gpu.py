@@ -1,0 +1,372 @@
+"""GPU parity of the particle-filter step against the oracle.
+
+Bar: resampling ancestors bit-exact for identical log-weights and uniforms;
+log-weights within 1e-5 relative (binary32 path); positions and estimates
+within 1 mm of the binary64 oracle.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import detmath, philox
+from oracle import pf as opf
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_cases.json")))
+SEED = 42
+
+
+def _dev():
+    from gnss_gpu_amd import _gnss_gpu_pf_device as dev
+
+    return dev
+
+
+def _stream_scenario(n_sat=8):
+    rng = np.random.RandomState(123)
+    d = rng.randn(n_sat, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    sat = d * 20_000_000.0
+    pr = np.linalg.norm(sat, axis=1) + 100.0
+    return sat, pr, np.ones(n_sat)
+
+
+# ------------------------------------------------ device building blocks ----
+def test_device_exp_and_philox_bit_exact(gpu):
+    import ctypes as C
+
+    from gnss_gpu_amd import _lib
+
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-745, 0, 5000), rng.uniform(-30, 5, 5000),
+                        np.array([0.0, -1e-300, -744.9, -708.5, -745.2])])
+    out = np.empty_like(x)
+    _lib.call("gg_debug_math", 0, _lib.dptr(x), _lib.dptr(out), x.size)
+    np.testing.assert_array_equal(out.view(np.uint64), detmath.exp64(x).view(np.uint64))
+    xf = rng.uniform(-100, 10, 5000).astype(np.float32).astype(np.float64)
+    out = np.empty_like(xf)
+    _lib.call("gg_debug_math", 1, _lib.dptr(xf), _lib.dptr(out), xf.size)
+    np.testing.assert_array_equal(out.astype(np.float32).view(np.uint32),
+                                  detmath.exp32(xf.astype(np.float32)).view(np.uint32))
+    idx = np.arange(1000, dtype=np.float64)
+    out = np.empty(4000)
+    _lib.call("gg_debug_math", 2, _lib.dptr(idx), _lib.dptr(out), 1000)
+    ref = philox.draw(0x123456789ABCDEF0, np.arange(1000), 7, 3, 1)
+    np.testing.assert_array_equal(out.reshape(1000, 4).astype(np.uint64), np.stack(ref, axis=1))
+    _ = C
+
+
+# -------------------------------------------------------- closed forms ------
+def _pfd(**kw):
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    args = dict(n_particles=1024, sigma_pos=1.0, sigma_cb=300.0, sigma_pr=5.0,
+                resampling="systematic", seed=SEED)
+    args.update(kw)
+    return ParticleFilterDevice(**args)
+
+
+def test_closed_form_gaussian_and_huber(gpu):
+    g = GOLD["stream_closed_forms"]
+    sat, pr, w = _stream_scenario(5)
+    pr_out = pr.copy()
+    pr_out[0] += 100.0
+    vals = []
+    for huber in (False, True):
+        pf = _pfd(per_particle_huber=huber, per_particle_huber_undiff_pr_k=1.5)
+        pf.initialize(np.zeros(3), clock_bias=100.0, spread_pos=0.0, spread_cb=0.0)
+        pf.update(sat, pr_out, w, resample=False)
+        lw = pf.get_log_weights()
+        assert np.all(lw == lw[0])
+        vals.append(float(lw[0]))
+    assert vals[0] == pytest.approx(g["gaussian_log_weight"], rel=1e-6)
+    assert vals[1] == pytest.approx(g["huber_log_weight"], rel=1e-6)
+
+
+def test_nlos_gate(gpu):
+    sat, pr, w = _stream_scenario(5)
+    out = pr.copy()
+    out[0] += 100.0
+    pf = _pfd(per_particle_nlos_gate=True, per_particle_nlos_undiff_pr_threshold_m=30.0)
+    pf.initialize(np.zeros(3), clock_bias=100.0, spread_pos=0.0, spread_cb=0.0)
+    pf.update(sat, out, w, resample=False)
+    assert float(np.max(np.abs(pf.get_log_weights()))) < 1e-6
+    pf2 = _pfd(per_particle_nlos_gate=True, per_particle_nlos_undiff_pr_threshold_m=30.0)
+    pf2.initialize(np.zeros(3), clock_bias=100.0, spread_pos=0.0, spread_cb=0.0)
+    pf2.update(sat, pr + 100.0, w, resample=False)
+    assert float(np.max(pf2.get_log_weights())) < -900.0
+
+
+def test_predict_deterministic_velocity(gpu):
+    """sigma = 0: every particle moves by v * dt (test_cuda_streams.py:239-255)."""
+    dev = _dev()
+    n = 2048
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 100.0, 0.0, 0.0, SEED)
+    dev.pf_device_predict(st, 2.0, -1.0, 0.5, 0.25, 0.0, 0.0, SEED, 1)
+    s = dev.pf_device_get_particle_states(st)
+    np.testing.assert_allclose(s[:, :3], np.tile([0.5, -0.25, 0.125], (n, 1)), atol=1e-12)
+    np.testing.assert_allclose(s[:, 3], 100.0)
+    np.testing.assert_allclose(s[:, 4:7], np.tile([2.0, -1.0, 0.5], (n, 1)))
+
+
+# ------------------------------------------------------- oracle parity ------
+def _setup(n, anchor=None, spread=30.0, spread_cb=20.0, seed=SEED, n_isb=0):
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    anchor = np.array(GOLD["tokyo_constellation"]["true_pos"]) if anchor is None else anchor
+    pf = ParticleFilterDevice(n_particles=n, sigma_pos=1.0, sigma_cb=3.0, sigma_pr=5.0,
+                              resampling="systematic", seed=seed, n_isb=n_isb)
+    pf.initialize(anchor, clock_bias=3000.0, spread_pos=spread, spread_cb=spread_cb)
+    orc = opf.OraclePF(n, seed=seed, n_isb=n_isb)
+    orc.initialize(anchor, 3000.0, spread, spread_cb)
+    return pf, orc
+
+
+def test_initialize_matches_oracle(gpu):
+    pf, orc = _setup(5000)
+    p = pf.get_particles()
+    np.testing.assert_allclose(p, orc.state, rtol=0, atol=1e-4)
+    # moments (test_particle_filter.py:51-79)
+    assert abs(p[:, 0].mean() - orc.state[:, 0].mean()) < 1e-4
+    assert abs(p[:, :3].std(axis=0).mean() - 30.0) < 3.0
+
+
+def test_predict_matches_oracle(gpu):
+    pf, orc = _setup(5000)
+    v = np.array([3.0, -2.0, 1.0])
+    for step in range(1, 4):
+        pf.predict(velocity=v, dt=1.0)
+        orc.predict(v, 1.0, 1.0, 3.0, step)
+    np.testing.assert_allclose(pf.get_particles(), orc.state, rtol=0, atol=1e-3)
+
+
+@pytest.mark.parametrize("kind", ["gauss", "cauchy", "huber"])
+def test_weight_ess_estimate_match_oracle(gpu, kind):
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    pr = np.linalg.norm(sat - tp, axis=1) + g["true_cb"] + np.random.default_rng(1).normal(0, 3, 8)
+    pf, orc = _setup(20000)
+    if kind == "cauchy":
+        pf.nu = 1.0
+    if kind == "huber":
+        pf.per_particle_huber = True
+    pf.update(sat, pr, resample=False)
+    lw = pf.get_log_weights()
+    st = pf.get_particles()
+    ref = opf.log_likelihood(st[:, :3], st[:, 3], np.zeros((len(st), 0)), sat, pr, np.ones(8), None,
+                             5.0, nu=1.0 if kind == "cauchy" else 0.0, huber=kind == "huber")
+    np.testing.assert_allclose(lw, ref, rtol=1e-5, atol=1e-5)
+    assert pf.get_ess() == pytest.approx(opf.ess(lw), rel=1e-9)
+    est = pf.estimate()
+    np.testing.assert_allclose(est, opf.weighted_mean(st, lw), rtol=0, atol=1e-3)
+    c = tp + 5.0
+    assert pf.get_position_spread(c) == pytest.approx(opf.position_spread(st[:, :3], lw, c), rel=1e-6)
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 1024, 1025, 4099, 100_000, 1_000_000])
+def test_systematic_ancestors_bit_exact(gpu, n):
+    dev = _dev()
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, -3957199.0, 3310205.0, 3737911.0, 3000.0, 40.0, 10.0, SEED)
+    lw = np.random.default_rng(n).normal(0.0, 4.0, n)
+    dev.pf_device_set_log_weights(st, lw)
+    before = dev.pf_device_get_particles(st)
+    dev.pf_device_resample_systematic(st, SEED)
+    anc = np.empty(n, np.int32)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    ref = detmath.systematic_ancestors(lw.astype(np.float32), n, detmath.reference_u0(SEED, n))
+    np.testing.assert_array_equal(anc, ref)
+    after = dev.pf_device_get_particles(st)
+    np.testing.assert_allclose(after, before[anc], rtol=0, atol=1e-3)
+    assert np.all(dev.pf_device_get_log_weights is not None)
+    out = np.empty(n)
+    dev.pf_device_get_log_weights(st, out)
+    assert np.all(out == 0.0)
+    assert dev.pf_device_ess(st) == pytest.approx(n)
+
+
+def test_systematic_fresh_u0_and_degenerate(gpu):
+    dev = _dev()
+    n = 3000
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 0.0, 10.0, 10.0, 7)
+    lw = np.full(n, -1e4)
+    lw[1234] = 0.0
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_systematic(st, 7, 1)
+    anc = np.empty(n, np.int32)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    assert np.all(anc == 1234)
+    lw = np.random.default_rng(3).normal(0, 2, n)
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_systematic(st, 7, 1)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    ref = detmath.systematic_ancestors(lw.astype(np.float32), n, detmath.fresh_u0(7, 1))
+    np.testing.assert_array_equal(anc, ref)
+
+
+@pytest.mark.parametrize("n,iters", [(1000, 15), (70_000, 32), (5, 3)])
+def test_megopolis_reference_rule_bit_exact(gpu, n, iters):
+    dev = _dev()
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 0.0, 10.0, 10.0, 11)
+    lw = np.random.default_rng(n).normal(0.0, 2.0, n)
+    dev.pf_device_set_log_weights(st, lw)
+    before = dev.pf_device_get_particles(st)
+    dev.pf_device_resample_megopolis(st, iters, 11)
+    anc = np.empty(n, np.int32)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    ref = detmath.megopolis_ref_ancestors(lw.astype(np.float32), n, iters, 11)
+    np.testing.assert_array_equal(anc, ref)
+    np.testing.assert_allclose(dev.pf_device_get_particles(st), before[anc], rtol=0, atol=1e-3)
+
+
+@pytest.mark.parametrize("n,iters", [(1000, 15), (70_000, 32), (300_000, 64)])
+def test_megopolis_chesser_bit_exact(gpu, n, iters):
+    dev = _dev()
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 0.0, 10.0, 10.0, 13)
+    lw = np.random.default_rng(n + 1).normal(0.0, 2.0, n)
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_megopolis_chesser(st, iters)
+    anc = np.empty(n, np.int32)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    ref = detmath.megopolis_ancestors(lw.astype(np.float32), n, iters, 13, 0)
+    np.testing.assert_array_equal(anc, ref)
+
+
+def test_state_roundtrip_and_frame_shifts(gpu):
+    dev = _dev()
+    n = 3000
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, -3957199.0, 3310205.0, 3737911.0, 3000.0, 50.0, 20.0, 3)
+    s = dev.pf_device_get_particle_states(st)
+    rng = np.random.default_rng(4)
+    s[:, :3] += rng.normal(0, 5.0, (n, 3))
+    s[:, 4:7] = [1.0, 2.0, 3.0]
+    dev.pf_device_set_particle_states(st, s)
+    s2 = dev.pf_device_get_particle_states(st)
+    np.testing.assert_allclose(s2[:, :4], s[:, :4], rtol=0, atol=1e-4)
+    np.testing.assert_allclose(s2[:, 4:7], s[:, 4:7], atol=1e-9)
+    lw = rng.normal(0, 1, n)
+    dev.pf_device_set_log_weights(st, lw)
+    out = np.empty(n)
+    dev.pf_device_get_log_weights(st, out)
+    np.testing.assert_allclose(out, lw.astype(np.float32).astype(np.float64))
+    e0 = dev.pf_device_estimate(st)
+    dev.pf_device_shift_clock_bias(st, 12.5)
+    dev.pf_device_shift_position(st, 1.0, -2.0, 3.0)
+    e1 = dev.pf_device_estimate(st)
+    np.testing.assert_allclose(e1 - e0, [1.0, -2.0, 3.0, 12.5], atol=1e-6)
+    p = dev.pf_device_get_particles(st)
+    np.testing.assert_allclose(p[:, 3], s[:, 3] + 12.5, atol=1e-4)
+
+
+def test_many_satellites_grow_capacity(gpu):
+    sat, pr, w = _stream_scenario(80)
+    dev = _dev()
+    st = dev.pf_device_create(10_000)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 100.0, 50.0, 200.0, SEED)
+    dev.pf_device_weight(st, sat.ravel(), pr, w, 80, 5.0)
+    est = dev.pf_device_estimate(st)
+    assert np.all(np.isfinite(est))
+    lw = np.empty(10_000)
+    dev.pf_device_get_log_weights(st, lw)
+    s = dev.pf_device_get_particles(st)
+    ref = opf.log_likelihood(s[:, :3], s[:, 3], np.zeros((10_000, 0)), sat, pr, w, None, 5.0)
+    np.testing.assert_allclose(lw, ref, rtol=1e-5, atol=1e-4)
+
+
+def test_ancestors_api_semantics(gpu):
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    pf = ParticleFilterDevice(n_particles=4096, resampling="megopolis", seed=1)
+    pf.initialize(np.zeros(3))
+    with pytest.raises(RuntimeError):
+        pf.get_resample_ancestors()
+    pf2 = ParticleFilterDevice(n_particles=4096, resampling="systematic", seed=1)
+    pf2.initialize(np.zeros(3))
+    sat, pr, w = _stream_scenario(8)
+    pf2.update(sat, pr, w)
+    a = pf2.get_resample_ancestors()
+    assert a.dtype == np.int32 and a.shape == (4096,)
+    assert a.min() >= 0 and a.max() < 4096
+
+
+def test_uninitialized_raises(gpu):
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    pf = ParticleFilterDevice(n_particles=100)
+    with pytest.raises(RuntimeError, match="not initialized"):
+        pf.estimate()
+    with pytest.raises(RuntimeError, match="not initialized"):
+        pf.predict()
+
+
+def test_isb_states_match_oracle(gpu):
+    sc_anchor = np.array([-3.96e6, 3.35e6, 3.70e6])
+    from gnss_gpu_amd import sim
+
+    sc = sim.scenario_multi_gnss(n_epochs=2, seed=2, anchor=sc_anchor)
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    n = 20_000
+    pf = ParticleFilterDevice(n_particles=n, sigma_pos=1.0, sigma_cb=3.0, sigma_pr=5.0, nu=1.0,
+                              resampling="systematic", seed=5, n_isb=3, sigma_isb=0.5)
+    pf.initialize(sc.truth[0], clock_bias=sc.cb, spread_pos=20.0, spread_cb=10.0,
+                  isb=sc.isb, spread_isb=5.0)
+    orc = opf.OraclePF(n, seed=5, n_isb=3)
+    orc.initialize(sc.truth[0], sc.cb, 20.0, 10.0, isb=sc.isb, spread_isb=5.0)
+    from gnss_gpu_amd import _gnss_gpu_pf_device as dev
+
+    np.testing.assert_allclose(dev.pf_device_get_particles_full(pf._state), orc.state, atol=1e-4)
+    pf.predict(velocity=sc.vel, dt=1.0)
+    orc.predict(sc.vel, 1.0, 1.0, 3.0, 1, sigma_isb=0.5)
+    full = dev.pf_device_get_particles_full(pf._state)
+    np.testing.assert_allclose(full, orc.state, atol=1e-3)
+    pf.update(sc.sat, sc.pr[1], system_ids=sc.sys, resample=False)
+    lw = pf.get_log_weights()
+    ref = opf.log_likelihood(full[:, :3], full[:, 3], full[:, 4:], sc.sat, sc.pr[1], np.ones(12),
+                             sc.sys, 5.0, nu=1.0)
+    np.testing.assert_allclose(lw, ref, rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(pf.estimate_full(), opf.weighted_mean(full, lw), atol=1e-3)
+
+
+def test_end_to_end_plain_filter_matches_oracle(gpu):
+    """Config-1 trajectory (N=10,000, 8 GPS, Gaussian 5 m, systematic at ESS<N/2)."""
+    from gnss_gpu_amd import ParticleFilterDevice, sim
+
+    sc = sim.scenario_plain(n_epochs=60, seed=0)
+    n = 10_000
+    pf = ParticleFilterDevice(n_particles=n, sigma_pos=1.0, sigma_cb=3.0, sigma_pr=5.0,
+                              resampling="systematic", ess_threshold=0.5, seed=0)
+    pf.initialize(sc.truth[0], clock_bias=sc.cb, spread_pos=10.0, spread_cb=10.0)
+    orc = opf.OraclePF(n, seed=0)
+    orc.initialize(sc.truth[0], sc.cb, 10.0, 10.0)
+    errs, diffs, mism, resampled = [], [], 0, 0
+    for t in range(sc.n_epochs):
+        if t > 0:
+            pf.predict(velocity=sc.vel, dt=1.0)
+            orc.predict(sc.vel, 1.0, 1.0, 3.0, t)
+        pf.update(sc.sat, sc.pr[t], resample=False)
+        orc.weight(sc.sat, sc.pr[t], sigma_pr=5.0)
+        lw_gpu = pf.get_log_weights()
+        np.testing.assert_allclose(lw_gpu, orc.lw, rtol=1e-4, atol=1e-3)
+        if pf.resample_if_needed():
+            resampled += 1
+            assert orc.ess() < 0.5 * n * 1.001
+            orc.resample_systematic(lw32=lw_gpu.astype(np.float32))
+            mism += int(np.sum(pf.get_resample_ancestors() != orc.ancestors))
+        est = pf.estimate()
+        diffs.append(np.abs(est - orc.estimate()).max())
+        errs.append(np.linalg.norm(est[:3] - sc.truth[t]))
+    assert resampled > 5
+    assert mism == 0
+    assert max(diffs) < 1e-3
+    assert np.median(errs) < 10.0
