@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Plain copy kernel of known size: HBM read + write of `mib` MiB each."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def cpu_info():
@@ -63,6 +78,10 @@ def main():
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the particle-filter "
                           "step is measured per GPU (--gpus 1)"}))
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
 
@@ -118,7 +137,7 @@ def main():
     # ---- per-kernel timing (hipEvents on the handle's stream)
     reps = args.stage_reps
     stage_ms = {}
-    stage_ms["predict_weight_decide"] = pf.time_stage(st, STAGE_WEIGHT_PREDICT, reps) / reps
+    stage_ms["predict_weight"] = pf.time_stage(st, STAGE_WEIGHT_PREDICT, reps) / reps
     if args.resampler == "systematic":
         stage_ms["cdf"] = pf.time_stage(st, STAGE_CDF, reps) / reps
         stage_ms["gather"] = pf.time_stage(st, STAGE_GATHER_SYSTEMATIC, reps) / reps
@@ -129,7 +148,7 @@ def main():
 
     # algorithmic HBM bytes per particle and launch (binary32 state, D = 7)
     bytes_pp = {
-        "predict_weight_decide": 2 * D * 4 + 4,  # read + write state, write lw
+        "predict_weight": 2 * D * 4 + 4,         # read + write state, write lw
         "cdf": 4 + 4,                            # read lw, write output ends
         "gather": 4 + 2 * D * 4 + 4,             # read ends, gather state, write ancestors
         "megopolis": 4 + 4 * args.mego_iters + 4,
@@ -141,8 +160,10 @@ def main():
                          "hbm_traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(dom)
-        except (OSError, ValueError):
+            rec = json.load(open(tpath)).get(dom)
+            if rec and rec.get("particles") == N and rec.get("satellites") == S:
+                traffic = rec["bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
             traffic = None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

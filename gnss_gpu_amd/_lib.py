@@ -151,6 +151,13 @@ EXPORTED = tuple(["gg_last_error", *_SIGS.keys()])
 
 
 def _load():
+    # PyTorch ships its own HIP runtime; import it first so the process has a
+    # single runtime and this library binds to it (torch tensors and our
+    # kernels then share devices and streams).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library with `python -m gnss_gpu_amd.build`"

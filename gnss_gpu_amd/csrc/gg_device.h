@@ -68,13 +68,14 @@ __host__ __device__ __forceinline__ double gg_u01d(uint32_t hi, uint32_t lo) {
   return (double)(m + 1) * 1.1102230246251565e-16;  // 2^-53
 }
 
-// Box-Muller pair from two (0,1) uniforms, binary32.
+// Box-Muller pair from two (0,1) uniforms, binary32, on the hardware
+// transcendental units: v_log_f32 (log2), v_sqrt_f32, and v_sin/v_cos_f32,
+// whose argument is in revolutions, so sin(2 pi u) is one instruction.
+// Relative error of the normals is about 1e-6 (noise generation only).
 __device__ __forceinline__ void gg_box_muller(float u1, float u2, float& z0, float& z1) {
-  const float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincospif(2.0f * u2, &s, &c);
-  z0 = r * c;
-  z1 = r * s;
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  z0 = r * __builtin_amdgcn_cosf(u2);
+  z1 = r * __builtin_amdgcn_sinf(u2);
 }
 
 // Four standard normals from one Philox block.
@@ -246,26 +247,4 @@ __device__ __forceinline__ int gg_block_prefix_max256_i(int v, int* lds) {
   v = max(v, pre);
   __syncthreads();
   return v;
-}
-
-// ---------------------------------------------------------------------------
-// "Last block" handshake: every block of a group publishes its partials, then
-// bumps the group's counter; the block that observes count == n-1 is the last
-// one and may read every partial.  It resets the counter for the next launch.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool gg_last_block(unsigned int* counter, unsigned int n_blocks,
-                                              int* lds_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int prev = atomicAdd(counter, 1u);
-    *lds_flag = (prev == n_blocks - 1u) ? 1 : 0;
-  }
-  __syncthreads();
-  const bool last = *lds_flag != 0;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-  }
-  return last;
 }

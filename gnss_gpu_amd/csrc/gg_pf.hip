@@ -196,7 +196,17 @@ void enu_rotation(const double* x, double* R) {
     } break;                          \
   }
 
-int launch_weight(gg_pf* h, const gg_step_args& s, bool pred) {
+int max_tiles(const gg_pf* h) {
+  int nt = 1;
+  for (const auto& f : h->filt_h) nt = std::max(nt, f.n_tiles);
+  return nt;
+}
+
+size_t gather_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(int32_t); }
+size_t decide_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(double); }
+
+// likelihood (+ predict) over all tiles, then the per-filter decision
+int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide = true) {
   const gg_arena a = arena_of(h);
   const size_t smem = sat_smem(h);
   if (pred) {
@@ -207,13 +217,24 @@ int launch_weight(gg_pf* h, const gg_step_args& s, bool pred) {
                                                   dim3(GG_BLOCK), smem, h->stream, a, s));
   }
   GG_HIP(hipGetLastError());
+  if (with_decide) {
+    const int mode = (pred ? 1 : 0) | 2;
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
+                                                  dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+                                                  mode));
+    GG_HIP(hipGetLastError());
+  }
   return 0;
 }
 
 int launch_reduce(gg_pf* h, const gg_step_args& s) {
   const gg_arena a = arena_of(h);
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_reduce<NI>), dim3(h->n_tiles),
-                                                dim3(GG_BLOCK), 0, h->stream, a, s));
+                                                dim3(GG_BLOCK), 0, h->stream, a));
+  GG_HIP(hipGetLastError());
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
+                                                dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+                                                0));
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -225,12 +246,19 @@ int launch_cdf(gg_pf* h, const gg_step_args& s, int force) {
   return 0;
 }
 
-int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf) {
+int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf,
+                  bool with_finalize = true) {
   const gg_arena a = arena_of(h);
+  const size_t smem = (anc_mode == 0) ? gather_smem(h) : 0;
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_gather<NI>), dim3(h->n_chunks),
-                                                dim3(GG_BLOCK), 0, h->stream, a, s, force,
+                                                dim3(GG_BLOCK), smem, h->stream, a, s, force,
                                                 anc_mode, src_buf));
   GG_HIP(hipGetLastError());
+  if (with_finalize) {
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_finalize<NI>), dim3(h->n_filters),
+                                                  dim3(GG_BLOCK), 0, h->stream, a, force));
+    GG_HIP(hipGetLastError());
+  }
   return 0;
 }
 
@@ -336,9 +364,9 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   int32_t tiles = 0, chunks = 0, rows = 0;
   for (int f = 0; f < n_filters; ++f) {
     const int n = n_particles[f];
-    if (n <= 0 || n > (1 << 30)) {
+    if (n <= 0 || n > GG_MAX_PARTICLES) {
       delete h;
-      return gg_fail(-2, "n_particles must be in [1, 2^30]");
+      return gg_fail(-2, "n_particles must be in [1, 7864320] per filter");
     }
     gg_filter F;
     std::memset(&F, 0, sizeof(F));
@@ -550,6 +578,8 @@ int gg_pf_device_predict(gg_pf* h, double vx, double vy, double vz, double dt, d
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_predict<NI>), dim3(h->n_tiles),
                                                 dim3(GG_BLOCK), 0, h->stream, a, s));
   GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_commit_predict, dim3(h->n_filters), dim3(64), 0, h->stream, a, s);
+  GG_HIP(hipGetLastError());
   h->est_valid = false;
   return 0;
 }
@@ -625,6 +655,8 @@ int gg_pf_device_position_spread(gg_pf* h, double cx, double cy, double cz, doub
   const gg_arena a = arena_of(h);
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_spread<NI>), dim3(h->n_tiles),
                                                 dim3(GG_BLOCK), 0, h->stream, a, cx, cy, cz));
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_spread_combine, dim3(h->n_filters), dim3(GG_BLOCK), 0, h->stream, a);
   GG_HIP(hipGetLastError());
   // the spread kernel reuses the tile-partial buffer
   h->decide_valid = false;
@@ -1009,12 +1041,19 @@ int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t
   for (int k = 0; k < reps; ++k) {
     int rc = 0;
     switch (stage) {
-      case 0: rc = launch_weight(h, *args, true); break;
-      case 1: rc = launch_weight(h, *args, false); break;
+      case 0: rc = launch_weight(h, *args, true, false); break;
+      case 1: rc = launch_weight(h, *args, false, false); break;
       case 2: rc = launch_cdf(h, *args, 1); break;
-      case 3: rc = launch_gather(h, *args, 1, 0, 0); break;
+      case 3: rc = launch_gather(h, *args, 1, 0, 0, false); break;
       case 4: rc = launch_mego(h, n_iter, 1); break;
-      case 5: rc = launch_gather(h, *args, 1, 1, 1); break;
+      case 5: rc = launch_gather(h, *args, 1, 1, 1, false); break;
+      case 6: {
+        const gg_arena a = arena_of(h);
+        GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
+                                                      dim3(GG_BLOCK), decide_smem(h), h->stream, a,
+                                                      *args, 0));
+        rc = hipGetLastError() == hipSuccess ? 0 : gg_fail(-1, "decide launch failed");
+      } break;
       default: return gg_fail(-2, "unknown stage");
     }
     if (rc) return rc;

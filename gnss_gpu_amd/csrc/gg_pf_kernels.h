@@ -260,14 +260,15 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
     double d[3];
     gg_rot_local(dy->R, dx, d);
     const double rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double inv_rho = 1.0 / rho;
     const int sys = si[r];
     const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
     double c = sd[3 * a.sat_rows + r] - rho - dy->cb_anchor;
     if (ib >= 0) c -= dy->isb_anchor[ib];
-    sv.ux[k] = (float)(d[0] / rho);
-    sv.uy[k] = (float)(d[1] / rho);
-    sv.uz[k] = (float)(d[2] / rho);
-    sv.irho[k] = (float)(1.0 / rho);
+    sv.ux[k] = (float)(d[0] * inv_rho);
+    sv.uy[k] = (float)(d[1] * inv_rho);
+    sv.uz[k] = (float)(d[2] * inv_rho);
+    sv.irho[k] = (float)inv_rho;
     sv.c[k] = (float)c;
     sv.w[k] = (float)sd[4 * a.sat_rows + r];
     sv.isb[k] = ib;
@@ -285,7 +286,7 @@ __device__ __forceinline__ float gg_residual(const gg_sat_view& sv, int k, const
   const float ud = sv.ux[k] * p[0] + sv.uy[k] * p[1] + sv.uz[k] * p[2];
   const float ir = sv.irho[k];
   const float q = pp * ir - 2.0f * ud;
-  const float dr = q / (1.0f + sqrtf(1.0f + q * ir));
+  const float dr = q * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f + q * ir));
   float e = sv.c[k] - dr - p[3];
   if (NISB > 0) {
     const int ib = sv.isb[k];
@@ -294,6 +295,12 @@ __device__ __forceinline__ float gg_residual(const gg_sat_view& sv, int k, const
       if (ib == j) e -= p[4 + j];
   }
   return e;
+}
+
+// natural log on the hardware log2 unit (relative error ~1e-7 for the
+// products of (1 + r^2/(nu s^2)) factors used here)
+__device__ __forceinline__ float gg_lnf(float x) {
+  return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
 }
 
 template <int NISB>
@@ -326,12 +333,12 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
       if (gate && fabsf(e) > s.gate_thr) continue;
       prod *= 1.0f + sv.w[k] * e * e * inv_nu_s2;
       if (++cnt == 3) {
-        acc += logf(prod);
+        acc += gg_lnf(prod);
         prod = 1.0f;
         cnt = 0;
       }
     }
-    acc += logf(prod);
+    acc += gg_lnf(prod);
     lw = -half_nup1 * acc;
   } else if (s.lik == GG_LIK_HUBER) {
     const float hk = s.huber_k;
@@ -428,12 +435,17 @@ __device__ __forceinline__ double gg_u0(const gg_step_args& s, uint64_t seed, in
   return (double)(seed % 1000000ull) / (double)den;
 }
 
-// Combine a filter's tile partials (run by the last block of the filter).
-// Chunked fixed order: thread t owns tiles [t*c, t*c+c), sums them left to
-// right from 0.0, and the per-thread values go through gg_block_scan256.
+// Combine a filter's tile partials (one workgroup per filter).
+//   pass 1 (strided, coalesced): M = max m_b;
+//   pass 2 (strided): g_b = exp64(m_b - M), T_b = S_b g_b staged in LDS (sT),
+//           ESS and weighted-mean sums accumulated in the strided order;
+//   pass 3: W and the CDF tile offsets P_b in the fixed chunked order (thread
+//           t owns tiles [t*c, t*c+c), sums them left to right from 0.0, the
+//           per-thread values go through gg_block_scan256), read from LDS.
+// sT needs 8 * n_tiles bytes of LDS.
 template <int D>
 __device__ void gg_decide(const gg_arena& a, const gg_step_args& s, const gg_filter& F,
-                          gg_dyn* dy, const double* anchor, double* lds) {
+                          gg_dyn* dy, const double* anchor, double* lds, double* sT) {
 #pragma clang fp contract(off)
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
@@ -441,36 +453,32 @@ __device__ void gg_decide(const gg_arena& a, const gg_step_args& s, const gg_fil
   double m = -__builtin_huge_val();
   for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
   const double M = gg_block_max256(m, lds);
-  const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
-  const int b0 = min(t * c, nt);
-  const int b1 = min(b0 + c, nt);
-  double T = 0.0, T2 = 0.0, Q[D];
+  double vv[D + 2];
 #pragma unroll
-  for (int d = 0; d < D; ++d) Q[d] = 0.0;
-  for (int b = b0; b < b1; ++b) {
+  for (int d = 0; d < D + 2; ++d) vv[d] = 0.0;
+  for (int b = t; b < nt; b += GG_BLOCK) {
     const double* pb = part + (int64_t)b * GG_NPART;
     const double mb = pb[0];
     const double gb = (mb > -__builtin_huge_val()) ? gg_exp64(mb - M) : 0.0;
     a.g[F.tile_off + b] = gb;
-    T = T + pb[1] * gb;
-    T2 = T2 + pb[2] * (gb * gb);
+    sT[b] = pb[1] * gb;
+    vv[0] = vv[0] + pb[2] * (gb * gb);
 #pragma unroll
-    for (int d = 0; d < D; ++d) Q[d] = Q[d] + pb[3 + d] * gb;
+    for (int d = 0; d < D; ++d) vv[2 + d] = vv[2 + d] + pb[3 + d] * gb;
   }
+  __syncthreads();
+  const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+  const int b0 = min(t * c, nt);
+  const int b1 = min(b0 + c, nt);
+  double T = 0.0;
+  for (int b = b0; b < b1; ++b) T = T + sT[b];
   double excl, W;
   gg_block_scan256(T, lds, excl, W);
-  // per-tile exclusive CDF offsets
   double run = 0.0;
   for (int b = b0; b < b1; ++b) {
-    const double* pb = part + (int64_t)b * GG_NPART;
     a.Pb[F.tile_off + b] = excl + run;
-    run = run + pb[1] * a.g[F.tile_off + b];
+    run = run + sT[b];
   }
-  double vv[D + 2];
-  vv[0] = T2;
-  vv[1] = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) vv[2 + d] = Q[d];
   double tot[D + 2];
   gg_block_totals<D>(vv, lds, tot);
   if (t == 0) {

@@ -47,16 +47,18 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_init(gg_arena a, float spread_p
 }
 
 // ---------------------------------------------------------------------------
-// predict only (drop-in path): in place on the current buffer.
+// predict only (drop-in path): in place on the current buffer.  The anchor and
+// velocity bookkeeping is committed afterwards by gg_k_commit_predict (kernel
+// boundaries are the only cross-workgroup synchronisation in this library: a
+// device-scope fence costs an L2 write-back per workgroup on gfx950).
 // ---------------------------------------------------------------------------
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_predict(gg_arena a, gg_step_args s) {
   constexpr int D = 4 + NISB;
-  __shared__ int sflag;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int cur = dy->cur;
   const int slot = s.slot_from_device ? (dy->epoch % s.n_slots) : 0;
@@ -72,34 +74,49 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_predict(gg_arena a, gg_step_arg
   for (int q = 0; q < GG_ITEMS; ++q)
     if (i0 + q < F.n) gg_predict_one<NISB>(a, cur, base + q, F.seed, i0 + q, step, s, pl, p[q]);
   gg_store4<D>(a, cur, base, p);
-  if (gg_last_block(&dy->cnt_d, (unsigned)F.n_tiles, &sflag) && threadIdx.x == 0) {
-    double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
-    double ve[3];
-    gg_rot_ecef(dy->R, vr, ve);
-    for (int k = 0; k < 3; ++k) {
-      dy->anchor[k] += ve[k];
-      dy->vel_u[k] = pl.v_new_u[k];
-    }
-    dy->vel_mode = pl.mode_out;
-    if (s.velocity_kf && pl.mode_out < 2)
-      for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
-    dy->est_valid = 0;
+}
+
+// Anchor/velocity commit of a predict (one thread per filter).  Recomputes the
+// plan from the same inputs the predict blocks used.
+__device__ __forceinline__ void gg_commit_predict_one(const gg_arena& a, const gg_step_args& s,
+                                                      int f, gg_dyn* dy, double* anchor_out) {
+  const int slot = s.slot_from_device ? (dy->epoch % s.n_slots) : 0;
+  const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+  double vge[3] = {vg[0], vg[1], vg[2]};
+  const gg_pred_plan pl = gg_make_plan(dy, s, vge);
+  double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
+  double ve[3];
+  gg_rot_ecef(dy->R, vr, ve);
+  for (int k = 0; k < 3; ++k) {
+    anchor_out[k] = dy->anchor[k] + ve[k];
+    dy->anchor[k] = anchor_out[k];
+    dy->vel_u[k] = pl.v_new_u[k];
   }
+  dy->vel_mode = pl.mode_out;
+  if (s.velocity_kf && pl.mode_out < 2)
+    for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
+  dy->est_valid = 0;
+}
+
+__global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
+  const int f = blockIdx.x;
+  if (threadIdx.x != 0 || f >= a.n_filters) return;
+  double anc[3];
+  gg_commit_predict_one(a, s, f, a.dyn + f, anc);
 }
 
 // ---------------------------------------------------------------------------
-// predict (optional) + likelihood + tile partials + on-device decision.
+// predict (optional) + likelihood + per-tile partials.
 // ---------------------------------------------------------------------------
 template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args s) {
   constexpr int D = 4 + NISB;
   extern __shared__ __align__(16) char smem[];
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ int sflag;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int cur = dy->cur;
   const int epoch = dy->epoch;
@@ -137,38 +154,19 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args
   for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? gg_loglik<NISB>(sv, ns, p[q], s) : 0.0f;
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
   gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
-
-  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
-    if (threadIdx.x == 0) {
-      if (PRED) {
-        for (int k = 0; k < 3; ++k) {
-          dy->anchor[k] = anchor[k];
-          dy->vel_u[k] = pl.v_new_u[k];
-        }
-        dy->vel_mode = pl.mode_out;
-        if (s.velocity_kf && pl.mode_out < 2)
-          for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
-      }
-      dy->lw_zero = 0;
-      dy->did_resample = 0;
-    }
-    gg_decide<D>(a, s, F, dy, anchor, lds);
-    if (threadIdx.x == 0 && (s.slot_from_device || s.step_from_device)) dy->epoch = epoch + 1;
-  }
 }
 
 // ---------------------------------------------------------------------------
-// Re-run the reduction and decision over the stored log-weights.
+// Tile partials over the stored log-weights (no likelihood evaluation).
 // ---------------------------------------------------------------------------
 template <int NISB>
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a, gg_step_args s) {
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a) {
   constexpr int D = 4 + NISB;
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ int sflag;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int cur = dy->cur;
   const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
@@ -187,9 +185,36 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a, gg_step_args
     lw[3] = v.w;
   }
   gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
-  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
-    double anchor[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
-    gg_decide<D>(a, s, F, dy, anchor, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Decision: one workgroup per filter combines the tile partials.
+// mode bit 0: commit the predict that preceded the weight launch;
+// mode bit 1: the partials come from a fresh likelihood (clears lw_zero,
+//             did_resample and advances the device epoch).
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args s, int mode) {
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
+  double* sT = reinterpret_cast<double*>(smem);
+  __shared__ double lds[8 + GG_BLOCK];
+  __shared__ double sanchor[3];
+  const int f = blockIdx.x;
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  if (threadIdx.x == 0) {
+    double anc[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
+    if (mode & 1) gg_commit_predict_one(a, s, f, dy, anc);
+    for (int k = 0; k < 3; ++k) sanchor[k] = anc[k];
+  }
+  __syncthreads();
+  double anchor[3] = {sanchor[0], sanchor[1], sanchor[2]};
+  gg_decide<D>(a, s, F, dy, anchor, lds, sT);
+  if (threadIdx.x == 0 && (mode & 2)) {
+    dy->lw_zero = 0;
+    dy->did_resample = 0;
+    if (s.slot_from_device || s.step_from_device) dy->epoch = dy->epoch + 1;
   }
 }
 
@@ -200,17 +225,19 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a, gg_step_args
 // k(cdf_i) = #{ j : (u0 + j) / n <= cdf_i } (n for the last particle, which
 // reproduces the hi = n-1 clamp of a binary search).  Ancestor of output j is
 // min{ i : cdf_i >= t_j } = #{ i : E_i <= j } with E the prefix max of k, so
-// no monotonicity of the floating-point CDF is assumed.
+// no monotonicity of the floating-point CDF is assumed.  Writes the
+// tile-local prefix max and the tile maximum; the gather completes the prefix
+// max across tiles.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
 #pragma clang fp contract(off)
   const double dn = (double)n;
-  if (!(c >= (u0 + 0.0) / dn)) return 0;
-  if (c >= (u0 + (double)(n - 1)) / dn) return n;
-  double est = floor(c * dn - u0);
-  int j = (est < 0.0) ? 0 : (est > (double)(n - 1) ? n - 1 : (int)est);
+  if (c != c) return 0;
+  // estimate of the last j with t_j <= c, then walk to the exact boundary
+  const double est = floor(c * dn - u0);
+  int j = (est < -1.0) ? -1 : (est > (double)(n - 1) ? n - 1 : (int)est);
   while (j + 1 < n && (u0 + (double)(j + 1)) / dn <= c) ++j;
-  while (j > 0 && (u0 + (double)j) / dn > c) --j;
+  while (j >= 0 && (u0 + (double)j) / dn > c) --j;
   return j + 1;
 }
 
@@ -218,11 +245,11 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
 #pragma clang fp contract(off)
   __shared__ double lds[8 + GG_BLOCK];
   __shared__ int ilds[8];
-  __shared__ int sflag;
+  __shared__ int tmax[GG_BLOCK];
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   if (!force && !dy->flag) return;
   const int lt = tile - F.tile_off;
   const int n = F.n;
@@ -258,9 +285,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
   for (int q = 0; q < GG_ITEMS; ++q) {
     const int i = i0 + q;
     int k;
-    if (i >= n) {
-      k = n;
-    } else if (i == n - 1) {
+    if (i >= n - 1) {
       k = n;
     } else {
       const double L = excl + l[q];
@@ -271,8 +296,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
     E[q] = rmax;
   }
   const int pre_incl = gg_block_prefix_max256_i(rmax, ilds);
-  // exclusive prefix max of the thread maxima: shift by one thread
-  __shared__ int tmax[GG_BLOCK];
   tmax[threadIdx.x] = pre_incl;
   __syncthreads();
   const int pre = (threadIdx.x == 0) ? 0 : tmax[threadIdx.x - 1];
@@ -280,96 +303,108 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
   for (int q = 0; q < GG_ITEMS; ++q) E[q] = max(E[q], pre);
   *reinterpret_cast<int4*>(a.E + base) = make_int4(E[0], E[1], E[2], E[3]);
   if (threadIdx.x == GG_BLOCK - 1) a.emax[tile] = E[GG_ITEMS - 1];
-  if (gg_last_block(&dy->cnt_b, (unsigned)F.n_tiles, &sflag)) {
-    // inclusive prefix max over the filter's tiles
-    int carry = 0;
-    for (int b0 = 0; b0 < F.n_tiles; b0 += GG_BLOCK) {
-      const int b = b0 + threadIdx.x;
-      const int v = (b < F.n_tiles) ? a.emax[F.tile_off + b] : 0;
-      int pm = gg_block_prefix_max256_i(v, ilds);
-      pm = max(pm, carry);
-      if (b < F.n_tiles) a.pmax[F.tile_off + b] = pm;
-      tmax[threadIdx.x] = pm;
-      __syncthreads();
-      carry = tmax[GG_BLOCK - 1];
-      __syncthreads();
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
 // Systematic resampling, part 2 / common gather.
+//
+// Work item = GG_QCHUNK consecutive steps of the merge of the particles'
+// positions pos_i = i + E_i (strictly increasing) with the outputs j = 0..n-1;
+// every item handles the same number of (input + output) steps whatever the
+// weights.  With M_b the inclusive prefix max of the tile maxima, the global
+// end of particle i in tile b is max(M_{b-1}, E_i) and the last position of
+// tile b is TE_b = last_i + M_b.  A work item:
+//   1. builds M_b for its filter in LDS,
+//   2. finds the tile A holding its start diagonal d0 by binary search of TE,
+//   3. loads the ends of tiles A and A+1 (its whole input range) into LDS and
+//      counts positions below d0 and d1 -> input range [i0, i1),
+//   4. ancestor of output j in [d0 - i0, d1 - i1) = i0 + #{ i in [i0, i1) : E_i <= j }.
+// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity.
+// Dynamic LDS: 4 * max(n_tiles) bytes for M.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int gg_Eglob(const gg_arena& a, const gg_filter& F, int i) {
-  const int b = i / GG_TILE;
-  const int pm = (b > 0) ? a.pmax[F.tile_off + b - 1] : 0;
-  return max(pm, a.E[F.p_off + i]);
-}
-
-// #{ i in [0,n) : i + E_i < d }  (i + E_i is strictly increasing), found by
-// one full wavefront with a 64-ary search.
-__device__ __forceinline__ int gg_merge_search(const gg_arena& a, const gg_filter& F, int64_t d) {
-  const int lane = threadIdx.x & 63;
-  int lo = 0, hi = F.n;
-  while (lo < hi) {
-    const int span = hi - lo;
-    const int step = (span + 63) >> 6;
-    const int64_t pidx = (int64_t)lo + (int64_t)lane * step;
-    bool pred = false;
-    if (pidx < hi) {
-      const int i = (int)pidx;
-      pred = ((int64_t)i + (int64_t)gg_Eglob(a, F, i)) < d;
-    }
-    const uint64_t m = __ballot(pred);
-    const int cnt = __popcll(m);
-    if (cnt == 0) {
-      hi = lo;
-    } else {
-      const int pl = lo + (cnt - 1) * step;
-      lo = pl + 1;
-      hi = min(pl + step, hi);
-    }
-  }
-  return lo;
-}
-
-// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args s, int force,
                                                         int anc_mode, int src_buf) {
   constexpr int D = 4 + NISB;
-  __shared__ int sE[GG_QCHUNK];
+  extern __shared__ __align__(16) char smem[];
+  int* spm = reinterpret_cast<int*>(smem);
+  constexpr int WT = GG_QCHUNK / GG_TILE + 1;  // tiles spanned by one item's inputs
+  __shared__ int sE[WT * GG_TILE];
   __shared__ int sA[GG_QCHUNK];
-  __shared__ int sI[2];
+  __shared__ int ilds[8];
+  __shared__ int sI[4];
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ int sflag;
   const int chunk = blockIdx.x;
   const int f = a.chunk_filter[chunk];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   if (!force && !dy->flag) return;
   const int cl = chunk - F.chunk_off;
   const int n = F.n;
   const int cur = dy->cur;
   const int nxt = 1 - cur;
+  const int t = threadIdx.x;
   int jbase, nout;
   if (anc_mode == 0) {
+    const int nt = F.n_tiles;
+    // 1. inclusive prefix max of the tile maxima
+    int carry = 0;
+    for (int b0 = 0; b0 < nt; b0 += GG_BLOCK) {
+      const int b = b0 + t;
+      const int v = (b < nt) ? a.emax[F.tile_off + b] : 0;
+      int pm = max(gg_block_prefix_max256_i(v, ilds), carry);
+      if (b < nt) spm[b] = pm;
+      if (t == GG_BLOCK - 1) ilds[4] = pm;
+      __syncthreads();
+      carry = ilds[4];
+      __syncthreads();
+    }
     const int64_t d0 = (int64_t)cl * GG_QCHUNK;
     const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
-    if (threadIdx.x < 128) {
-      const int w = threadIdx.x >> 6;
-      const int i = gg_merge_search(a, F, w ? d1 : d0);
-      if ((threadIdx.x & 63) == 0) sI[w] = i;
+    // 2. first tile whose last position reaches d0
+    if (t == 0) {
+      int lo = 0, hi = nt - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int64_t last = (int64_t)min((mid + 1) * GG_TILE, n) - 1;
+        if (last + spm[mid] < d0)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      sI[0] = lo;
     }
     __syncthreads();
-    const int ia = sI[0], ib = sI[1];
+    const int ta = sI[0];
+    // 3. ends of tiles ta .. ta+WT-1 (global values), positions below d0 / d1
+    int c0 = 0, c1 = 0;
+    for (int k0 = 4 * t; k0 < WT * GG_TILE; k0 += 4 * GG_BLOCK) {
+      const int b = ta + k0 / GG_TILE;
+      const int i0 = ta * GG_TILE + k0;
+      int4 ev = make_int4(0x3fffffff, 0x3fffffff, 0x3fffffff, 0x3fffffff);
+      if (b < nt) ev = *reinterpret_cast<const int4*>(a.E + F.p_off + i0);
+      const int pm = (b > 0 && b < nt) ? spm[b - 1] : 0;
+      int e4[4] = {ev.x, ev.y, ev.z, ev.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + q;
+        const int e = (b < nt && i < n) ? max(pm, e4[q]) : 0x3fffffff;
+        sE[k0 + q] = e;
+        const int64_t pos = (int64_t)i + e;
+        c0 += (i < n && pos < d0) ? 1 : 0;
+        c1 += (i < n && pos < d1) ? 1 : 0;
+      }
+    }
+    c0 = gg_block_sum256_i(c0, ilds);
+    c1 = gg_block_sum256_i(c1, ilds);
+    const int ia = ta * GG_TILE + c0;
+    const int ib = ta * GG_TILE + c1;
     const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
-    for (int k = threadIdx.x; k < ib - ia; k += GG_BLOCK) sE[k] = gg_Eglob(a, F, ia + k);
-    __syncthreads();
-    const int na = ib - ia;
-    for (int k = threadIdx.x; k < j1 - j0; k += GG_BLOCK) {
+    const int la = c0, lb = c1;  // local range in sE
+    // 4. ancestors of this item's outputs
+    for (int k = t; k < j1 - j0; k += GG_BLOCK) {
       const int j = j0 + k;
-      int lo = 0, hi = na;
+      int lo = la, hi = lb;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (sE[mid] <= j)
@@ -377,7 +412,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
         else
           hi = mid;
       }
-      sA[k] = ia + lo;
+      sA[k] = ta * GG_TILE + lo;
     }
     __syncthreads();
     jbase = j0;
@@ -394,7 +429,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   double acc[D + 2];
 #pragma unroll
   for (int d = 0; d < D + 2; ++d) acc[d] = 0.0;
-  for (int k = threadIdx.x; k < nout; k += GG_BLOCK) {
+  for (int k = t; k < nout; k += GG_BLOCK) {
     const int j = jbase + k;
     const int src = (anc_mode == 0) ? sA[k] : (anc_mode == 1 ? asrc[j] : j);
 #pragma unroll
@@ -412,45 +447,55 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   }
   double tot[D + 2];
   gg_block_totals<D>(acc, lds, tot);
-  if (threadIdx.x == 0)
+  if (t == 0)
     for (int d = 0; d < D; ++d) a.csum[(int64_t)chunk * GG_MAX_D + d] = tot[2 + d];
-  if (gg_last_block(&dy->cnt_c, (unsigned)F.n_chunks, &sflag)) {
-    const int nc = F.n_chunks;
-    const int c = (nc + GG_BLOCK - 1) / GG_BLOCK;
-    const int c0 = min((int)threadIdx.x * c, nc);
-    const int c1 = min(c0 + c, nc);
-    double v[D + 2];
+}
+
+// After the gather: mean of the resampled particles, anchor re-centring, buffer
+// swap, weights reset (one workgroup per filter).
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_finalize(gg_arena a, int force) {
+  constexpr int D = 4 + NISB;
+  __shared__ double lds[8 + GG_BLOCK];
+  const int f = blockIdx.x;
+  const gg_filter F = a.filt[f];
+  gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int n = F.n;
+  const int nc = F.n_chunks;
+  double v[D + 2];
 #pragma unroll
-    for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
-    for (int cc = c0; cc < c1; ++cc)
+  for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
+  for (int cc = threadIdx.x; cc < nc; cc += GG_BLOCK)
 #pragma unroll
-      for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + a.csum[(int64_t)(F.chunk_off + cc) * GG_MAX_D + d];
-    gg_block_totals<D>(v, lds, tot);
-    if (threadIdx.x == 0) {
-      double shp[3] = {(double)sh[0], (double)sh[1], (double)sh[2]};
-      double she[3];
-      gg_rot_ecef(dy->R, shp, she);
-      for (int k = 0; k < 3; ++k) dy->anchor[k] += she[k];
-      dy->cb_anchor += (double)sh[3];
-      for (int d = 4; d < D; ++d) dy->isb_anchor[d - 4] += (double)sh[d];
-      for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / (double)n;
-      double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
-      double me[3];
-      gg_rot_ecef(dy->R, mp, me);
-      for (int k = 0; k < 3; ++k) dy->est[k] = dy->anchor[k] + me[k];
-      dy->est[3] = dy->cb_anchor + dy->mean[3];
-      for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
-      dy->cur = nxt;
-      dy->lw_zero = 1;
-      dy->flag = 0;
-      dy->did_resample = 1;
-      dy->resample_count += 1;
-      dy->ess = (double)n;
-      dy->M = 0.0;
-      dy->W = (double)n;
-      dy->W2 = (double)n;
-      dy->est_valid = 1;
-    }
+    for (int d = 0; d < D; ++d)
+      v[2 + d] = v[2 + d] + a.csum[(int64_t)(F.chunk_off + cc) * GG_MAX_D + d];
+  double tot[D + 2];
+  gg_block_totals<D>(v, lds, tot);
+  if (threadIdx.x == 0) {
+    double shp[3] = {(double)dy->shift[0], (double)dy->shift[1], (double)dy->shift[2]};
+    double she[3];
+    gg_rot_ecef(dy->R, shp, she);
+    for (int k = 0; k < 3; ++k) dy->anchor[k] += she[k];
+    dy->cb_anchor += (double)dy->shift[3];
+    for (int d = 4; d < D; ++d) dy->isb_anchor[d - 4] += (double)dy->shift[d];
+    for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / (double)n;
+    double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
+    double me[3];
+    gg_rot_ecef(dy->R, mp, me);
+    for (int k = 0; k < 3; ++k) dy->est[k] = dy->anchor[k] + me[k];
+    dy->est[3] = dy->cb_anchor + dy->mean[3];
+    for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
+    dy->cur = 1 - dy->cur;
+    dy->lw_zero = 1;
+    dy->flag = 0;
+    dy->did_resample = 1;
+    dy->resample_count += 1;
+    dy->ess = (double)n;
+    dy->M = 0.0;
+    dy->W = (double)n;
+    dy->W2 = (double)n;
+    dy->est_valid = 1;
   }
 }
 
@@ -557,19 +602,19 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
 }
 
 // ---------------------------------------------------------------------------
-// Weighted RMS distance of the positions to a centre (local frame, binary32).
-// Writes scratch_f[f*16 + 0].
+// Weighted RMS distance of the positions to a centre: per-tile partials
+// (m_b, S_b, sum e*d^2) and a one-workgroup-per-filter combine that writes
+// scratch_f[f*16 + 0].
 // ---------------------------------------------------------------------------
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_spread(gg_arena a, double cx, double cy,
                                                         double cz) {
 #pragma clang fp contract(off)
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ int sflag;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
+  const gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int cur = dy->cur;
   double cl[3];
@@ -590,7 +635,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_spread(gg_arena a, double cx, d
   bool valid[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
-  // partial: m_b, S_b, sum e*d^2 (binary64 distances)
   double m = -__builtin_huge_val();
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
@@ -615,28 +659,34 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_spread(gg_arena a, double cx, d
     pb[1] = tot[0];
     pb[2] = tot[2];
   }
-  if (gg_last_block(&dy->cnt_a, (unsigned)F.n_tiles, &sflag)) {
-    const int nt = F.n_tiles;
-    const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
-    double mm = -__builtin_huge_val();
-    for (int b = threadIdx.x; b < nt; b += GG_BLOCK) mm = fmax(mm, part[(int64_t)b * GG_NPART]);
-    const double M = gg_block_max256(mm, lds);
-    const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
-    const int b0 = min((int)threadIdx.x * c, nt);
-    const int b1 = min(b0 + c, nt);
-    double w[3] = {0.0, 0.0, 0.0};
-    for (int b = b0; b < b1; ++b) {
-      const double* pb = part + (int64_t)b * GG_NPART;
-      const double gb = (pb[0] > -__builtin_huge_val()) ? gg_exp64(pb[0] - M) : 0.0;
-      w[0] = w[0] + pb[1] * gb;
-      w[2] = w[2] + pb[2] * gb;
-    }
-    gg_block_totals<1>(w, lds, tot);
-    if (threadIdx.x == 0) {
-      double r = 0.0;
-      if (tot[0] > 0.0 && isfinite(tot[2])) r = sqrt(fmax(tot[2] / tot[0], 0.0));
-      a.scratch_f[(int64_t)f * 16 + 0] = r;
-    }
+}
+
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_spread_combine(gg_arena a) {
+#pragma clang fp contract(off)
+  __shared__ double lds[8 + GG_BLOCK];
+  const int f = blockIdx.x;
+  const gg_filter F = a.filt[f];
+  const int nt = F.n_tiles;
+  const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
+  double mm = -__builtin_huge_val();
+  for (int b = threadIdx.x; b < nt; b += GG_BLOCK) mm = fmax(mm, part[(int64_t)b * GG_NPART]);
+  const double M = gg_block_max256(mm, lds);
+  const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+  const int b0 = min((int)threadIdx.x * c, nt);
+  const int b1 = min(b0 + c, nt);
+  double w[3] = {0.0, 0.0, 0.0};
+  for (int b = b0; b < b1; ++b) {
+    const double* pb = part + (int64_t)b * GG_NPART;
+    const double gb = (pb[0] > -__builtin_huge_val()) ? gg_exp64(pb[0] - M) : 0.0;
+    w[0] = w[0] + pb[1] * gb;
+    w[2] = w[2] + pb[2] * gb;
+  }
+  double tot[3];
+  gg_block_totals<1>(w, lds, tot);
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    if (tot[0] > 0.0 && isfinite(tot[2])) r = sqrt(fmax(tot[2] / tot[0], 0.0));
+    a.scratch_f[(int64_t)f * 16 + 0] = r;
   }
 }
 

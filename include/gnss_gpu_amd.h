@@ -124,8 +124,9 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
 int gg_pfb_replay(gg_pf* h, int32_t n_replays);
 /* Device time of n_replays graph launches (hipEvents on the handle's stream). */
 int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total);
-/* Device time of `reps` launches of one stage: 0 weight+predict, 1 weight,
- * 2 cdf, 3 gather (systematic), 4 Megopolis (Chesser), 5 gather (ancestors). */
+/* Device time of `reps` launches of one kernel: 0 weight+predict, 1 weight,
+ * 2 cdf, 3 gather (systematic), 4 Megopolis (Chesser), 5 gather (ancestors),
+ * 6 decision (one workgroup per filter). */
 int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
                       int32_t reps, float* ms_total);
 int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out);

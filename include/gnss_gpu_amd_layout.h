@@ -24,7 +24,8 @@
 #define GG_BLOCK 256
 #define GG_ITEMS 4
 #define GG_TILE (GG_BLOCK * GG_ITEMS)
-#define GG_QCHUNK GG_TILE
+#define GG_QCHUNK (2 * GG_TILE)  // merge-path steps per gather work item
+#define GG_MAX_PARTICLES (7680 * GG_TILE)  // per filter (per-tile values kept in LDS)
 #define GG_MAX_ISB 3
 #define GG_MAX_D (4 + GG_MAX_ISB)
 #define GG_NVEL 9
