@@ -15,6 +15,8 @@ Contents
   wls.py      binary64 restatement of src/positioning/wls.cu.
   cpu_pf.c    C restatement used as the timed CPU baseline (OpenMP).
 
-Each function cites the reference file:line it follows.  Parity status of
-every function (pinned by which check) is listed in pinning.py.
+Each function cites the reference file:line it follows.  pinning.py lists,
+for every function, the check that pins it; what only an import of the
+reference could pin (its cuRAND streams, thrust's scan order) is listed there
+as parity unpinned.
 """

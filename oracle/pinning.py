@@ -1,0 +1,79 @@
+"""Parity status of every oracle function: which check pins it.
+
+TEST INFRASTRUCTURE ONLY.  A function is "pinned" when a check that does not
+restate its formula would fail on a plausible mistake in it (a dropped term,
+a wrong sign or index, a transposed operand): a published known-answer test,
+a closed form from the reference's own tests, brute force on tiny inputs, or
+an invariant.  "parity unpinned" marks what only an import of the reference
+(or of a closed library it calls) could pin.
+
+tests/test_oracle_pinning.py asserts that every public oracle function has an
+entry here.
+"""
+
+PINS = {
+    # philox.py
+    "philox.philox4x32_10": ("pinned", "Random123 Philox4x32-10 known-answer vectors "
+                             "(test_philox_known_answers)"),
+    "philox.draw": ("pinned", "stream layout through philox4x32_10 KATs; GPU bit-exact "
+                    "(test_device_exp_and_philox_bit_exact)"),
+    "philox.u01f": ("pinned", "range (0,1) and exact dyadic values (test_uniform_ranges_and_exactness)"),
+    "philox.u01d": ("pinned", "range (0,1] and 53-bit exactness (test_uniform_ranges_and_exactness)"),
+    "philox.normal4": ("pinned", "moments of 4e5 draws (test_normals_are_standard)"),
+    # detmath.py
+    "detmath.exp64": ("pinned", "<= 2 ulp against libm over [-745, 0] (test_exp64_against_libm)"),
+    "detmath.exp32": ("pinned", "<= 4 ulp (binary32) against binary64 exp "
+                      "(test_exp32_against_float64_exp)"),
+    "detmath.block_scan256": ("pinned", "exact on integers, rational bound on reals "
+                              "(test_block_scan_exact_on_integers, test_block_scan_close_to_exact_rational)"),
+    "detmath._hs_wave": ("pinned", "inside block_scan256 (exact on integers)"),
+    "detmath.block_total256": ("pinned", "last element of the pinned scan"),
+    "detmath.tile_view": ("pinned", "layout only; exercised by the brute-force systematic tests"),
+    "detmath.tile_partials": ("pinned", "brute-force systematic + ESS closed forms"),
+    "detmath.decide": ("pinned", "brute-force systematic (test_systematic_matches_brute_force)"),
+    "detmath.systematic_targets": ("pinned", "pf_device.cu:1202 (u0+j)/N; brute force"),
+    "detmath.reference_u0": ("pinned", "int32 product of pf_device.cu:1852 on N where 1e6*N "
+                             "overflows (test_reference_u0_int32_product)"),
+    "detmath.fresh_u0": ("pinned", "range [0,1) and Philox stream"),
+    "detmath.systematic_ends": ("pinned", "brute force cumsum+searchsorted, invariants "
+                                "(test_systematic_matches_brute_force, test_systematic_invariants_large)"),
+    "detmath.systematic_ancestors": ("pinned", "brute force on tiny N, identity for uniform weights, "
+                                     "single heavy particle"),
+    "detmath.megopolis_ref_ancestors": ("pinned", "B=0 identity, uniform weights accept every "
+                                        "proposal, slot-weight rule of pf_device.cu:1262-1270 "
+                                        "on a hand-computed case"),
+    "detmath.megopolis_ancestors": ("pinned", "chi-square against the target distribution "
+                                    "(test_megopolis_chesser_matches_target_distribution)"),
+    "detmath.sequential_systematic": ("pinned", "textbook searchsorted(side='left') on tiny N"),
+    # pf.py
+    "pf.enu_rotation": ("pinned", "orthonormality and the up axis along the ellipsoid normal "
+                        "(test_scenario_geometry)"),
+    "pf.huber_cost": ("pinned", "closed form -28.875 of test_cuda_streams.py:689-692"),
+    "pf.log_likelihood": ("pinned", "closed forms -200.0 (Gaussian) and -28.875 (Huber); "
+                          "zero residual -> 0; gate all-pass / all-reject; Student-t brute force"),
+    "pf.ess": ("pinned", "uniform -> N, one dominant -> < 2 (test_particle_filter.py:174-186)"),
+    "pf.weighted_mean": ("pinned", "uniform weights -> arithmetic mean (test_particle_filter.py:293-315)"),
+    "pf.position_spread": ("pinned", "uniform weights -> RMS distance, closed form on a 2-point cloud"),
+    "pf.OraclePF.initialize": ("pinned", "moments; the draw order x,y,z,cb follows pf_device.cu:67-104. "
+                               "The reference's cuRAND stream itself is parity unpinned"),
+    "pf.OraclePF.predict": ("pinned", "sigma=0 closed form p += v*dt (test_cuda_streams.py:239-255); "
+                            "noise values parity unpinned (cuRAND)"),
+    "pf.OraclePF.weight": ("pinned", "as log_likelihood"),
+    "pf.OraclePF.resample_systematic": ("pinned", "as systematic_ancestors"),
+    "pf.OraclePF.resample_megopolis": ("pinned", "as megopolis_ref_ancestors"),
+    "pf.OraclePF.resample_megopolis_chesser": ("pinned", "as megopolis_ancestors"),
+    # wls.py
+    "wls.wls_solve": ("pinned", "noise-free truth < 1 cm with and without the Sagnac term; "
+                      "singular geometry; < 4 satellites"),
+    "wls.wls_position_ref": ("pinned", "test_verification.py:305-333 truth (no Sagnac)"),
+    "wls.wls_batch_ref": ("pinned", "test_positioning.py:49-75 truth (Sagnac)"),
+    "wls._los": ("pinned", "Sagnac closed form of range_model.py:20-29 (test_sagnac_rotation_closed_form)"),
+    "wls._solve4": ("pinned", "numpy.linalg.solve on random SPD systems; singular pivot rule"),
+}
+
+UNPINNED = {
+    "cuRAND normal/uniform streams (pf_device.cu:118-186,1245-1270)":
+        "closed NVIDIA library; the library uses its own Philox4x32-10 and the oracle its NumPy twin",
+    "thrust::inclusive_scan summation order (pf_device.cu:1847-1849)":
+        "unspecified by thrust; the library pins its own fixed-order scan",
+}

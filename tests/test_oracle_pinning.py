@@ -328,3 +328,61 @@ def test_sagnac_rotation_closed_form():
     th = OMEGA_E * np.linalg.norm(rx - sat, axis=1) / C_LIGHT
     ang = np.arctan2(sat[:, 1], sat[:, 0]) - np.arctan2(rot[:, 1], rot[:, 0])
     np.testing.assert_allclose(ang, th, rtol=1e-9)
+
+
+def test_position_spread_two_point_cloud():
+    """Two particles at +-d along x with weights w, 1-w around the origin: the
+    weighted RMS distance is d whatever w; around (d, 0, 0) it is 2d*sqrt(1-w)."""
+    d, w = 3.0, 0.25
+    pos = np.array([[d, 0.0, 0.0], [-d, 0.0, 0.0]])
+    lw = np.log([w, 1.0 - w])
+    assert opf.position_spread(pos, lw, [0.0, 0.0, 0.0]) == pytest.approx(d, rel=1e-14)
+    assert opf.position_spread(pos, lw, [d, 0.0, 0.0]) == pytest.approx(2 * d * math.sqrt(1 - w), rel=1e-14)
+
+
+def test_solve4_against_numpy_and_pivot_rule():
+    rng = np.random.default_rng(3)
+    B = rng.normal(size=(64, 4, 4))
+    A = np.einsum("eij,ekj->eik", B, B) + 0.1 * np.eye(4)
+    b = rng.normal(size=(64, 4))
+    x, ok = owls._solve4(np.concatenate([A, b[:, :, None]], axis=2), cpu_rule=False)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b[:, :, None])[:, :, 0], rtol=1e-9, atol=1e-12)
+    assert ok.all()
+    # a zero column: the CPU rule flags the system, the batch rule skips the pivot
+    A0 = A[:1].copy()
+    A0[0, :, 2] = 0.0
+    _, ok0 = owls._solve4(np.concatenate([A0, b[:1, :, None]], axis=2), cpu_rule=True)
+    assert not ok0[0]
+
+
+def test_oracle_predict_deterministic_closed_form():
+    """sigma = 0: every particle moves by v*dt exactly (test_cuda_streams.py:239-255)."""
+    pf = opf.OraclePF(100, seed=1)
+    pf.initialize(np.array([1.0e6, 2.0e6, 3.0e6]), cb=5.0, spread_pos=4.0, spread_cb=2.0)
+    before = pf.state.copy()
+    pf.predict(np.array([1.5, -2.0, 0.25]), 2.0, 0.0, 0.0, step=1)
+    np.testing.assert_allclose(pf.state[:, :3] - before[:, :3],
+                               np.broadcast_to(np.array([3.0, -4.0, 0.5]), (100, 3)), atol=1e-9)
+    np.testing.assert_array_equal(pf.state[:, 3], before[:, 3])
+
+
+def test_every_oracle_function_has_a_pinning_entry():
+    import inspect
+
+    from oracle import pinning
+
+    mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls}
+    missing = []
+    for name, mod in mods.items():
+        for fn, obj in vars(mod).items():
+            if inspect.isfunction(obj) and obj.__module__ == mod.__name__:
+                if f"{name}.{fn}" not in pinning.PINS:
+                    missing.append(f"{name}.{fn}")
+            if inspect.isclass(obj) and obj.__module__ == mod.__name__:
+                for m, mo in vars(obj).items():
+                    if inspect.isfunction(mo) and not m.startswith("_") and m not in ("ess", "estimate"):
+                        if f"{name}.{fn}.{m}" not in pinning.PINS:
+                            missing.append(f"{name}.{fn}.{m}")
+    assert not missing, missing
+    for key, (status, how) in pinning.PINS.items():
+        assert status in ("pinned", "parity unpinned") and how
