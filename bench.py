@@ -37,10 +37,13 @@ def parse():
     ap.add_argument("--resampler", default="systematic",
                     choices=["systematic", "megopolis", "megopolis_ref"])
     ap.add_argument("--mego-iters", type=int, default=32)
-    ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="consecutive steps captured in one hipGraph (must divide --steps)")
+    ap.add_argument("--profile-steps", type=int, default=100,
+                    help="eager steps timed kernel by kernel (after the timed replays)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -71,6 +74,38 @@ def cpu_info():
     return model
 
 
+def cpu_steps(sc, init, N, S, T, threads, seconds, max_steps):
+    """Median wall ms per step of the OpenMP C restatement on the host cores."""
+    import ctypes as C
+
+    from oracle import build_cpu
+
+    lib = build_cpu.load()
+    h = lib.cpu_pf_create(N, 3, 12345, threads)
+    DP = C.POINTER(C.c_double)
+    ini = np.ascontiguousarray(init[0])
+    lib.cpu_pf_initialize(h, ini.ctypes.data_as(DP), 10.0, 10.0, 5.0)
+    sat = np.ascontiguousarray(sc.sat.ravel())
+    sysi = np.ascontiguousarray(sc.sys.astype(np.int32))
+    vel = np.ascontiguousarray(sc.vel)
+    est = np.zeros(8)
+    times = []
+    t_start = time.perf_counter()
+    t = 0
+    while (time.perf_counter() - t_start < seconds and t < max_steps) or t < 3:
+        pr = np.ascontiguousarray(sc.pr[t % T])
+        t0 = time.perf_counter()
+        lib.cpu_pf_step(h, vel.ctypes.data_as(DP), 1.0, 1.0, 1.0, 0.1, t + 1,
+                        sat.ctypes.data_as(DP), pr.ctypes.data_as(DP),
+                        sysi.ctypes.data_as(C.POINTER(C.c_int)), S, 5.0, 1.0, 0.5,
+                        est.ctypes.data_as(DP))
+        times.append((time.perf_counter() - t0) * 1e3)
+        t += 1
+    used = int(lib.cpu_pf_threads())
+    lib.cpu_pf_destroy(h)
+    return float(np.median(times)), len(times), used
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -86,13 +121,11 @@ def main():
     import torch
 
     from gnss_gpu_amd import sim
-    from gnss_gpu_amd.batched import (BatchedParticleFilter, STAGE_CDF,
-                                      STAGE_GATHER_ANCESTORS, STAGE_GATHER_SYSTEMATIC,
-                                      STAGE_MEGOPOLIS, STAGE_WEIGHT_PREDICT)
+    from gnss_gpu_amd.batched import BatchedParticleFilter
     from gnss_gpu_amd._lib import LIK_STUDENT
 
     N = args.particles
-    T = args.warmup + args.steps + args.stage_reps * 4 + 8
+    T = args.warmup + args.steps + args.profile_steps + 2 * max(args.graph_steps, 1) + 8
     sc = sim.scenario_multi_gnss(n_epochs=T, seed=0)
     S = sc.n_sat
     D = 7
@@ -121,39 +154,41 @@ def main():
         pf.initialize(init, [12345], spread_pos=10.0, spread_cb=10.0, spread_isb=5.0)
         pf.set_epoch(0)
 
-    # ---- timed region: graph replays of one full step
-    pf.capture(st, args.resampler, args.mego_iters, steps=1)
-    pf.replay(args.warmup)
+    # ---- timed region: graph replays, G consecutive steps per graph (the
+    # trajectory stays on the device; the epoch counter advances in the graph),
+    # one hipEvent pair per replay
+    G = args.graph_steps if (args.graph_steps > 0 and args.steps % args.graph_steps == 0) else 1
+    pf.capture(st, args.resampler, args.mego_iters, steps=G)
+    pf.replay(max(1, -(-args.warmup // G)))
     pf.sync()
     torch.cuda.synchronize()
     rc0 = pf.read_dyn()[0].resample_count
-    ms_total = pf.time_replays(args.steps)
+    per_replay = pf.time_replays_each(args.steps // G)
     pf.sync()
     torch.cuda.synchronize()
     rc1 = pf.read_dyn()[0].resample_count
-    ms_step = ms_total / args.steps
+    per_step = per_replay / G
+    ms_step = float(per_replay.sum()) / args.steps
     resample_fraction = (rc1 - rc0) / args.steps
 
-    # ---- per-kernel timing (hipEvents on the handle's stream)
-    reps = args.stage_reps
-    stage_ms = {}
-    stage_ms["predict_weight"] = pf.time_stage(st, STAGE_WEIGHT_PREDICT, reps) / reps
-    if args.resampler == "systematic":
-        stage_ms["cdf"] = pf.time_stage(st, STAGE_CDF, reps) / reps
-        stage_ms["gather"] = pf.time_stage(st, STAGE_GATHER_SYSTEMATIC, reps) / reps
-    elif args.resampler == "megopolis":
-        stage_ms["megopolis"] = pf.time_stage(st, STAGE_MEGOPOLIS, reps, args.mego_iters) / reps
-        stage_ms["gather"] = pf.time_stage(st, STAGE_GATHER_ANCESTORS, reps) / reps
+    # ---- per-kernel device time inside real steps (eager steps, hipEvents
+    # around each launch group on the handle's stream)
+    prof_steps = max(args.profile_steps, 1)
+    ms_res, ms_skip, (n_res, n_skip) = pf.profile_steps(st, prof_steps, args.resampler,
+                                                         args.mego_iters)
     pf.sync()
+    tot = max(n_res + n_skip, 1)
+    stage_ms = {k: (ms_res[k] * n_res + ms_skip[k] * n_skip) / tot for k in ms_res}
 
     # algorithmic HBM bytes per particle and launch (binary32 state, D = 7)
-    bytes_pp = {
-        "predict_weight": 2 * D * 4 + 4,         # read + write state, write lw
-        "cdf": 4 + 4,                            # read lw, write output ends
-        "gather": 4 + 2 * D * 4 + 4,             # read ends, gather state, write ancestors
-        "megopolis": 4 + 4 * args.mego_iters + 4,
-    }
-    dom = max(stage_ms, key=stage_ms.get)
+    gather_key = "gather" if args.resampler == "systematic" else "megopolis_gather"
+    resample_bytes = {
+        "systematic": 4 + 2 * D * 4 + 4,          # read ends, gather state, write ancestors
+        "megopolis": 4 + 4 * args.mego_iters + 2 * D * 4 + 4,
+        "megopolis_ref": 4 + 8 * args.mego_iters + 2 * D * 4 + 4,
+    }[args.resampler]
+    bytes_pp = {"predict_weight": 2 * D * 4 + 4}  # read + write state, write lw
+    dom = "predict_weight"
     achieved = bytes_pp[dom] * N / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -165,45 +200,24 @@ def main():
                 traffic = rec["bytes_per_launch"]
         except (OSError, ValueError, KeyError):
             traffic = None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes": bytes_pp[dom] * N,
-                "avg_launch_ms": round(stage_ms[dom], 5)}
+    roofline = {"bound": "hbm", "kernel": "gg_k_weight (predict + likelihood + tile partials)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": bytes_pp[dom] * N, "avg_launch_ms": round(stage_ms[dom], 5)}
+    resample_gbs = (resample_bytes * N / (ms_res[gather_key] * 1e-3) / 1e9) if n_res else None
 
     # ---- CPU baseline (bounded sample of the same workload, host cores)
     cpu = None
     if not args.no_cpu:
-        from oracle import build_cpu
-
-        lib = build_cpu.load()
-        import ctypes as C
-
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-        h = lib.cpu_pf_create(N, 3, 12345, threads)
-        ini = np.ascontiguousarray(init[0])
-        DP = C.POINTER(C.c_double)
-        lib.cpu_pf_initialize(h, ini.ctypes.data_as(DP), 10.0, 10.0, 5.0)
-        sat = np.ascontiguousarray(sc.sat.ravel())
-        sysi = np.ascontiguousarray(sc.sys.astype(np.int32))
-        vel = np.ascontiguousarray(sc.vel)
-        est = np.zeros(8)
-        times = []
-        t_start = time.perf_counter()
-        t = 0
-        while time.perf_counter() - t_start < args.cpu_seconds and t < 200:
-            pr = np.ascontiguousarray(sc.pr[t % T])
-            t0 = time.perf_counter()
-            lib.cpu_pf_step(h, vel.ctypes.data_as(DP), 1.0, 1.0, 1.0, 0.1, t + 1,
-                            sat.ctypes.data_as(DP), pr.ctypes.data_as(DP),
-                            sysi.ctypes.data_as(C.POINTER(C.c_int)), S, 5.0, 1.0, 0.5,
-                            est.ctypes.data_as(DP))
-            times.append((time.perf_counter() - t0) * 1e3)
-            t += 1
-        lib.cpu_pf_destroy(h)
-        cpu = {"value": round(float(np.median(times)), 3), "unit": "ms",
-               "cores": int(lib.cpu_pf_threads()), "kind": "port",
-               "sample": f"{len(times)} steps of the same 1M x 12 workload (binary64, "
-                         f"OpenMP C restatement), median ms/step; CPU {cpu_info()}"}
+        ms_all, n_all, used = cpu_steps(sc, init, N, S, T, threads, args.cpu_seconds, 200)
+        ms_one, n_one, _ = cpu_steps(sc, init, N, S, T, 1, args.cpu_seconds / 2, 50)
+        cpu = {"value": round(ms_all, 3), "unit": "ms", "cores": used, "kind": "port",
+               "single_thread_ms": round(ms_one, 3),
+               "sample": f"median ms/step over {n_all} steps ({used} OpenMP threads) and "
+                         f"{n_one} steps (1 thread) of the same 1M x 12 workload: binary64 "
+                         f"C restatement of the reference step (oracle/cpu_pf.c, gcc -O3 "
+                         f"-march=x86-64-v3 -fopenmp); CPU {cpu_info()}"}
 
     out = {
         "metric": METRIC,
@@ -221,10 +235,16 @@ def main():
         "config": {"workload": "pf_step_1Mx12_mgnss_cauchy", "particles": N, "satellites": S,
                    "state_dim": D, "likelihood": "cauchy(scale=5m)",
                    "resampler": args.resampler, "ess_threshold": 0.5, "nlos_fraction": 0.3,
-                   "graph_steps": 1},
+                   "graph_steps": G},
         "particles_per_s": round(N / (ms_step * 1e-3), 1),
+        "median_ms": round(float(np.median(per_step)), 5),
+        "p90_ms": round(float(np.percentile(per_step, 90)), 5),
+        "per_step_stats_over": f"{len(per_replay)} replays of {G} steps",
         "resample_fraction": round(resample_fraction, 3),
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "stage_ms_resampling_steps": {k: round(v, 5) for k, v in ms_res.items()},
+        "stage_ms_other_steps": {k: round(v, 5) for k, v in ms_skip.items()},
+        "resample_kernel_gbs": None if resample_gbs is None else round(resample_gbs, 1),
         "p50_err_m": None if p50 is None else round(p50, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -136,6 +136,9 @@ _SIGS = {
     "gg_pfb_capture": [_P, C.POINTER(StepArgs), _I, _I, _I],
     "gg_pfb_replay": [_P, _I],
     "gg_pfb_time_replays": [_P, _I, C.POINTER(C.c_float)],
+    "gg_pfb_time_replays_each": [_P, _I, C.POINTER(C.c_float)],
+    "gg_pfb_profile_steps": [_P, C.POINTER(StepArgs), _I, _I, _I, C.POINTER(C.c_float),
+                             C.POINTER(C.c_int32)],
     "gg_pfb_time_stage": [_P, C.POINTER(StepArgs), _I, _I, _I, C.POINTER(C.c_float)],
     "gg_pfb_read_dyn": [_P, C.POINTER(Dyn)],
     "gg_pfb_set_epoch": [_P, _I],

@@ -28,6 +28,7 @@ STAGE_CDF = 2
 STAGE_GATHER_SYSTEMATIC = 3
 STAGE_MEGOPOLIS = 4
 STAGE_GATHER_ANCESTORS = 5
+STAGE_DECIDE = 6
 
 RESAMPLERS = {"systematic": 0, "megopolis": 1, "megopolis_ref": 2}
 
@@ -118,6 +119,31 @@ class BatchedParticleFilter:
         ms = C.c_float()
         call("gg_pfb_time_replays", self._h, int(n), C.byref(ms))
         return float(ms.value)
+
+    def time_replays_each(self, n):
+        """Device ms of each of n graph replays (hipEvents on the handle's stream)."""
+        out = np.zeros(int(n), dtype=np.float32)
+        call("gg_pfb_time_replays_each", self._h, int(n),
+             out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def profile_steps(self, args, n_steps, resampler="systematic", n_iter=32):
+        """Eager steps with hipEvents around each launch group.  Returns
+        ({stage: ms} for steps that resampled, {stage: ms} for the others,
+        (n_resampled, n_other)); stages are predict_weight, decide_cdf, gather,
+        finalize (systematic) or predict_weight, decide, megopolis_gather,
+        finalize."""
+        ms = np.zeros(8, dtype=np.float32)
+        cnt = np.zeros(2, dtype=np.int32)
+        call("gg_pfb_profile_steps", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
+             int(n_steps), ms.ctypes.data_as(C.POINTER(C.c_float)),
+             cnt.ctypes.data_as(C.POINTER(C.c_int32)))
+        if resampler == "systematic":
+            names = ("predict_weight", "decide_cdf", "gather", "finalize")
+        else:
+            names = ("predict_weight", "decide", "megopolis_gather", "finalize")
+        return (dict(zip(names, map(float, ms[:4]))), dict(zip(names, map(float, ms[4:]))),
+                (int(cnt[0]), int(cnt[1])))
 
     def time_stage(self, args, stage, reps, n_iter=32):
         ms = C.c_float()

@@ -35,7 +35,6 @@ struct gg_pf {
   double* g = nullptr;
   double* Pb = nullptr;
   int32_t* emax = nullptr;
-  int32_t* pmax = nullptr;
   double* csum = nullptr;
   double* scratch_f = nullptr;
   gg_filter* filt = nullptr;
@@ -80,13 +79,12 @@ gg_arena arena_of(const gg_pf* h) {
   gg_arena a;
   a.state = h->state;
   a.lw = h->lw;
-  a.E = h->E;
   a.anc = h->anc;
   a.part = h->part;
   a.g = h->g;
   a.Pb = h->Pb;
   a.emax = h->emax;
-  a.pmax = h->pmax;
+  a.E = h->E;
   a.csum = h->csum;
   a.scratch_f = h->scratch_f;
   a.filt = h->filt;
@@ -239,9 +237,29 @@ int launch_reduce(gg_pf* h, const gg_step_args& s) {
   return 0;
 }
 
-int launch_cdf(gg_pf* h, const gg_step_args& s, int force) {
+int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
   const gg_arena a = arena_of(h);
-  hipLaunchKernelGGL(gg_k_cdf, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, s, force);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
+                                                dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+                                                mode));
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+// per-tile CDF and ends; with_decide: every workgroup takes the decision itself
+int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
+  const gg_arena a = arena_of(h);
+  const size_t smem = with_decide ? decide_smem(h) : 0;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_cdf<NI>), dim3(h->n_tiles), dim3(GG_BLOCK),
+                                                smem, h->stream, a, s, force, with_decide));
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_finalize(gg_pf* h, int force) {
+  const gg_arena a = arena_of(h);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_finalize<NI>), dim3(h->n_filters),
+                                                dim3(GG_BLOCK), 0, h->stream, a, force));
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -254,12 +272,7 @@ int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int 
                                                 dim3(GG_BLOCK), smem, h->stream, a, s, force,
                                                 anc_mode, src_buf));
   GG_HIP(hipGetLastError());
-  if (with_finalize) {
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_finalize<NI>), dim3(h->n_filters),
-                                                  dim3(GG_BLOCK), 0, h->stream, a, force));
-    GG_HIP(hipGetLastError());
-  }
-  return 0;
+  return with_finalize ? launch_finalize(h, force) : 0;
 }
 
 int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
@@ -282,21 +295,35 @@ int launch_mego(gg_pf* h, int n_iter, int force) {
   return 0;
 }
 
-// One full device-driven step (no host synchronisation).
-int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, bool pred) {
-  if (launch_weight(h, s, pred)) return -1;
-  if (resampler == 0) {
-    if (launch_cdf(h, s, 0)) return -1;
-    return launch_gather(h, s, 0, 0, 0);
-  }
+// Megopolis part of a step, after the decision kernel.
+int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
   if (resampler == 1) {
     if (launch_mego(h, n_iter, 0)) return -1;
-    return launch_gather(h, s, 0, 1, 1);
+    return launch_gather(h, s, 0, 1, 1, false);
   }
   int fb = 0;
-  if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0);
+  if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0, false);
   if (launch_mego_ref(h, n_iter, 0, 0, 0, &fb)) return -1;
-  return launch_gather(h, s, 0, 1, fb);
+  return launch_gather(h, s, 0, 1, fb, false);
+}
+
+// Resampling part of a device-driven step, after the weight launch, up to (not
+// including) finalize.  Systematic: the CDF kernel takes the decision in every
+// workgroup (and commits the predict's frame), so no decision kernel runs.
+int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
+  if (resampler == 0) {
+    if (launch_cdf(h, s, 0, 1)) return -1;
+    return launch_gather(h, s, 0, 0, 0, false);
+  }
+  if (launch_decide(h, s, 3)) return -1;
+  return enqueue_resample_after_decide(h, s, resampler, n_iter);
+}
+
+// One full device-driven step (no host synchronisation).
+int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
+  if (launch_weight(h, s, true, false)) return -1;
+  if (enqueue_resample(h, s, resampler, n_iter)) return -1;
+  return launch_finalize(h, 0);
 }
 
 gg_step_args default_args() {
@@ -396,13 +423,12 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   h->own_stream = true;
   if (!rc) rc = dev_alloc(&h->state, (size_t)2 * h->ncomp * h->P);
   if (!rc) rc = dev_alloc(&h->lw, (size_t)h->P);
-  if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
   if (!rc) rc = dev_alloc(&h->anc, (size_t)2 * h->P);
   if (!rc) rc = dev_alloc(&h->part, (size_t)tiles * GG_NPART);
   if (!rc) rc = dev_alloc(&h->g, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->Pb, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
-  if (!rc) rc = dev_alloc(&h->pmax, (size_t)tiles);
+  if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
   if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
   if (!rc) rc = dev_alloc(&h->scratch_f, (size_t)n_filters * 16);
   if (!rc) rc = dev_alloc(&h->filt, (size_t)n_filters);
@@ -447,8 +473,8 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
-  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,         h->part,         h->g,
-                  h->Pb,    h->emax,  h->pmax,      h->csum,        h->scratch_f,    h->filt,
+  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->g, h->Pb,
+                  h->emax,  h->csum,  h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->io_tmp};
   for (void* b : bufs)
@@ -578,7 +604,8 @@ int gg_pf_device_predict(gg_pf* h, double vx, double vy, double vz, double dt, d
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_predict<NI>), dim3(h->n_tiles),
                                                 dim3(GG_BLOCK), 0, h->stream, a, s));
   GG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gg_k_commit_predict, dim3(h->n_filters), dim3(64), 0, h->stream, a, s);
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_commit_predict<NI>), dim3(h->n_filters),
+                                                dim3(64), 0, h->stream, a, s));
   GG_HIP(hipGetLastError());
   h->est_valid = false;
   return 0;
@@ -687,7 +714,7 @@ int gg_pf_device_resample_systematic(gg_pf* h, uint64_t seed, int32_t u0_mode) {
     s.u0_mode = 2;
     s.u0_value = gg_u01d(r.v[0], r.v[1]) - 1.1102230246251565e-16;
   }
-  if (launch_cdf(h, s, 1)) return -1;
+  if (launch_cdf(h, s, 1, 0)) return -1;
   if (launch_gather(h, s, 1, 0, 0)) return -1;
   h->decide_valid = false;
   h->ess_valid = true;
@@ -793,7 +820,7 @@ int gg_pf_device_set_particle_states(gg_pf* h, const double* in) {
   d.anchor[1] = m[1];
   d.anchor[2] = m[2];
   d.cb_anchor = m[3];
-  for (int k = 0; k < GG_MAX_ISB; ++k) d.isb_anchor[k] = d.isb_anchor[k];
+  for (int k = 0; k < GG_MAX_D; ++k) d.mean[k] = 0.0;  // the offsets are centred on the new anchor
   const double* R = d.R;
   d.vel_u[0] = R[0] * in[4] + R[1] * in[5] + R[2] * in[6];
   d.vel_u[1] = R[3] * in[4] + R[4] * in[5] + R[5] * in[6];
@@ -980,7 +1007,7 @@ int gg_pfb_step(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n
   if (!h || !args) return gg_fail(-2, "null argument");
   if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
   if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
-  if (enqueue_step(h, *args, resampler, n_iter, true)) return -1;
+  if (enqueue_step(h, *args, resampler, n_iter)) return -1;
   h->decide_valid = h->ess_valid = h->est_valid = false;
   return 0;
 }
@@ -1002,7 +1029,7 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
   }
   GG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter, true);
+  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (rc) {
@@ -1033,6 +1060,92 @@ int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total) {
   return 0;
 }
 
+int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each) {
+  if (!h || !h->graph_exec) return gg_fail(-2, "no captured graph");
+  if (n_replays <= 0 || !ms_each) return gg_fail(-2, "bad n_replays");
+  std::vector<hipEvent_t> ev((size_t)n_replays + 1, nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = gg_fail(-1, "hipEventCreate failed");
+  for (int k = 0; rc == 0 && k < n_replays; ++k) {
+    if (hipEventRecord(ev[k], h->stream) != hipSuccess ||
+        hipGraphLaunch(h->graph_exec, h->stream) != hipSuccess)
+      rc = gg_fail(-1, "graph replay failed");
+  }
+  if (rc == 0 && hipEventRecord(ev[n_replays], h->stream) != hipSuccess)
+    rc = gg_fail(-1, "hipEventRecord failed");
+  if (rc == 0 && hipEventSynchronize(ev[n_replays]) != hipSuccess)
+    rc = gg_fail(-1, "hipEventSynchronize failed");
+  for (int k = 0; rc == 0 && k < n_replays; ++k)
+    if (hipEventElapsedTime(&ms_each[k], ev[k], ev[k + 1]) != hipSuccess)
+      rc = gg_fail(-1, "hipEventElapsedTime failed");
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return rc;
+}
+
+int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                         int32_t n_steps, float* stage_ms, int32_t* counts) {
+  if (!h || !args || !stage_ms || !counts) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = gg_fail(-1, "hipEventCreate failed");
+  double acc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  int cnt[2] = {0, 0};
+  std::vector<gg_dyn> dv(h->n_filters);
+  const gg_step_args s = *args;
+  for (int k = 0; rc == 0 && k < n_steps; ++k) {
+    // stages: weight | decision (+ CDF for systematic) | gather (+ Megopolis) | finalize
+    bool bad = hipEventRecord(ev[0], h->stream) != hipSuccess || launch_weight(h, s, true, false) ||
+               hipEventRecord(ev[1], h->stream) != hipSuccess;
+    if (!bad) {
+      if (resampler == 0)
+        bad = launch_cdf(h, s, 0, 1) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
+              launch_gather(h, s, 0, 0, 0, false);
+      else
+        bad = launch_decide(h, s, 3) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
+              enqueue_resample_after_decide(h, s, resampler, n_iter);
+    }
+    if (bad ||
+        hipEventRecord(ev[3], h->stream) != hipSuccess || launch_finalize(h, 0) ||
+        hipEventRecord(ev[4], h->stream) != hipSuccess) {
+      rc = gg_fail(-1, "profiled step launch failed");
+      break;
+    }
+    if (hipEventSynchronize(ev[4]) != hipSuccess ||
+        hipMemcpy(dv.data(), h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+      rc = gg_fail(-1, "profiled step failed");
+      break;
+    }
+    int any = 0;
+    for (const auto& d : dv) any |= d.did_resample;
+    const int r = any ? 0 : 1;
+    cnt[r] += 1;
+    for (int st = 0; st < 4; ++st) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[st], ev[st + 1]) != hipSuccess) {
+        rc = gg_fail(-1, "hipEventElapsedTime failed");
+        break;
+      }
+      acc[r][st] += ms;
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (int r = 0; r < 2; ++r)
+    for (int st = 0; st < 4; ++st)
+      stage_ms[r * 4 + st] = cnt[r] ? (float)(acc[r][st] / cnt[r]) : 0.0f;
+  counts[0] = cnt[0];
+  counts[1] = cnt[1];
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return rc;
+}
+
 int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
                       int32_t reps, float* ms_total) {
   if (!h || !args) return gg_fail(-2, "null argument");
@@ -1043,17 +1156,11 @@ int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t
     switch (stage) {
       case 0: rc = launch_weight(h, *args, true, false); break;
       case 1: rc = launch_weight(h, *args, false, false); break;
-      case 2: rc = launch_cdf(h, *args, 1); break;
+      case 2: rc = launch_cdf(h, *args, 1, 1); break;
       case 3: rc = launch_gather(h, *args, 1, 0, 0, false); break;
       case 4: rc = launch_mego(h, n_iter, 1); break;
       case 5: rc = launch_gather(h, *args, 1, 1, 1, false); break;
-      case 6: {
-        const gg_arena a = arena_of(h);
-        GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
-                                                      dim3(GG_BLOCK), decide_smem(h), h->stream, a,
-                                                      *args, 0));
-        rc = hipGetLastError() == hipSuccess ? 0 : gg_fail(-1, "decide launch failed");
-      } break;
+      case 6: rc = launch_decide(h, *args, 0); break;
       default: return gg_fail(-2, "unknown stage");
     }
     if (rc) return rc;
@@ -1074,6 +1181,7 @@ int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out) {
 
 int gg_pfb_set_epoch(gg_pf* h, int32_t epoch) {
   if (!h) return gg_fail(-2, "null handle");
+  if (epoch < 0) return gg_fail(-2, "epoch must be >= 0");
   GG_HIP(hipStreamSynchronize(h->stream));
   std::vector<gg_dyn> dv(h->n_filters);
   GG_HIP(hipMemcpy(dv.data(), h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost));

@@ -5,16 +5,23 @@
 // particles and every SoA load/store is a 16-byte vector access that the
 // wavefront coalesces into one contiguous 1 KiB transaction.
 //
-// The step for a batch of filters is three launches:
-//   gg_k_weight   predict (optional) + pseudorange likelihood + per-tile
-//                 partials; the last tile of each filter combines the partials
-//                 (max, sum, ESS, weighted mean, CDF tile offsets) and takes
-//                 the resampling decision on the device.
-//   gg_k_cdf      systematic resampling, part 1: per-tile f64 CDF in a fixed
-//                 order and the output "end" index of every particle.
-//   gg_k_gather   merge-path partition of (ends, outputs), ancestor indices,
-//                 state gather with re-centring, post-resample mean.
-// Megopolis replaces gg_k_cdf by gg_k_mego (or B launches of gg_k_mego_ref).
+// The systematic step for a batch of filters is four launches, each over all
+// filters:
+//   gg_k_weight    predict (optional) + pseudorange likelihood + per-tile
+//                  partials (max, sums of exp32(lw - m_b), ESS and mean sums).
+//   gg_k_cdf       every workgroup combines its filter's partials (max, total,
+//                  ESS, CDF tile offset) itself and takes the resampling
+//                  decision; the first tile of each filter writes the decision
+//                  and estimate; when resampling, per-tile binary64 CDF and
+//                  output ends.
+//   gg_k_gather    merge-path partition of (ends, outputs), ancestor indices,
+//                  state gather.
+//   gg_k_finalize  mean of the resampled particles, buffer swap.
+// Kernel boundaries are the only cross-workgroup synchronisation: the L2s of
+// the 8 XCDs are not coherent, and an agent-scope fence in every workgroup
+// (L2 writeback + invalidate) costs more than a launch inside a graph.
+// Megopolis: gg_k_weight, gg_k_decide (one workgroup per filter), gg_k_mego
+// (or B launches of gg_k_mego_ref), the ancestor-driven gather, finalize.
 #pragma once
 
 #include "gg_device.h"
@@ -22,13 +29,12 @@
 struct gg_arena {
   float* state;           // [2][ncomp][P]
   float* lw;              // [P]
-  int32_t* E;             // [P]
+  int32_t* E;             // [P] tile-local prefix max of the output ends
   int32_t* anc;           // [2][P]
   double* part;           // [n_tiles_total][GG_NPART]
   double* g;              // [n_tiles_total]
   double* Pb;             // [n_tiles_total]
-  int32_t* emax;          // [n_tiles_total]
-  int32_t* pmax;          // [n_tiles_total]
+  int32_t* emax;          // [n_tiles_total] tile maxima of E
   double* csum;           // [n_chunks_total][GG_MAX_D]
   double* scratch_f;      // [n_filters][16] misc per-filter results
   const gg_filter* filt;  // [n_filters]
@@ -239,10 +245,48 @@ __device__ __forceinline__ void gg_predict_one(const gg_arena& a, int cur, int64
   }
 }
 
+// Predict the thread's four particles.  With a uniform velocity carried by the
+// anchor (plan mode 0) and no velocity filter, the noise draws of the four
+// particles are independent straight-line code, so their Philox chains
+// interleave; every other mode goes particle by particle.  Both branches give
+// the same values for the same particle.
+template <int NISB>
+__device__ __forceinline__ void gg_predict4(const gg_arena& a, int cur, int64_t base, int n,
+                                            uint64_t seed, int i0, uint32_t step,
+                                            const gg_step_args& s, const gg_pred_plan& pl,
+                                            float (&p)[GG_ITEMS][4 + NISB]) {
+  if (pl.mode_out == 0 && !s.velocity_kf) {
+    float z[GG_ITEMS][4];
+    float zi[GG_ITEMS][4];
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      gg_normal4(seed, (uint32_t)(i0 + q), step, GG_PURPOSE_PREDICT, 0u, z[q]);
+      if (NISB > 0) gg_normal4(seed, (uint32_t)(i0 + q), step, GG_PURPOSE_PREDICT, 2u, zi[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const float n0 = z[q][0] * s.sigma_pos;
+      const float n1 = z[q][1] * s.sigma_pos;
+      const float n2 = z[q][2] * s.sigma_pos;
+      p[q][0] += 0.0f + n0;
+      p[q][1] += 0.0f + n1;
+      p[q][2] += 0.0f + n2;
+      p[q][3] += z[q][3] * s.sigma_cb;
+#pragma unroll
+      for (int k = 0; k < NISB; ++k) p[q][4 + k] += zi[q][k] * s.sigma_isb;
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    if (i0 + q < n) gg_predict_one<NISB>(a, cur, base + q, seed, (uint32_t)(i0 + q), step, s, pl, p[q]);
+}
+
 // Stage the filter's satellites for this epoch slot in LDS.
 __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_filter& F,
-                                             const gg_dyn* dy, const double* anchor, int slot,
-                                             gg_sat_view& sv, char* smem) {
+                                             const gg_dyn* dy, double ax, double ay, double az,
+                                             double cb_anchor, double isb0, double isb1,
+                                             double isb2, int slot, gg_sat_view& sv, char* smem) {
   const double* sd = a.satd + (int64_t)slot * 5 * a.sat_rows;
   const int32_t* si = a.sati + (int64_t)slot * (a.sat_rows + a.n_filters);
   const int ns = si[a.sat_rows + f];
@@ -256,15 +300,15 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   sv.isb = (int*)(sv.w + cap);
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
     const int64_t r = F.sat_off + k;
-    double dx[3] = {sd[r * 3 + 0] - anchor[0], sd[r * 3 + 1] - anchor[1], sd[r * 3 + 2] - anchor[2]};
+    double dx[3] = {sd[r * 3 + 0] - ax, sd[r * 3 + 1] - ay, sd[r * 3 + 2] - az};
     double d[3];
     gg_rot_local(dy->R, dx, d);
     const double rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const double inv_rho = 1.0 / rho;
     const int sys = si[r];
     const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
-    double c = sd[3 * a.sat_rows + r] - rho - dy->cb_anchor;
-    if (ib >= 0) c -= dy->isb_anchor[ib];
+    double c = sd[3 * a.sat_rows + r] - rho - cb_anchor;
+    if (ib >= 0) c -= (ib == 0) ? isb0 : ((ib == 1) ? isb1 : isb2);
     sv.ux[k] = (float)(d[0] * inv_rho);
     sv.uy[k] = (float)(d[1] * inv_rho);
     sv.uz[k] = (float)(d[2] * inv_rho);
@@ -359,19 +403,99 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
   return lw;
 }
 
+// Likelihood of the thread's four particles with the satellites in the outer
+// loop, so that the four residual chains interleave.  Per particle the
+// arithmetic and its order are those of gg_loglik; with the NLOS gate on the
+// kept set differs per particle and gg_loglik is used as is.
+template <int NISB>
+__device__ __forceinline__ void gg_loglik4(const gg_sat_view& sv, int ns,
+                                           const float (&p)[GG_ITEMS][4 + NISB],
+                                           const gg_step_args& s, float (&out)[GG_ITEMS]) {
+  if (s.gate_thr > 0.0f) {
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = gg_loglik<NISB>(sv, ns, p[q], s);
+    return;
+  }
+  float pp[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
+  const float sig = s.sigma_pr;
+  if (s.lik == GG_LIK_STUDENT) {
+    const float half_nup1 = 0.5f * (s.nu + 1.0f);
+    const float inv_nu_s2 = 1.0f / (s.nu * sig * sig);
+    float acc[GG_ITEMS], prod[GG_ITEMS];
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      acc[q] = 0.0f;
+      prod[q] = 1.0f;
+    }
+    int cnt = 0;
+    for (int k = 0; k < ns; ++k) {
+      const float wk = sv.w[k];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
+        prod[q] *= 1.0f + wk * e * e * inv_nu_s2;
+      }
+      if (++cnt == 3) {
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          acc[q] += gg_lnf(prod[q]);
+          prod[q] = 1.0f;
+        }
+        cnt = 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      acc[q] += gg_lnf(prod[q]);
+      out[q] = -half_nup1 * acc[q];
+    }
+  } else if (s.lik == GG_LIK_HUBER) {
+    const float hk = s.huber_k;
+    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < ns; ++k) {
+      const float wk = sv.w[k];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
+        const float z = fabsf(e) / sig;
+        const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
+        lw[q] -= wk * rho;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
+  } else {
+    const float inv_s2 = 1.0f / (sig * sig);
+    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < ns; ++k) {
+      const float wk = sv.w[k];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
+        lw[q] += -0.5f * wk * e * e * inv_s2;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tile partials: m_b = max lw, S_b = sum e, S2_b = sum e^2, sum_d = sum e p_d
-// with e = exp64(lw - m_b), each sum in the fixed order: per-thread left to
+// with e = exp32(lw - m_b) (binary32, like lw itself) accumulated in binary64,
+// each sum in the fixed order: per-thread left to
 // right over its 4 particles, then gg_block_scan256's total.
 // ---------------------------------------------------------------------------
-template <int D>
-__device__ __forceinline__ void gg_block_totals(double (&v)[D + 2], double* lds, double* out) {
+template <int NV>
+__device__ __forceinline__ void gg_block_totals_n(const double (&v)[NV], double* lds, double* out) {
 #pragma clang fp contract(off)
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
 #pragma unroll
-  for (int q = 0; q < D + 2; ++q) {
+  for (int q = 0; q < NV; ++q) {
     double x = v[q];
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -382,9 +506,15 @@ __device__ __forceinline__ void gg_block_totals(double (&v)[D + 2], double* lds,
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < D + 2; ++q)
+  for (int q = 0; q < NV; ++q)
     out[q] = ((lds[q * 4 + 0] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
   __syncthreads();
+}
+
+// Each value is reduced on its own (same operations whatever the count).
+template <int D>
+__device__ __forceinline__ void gg_block_totals(const double (&v)[D + 2], double* lds, double* out) {
+  gg_block_totals_n<D + 2>(v, lds, out);
 }
 
 template <int D>
@@ -393,15 +523,15 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
                                                  const float (&p)[GG_ITEMS][D], double* lds,
                                                  double* part_out) {
 #pragma clang fp contract(off)
-  double m = -__builtin_huge_val();
+  float m = -__builtin_huge_valf();
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
-    if (valid[q]) m = fmax(m, (double)lw[q]);
-  const double mb = gg_block_max256(m, lds);
+    if (valid[q]) m = fmaxf(m, lw[q]);
+  const float mb = (float)gg_block_max256((double)m, lds);
   double e[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
-    e[q] = (valid[q] && mb > -__builtin_huge_val()) ? gg_exp64((double)lw[q] - mb) : 0.0;
+    e[q] = (valid[q] && mb > -__builtin_huge_valf()) ? (double)gg_exp32(lw[q] - mb) : 0.0;
   double v[D + 2];
   v[0] = e[0];
   v[1] = e[0] * e[0];
@@ -417,7 +547,7 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
   double tot[D + 2];
   gg_block_totals<D>(v, lds, tot);
   if (threadIdx.x == 0) {
-    part_out[0] = mb;
+    part_out[0] = (double)mb;
 #pragma unroll
     for (int q = 0; q < D + 2; ++q) part_out[1 + q] = tot[q];
   }
@@ -435,36 +565,124 @@ __device__ __forceinline__ double gg_u0(const gg_step_args& s, uint64_t seed, in
   return (double)(seed % 1000000ull) / (double)den;
 }
 
-// Combine a filter's tile partials (one workgroup per filter).
-//   pass 1 (strided, coalesced): M = max m_b;
+// #{ j in [0, n) : (u0 + j) / n <= c } with the targets rounded exactly as
+// the reference computes them (pf_device.cu:1197); NaN counts nothing.  The
+// estimate floor(c n - u0) is corrected by walking to the exact boundary.
+__device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
+#pragma clang fp contract(off)
+  const double dn = (double)n;
+  if (c != c) return 0;
+  const double est = floor(c * dn - u0);
+  int j = (est < -1.0) ? -1 : (est > (double)(n - 1) ? n - 1 : (int)est);
+  while (j + 1 < n && (u0 + (double)(j + 1)) / dn <= c) ++j;
+  while (j >= 0 && (u0 + (double)j) / dn > c) --j;
+  return j + 1;
+}
+
+// Copy a filter record between global memory and LDS, one 32-bit word per
+// thread (the record is a multiple of 4 bytes).
+__device__ __forceinline__ void gg_dyn_copy(gg_dyn* dst, const gg_dyn* src) {
+  constexpr int NW = (int)(sizeof(gg_dyn) / 4);
+  static_assert(sizeof(gg_dyn) % 4 == 0, "gg_dyn must be a whole number of words");
+  static_assert(NW <= GG_BLOCK, "gg_dyn larger than one word per thread");
+  if (threadIdx.x < NW)
+    reinterpret_cast<uint32_t*>(dst)[threadIdx.x] =
+        reinterpret_cast<const uint32_t*>(src)[threadIdx.x];
+}
+
+// Frame of one step.  Predict moves the anchor by the uniform velocity and
+// re-centres the offsets on the last posterior mean (shift = mean as binary32,
+// added to the binary64 anchors), so the offsets stay at metre scale.  Every
+// kernel of the step derives the same frame from the same record.
+struct gg_frame {
+  double anchor[3];
+  double cb_anchor;
+  double isb_anchor[GG_MAX_ISB];
+  float shift[GG_MAX_D];
+  gg_pred_plan pl;
+};
+
+template <int D>
+__device__ __forceinline__ void gg_step_frame(const gg_dyn* dy, const gg_step_args& s,
+                                              const double* vge, gg_frame& fr) {
+#pragma clang fp contract(off)
+  fr.pl = gg_make_plan(dy, s, vge);
+  double vr[3] = {fr.pl.v_ref[0] * s.dt, fr.pl.v_ref[1] * s.dt, fr.pl.v_ref[2] * s.dt};
+  double ve[3];
+  gg_rot_ecef(dy->R, vr, ve);
+#pragma unroll
+  for (int d = 0; d < GG_MAX_D; ++d) fr.shift[d] = (s.recenter && d < D) ? (float)dy->mean[d] : 0.0f;
+  double sp[3] = {(double)fr.shift[0], (double)fr.shift[1], (double)fr.shift[2]};
+  double se[3];
+  gg_rot_ecef(dy->R, sp, se);
+  for (int k = 0; k < 3; ++k) fr.anchor[k] = (dy->anchor[k] + ve[k]) + se[k];
+  fr.cb_anchor = dy->cb_anchor + (double)fr.shift[3];
+  for (int k = 0; k < GG_MAX_ISB; ++k) fr.isb_anchor[k] = dy->isb_anchor[k] + (double)fr.shift[4 + k];
+}
+
+// Commit a step's frame to the record (one thread).  The old posterior mean is
+// carried into the new frame.
+template <int D>
+__device__ __forceinline__ void gg_commit_frame(const gg_step_args& s, gg_dyn* dy,
+                                                const gg_frame& fr) {
+  for (int k = 0; k < 3; ++k) {
+    dy->anchor[k] = fr.anchor[k];
+    dy->vel_u[k] = fr.pl.v_new_u[k];
+  }
+  dy->cb_anchor = fr.cb_anchor;
+  for (int k = 0; k < GG_MAX_ISB; ++k) dy->isb_anchor[k] = fr.isb_anchor[k];
+  for (int d = 0; d < GG_MAX_D; ++d) dy->shift[d] = fr.shift[d];
+  for (int d = 0; d < D; ++d) dy->mean[d] = dy->mean[d] - (double)fr.shift[d];
+  dy->vel_mode = fr.pl.mode_out;
+  if (s.velocity_kf && fr.pl.mode_out < 2)
+    for (int k = 0; k < 6; ++k) dy->vcov_u[k] = fr.pl.vcov_new[k];
+  dy->est_valid = 0;
+}
+
+// Result of combining a filter's tile partials.
+struct gg_comb {
+  double M, W, W2, g_own, P_own;
+  double sum[GG_MAX_D];
+};
+
+// Combine a filter's tile partials (one workgroup per filter, or redundantly
+// in every workgroup that needs the result; the operations and their order
+// are the same wherever it runs):
+//   pass 1 (strided): M = max m_b;
 //   pass 2 (strided): g_b = exp64(m_b - M), T_b = S_b g_b staged in LDS (sT),
-//           ESS and weighted-mean sums accumulated in the strided order;
+//           S2 and (FULL) weighted-mean sums accumulated in the strided order;
 //   pass 3: W and the CDF tile offsets P_b in the fixed chunked order (thread
 //           t owns tiles [t*c, t*c+c), sums them left to right from 0.0, the
-//           per-thread values go through gg_block_scan256), read from LDS.
-// sT needs 8 * n_tiles bytes of LDS.
-template <int D>
-__device__ void gg_decide(const gg_arena& a, const gg_step_args& s, const gg_filter& F,
-                          gg_dyn* dy, const double* anchor, double* lds, double* sT) {
+//           per-thread values go through gg_block_scan256, then
+//           P_b = excl_t + (T_{t*c} + ... + T_{b-1})).
+// ALL writes g_b and P_b of every tile; otherwise only tile `own`'s g and P
+// are returned.  sT needs 8 * n_tiles bytes of LDS, sx two doubles.
+template <int D, bool FULL, bool ALL>
+__device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, double* lds,
+                           double* sT, double* sx, gg_comb& r) {
 #pragma clang fp contract(off)
+  constexpr int NV = FULL ? D + 2 : 1;
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
   const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
   double m = -__builtin_huge_val();
   for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
   const double M = gg_block_max256(m, lds);
-  double vv[D + 2];
+  double vv[NV];
 #pragma unroll
-  for (int d = 0; d < D + 2; ++d) vv[d] = 0.0;
+  for (int d = 0; d < NV; ++d) vv[d] = 0.0;
   for (int b = t; b < nt; b += GG_BLOCK) {
     const double* pb = part + (int64_t)b * GG_NPART;
     const double mb = pb[0];
     const double gb = (mb > -__builtin_huge_val()) ? gg_exp64(mb - M) : 0.0;
-    a.g[F.tile_off + b] = gb;
+    if (ALL) a.g[F.tile_off + b] = gb;
+    if (!ALL && b == own) sx[1] = gb;
     sT[b] = pb[1] * gb;
     vv[0] = vv[0] + pb[2] * (gb * gb);
+    if (FULL) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) vv[2 + d] = vv[2 + d] + pb[3 + d] * gb;
+      for (int d = 0; d < D; ++d) vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + pb[3 + d] * gb;
+    }
   }
   __syncthreads();
   const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
@@ -474,34 +692,56 @@ __device__ void gg_decide(const gg_arena& a, const gg_step_args& s, const gg_fil
   for (int b = b0; b < b1; ++b) T = T + sT[b];
   double excl, W;
   gg_block_scan256(T, lds, excl, W);
-  double run = 0.0;
-  for (int b = b0; b < b1; ++b) {
-    a.Pb[F.tile_off + b] = excl + run;
-    run = run + sT[b];
+  if (ALL) {
+    double run = 0.0;
+    for (int b = b0; b < b1; ++b) {
+      a.Pb[F.tile_off + b] = excl + run;
+      run = run + sT[b];
+    }
+  } else if (own >= b0 && own < b1) {
+    double run = 0.0;
+    for (int b = b0; b < own; ++b) run = run + sT[b];
+    sx[0] = excl + run;
   }
-  double tot[D + 2];
-  gg_block_totals<D>(vv, lds, tot);
-  if (t == 0) {
-    const double W2 = tot[0];
-    dy->M = M;
-    dy->W = W;
-    dy->W2 = W2;
-    const double ess = (W2 > 0.0) ? (W * W) / W2 : 0.0;
-    dy->ess = ess;
-    for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / W;
-    double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
-    double me[3];
-    gg_rot_ecef(dy->R, mp, me);
-    dy->est[0] = anchor[0] + me[0];
-    dy->est[1] = anchor[1] + me[1];
-    dy->est[2] = anchor[2] + me[2];
-    dy->est[3] = dy->cb_anchor + dy->mean[3];
-    for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
-    for (int d = 0; d < D; ++d) dy->shift[d] = s.recenter ? (float)dy->mean[d] : 0.0f;
-    if (s.auto_flag) dy->flag = (ess < s.ess_frac * (double)F.n) ? 1 : 0;
-    dy->u0 = gg_u0(s, F.seed, F.n, dy->resample_count);
-    dy->est_valid = 1;
+  double tot[NV];
+  gg_block_totals_n<NV>(vv, lds, tot);  // synchronises: sx is visible below
+  r.M = M;
+  r.W = W;
+  r.W2 = tot[0];
+  if (!ALL) {
+    r.P_own = sx[0];
+    r.g_own = sx[1];
   }
+  if (FULL)
+    for (int d = 0; d < D; ++d) r.sum[d] = tot[(FULL ? 2 : 0) + d];
+}
+
+// Decision part of a combine, written to the (LDS) record by one thread.
+template <int D>
+__device__ __forceinline__ void gg_store_decision(const gg_step_args& s, const gg_filter& F,
+                                                  const gg_comb& r, double u0, gg_dyn* sd) {
+  sd->M = r.M;
+  sd->W = r.W;
+  sd->W2 = r.W2;
+  const double ess = (r.W2 > 0.0) ? (r.W * r.W) / r.W2 : 0.0;
+  sd->ess = ess;
+  for (int d = 0; d < D; ++d) sd->mean[d] = r.sum[d] / r.W;
+  if (s.auto_flag) sd->flag = (ess < s.ess_frac * (double)F.n) ? 1 : 0;
+  sd->u0 = u0;
+}
+
+// Absolute estimate from the local mean (thread 0, after the anchor is final).
+template <int D>
+__device__ __forceinline__ void gg_make_estimate(gg_dyn* sd) {
+  double mp[3] = {sd->mean[0], sd->mean[1], sd->mean[2]};
+  double me[3];
+  gg_rot_ecef(sd->R, mp, me);
+  sd->est[0] = sd->anchor[0] + me[0];
+  sd->est[1] = sd->anchor[1] + me[1];
+  sd->est[2] = sd->anchor[2] + me[2];
+  sd->est[3] = sd->cb_anchor + sd->mean[3];
+  for (int d = 4; d < D; ++d) sd->est[d] = sd->isb_anchor[d - 4] + sd->mean[d];
+  sd->est_valid = 1;
 }
 
 // ---------------------------------------------------------------------------

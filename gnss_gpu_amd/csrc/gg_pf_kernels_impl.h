@@ -47,11 +47,26 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_init(gg_arena a, float spread_p
 }
 
 // ---------------------------------------------------------------------------
-// predict only (drop-in path): in place on the current buffer.  The anchor and
-// velocity bookkeeping is committed afterwards by gg_k_commit_predict (kernel
-// boundaries are the only cross-workgroup synchronisation in this library: a
-// device-scope fence costs an L2 write-back per workgroup on gfx950).
+// predict only (drop-in path): in place on the current buffer, offsets
+// re-centred on the step frame.  The frame is committed afterwards by
+// gg_k_commit_predict (kernel boundaries are the only cross-workgroup
+// synchronisation in this library: a device-scope fence costs an L2
+// write-back per workgroup on gfx950).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ const double* gg_guide_ptr(const gg_arena& a, const gg_step_args& s,
+                                                      int f, int epoch) {
+  const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
+  return a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+}
+
+template <int D>
+__device__ __forceinline__ void gg_apply_shift(const gg_frame& fr, float (&p)[GG_ITEMS][D]) {
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+#pragma unroll
+    for (int d = 0; d < D; ++d) p[q][d] -= fr.shift[d];
+}
+
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_predict(gg_arena a, gg_step_args s) {
   constexpr int D = 4 + NISB;
@@ -61,48 +76,33 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_predict(gg_arena a, gg_step_arg
   const gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int cur = dy->cur;
-  const int slot = s.slot_from_device ? (dy->epoch % s.n_slots) : 0;
   const uint32_t step = s.step_from_device ? (uint32_t)(dy->epoch + 1) : (uint32_t)s.step;
-  const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+  const double* vg = gg_guide_ptr(a, s, f, dy->epoch);
   double vge[3] = {vg[0], vg[1], vg[2]};
-  const gg_pred_plan pl = gg_make_plan(dy, s, vge);
+  gg_frame fr;
+  gg_step_frame<D>(dy, s, vge, fr);
   const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
   const int64_t base = F.p_off + i0;
   float p[GG_ITEMS][D];
   gg_load4<D>(a, cur, base, p);
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q)
-    if (i0 + q < F.n) gg_predict_one<NISB>(a, cur, base + q, F.seed, i0 + q, step, s, pl, p[q]);
+  gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, fr.pl, p);
+  gg_apply_shift<D>(fr, p);
   gg_store4<D>(a, cur, base, p);
 }
 
-// Anchor/velocity commit of a predict (one thread per filter).  Recomputes the
-// plan from the same inputs the predict blocks used.
-__device__ __forceinline__ void gg_commit_predict_one(const gg_arena& a, const gg_step_args& s,
-                                                      int f, gg_dyn* dy, double* anchor_out) {
-  const int slot = s.slot_from_device ? (dy->epoch % s.n_slots) : 0;
-  const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
-  double vge[3] = {vg[0], vg[1], vg[2]};
-  const gg_pred_plan pl = gg_make_plan(dy, s, vge);
-  double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
-  double ve[3];
-  gg_rot_ecef(dy->R, vr, ve);
-  for (int k = 0; k < 3; ++k) {
-    anchor_out[k] = dy->anchor[k] + ve[k];
-    dy->anchor[k] = anchor_out[k];
-    dy->vel_u[k] = pl.v_new_u[k];
-  }
-  dy->vel_mode = pl.mode_out;
-  if (s.velocity_kf && pl.mode_out < 2)
-    for (int k = 0; k < 6; ++k) dy->vcov_u[k] = pl.vcov_new[k];
-  dy->est_valid = 0;
-}
-
+// Commit of a predict's frame (one thread per filter), from the same record
+// the predict blocks read.
+template <int NISB>
 __global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
+  constexpr int D = 4 + NISB;
   const int f = blockIdx.x;
   if (threadIdx.x != 0 || f >= a.n_filters) return;
-  double anc[3];
-  gg_commit_predict_one(a, s, f, a.dyn + f, anc);
+  gg_dyn* dy = a.dyn + f;
+  const double* vg = gg_guide_ptr(a, s, f, dy->epoch);
+  double vge[3] = {vg[0], vg[1], vg[2]};
+  gg_frame fr;
+  gg_step_frame<D>(dy, s, vge, fr);
+  gg_commit_frame<D>(s, dy, fr);
 }
 
 // ---------------------------------------------------------------------------
@@ -122,36 +122,35 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args
   const int epoch = dy->epoch;
   const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
   const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
-  double anchor[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
-  gg_pred_plan pl;
+  gg_frame fr;
   if (PRED) {
-    const double* vg = a.vel_guide + ((int64_t)slot * a.n_filters + f) * 3;
+    const double* vg = gg_guide_ptr(a, s, f, epoch);
     double vge[3] = {vg[0], vg[1], vg[2]};
-    pl = gg_make_plan(dy, s, vge);
-    double vr[3] = {pl.v_ref[0] * s.dt, pl.v_ref[1] * s.dt, pl.v_ref[2] * s.dt};
-    double ve[3];
-    gg_rot_ecef(dy->R, vr, ve);
-    for (int k = 0; k < 3; ++k) anchor[k] += ve[k];
+    gg_step_frame<D>(dy, s, vge, fr);
+  } else {
+    for (int k = 0; k < 3; ++k) fr.anchor[k] = dy->anchor[k];
+    fr.cb_anchor = dy->cb_anchor;
+    for (int k = 0; k < GG_MAX_ISB; ++k) fr.isb_anchor[k] = dy->isb_anchor[k];
   }
-  gg_sat_view sv;
-  const int ns = gg_stage_sats(a, f, F, dy, anchor, slot, sv, smem);
-
   const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
   const int64_t base = F.p_off + i0;
   float p[GG_ITEMS][D];
   gg_load4<D>(a, cur, base, p);
+  gg_sat_view sv;
+  const int ns = gg_stage_sats(a, f, F, dy, fr.anchor[0], fr.anchor[1], fr.anchor[2], fr.cb_anchor,
+                               fr.isb_anchor[0], fr.isb_anchor[1], fr.isb_anchor[2], slot, sv, smem);
   bool valid[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
   if (PRED) {
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q)
-      if (valid[q]) gg_predict_one<NISB>(a, cur, base + q, F.seed, i0 + q, step, s, pl, p[q]);
+    gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, fr.pl, p);
+    gg_apply_shift<D>(fr, p);
     gg_store4<D>(a, cur, base, p);
   }
   float lw[GG_ITEMS];
+  gg_loglik4<NISB>(sv, ns, p, s, lw);
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? gg_loglik<NISB>(sv, ns, p[q], s) : 0.0f;
+  for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? lw[q] : 0.0f;
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
   gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
 }
@@ -197,83 +196,147 @@ template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args s, int mode) {
   constexpr int D = 4 + NISB;
   extern __shared__ __align__(16) char smem[];
-  double* sT = reinterpret_cast<double*>(smem);
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ double sanchor[3];
+  __shared__ double sx[2];
+  __shared__ gg_dyn sd;
+  __shared__ double svg[3];
   const int f = blockIdx.x;
   const gg_filter F = a.filt[f];
+  double* sT = reinterpret_cast<double*>(smem);
   gg_dyn* dy = a.dyn + f;
+  gg_dyn_copy(&sd, dy);
+  __syncthreads();
+  // the guide velocity load overlaps the combine
+  double vg = 0.0;
+  if ((mode & 1) && threadIdx.x < 3) vg = gg_guide_ptr(a, s, f, sd.epoch)[threadIdx.x];
+  gg_comb r;
+  gg_combine<D, true, true>(a, F, 0, lds, sT, sx, r);
+  if ((mode & 1) && threadIdx.x < 3) svg[threadIdx.x] = vg;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double anc[3] = {dy->anchor[0], dy->anchor[1], dy->anchor[2]};
-    if (mode & 1) gg_commit_predict_one(a, s, f, dy, anc);
-    for (int k = 0; k < 3; ++k) sanchor[k] = anc[k];
+    if (mode & 1) {
+      gg_frame fr;
+      gg_step_frame<D>(&sd, s, svg, fr);
+      gg_commit_frame<D>(s, &sd, fr);
+    }
+    gg_store_decision<D>(s, F, r, gg_u0(s, F.seed, F.n, sd.resample_count), &sd);
+    gg_make_estimate<D>(&sd);
+    if (mode & 2) {
+      sd.lw_zero = 0;
+      sd.did_resample = 0;
+      if (s.slot_from_device || s.step_from_device) sd.epoch = sd.epoch + 1;
+    }
   }
   __syncthreads();
-  double anchor[3] = {sanchor[0], sanchor[1], sanchor[2]};
-  gg_decide<D>(a, s, F, dy, anchor, lds, sT);
-  if (threadIdx.x == 0 && (mode & 2)) {
-    dy->lw_zero = 0;
-    dy->did_resample = 0;
-    if (s.slot_from_device || s.step_from_device) dy->epoch = dy->epoch + 1;
-  }
+  gg_dyn_copy(dy, &sd);
 }
 
 // ---------------------------------------------------------------------------
-// Systematic resampling, part 1.
+// Systematic resampling, part 1: per-tile CDF and output ends.
 // cdf_i = (P_b + g_b * L_i) / W with L the fixed-order inclusive scan of
-// exp64(lw - m_b) inside the tile.  The particle's output end is
+// e_i = exp32(lw_i - m_b) inside the tile.  The particle's output end is
 // k(cdf_i) = #{ j : (u0 + j) / n <= cdf_i } (n for the last particle, which
-// reproduces the hi = n-1 clamp of a binary search).  Ancestor of output j is
+// reproduces the hi = n-1 clamp of the reference's binary search,
+// pf_device.cu:1200-1215).  The ancestor of output j is
 // min{ i : cdf_i >= t_j } = #{ i : E_i <= j } with E the prefix max of k, so
 // no monotonicity of the floating-point CDF is assumed.  Writes the
 // tile-local prefix max and the tile maximum; the gather completes the prefix
 // max across tiles.
+//
+// with_decide (the device-driven step): every workgroup combines its filter's
+// tile partials itself (gg_combine, identical in every workgroup) instead of
+// waiting for a one-workgroup decision kernel, and the filter's first tile
+// commits the step frame and writes the decision and estimate to the record.
+// Dynamic LDS: 8 * max(n_tiles) bytes (with_decide).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s, int force,
+                                                     int with_decide) {
 #pragma clang fp contract(off)
-  const double dn = (double)n;
-  if (c != c) return 0;
-  // estimate of the last j with t_j <= c, then walk to the exact boundary
-  const double est = floor(c * dn - u0);
-  int j = (est < -1.0) ? -1 : (est > (double)(n - 1) ? n - 1 : (int)est);
-  while (j + 1 < n && (u0 + (double)(j + 1)) / dn <= c) ++j;
-  while (j >= 0 && (u0 + (double)j) / dn > c) --j;
-  return j + 1;
-}
-
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s, int force) {
-#pragma clang fp contract(off)
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
   __shared__ double lds[8 + GG_BLOCK];
+  __shared__ double sx[2];
   __shared__ int ilds[8];
   __shared__ int tmax[GG_BLOCK];
+  __shared__ gg_dyn sd;
+  __shared__ double svg[3];
+  __shared__ int sflag;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
-  const gg_dyn* dy = a.dyn + f;
-  if (!force && !dy->flag) return;
+  gg_dyn* dy = a.dyn + f;
   const int lt = tile - F.tile_off;
   const int n = F.n;
-  const double mb = a.part[(int64_t)tile * GG_NPART];
-  const double gb = a.g[tile];
-  const double Pb = a.Pb[tile];
-  const double W = dy->W;
-  const double u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
-  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int t = threadIdx.x;
+  const int i0 = lt * GG_TILE + t * GG_ITEMS;
   const int64_t base = F.p_off + i0;
-  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (!dy->lw_zero) {
-    const float4 v = *reinterpret_cast<const float4*>(a.lw + base);
-    lw[0] = v.x;
-    lw[1] = v.y;
-    lw[2] = v.z;
-    lw[3] = v.w;
+  // this tile's log-weights; the load overlaps the combine
+  const float4 lwv = *reinterpret_cast<const float4*>(a.lw + base);
+  double mb, gb, Pb, W, u0;
+  bool lwz;
+  if (with_decide) {
+    double* sT = reinterpret_cast<double*>(smem);
+    const bool first = (lt == 0);
+    if (first) gg_dyn_copy(&sd, dy);
+    double vg = 0.0;
+    if (first && t < 3) vg = gg_guide_ptr(a, s, f, dy->epoch)[t];
+    gg_comb r;
+    if (first)
+      gg_combine<D, true, false>(a, F, lt, lds, sT, sx, r);
+    else
+      gg_combine<D, false, false>(a, F, lt, lds, sT, sx, r);
+    mb = a.part[(int64_t)tile * GG_NPART];
+    gb = r.g_own;
+    Pb = r.P_own;
+    W = r.W;
+    u0 = gg_u0(s, F.seed, n, dy->resample_count);
+    lwz = false;  // the step's weight launch wrote fresh log-weights
+    if (t == 0) {
+      const double ess = (r.W2 > 0.0) ? (r.W * r.W) / r.W2 : 0.0;
+      sflag = s.auto_flag ? ((ess < s.ess_frac * (double)n) ? 1 : 0) : dy->flag;
+    }
+    if (first && t < 3) svg[t] = vg;
+    __syncthreads();
+    if (first) {
+      if (t == 0) {
+        gg_frame fr;
+        gg_step_frame<D>(&sd, s, svg, fr);
+        gg_commit_frame<D>(s, &sd, fr);
+        gg_store_decision<D>(s, F, r, u0, &sd);
+        gg_make_estimate<D>(&sd);
+        sd.lw_zero = 0;
+        sd.did_resample = 0;
+        if (s.slot_from_device || s.step_from_device) sd.epoch = sd.epoch + 1;
+      }
+      __syncthreads();
+      gg_dyn_copy(dy, &sd);
+    }
+    if (!force && !sflag) return;
+  } else {
+    if (!force && !dy->flag) return;
+    mb = a.part[(int64_t)tile * GG_NPART];
+    gb = a.g[tile];
+    Pb = a.Pb[tile];
+    W = dy->W;
+    u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
+    lwz = dy->lw_zero != 0;
   }
+  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!lwz) {
+    lw[0] = lwv.x;
+    lw[1] = lwv.y;
+    lw[2] = lwv.z;
+    lw[3] = lwv.w;
+  }
+  const float mbf = (float)mb;
   double l[GG_ITEMS];
   double run = 0.0;
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) {
     const bool valid = (i0 + q) < n;
-    const double e = (valid && mb > -__builtin_huge_val()) ? gg_exp64((double)lw[q] - mb) : 0.0;
+    const double e =
+        (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[q] - mbf) : 0.0;
     run = (q == 0) ? e : run + e;
     l[q] = run;
   }
@@ -289,39 +352,65 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
       k = n;
     } else {
       const double L = excl + l[q];
-      const double cdf = (Pb + gb * L) / W;
-      k = gg_kcount(cdf, u0, n);
+      k = gg_kcount((Pb + gb * L) / W, u0, n);
     }
     rmax = max(rmax, k);
     E[q] = rmax;
   }
   const int pre_incl = gg_block_prefix_max256_i(rmax, ilds);
-  tmax[threadIdx.x] = pre_incl;
+  tmax[t] = pre_incl;
   __syncthreads();
-  const int pre = (threadIdx.x == 0) ? 0 : tmax[threadIdx.x - 1];
+  const int pre = (t == 0) ? 0 : tmax[t - 1];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) E[q] = max(E[q], pre);
   *reinterpret_cast<int4*>(a.E + base) = make_int4(E[0], E[1], E[2], E[3]);
-  if (threadIdx.x == GG_BLOCK - 1) a.emax[tile] = E[GG_ITEMS - 1];
+  if (t == GG_BLOCK - 1) a.emax[tile] = E[GG_ITEMS - 1];
 }
 
 // ---------------------------------------------------------------------------
-// Systematic resampling, part 2 / common gather.
+// Resampling gather (systematic merge path, or ancestors from Megopolis).
 //
-// Work item = GG_QCHUNK consecutive steps of the merge of the particles'
-// positions pos_i = i + E_i (strictly increasing) with the outputs j = 0..n-1;
-// every item handles the same number of (input + output) steps whatever the
-// weights.  With M_b the inclusive prefix max of the tile maxima, the global
-// end of particle i in tile b is max(M_{b-1}, E_i) and the last position of
-// tile b is TE_b = last_i + M_b.  A work item:
-//   1. builds M_b for its filter in LDS,
-//   2. finds the tile A holding its start diagonal d0 by binary search of TE,
-//   3. loads the ends of tiles A and A+1 (its whole input range) into LDS and
-//      counts positions below d0 and d1 -> input range [i0, i1),
-//   4. ancestor of output j in [d0 - i0, d1 - i1) = i0 + #{ i in [i0, i1) : E_i <= j }.
+// Systematic: work item = GG_QCHUNK consecutive steps of the merge of the
+// particles' positions pos_i = i + E_i (strictly increasing) with the outputs
+// j = 0..n-1; every item handles the same number of (input + output) steps
+// whatever the weights, so its inputs lie in at most WT consecutive tiles.
+// With PM_b the inclusive prefix max of the tile maxima, the global end of
+// particle i in tile b is max(PM_{b-1}, E_i) and the last position of tile b
+// is TE_b = last_b + PM_b.  A work item:
+//   1. builds PM for its filter in LDS (thread t owns tiles [t*c, t*c+c)),
+//   2. finds the tile A holding its start diagonal d0 (the tile with
+//      TE_{A-1} < d0 <= TE_A; each thread tests its own tiles),
+//   3. loads the ends of tiles A .. A+WT-1 into LDS and counts positions below
+//      d0 and d1 -> input range [i0, i1),
+//   4. ancestor of output j in [d0 - i0, d1 - i1) = i0 + #{ i in [i0, i1) : E_i <= j },
+//      each thread handling a run of consecutive outputs with an exponential
+//      search from the previous ancestor,
+//   5. copies the state of the ancestors (all loads of a thread in flight
+//      before its stores).
 // anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity.
-// Dynamic LDS: 4 * max(n_tiles) bytes for M.
+// Dynamic LDS: 4 * max(n_tiles) bytes for PM.
 // ---------------------------------------------------------------------------
+
+// First index in [lo, hi) with e[idx] > j, searching forward from lo.
+__device__ __forceinline__ int gg_first_greater(const int* e, int lo, int hi, int j) {
+  if (lo >= hi || e[lo] > j) return lo;
+  int step = 1;
+  int last_le = lo;  // e[last_le] <= j
+  while (last_le + step < hi && e[last_le + step] <= j) {
+    last_le += step;
+    step <<= 1;
+  }
+  int a = last_le + 1, b = min(last_le + step, hi);  // answer in [a, b]
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (e[mid] <= j)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return a;
+}
+
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args s, int force,
                                                         int anc_mode, int src_buf) {
@@ -329,6 +418,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   extern __shared__ __align__(16) char smem[];
   int* spm = reinterpret_cast<int*>(smem);
   constexpr int WT = GG_QCHUNK / GG_TILE + 1;  // tiles spanned by one item's inputs
+  constexpr int RUN = GG_QCHUNK / GG_BLOCK;    // consecutive outputs per thread
   __shared__ int sE[WT * GG_TILE];
   __shared__ int sA[GG_QCHUNK];
   __shared__ int ilds[8];
@@ -347,32 +437,34 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   int jbase, nout;
   if (anc_mode == 0) {
     const int nt = F.n_tiles;
-    // 1. inclusive prefix max of the tile maxima
-    int carry = 0;
-    for (int b0 = 0; b0 < nt; b0 += GG_BLOCK) {
-      const int b = b0 + t;
-      const int v = (b < nt) ? a.emax[F.tile_off + b] : 0;
-      int pm = max(gg_block_prefix_max256_i(v, ilds), carry);
-      if (b < nt) spm[b] = pm;
-      if (t == GG_BLOCK - 1) ilds[4] = pm;
-      __syncthreads();
-      carry = ilds[4];
-      __syncthreads();
+    // 1. inclusive prefix max of the tile maxima, chunked per thread
+    const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+    const int b0 = min(t * c, nt);
+    const int b1 = min(b0 + c, nt);
+    if (t == 0) sI[0] = nt - 1;  // overwritten by the search below
+    int run = 0;
+    for (int b = b0; b < b1; ++b) {
+      run = max(run, a.emax[F.tile_off + b]);
+      spm[b] = run;
     }
+    const int incl = gg_block_prefix_max256_i(run, ilds);
+    const int prev = __shfl_up(incl, 1, 64);
+    if ((t & 63) == 63) ilds[4 + (t >> 6)] = incl;
+    __syncthreads();
+    int pre = 0;
+    if ((t & 63) != 0)
+      pre = prev;
+    else if (t > 0)
+      pre = ilds[4 + (t >> 6) - 1];
+    for (int b = b0; b < b1; ++b) spm[b] = max(spm[b], pre);
+    __syncthreads();
     const int64_t d0 = (int64_t)cl * GG_QCHUNK;
     const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
-    // 2. first tile whose last position reaches d0
-    if (t == 0) {
-      int lo = 0, hi = nt - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int64_t last = (int64_t)min((mid + 1) * GG_TILE, n) - 1;
-        if (last + spm[mid] < d0)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      sI[0] = lo;
+    // 2. the tile holding diagonal d0 (TE is strictly increasing, TE_{nt-1} = 2n - 1)
+    for (int b = b0; b < b1; ++b) {
+      const int64_t te = (int64_t)min((b + 1) * GG_TILE, n) - 1 + spm[b];
+      const int64_t tp = (b == 0) ? -1 : (int64_t)(b * GG_TILE - 1) + spm[b - 1];
+      if (tp < d0 && te >= d0) sI[0] = b;
     }
     __syncthreads();
     const int ta = sI[0];
@@ -401,18 +493,15 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
     const int ib = ta * GG_TILE + c1;
     const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
     const int la = c0, lb = c1;  // local range in sE
-    // 4. ancestors of this item's outputs
-    for (int k = t; k < j1 - j0; k += GG_BLOCK) {
-      const int j = j0 + k;
-      int lo = la, hi = lb;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (sE[mid] <= j)
-          lo = mid + 1;
-        else
-          hi = mid;
+    // 4. ancestors: thread t owns outputs [t*RUN, t*RUN + RUN)
+    {
+      int idx = la;
+      const int kb = t * RUN;
+      const int ke = min(kb + RUN, j1 - j0);
+      for (int k = kb; k < ke; ++k) {
+        idx = gg_first_greater(sE, idx, lb, j0 + k);
+        sA[k] = ta * GG_TILE + idx;
       }
-      sA[k] = ta * GG_TILE + lo;
     }
     __syncthreads();
     jbase = j0;
@@ -421,29 +510,48 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
     jbase = cl * GG_QCHUNK;
     nout = min(max(n - jbase, 0), GG_QCHUNK);
   }
-  float sh[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) sh[d] = dy->shift[d];
-  const int vm = dy->vel_mode;
   const int32_t* asrc = a.anc + (int64_t)src_buf * a.P + F.p_off;
   double acc[D + 2];
 #pragma unroll
   for (int d = 0; d < D + 2; ++d) acc[d] = 0.0;
-  for (int k = t; k < nout; k += GG_BLOCK) {
-    const int j = jbase + k;
-    const int src = (anc_mode == 0) ? sA[k] : (anc_mode == 1 ? asrc[j] : j);
+  // 5. copy: four outputs per thread and round, every load issued before the stores
+  constexpr int U = 4;
+  for (int k0 = 0; k0 < nout; k0 += U * GG_BLOCK) {
+    int src[U];
+    bool ok[U];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const float v = gg_comp(a, cur, d)[F.p_off + src] - sh[d];
-      gg_comp(a, nxt, d)[F.p_off + j] = v;
-      acc[2 + d] += (double)v;
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * GG_BLOCK + t;
+      ok[u] = k < nout;
+      const int j = jbase + (ok[u] ? k : 0);
+      src[u] = (anc_mode == 0) ? sA[ok[u] ? k : 0] : (anc_mode == 1 ? asrc[j] : j);
     }
-    if (vm >= 1) {
-      const int nv = (vm == 2) ? 9 : 3;
+    float v[U][D];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int d = 0; d < D; ++d) v[u][d] = ok[u] ? gg_comp(a, cur, d)[F.p_off + src[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const int j = jbase + k0 + u * GG_BLOCK + t;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        gg_comp(a, nxt, d)[F.p_off + j] = v[u][d];
+        acc[2 + d] += (double)v[u][d];
+      }
+      a.anc[F.p_off + j] = src[u];
+    }
+  }
+  const int vm = dy->vel_mode;
+  if (vm >= 1) {
+    const int nv = (vm == 2) ? 9 : 3;
+    for (int k = t; k < nout; k += GG_BLOCK) {
+      const int j = jbase + k;
+      const int sj = a.anc[F.p_off + j];  // this thread wrote it above
       for (int k2 = 0; k2 < nv; ++k2)
-        gg_comp(a, nxt, D + k2)[F.p_off + j] = gg_comp(a, cur, D + k2)[F.p_off + src];
+        gg_comp(a, nxt, D + k2)[F.p_off + j] = gg_comp(a, cur, D + k2)[F.p_off + sj];
     }
-    a.anc[F.p_off + j] = src;
   }
   double tot[D + 2];
   gg_block_totals<D>(acc, lds, tot);
@@ -451,16 +559,18 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
     for (int d = 0; d < D; ++d) a.csum[(int64_t)chunk * GG_MAX_D + d] = tot[2 + d];
 }
 
-// After the gather: mean of the resampled particles, anchor re-centring, buffer
-// swap, weights reset (one workgroup per filter).
+// After the gather: mean of the resampled particles, buffer swap, weights
+// reset (one workgroup per filter, record staged in LDS).
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_finalize(gg_arena a, int force) {
   constexpr int D = 4 + NISB;
   __shared__ double lds[8 + GG_BLOCK];
+  __shared__ gg_dyn sd;
   const int f = blockIdx.x;
   const gg_filter F = a.filt[f];
   gg_dyn* dy = a.dyn + f;
   if (!force && !dy->flag) return;
+  gg_dyn_copy(&sd, dy);
   const int n = F.n;
   const int nc = F.n_chunks;
   double v[D + 2];
@@ -471,32 +581,22 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_finalize(gg_arena a, int force)
     for (int d = 0; d < D; ++d)
       v[2 + d] = v[2 + d] + a.csum[(int64_t)(F.chunk_off + cc) * GG_MAX_D + d];
   double tot[D + 2];
-  gg_block_totals<D>(v, lds, tot);
+  gg_block_totals<D>(v, lds, tot);  // synchronises, so sd is complete here
   if (threadIdx.x == 0) {
-    double shp[3] = {(double)dy->shift[0], (double)dy->shift[1], (double)dy->shift[2]};
-    double she[3];
-    gg_rot_ecef(dy->R, shp, she);
-    for (int k = 0; k < 3; ++k) dy->anchor[k] += she[k];
-    dy->cb_anchor += (double)dy->shift[3];
-    for (int d = 4; d < D; ++d) dy->isb_anchor[d - 4] += (double)dy->shift[d];
-    for (int d = 0; d < D; ++d) dy->mean[d] = tot[2 + d] / (double)n;
-    double mp[3] = {dy->mean[0], dy->mean[1], dy->mean[2]};
-    double me[3];
-    gg_rot_ecef(dy->R, mp, me);
-    for (int k = 0; k < 3; ++k) dy->est[k] = dy->anchor[k] + me[k];
-    dy->est[3] = dy->cb_anchor + dy->mean[3];
-    for (int d = 4; d < D; ++d) dy->est[d] = dy->isb_anchor[d - 4] + dy->mean[d];
-    dy->cur = 1 - dy->cur;
-    dy->lw_zero = 1;
-    dy->flag = 0;
-    dy->did_resample = 1;
-    dy->resample_count += 1;
-    dy->ess = (double)n;
-    dy->M = 0.0;
-    dy->W = (double)n;
-    dy->W2 = (double)n;
-    dy->est_valid = 1;
+    for (int d = 0; d < D; ++d) sd.mean[d] = tot[2 + d] / (double)n;
+    gg_make_estimate<D>(&sd);
+    sd.cur = 1 - sd.cur;
+    sd.lw_zero = 1;
+    sd.flag = 0;
+    sd.did_resample = 1;
+    sd.resample_count += 1;
+    sd.ess = (double)n;
+    sd.M = 0.0;
+    sd.W = (double)n;
+    sd.W2 = (double)n;
   }
+  __syncthreads();
+  gg_dyn_copy(dy, &sd);
 }
 
 // ---------------------------------------------------------------------------

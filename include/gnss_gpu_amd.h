@@ -124,9 +124,18 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
 int gg_pfb_replay(gg_pf* h, int32_t n_replays);
 /* Device time of n_replays graph launches (hipEvents on the handle's stream). */
 int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total);
+/* Device time of each of n_replays graph launches (ms_each[n_replays]). */
+int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each);
+/* n_steps eager steps with hipEvents around each launch group on the handle's
+ * stream: stage_ms[0..3] = mean ms of (predict+weight, decision [+ CDF for
+ * systematic], gather [+ Megopolis], finalize) over steps in which some filter
+ * resampled, stage_ms[4..7] over the other steps; counts[0..1] = the two step
+ * counts. */
+int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                         int32_t n_steps, float* stage_ms, int32_t* counts);
 /* Device time of `reps` launches of one kernel: 0 weight+predict, 1 weight,
- * 2 cdf, 3 gather (systematic), 4 Megopolis (Chesser), 5 gather (ancestors),
- * 6 decision (one workgroup per filter). */
+ * 2 CDF with the per-workgroup decision, 3 gather (systematic), 4 Megopolis
+ * (Chesser), 5 gather (ancestors), 6 decision (one workgroup per filter). */
 int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
                       int32_t reps, float* ms_total);
 int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out);

@@ -127,23 +127,36 @@ def tile_view(lw, n):
 
 
 def tile_partials(lw, n):
-    """Per-tile m_b and S_b = sum exp64(lw - m_b) (pf_device.cu:924-965 pattern,
-    library order).  Returns (m[nt], S[nt], e[nt,256,4])."""
+    """Per-tile m_b = max lw (binary32, exact), e_i = exp32(lw_i - m_b) and
+    S_b = sum e_i accumulated in binary64 (pf_device.cu:924-965 pattern,
+    library order: per thread left to right over its 4 particles, then
+    gg_block_scan256's total).  Returns (m[nt] as binary64, S[nt],
+    e[nt,256,4] as binary64)."""
     lwt, val = tile_view(lw, n)
-    lw64 = lwt.astype(np.float64)
-    m = np.where(val, lw64, -np.inf).reshape(lw64.shape[0], -1).max(axis=1)
-    e = np.zeros_like(lw64)
-    ok = val & (m[:, None, None] > -np.inf)
-    e[ok] = exp64((lw64 - m[:, None, None])[ok])
-    s = e[..., 0].copy()
+    nt = lwt.shape[0]
+    m32 = np.fmax.reduce(np.where(val, lwt, _F(-np.inf)).reshape(nt, -1), axis=1).astype(np.float32)
+    e = np.zeros(lwt.shape, dtype=np.float32)
+    ok = val & (m32[:, None, None] > -np.inf)
+    diff = (lwt - m32[:, None, None]).astype(np.float32)
+    e[ok] = exp32(diff[ok])
+    ed = e.astype(np.float64)
+    s = ed[..., 0].copy()
     for q in range(1, ITEMS):
-        s = s + e[..., q]
+        s = s + ed[..., q]
     S = block_total256(s)
-    return m, S, e
+    return m32.astype(np.float64), S, ed
+
+
+def kcount(c, n, u0):
+    """#{j in [0, N) : (u0 + j) / N <= c}, NaN -> 0 (twin of gg_kcount)."""
+    c = np.asarray(c, dtype=np.float64)
+    k = np.searchsorted(systematic_targets(n, u0), c, side="right").astype(np.int64)
+    return np.where(np.isnan(c), 0, k)
 
 
 def decide(m, S):
-    """Twin of gg_decide's CDF part: M, g_b, P_b, W."""
+    """Twin of gg_combine: M, g_b = exp64(m_b - M), the tile offsets P_b and
+    the total W in the fixed chunked order."""
     nt = m.shape[0]
     M = float(np.max(m))
     g = np.zeros(nt)
@@ -188,7 +201,11 @@ def fresh_u0(seed, resample_count):
 
 
 def systematic_ends(lw, n, u0):
-    """Per-particle output ends E_i (prefix max of #{j : t_j <= cdf_i})."""
+    """Per-particle output ends E_i of the library's systematic resampler:
+    cdf_i = (P_b + g_b L_i) / W with L_i the in-tile inclusive scan of e,
+    k_i = #{j : t_j <= cdf_i} (N for the last particle), E = prefix max of k.
+    The ancestors a_j = #{i : E_i <= j} equal min{i : cdf_i >= t_j} wherever
+    the rounded cdf is monotone."""
     m, S, e = tile_partials(lw, n)
     M, g, P, W = decide(m, S)
     nt = m.shape[0]
@@ -197,11 +214,8 @@ def systematic_ends(lw, n, u0):
         l[..., q] = l[..., q - 1] + e[..., q]
     _, excl, _ = block_scan256(l[..., ITEMS - 1])
     L = excl[..., None] + l
-    cdf = (P[:, None, None] + g[:, None, None] * L) / W
-    cdf = cdf.reshape(nt * TILE)[:n]
-    t = systematic_targets(n, u0)
-    k = np.searchsorted(t, cdf, side="right").astype(np.int64)
-    k[np.isnan(cdf)] = 0
+    cdf = ((P[:, None, None] + g[:, None, None] * L) / W).reshape(nt * TILE)[:n]
+    k = kcount(cdf, n, u0)
     k[n - 1] = n
     return np.maximum.accumulate(k), cdf
 

@@ -30,6 +30,8 @@ PINS = {
     "detmath.block_total256": ("pinned", "last element of the pinned scan"),
     "detmath.tile_view": ("pinned", "layout only; exercised by the brute-force systematic tests"),
     "detmath.tile_partials": ("pinned", "brute-force systematic + ESS closed forms"),
+    "detmath.kcount": ("pinned", "searchsorted over the exactly rounded targets; brute force "
+                       "through the systematic tests"),
     "detmath.decide": ("pinned", "brute-force systematic (test_systematic_matches_brute_force)"),
     "detmath.systematic_targets": ("pinned", "pf_device.cu:1202 (u0+j)/N; brute force"),
     "detmath.reference_u0": ("pinned", "int32 product of pf_device.cu:1852 on N where 1e6*N "

@@ -136,3 +136,35 @@ def test_batched_megopolis_steps_match_oracle_ancestors(gpu, resampler):
     assert np.linalg.norm(b.estimates()[0, :3] - truth[steps - 1]) < 30.0
     _ = detmath
     _ = opf
+
+
+def test_multi_step_graph_matches_eager_steps(gpu):
+    """A graph holding several consecutive steps advances the device epoch
+    counter inside the graph; its replays equal the eager steps."""
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+
+    ns = [7000, 3000]
+    steps, per_graph = 12, 4
+    scen = [_scenario(300 + f, 9, steps) for f in range(2)]
+
+    def make():
+        b = BatchedParticleFilter(ns, sat_cap=9, n_slots=steps)
+        b.load_epochs([[scen[f][0] for f in range(2)] for _ in range(steps)],
+                      [[scen[f][1][t] for f in range(2)] for t in range(steps)], [[9, 9]] * steps,
+                      vel=np.array([[scen[f][3] for f in range(2)]] * steps))
+        init = np.array([np.concatenate([scen[f][2][0] - scen[f][3], [50.0]]) for f in range(2)])
+        b.initialize(init, [5, 6], 8.0, 8.0)
+        return b
+
+    eager = make()
+    st = eager.step_args(sigma_pos=1.0, sigma_cb=2.0, sigma_pr=5.0)
+    ref = []
+    for t in range(steps):
+        eager.step(st)
+        if (t + 1) % per_graph == 0:
+            ref.append(eager.estimates())
+    g = make()
+    g.capture(st, steps=per_graph)
+    for r in range(steps // per_graph):
+        g.replay(1)
+        np.testing.assert_array_equal(g.estimates(), ref[r])

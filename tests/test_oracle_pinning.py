@@ -117,15 +117,16 @@ def _brute_systematic(w, u0):
     for x in w:
         run += Fraction(x)
         cdf.append(run / tot)
+    import bisect
+
     out = []
     for j in range(n):
         t = (Fraction(u0) + j) / n
-        i = next((k for k, c in enumerate(cdf) if c >= t), n - 1)
-        out.append(i)
+        out.append(min(bisect.bisect_left(cdf, t), n - 1))
     return np.array(out)
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 300, 1024, 1500])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 300, 1024, 1025, 1500, 4099])
 def test_systematic_matches_brute_force(n):
     rng = np.random.default_rng(n)
     lw = rng.normal(0.0, 2.0, n).astype(np.float32)
