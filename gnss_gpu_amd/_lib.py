@@ -98,9 +98,18 @@ _I = C.c_int32
 _U64 = C.c_uint64
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
+_FP = C.POINTER(C.c_float)
 
 # name -> argtypes (all return int status)
 _SIGS = {
+    "gg_mlp_create": [C.POINTER(_P)],
+    "gg_mlp_destroy": [_P],
+    "gg_mlp_set_params": [_P, _FP],
+    "gg_mlp_get_params": [_P, _FP],
+    "gg_mlp_eval": [_P, _I, _FP, _FP, _FP, _FP],
+    "gg_mlp_load_data": [_P, _I, _I, _FP, _FP, _FP, _IP],
+    "gg_mlp_train": [_P, _I, _I, C.c_float, _U64, C.c_float, C.c_float, C.c_float, _FP],
+    "gg_mlp_loss_grad": [_P, _I, _FP, _FP, _FP],
     "gg_abi_version": [],
     "gg_struct_sizes": [_IP, _IP, _IP],
     "gg_device_count": [_IP],
@@ -212,6 +221,14 @@ def dptr(a: np.ndarray):
 
 def iptr(a: np.ndarray):
     return a.ctypes.data_as(_IP)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
 def f64(a) -> np.ndarray:

@@ -1,0 +1,162 @@
+// Learned pseudorange likelihood: a 8 -> 64 -> 64 -> 2 MLP per (particle,
+// satellite) pair on the gfx950 matrix cores (bf16 operands, binary32
+// accumulation), laid out so that one wavefront evaluates 16 pairs per pass
+// with no data exchange between the layers.
+//
+// Network (parameters as a binary32 vector, offsets below):
+//   x  = [clamp(0.1 e, -20, 20), f_1..f_7]      f: per-satellite features
+//   z1 = W1 x + b1 = w_e x_0 + c_s,  c_s = b1 + sum_j W1[:,1+j] f_j  (per satellite)
+//   z2 = W2 relu(z1) + b2,  z3 = W3 relu(z2) + b3
+//   mu = 10 z3_0,  l = clamp(z3_1, -4, 12),  ll = -(e - mu)^2 exp(-l) / 2 - l / 2
+//
+// MFMA v_mfma_f32_16x16x32_bf16 operand layout (lane l, g = l / 16):
+//   A (16 x 32): lane holds row l%16, k = 8g .. 8g+7
+//   B (32 x 16): lane holds column l%16, k = 8g .. 8g+7
+//   C (16 x 16): lane holds column l%16, rows 4g .. 4g+3
+// Every layer is computed transposed (units x pairs), so a lane always holds
+// one pair (column l%16) and 16 of its units: 16 mb + 4 g + j (mb = 0..3,
+// j = 0..3).  The next layer takes those as its B operand directly by
+// ordering its K dimension as unit(ks, g, s) = 16 (2 ks + s/4) + 4 g + s%4
+// (K-step ks = 0..1, slot s = 0..7); its A operand (the weights) is stored in
+// the same K order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define GG_MLP_H 64
+#define GG_MLP_NF 8
+#define GG_MLP_OFF_W1 0
+#define GG_MLP_OFF_B1 (GG_MLP_OFF_W1 + GG_MLP_H * GG_MLP_NF)
+#define GG_MLP_OFF_W2 (GG_MLP_OFF_B1 + GG_MLP_H)
+#define GG_MLP_OFF_B2 (GG_MLP_OFF_W2 + GG_MLP_H * GG_MLP_H)
+#define GG_MLP_OFF_W3 (GG_MLP_OFF_B2 + GG_MLP_H)
+#define GG_MLP_OFF_B3 (GG_MLP_OFF_W3 + 2 * GG_MLP_H)
+#define GG_MLP_NPARAMS (GG_MLP_OFF_B3 + 2)
+#define GG_MLP_LV_MIN (-4.0f)
+#define GG_MLP_LV_MAX (12.0f)
+#define GG_MLP_MU_SCALE 10.0f
+#define GG_MLP_X_CLAMP 20.0f
+
+typedef __bf16 gg_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float gg_f32x4 __attribute__((ext_vector_type(4)));
+
+// Packed weights (built by gg_k_mlp_pack from the binary32 parameters).
+struct gg_mlp_w {
+  gg_bf16x8 A2[4][2][64];   // W2 as A fragments: [mb][ks][lane]
+  gg_bf16x8 A2t[4][2][64];  // W2^T as A fragments (backward)
+  gg_bf16x8 A3[2][64];      // W3 (rows 0, 1; rows 2..15 zero)
+  float b2[GG_MLP_H];
+  float b3[4];
+  float w_e[GG_MLP_H];      // bf16(W1[:, 0]) as binary32
+  float W1[GG_MLP_H][GG_MLP_NF];
+  float b1[GG_MLP_H];
+  float W3b[2][GG_MLP_H];   // bf16(W3) as binary32 (backward)
+};
+
+__host__ __device__ __forceinline__ int gg_mlp_unit(int ks, int g, int s) {
+  return 16 * (2 * ks + (s >> 2)) + 4 * g + (s & 3);
+}
+
+__device__ __forceinline__ gg_f32x4 gg_mfma(const gg_bf16x8& a, const gg_bf16x8& b,
+                                            const gg_f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// First-layer bias of one satellite: c = b1 + sum_j W1[:,1+j] f_j, binary32,
+// j left to right.
+__device__ __forceinline__ float gg_mlp_c(const gg_mlp_w& w, int u, const float* f) {
+#pragma clang fp contract(off)
+  float c = w.b1[u];
+#pragma unroll
+  for (int j = 0; j < GG_MLP_NF - 1; ++j) c = c + w.W1[u][1 + j] * f[j];
+  return c;
+}
+
+__device__ __forceinline__ float gg_mlp_xe(float e) {
+#pragma clang fp contract(off)
+  return fminf(fmaxf(e * 0.1f, -GG_MLP_X_CLAMP), GG_MLP_X_CLAMP);
+}
+
+// ReLU of a lane's 16 units (C layout, c[mb][j]) packed as the two B
+// fragments of the next layer.
+__device__ __forceinline__ void gg_mlp_relu_pack(const gg_f32x4 (&c)[4], gg_bf16x8 (&b)[2]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) b[ks][s] = (__bf16)fmaxf(c[2 * ks + (s >> 2)][s & 3], 0.0f);
+}
+
+// One wavefront, 16 pairs (pair = lane % 16).  xe: the pair's scaled residual
+// (used by lanes 0..15); a1: the satellite's first-layer A fragments.  lds_w:
+// the packed weights (LDS or global).  Outputs the layer activations the
+// backward pass needs and, in lanes 0..15, z3_0 / z3_1.
+struct gg_mlp_act {
+  gg_f32x4 z1[4];
+  gg_f32x4 z2[4];
+  gg_bf16x8 h1[2];
+  gg_bf16x8 h2[2];
+  float z30, z31;
+};
+
+__device__ __forceinline__ void gg_mlp_fwd16(const gg_mlp_w& w, const gg_bf16x8 (&a1)[4], float xe,
+                                             gg_mlp_act& act) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4;
+  gg_bf16x8 b1;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) b1[s] = (__bf16)0.0f;
+  if (g == 0) {
+    b1[0] = (__bf16)xe;
+    b1[1] = (__bf16)1.0f;
+  }
+  const gg_f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) act.z1[mb] = gg_mfma(a1[mb], b1, zero);
+  gg_mlp_relu_pack(act.z1, act.h1);
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    gg_f32x4 c;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = w.b2[16 * mb + 4 * g + j];
+    c = gg_mfma(w.A2[mb][0][l], act.h1[0], c);
+    c = gg_mfma(w.A2[mb][1][l], act.h1[1], c);
+    act.z2[mb] = c;
+  }
+  gg_mlp_relu_pack(act.z2, act.h2);
+  gg_f32x4 c3 = zero;
+  if (g == 0) {
+    c3[0] = w.b3[0];
+    c3[1] = w.b3[1];
+  }
+  c3 = gg_mfma(w.A3[0][l], act.h2[0], c3);
+  c3 = gg_mfma(w.A3[1][l], act.h2[1], c3);
+  act.z30 = c3[0];
+  act.z31 = c3[1];
+}
+
+// First-layer A fragments of one satellite: lanes 0..15 hold k = 0 (w_e) and
+// k = 1 (c_s) of unit 16 mb + lane; everything else is zero.
+__device__ __forceinline__ void gg_mlp_a1(const float* w_e, const float* c_s,
+                                          gg_bf16x8 (&a1)[4]) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a1[mb][s] = (__bf16)0.0f;
+    if (l < 16) {
+      a1[mb][0] = (__bf16)w_e[16 * mb + l];
+      a1[mb][1] = (__bf16)c_s[16 * mb + l];
+    }
+  }
+}
+
+// Pair log-likelihood from the network output (lanes 0..15).
+__device__ __forceinline__ float gg_mlp_ll(float e, float z30, float z31) {
+#pragma clang fp contract(off)
+  const float mu = GG_MLP_MU_SCALE * z30;
+  const float lv = fminf(fmaxf(z31, GG_MLP_LV_MIN), GG_MLP_LV_MAX);
+  const float r = e - mu;
+  return -0.5f * r * r * __expf(-lv) - 0.5f * lv;
+}

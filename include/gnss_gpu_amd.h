@@ -25,6 +25,7 @@ extern "C" {
 #endif
 
 typedef struct gg_pf gg_pf;
+typedef struct gg_mlp gg_mlp;
 
 /* --- library ----------------------------------------------------------- */
 const char* gg_last_error(void);
@@ -161,6 +162,36 @@ int gg_wls_batch_device(const double* sat_ecef, const double* pseudoranges,
 int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const double* weights,
                     int32_t n_sat, int32_t max_iter, double tol, double* result,
                     int32_t* iters);
+
+/* --- learned likelihood (no reference counterpart) -------------------------- */
+/* An 8 -> 64 -> 64 -> 2 MLP per (particle, satellite) pair predicting the NLOS
+ * bias mean and log-variance of the pseudorange residual; it replaces the
+ * Student-t term of pfd_weight_kernel (src/particle_filter/pf_device.cu:262-266)
+ * when a filter is given one.  Parameters: GG_MLP_NPARAMS binary32 values
+ * (W1[64][8] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]); bf16 operands with
+ * binary32 accumulation on the matrix cores. */
+int gg_mlp_create(gg_mlp** out);
+int gg_mlp_destroy(gg_mlp* h);
+int gg_mlp_set_params(gg_mlp* h, const float* params);
+int gg_mlp_get_params(gg_mlp* h, float* params);
+/* Pair log-likelihoods for n_groups groups of 16 pairs sharing one satellite:
+ * e[n_groups*16] residuals [m], f[n_groups*7] satellite features; outputs
+ * ll[n_groups*16] and z3[n_groups*16*2] (network outputs). */
+int gg_mlp_eval(gg_mlp* h, int32_t n_groups, const float* e, const float* f, float* ll, float* z3);
+
+/* Training set of n_epochs epochs with up to s_max (<= 16) satellites each:
+ * f[E][s_max][7] features, r0[E][s_max] residual at the true position [m],
+ * u[E][s_max][3] unit vectors to the satellites, n_sat[E]. */
+int gg_mlp_load_data(gg_mlp* h, int32_t n_epochs, int32_t s_max, const float* f, const float* r0,
+                     const float* u, const int32_t* n_sat);
+/* n_steps Adam steps (batch epochs each, 64 candidate offsets per epoch drawn
+ * from N(0, diag(sh^2, sh^2, sv^2, scb^2)), candidate 0 = the truth); loss_out
+ * (may be NULL) gets the mean batch loss of every step. */
+int gg_mlp_train(gg_mlp* h, int32_t n_steps, int32_t batch, float lr, uint64_t seed,
+                 float cand_sigma_h, float cand_sigma_v, float cand_sigma_cb, float* loss_out);
+/* Loss and gradient (sum over epochs) of the first n_epochs loaded epochs for
+ * given candidates cand[n_epochs][64][4] (dx dy dz dcb); loss[n_epochs]. */
+int gg_mlp_loss_grad(gg_mlp* h, int32_t n_epochs, const float* cand, float* grad, float* loss);
 
 /* --- test hook (no reference counterpart) ----------------------------------- */
 /* Evaluates a deterministic device building block for n inputs so tests can

@@ -71,6 +71,22 @@ PINS = {
     "wls.wls_batch_ref": ("pinned", "test_positioning.py:49-75 truth (Sagnac)"),
     "wls._los": ("pinned", "Sagnac closed form of range_model.py:20-29 (test_sagnac_rotation_closed_form)"),
     "wls._solve4": ("pinned", "numpy.linalg.solve on random SPD systems; singular pivot rule"),
+    # learned.py
+    "learned.unpack": ("pinned", "inverse of pack (round trip in the GPU parameter test)"),
+    "learned.pack": ("pinned", "inverse of unpack"),
+    "learned.init_params": ("pinned", "shapes and the b3 = [0, ln 25] start; no numerics"),
+    "learned.bf16": ("pinned", "round-to-nearest-even on hand-checked bit patterns "
+                     "(test_bf16_rounding_known_values)"),
+    "learned.first_layer_bias": ("pinned", "binary32 left-to-right sum against a binary64 sum"),
+    "learned.scaled_residual": ("pinned", "clamp bounds"),
+    "learned.mlp_forward": ("pinned", "exact mode: a ReLU network is affine on a fixed activation "
+                            "pattern (test_mlp_forward_affine_on_fixed_pattern)"),
+    "learned.pair_loglik": ("pinned", "closed form of a Gaussian log-density"),
+    "learned.mlp_backward": ("pinned", "central finite differences in binary64 "
+                             "(test_contrastive_gradient_finite_differences)"),
+    "learned.candidate_residuals": ("pinned", "u.d - dcb against binary64"),
+    "learned.contrastive_loss_and_grad": ("pinned", "finite differences; loss >= 0 and = 0 "
+                                          "for a perfect softmax match"),
 }
 
 UNPINNED = {

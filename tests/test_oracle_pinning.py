@@ -372,7 +372,9 @@ def test_every_oracle_function_has_a_pinning_entry():
 
     from oracle import pinning
 
-    mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls}
+    from oracle import learned as olearned
+
+    mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls, "learned": olearned}
     missing = []
     for name, mod in mods.items():
         for fn, obj in vars(mod).items():
@@ -387,3 +389,75 @@ def test_every_oracle_function_has_a_pinning_entry():
     assert not missing, missing
     for key, (status, how) in pinning.PINS.items():
         assert status in ("pinned", "parity unpinned") and how
+
+
+# -------------------------------------------------------- learned likelihood ----
+def test_bf16_rounding_known_values():
+    from oracle import learned as ol
+
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: ties to even -> 1.0
+    assert ol.bf16(np.float32(1.0 + 2.0 ** -8)) == 1.0
+    # 1 + 3 * 2^-8 ties between 1 + 2^-7 and 1 + 2^-6: even mantissa is 1 + 2^-6
+    assert ol.bf16(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6
+    assert ol.bf16(np.float32(1.0 + 2.0 ** -8 + 2.0 ** -12)) == 1.0 + 2.0 ** -7
+    assert ol.bf16(np.float32(-2.5)) == -2.5
+    assert np.isnan(ol.bf16(np.float32(np.nan)))
+
+
+def test_first_layer_bias_close_to_binary64():
+    from oracle import learned as ol
+
+    p = ol.init_params(2)
+    W1, b1 = ol.unpack(p)[:2]
+    f = np.random.default_rng(1).normal(size=(5, 7))
+    c = ol.first_layer_bias(W1, b1, f)
+    np.testing.assert_allclose(c, b1 + f @ W1[:, 1:].T, rtol=1e-5, atol=1e-5)
+
+
+def test_mlp_forward_affine_on_fixed_pattern():
+    """With every unit active the exact network is affine: z3 = A x + c with
+    A = W3 W2 W1, checked on inputs where all pre-activations are positive."""
+    from oracle import learned as ol
+
+    p = ol.init_params(0).astype(np.float64)
+    W1, b1, W2, b2, W3, b3 = ol.unpack(p)
+    W1, W2 = np.abs(W1) * 0.1, np.abs(W2) * 0.1
+    b1, b2 = np.ones(ol.H), np.ones(ol.H)
+    p = ol.pack(W1, b1, W2, b2, W3, b3).astype(np.float64)
+    e = np.array([3.0, 17.0, 42.0])
+    f = np.abs(np.random.default_rng(0).normal(size=(3, 7)))
+    fw = ol.mlp_forward(p, e, f, mode="exact")
+    x = np.concatenate([(e * np.float32(0.1))[:, None], f], axis=1)
+    z3 = x @ (W3 @ W2 @ W1).T + (W3 @ (W2 @ b1 + b2) + b3)
+    np.testing.assert_allclose(fw["z3"], z3, rtol=1e-6)
+    np.testing.assert_allclose(fw["mu"], 10 * z3[:, 0], rtol=1e-6)
+    # the pair log-likelihood is the Gaussian log-density of e under (mu, exp(l))
+    from math import exp, log, pi, sqrt
+    for i in range(3):
+        sd = sqrt(exp(fw["lv"][i]))
+        logpdf = -0.5 * ((e[i] - fw["mu"][i]) / sd) ** 2 - log(sd) - 0.5 * log(2 * pi)
+        assert fw["ll"][i] == pytest.approx(logpdf + 0.5 * log(2 * pi), rel=1e-12, abs=1e-12)
+
+
+def test_contrastive_gradient_finite_differences():
+    from oracle import learned as ol
+
+    p = ol.init_params(1).astype(np.float64)
+    rng = np.random.default_rng(0)
+    S, K = 5, 7
+    r0 = rng.normal(0, 10, S)
+    u = rng.normal(size=(S, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    f = rng.normal(size=(S, 7))
+    dpos, dcb = rng.normal(0, 5, (K, 3)), rng.normal(0, 5, K)
+    dpos[0], dcb[0] = 0, 0
+    loss, g, _ = ol.contrastive_loss_and_grad(p, r0, u, f, dpos, dcb)
+    assert loss > 0
+    h = 1e-6
+    for i in range(0, ol.N_PARAMS, 53):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        num = (ol.contrastive_loss_and_grad(pp, r0, u, f, dpos, dcb)[0]
+               - ol.contrastive_loss_and_grad(pm, r0, u, f, dpos, dcb)[0]) / (2 * h)
+        assert num == pytest.approx(g[i], abs=1e-7, rel=1e-5)
