@@ -103,6 +103,29 @@ def pf_device_weight(state, sat_ecef, pseudoranges, weights_sat, n_sat, sigma_pr
          float(per_particle_huber_k))
 
 
+def pf_device_weight_learned(state, sat_ecef, pseudoranges, weights_sat, n_sat, cn0, los_flag,
+                             system_ids=None):
+    """Weight update with the attached learned likelihood (no reference
+    counterpart; replaces the Student-t term of pf_device.cu:262-266)."""
+    n_sat = int(n_sat)
+    sat = f64(sat_ecef).ravel()
+    pr = f64(pseudoranges).ravel()
+    w = f64(weights_sat).ravel()
+    c = np.ascontiguousarray(np.asarray(cn0, dtype=np.float32).ravel())
+    fl = np.ascontiguousarray(np.asarray(los_flag, dtype=np.float32).ravel())
+    if sat.size < 3 * n_sat or pr.size < n_sat or w.size < n_sat or c.size < n_sat or fl.size < n_sat:
+        raise ValueError("satellite arrays are shorter than n_sat")
+    sys = None if system_ids is None else i32(system_ids).ravel()
+    F = C.POINTER(C.c_float)
+    call("gg_pf_device_weight_learned", state.handle, dptr(sat), dptr(pr), dptr(w),
+         iptr(sys) if sys is not None else None, n_sat, c.ctypes.data_as(F),
+         fl.ctypes.data_as(F))
+
+
+def pf_device_set_mlp(state, model):
+    call("gg_pf_set_mlp", state.handle, None if model is None else model.handle)
+
+
 def pf_device_ess(state):
     out = C.c_double()
     call("gg_pf_device_ess", state.handle, C.byref(out))

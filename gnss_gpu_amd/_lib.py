@@ -23,6 +23,7 @@ GG_MAX_D = 4 + GG_MAX_ISB
 LIK_GAUSS = 0
 LIK_STUDENT = 1
 LIK_HUBER = 2
+LIK_LEARNED = 3
 
 RESAMPLER_SYSTEMATIC = 0
 RESAMPLER_MEGOPOLIS = 1
@@ -110,6 +111,9 @@ _SIGS = {
     "gg_mlp_load_data": [_P, _I, _I, _FP, _FP, _FP, _IP],
     "gg_mlp_train": [_P, _I, _I, C.c_float, _U64, C.c_float, C.c_float, C.c_float, _FP],
     "gg_mlp_loss_grad": [_P, _I, _FP, _FP, _FP],
+    "gg_pf_set_mlp": [_P, _P],
+    "gg_pfb_load_features": [_P, _I, _FP, _FP],
+    "gg_pf_device_weight_learned": [_P, _DP, _DP, _DP, _IP, _I, _FP, _FP],
     "gg_abi_version": [],
     "gg_struct_sizes": [_IP, _IP, _IP],
     "gg_device_count": [_IP],

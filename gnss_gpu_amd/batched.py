@@ -100,6 +100,28 @@ class BatchedParticleFilter:
              dptr(v))
         self.loaded_slots = T
 
+    def set_learned(self, model):
+        """Attach a LearnedLikelihood (None detaches); steps use it with
+        step_args(lik=LIK_LEARNED).  The model must outlive its use."""
+        self._mlp = model
+        call("gg_pf_set_mlp", self._h, None if model is None else model.handle)
+
+    def load_features(self, cn0, los_flag):
+        """Per-satellite C/N0 [dB-Hz] and line-of-sight flags of the learned
+        likelihood, epoch-major like load_epochs: cn0[t][f] -> (S_tf,)."""
+        T = len(cn0)
+        rows = self.sat_rows
+        c = np.zeros((T, rows), dtype=np.float32)
+        fl = np.zeros((T, rows), dtype=np.float32)
+        for t in range(T):
+            for f in range(self.n_filters):
+                o = int(self.sat_off[f])
+                v = np.asarray(cn0[t][f], dtype=np.float32).ravel()
+                c[t, o:o + v.size] = v
+                fl[t, o:o + v.size] = np.asarray(los_flag[t][f], dtype=np.float32).ravel()
+        call("gg_pfb_load_features", self._h, T, c.ctypes.data_as(C.POINTER(C.c_float)),
+             fl.ctypes.data_as(C.POINTER(C.c_float)))
+
     # -- stepping --------------------------------------------------------------
     def step_args(self, **kw):
         kw.setdefault("n_slots", getattr(self, "loaded_slots", self.n_slots))

@@ -9,6 +9,10 @@ int gg_fail(int code, const char* msg);
 int gg_fail_hip(hipError_t e, const char* expr, const char* file, int line);
 std::string gg_last_error_str();
 
+// Packed device weights of a learned-likelihood handle (gg_learn.hip).
+struct gg_mlp;
+const void* gg_mlp_packed_weights(const gg_mlp* m);
+
 #define GG_HIP(expr)                                                   \
   do {                                                                 \
     const hipError_t gg_e_ = (expr);                                   \

@@ -424,6 +424,8 @@ int alloc(T** p, size_t n) {
 
 }  // namespace
 
+const void* gg_mlp_packed_weights(const gg_mlp* m) { return m ? (const void*)m->w : nullptr; }
+
 extern "C" {
 
 int gg_mlp_create(gg_mlp** out) {

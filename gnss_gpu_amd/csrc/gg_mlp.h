@@ -55,6 +55,8 @@ struct gg_mlp_w {
   float W3b[2][GG_MLP_H];   // bf16(W3) as binary32 (backward)
 };
 
+static_assert(sizeof(gg_mlp_w) % 16 == 0, "gg_mlp_w is copied in 16-byte words");
+
 __host__ __device__ __forceinline__ int gg_mlp_unit(int ks, int g, int s) {
   return 16 * (2 * ks + (s >> 2)) + 4 * g + (s & 3);
 }

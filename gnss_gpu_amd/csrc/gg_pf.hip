@@ -44,6 +44,8 @@ struct gg_pf {
   double* satd = nullptr;
   int32_t* sati = nullptr;
   double* vel_guide = nullptr;
+  float* satf = nullptr;        // [n_slots][2][rows] C/N0 | LOS flag (learned likelihood)
+  const void* mlp_w = nullptr;  // packed network of the attached gg_mlp
   double* io_tmp = nullptr;
   size_t io_tmp_bytes = 0;
   // pinned host staging
@@ -92,6 +94,8 @@ gg_arena arena_of(const gg_pf* h) {
   a.tile_filter = h->tile_filter;
   a.chunk_filter = h->chunk_filter;
   a.satd = h->satd;
+  a.satf = h->satf;
+  a.mlp = (const gg_mlp_w*)h->mlp_w;
   a.sati = h->sati;
   a.vel_guide = h->vel_guide;
   a.P = h->P;
@@ -104,10 +108,12 @@ gg_arena arena_of(const gg_pf* h) {
   return a;
 }
 
-size_t sat_smem(const gg_pf* h) {
+// LDS of the staged satellites: 7 binary32 arrays (+ 64 first-layer biases
+// per satellite for the learned likelihood)
+size_t sat_smem(const gg_pf* h, bool learned = false) {
   int cap = 0;
   for (const auto& f : h->filt_h) cap = std::max(cap, f.sat_cap);
-  return (size_t)std::max(cap, 1) * 7 * sizeof(float);
+  return (size_t)std::max(cap, 1) * (7 + (learned ? GG_MLP_H : 0)) * sizeof(float);
 }
 
 int ensure_pinned(gg_pf* h, size_t bytes) {
@@ -206,8 +212,18 @@ size_t decide_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(double
 // likelihood (+ predict) over all tiles, then the per-filter decision
 int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide = true) {
   const gg_arena a = arena_of(h);
-  const size_t smem = sat_smem(h);
-  if (pred) {
+  const bool learned = s.lik == GG_LIK_LEARNED;
+  if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
+  const size_t smem = sat_smem(h, learned);
+  if (learned) {
+    if (pred) {
+      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, true>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+    } else {
+      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, false>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+    }
+  } else if (pred) {
     GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, true>), dim3(h->n_tiles),
                                                   dim3(GG_BLOCK), smem, h->stream, a, s));
   } else {
@@ -437,6 +453,7 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->chunk_filter, (size_t)chunks);
   if (!rc) rc = dev_alloc(&h->satd, (size_t)n_slots * 5 * rows);
   if (!rc) rc = dev_alloc(&h->sati, (size_t)n_slots * (rows + n_filters));
+  if (!rc) rc = dev_alloc(&h->satf, (size_t)n_slots * 2 * rows);
   if (!rc) rc = dev_alloc(&h->vel_guide, (size_t)n_slots * n_filters * 3);
   if (!rc) rc = hipEventCreateWithFlags(&h->pin_done, hipEventDisableTiming) == hipSuccess ? 0 : -1;
   if (!rc) rc = hipEventCreate(&h->ev0) == hipSuccess ? 0 : -1;
@@ -476,6 +493,7 @@ int gg_pf_destroy(gg_pf* h) {
   void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->g, h->Pb,
                   h->emax,  h->csum,  h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
+                  h->satf,
                   h->io_tmp};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -550,7 +568,7 @@ int gg_pf_device_initialize(gg_pf* h, double init_x, double init_y, double init_
     d.vcov_u[5] = vv;
     d.vel_mode = (spread_vel > 0.0) ? 1 : 0;
     d.lw_zero = 1;
-    d.ess = 0.0;
+    d.ess = (double)h->filt_h[f].n;  // uniform weights
     GG_HIP(hipMemcpy(h->dyn + f, &d, sizeof(d), hipMemcpyHostToDevice));
   }
   const gg_arena a = arena_of(h);
@@ -611,31 +629,38 @@ int gg_pf_device_predict(gg_pf* h, double vx, double vy, double vz, double dt, d
   return 0;
 }
 
-int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
-                        const double* weights_sat, const int32_t* sys, int32_t n_sat,
-                        double sigma_pr, double nu, double per_particle_nlos_threshold_m,
-                        int32_t per_particle_huber, double per_particle_huber_k) {
-  if (int rc = check_single(h)) return rc;
+}  // extern "C"
+
+namespace {
+
+// Satellites of the single-filter API into epoch slot 0 (grows the table past
+// its capacity like the reference, pf_device.cu:1533-1545).
+int stage_single(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                 const double* weights_sat, const int32_t* sys, int32_t n_sat, const float* cn0,
+                 const float* los_flag) {
   if (n_sat < 0) return gg_fail(-2, "n_sat must be >= 0");
   if (h->n_filters != 1) return gg_fail(-2, "single-filter call on a batch handle");
   if (n_sat > h->filt_h[0].sat_cap) {
-    // grow the satellite table (rare path; the reference grows past 64 too)
     GG_HIP(hipStreamSynchronize(h->stream));
     const int cap = std::max(n_sat, 2 * h->filt_h[0].sat_cap);
     h->filt_h[0].sat_cap = cap;
     h->sat_rows = cap;
     GG_HIP(hipFree(h->satd));
     GG_HIP(hipFree(h->sati));
+    GG_HIP(hipFree(h->satf));
     h->satd = nullptr;
     h->sati = nullptr;
+    h->satf = nullptr;
     if (dev_alloc(&h->satd, (size_t)h->n_slots * 5 * cap)) return -1;
     if (dev_alloc(&h->sati, (size_t)h->n_slots * (cap + 1))) return -1;
+    if (dev_alloc(&h->satf, (size_t)h->n_slots * 2 * cap)) return -1;
     GG_HIP(hipMemcpy(h->filt, h->filt_h.data(), sizeof(gg_filter), hipMemcpyHostToDevice));
   }
   const int rows = h->sat_rows;
   const size_t dbytes = sizeof(double) * 5 * rows;
   const size_t ibytes = sizeof(int32_t) * (rows + 1);
-  if (ensure_pinned(h, std::max(h->pin_bytes, dbytes + ibytes + 64))) return -1;
+  const size_t fbytes = sizeof(float) * 2 * rows;
+  if (ensure_pinned(h, std::max(h->pin_bytes, dbytes + ibytes + fbytes + 64))) return -1;
   double* pd = (double*)h->pin;
   std::memset(pd, 0, dbytes);
   for (int k = 0; k < n_sat; ++k) {
@@ -650,7 +675,29 @@ int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudora
   pi[rows] = n_sat;
   GG_HIP(hipMemcpyAsync(h->satd, pd, dbytes, hipMemcpyHostToDevice, h->stream));
   GG_HIP(hipMemcpyAsync(h->sati, pi, ibytes, hipMemcpyHostToDevice, h->stream));
+  if (cn0 && los_flag) {
+    float* pf = (float*)(h->pin + dbytes + ibytes);
+    for (int k = 0; k < rows; ++k) {
+      pf[k] = (k < n_sat) ? cn0[k] : 0.0f;
+      pf[rows + k] = (k < n_sat) ? los_flag[k] : 0.0f;
+    }
+    GG_HIP(hipMemcpyAsync(h->satf, pf, fbytes, hipMemcpyHostToDevice, h->stream));
+  }
   GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                        const double* weights_sat, const int32_t* sys, int32_t n_sat,
+                        double sigma_pr, double nu, double per_particle_nlos_threshold_m,
+                        int32_t per_particle_huber, double per_particle_huber_k) {
+  if (int rc = check_single(h)) return rc;
+  if (stage_single(h, sat_ecef, pseudoranges, weights_sat, sys, n_sat, nullptr, nullptr))
+    return -1;
   gg_step_args s = default_args();
   s.sigma_pr = (float)sigma_pr;
   s.nu = (float)nu;
@@ -665,6 +712,43 @@ int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudora
   h->decide_valid = true;
   h->ess_valid = true;
   h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_weight_learned(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                                const double* weights_sat, const int32_t* sys, int32_t n_sat,
+                                const float* cn0, const float* los_flag) {
+  if (int rc = check_single(h)) return rc;
+  if (!cn0 || !los_flag) return gg_fail(-2, "learned likelihood needs cn0 and los_flag");
+  if (stage_single(h, sat_ecef, pseudoranges, weights_sat, sys, n_sat, cn0, los_flag)) return -1;
+  gg_step_args s = default_args();
+  s.lik = GG_LIK_LEARNED;
+  s.auto_flag = 0;
+  if (launch_weight(h, s, false)) return -1;
+  h->decide_valid = true;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_set_mlp(gg_pf* h, gg_mlp* m) {
+  if (!h) return gg_fail(-2, "null handle");
+  h->mlp_w = gg_mlp_packed_weights(m);
+  return 0;
+}
+
+int gg_pfb_load_features(gg_pf* h, int32_t n_slots, const float* cn0, const float* los_flag) {
+  if (!h || !cn0 || !los_flag) return gg_fail(-2, "null argument");
+  if (n_slots <= 0 || n_slots > h->n_slots) return gg_fail(-2, "n_slots out of range");
+  const int rows = h->sat_rows;
+  std::vector<float> buf((size_t)n_slots * 2 * rows);
+  for (int t = 0; t < n_slots; ++t)
+    for (int k = 0; k < rows; ++k) {
+      buf[(size_t)t * 2 * rows + k] = cn0[(size_t)t * rows + k];
+      buf[(size_t)t * 2 * rows + rows + k] = los_flag[(size_t)t * rows + k];
+    }
+  GG_HIP(hipStreamSynchronize(h->stream));
+  GG_HIP(hipMemcpy(h->satf, buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -843,6 +927,7 @@ int gg_pf_device_set_particle_states(gg_pf* h, const double* in) {
   }
   d.vel_mode = !cov_uniform ? 2 : (!vel_uniform ? 1 : 0);
   d.lw_zero = 1;
+  d.ess = (double)n;
   d.flag = 0;
   d.est_valid = 0;
   d.cnt_a = d.cnt_b = d.cnt_c = d.cnt_d = 0;
@@ -950,6 +1035,7 @@ int gg_pfb_initialize(gg_pf* h, const double* init, const uint64_t* seeds, doubl
     for (int k = 0; k < h->n_isb; ++k) d.isb_anchor[k] = x[4 + k];
     enu_rotation(d.anchor, d.R);
     d.lw_zero = 1;
+    d.ess = (double)h->filt_h[f].n;  // uniform weights
   }
   GG_HIP(hipMemcpy(h->dyn, dv.data(), sizeof(gg_dyn) * h->n_filters, hipMemcpyHostToDevice));
   const gg_arena a = arena_of(h);

@@ -25,6 +25,7 @@
 #pragma once
 
 #include "gg_device.h"
+#include "gg_mlp.h"
 
 struct gg_arena {
   float* state;           // [2][ncomp][P]
@@ -47,6 +48,10 @@ struct gg_arena {
   const double* satd;
   const int32_t* sati;
   const double* vel_guide;  // [n_slots][n_filters][3] (ECEF)
+  // learned likelihood: packed network and per-row satellite inputs
+  //   satf + s*2*sat_rows: cn0[sat_rows] | los_flag[sat_rows]   (binary32)
+  const gg_mlp_w* mlp;
+  const float* satf;
   int64_t P;
   int32_t ncomp;
   int32_t n_isb;
@@ -69,6 +74,7 @@ struct gg_sat_view {
   float* c;     // pr - |sat - anchor| - cb_anchor - isb_anchor
   float* w;
   int* isb;     // index into the ISB offsets, -1 for the reference system
+  float* cs;    // learned likelihood: first-layer bias per satellite [cap][64]
 };
 
 // ---------------------------------------------------------------------------
@@ -283,10 +289,12 @@ __device__ __forceinline__ void gg_predict4(const gg_arena& a, int cur, int64_t 
 }
 
 // Stage the filter's satellites for this epoch slot in LDS.
+// anchor[3], cb_anchor, isb_anchor[3]: the step frame's binary64 origins
+// (usually in LDS).
 __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_filter& F,
-                                             const gg_dyn* dy, double ax, double ay, double az,
-                                             double cb_anchor, double isb0, double isb1,
-                                             double isb2, int slot, gg_sat_view& sv, char* smem) {
+                                             const gg_dyn* dy, const double* anchor,
+                                             const double* cb_anchor, const double* isb_anchor,
+                                             int slot, gg_sat_view& sv, char* smem) {
   const double* sd = a.satd + (int64_t)slot * 5 * a.sat_rows;
   const int32_t* si = a.sati + (int64_t)slot * (a.sat_rows + a.n_filters);
   const int ns = si[a.sat_rows + f];
@@ -298,17 +306,18 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   sv.c = sv.irho + cap;
   sv.w = sv.c + cap;
   sv.isb = (int*)(sv.w + cap);
+  sv.cs = (float*)(sv.isb + cap);
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
     const int64_t r = F.sat_off + k;
-    double dx[3] = {sd[r * 3 + 0] - ax, sd[r * 3 + 1] - ay, sd[r * 3 + 2] - az};
+    double dx[3] = {sd[r * 3 + 0] - anchor[0], sd[r * 3 + 1] - anchor[1], sd[r * 3 + 2] - anchor[2]};
     double d[3];
     gg_rot_local(dy->R, dx, d);
     const double rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const double inv_rho = 1.0 / rho;
     const int sys = si[r];
     const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
-    double c = sd[3 * a.sat_rows + r] - rho - cb_anchor;
-    if (ib >= 0) c -= (ib == 0) ? isb0 : ((ib == 1) ? isb1 : isb2);
+    double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
+    if (ib >= 0) c -= isb_anchor[ib];
     sv.ux[k] = (float)(d[0] * inv_rho);
     sv.uy[k] = (float)(d[1] * inv_rho);
     sv.uz[k] = (float)(d[2] * inv_rho);
@@ -480,6 +489,70 @@ __device__ __forceinline__ void gg_loglik4(const gg_sat_view& sv, int ns,
 #pragma unroll
     for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
   }
+}
+
+// Learned likelihood, per-satellite part: features (elevation and azimuth
+// from the staged unit vector, C/N0 and line-of-sight flag of the epoch slot,
+// satellite count, ESS fraction of the last decision) and the first-layer
+// bias c_s = b1 + W1[:, 1:] f of every satellite, into sv.cs.
+__device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_filter& F,
+                                                 const gg_dyn* dy, int slot, int ns,
+                                                 const gg_mlp_w& w, gg_sat_view& sv) {
+  const float* sf = a.satf + (int64_t)slot * 2 * a.sat_rows;
+  const float essf = (F.n > 0) ? (float)(dy->ess / (double)F.n) : 1.0f;
+  for (int i = threadIdx.x; i < ns * GG_MLP_H; i += blockDim.x) {
+    const int k = i / GG_MLP_H;
+    const int u = i % GG_MLP_H;
+    const float ux = sv.ux[k], uy = sv.uy[k], uz = sv.uz[k];
+    const float ch = sqrtf(ux * ux + uy * uy);
+    float f[GG_MLP_NF - 1];
+    f[0] = asinf(fminf(fmaxf(uz, -1.0f), 1.0f)) * 0.63661977236758134f;  // el / (pi/2)
+    f[1] = (ch > 0.0f) ? ux / ch : 0.0f;                                  // sin az
+    f[2] = (ch > 0.0f) ? uy / ch : 1.0f;                                  // cos az
+    f[3] = (sf[F.sat_off + k] - 35.0f) * 0.1f;
+    f[4] = sf[a.sat_rows + F.sat_off + k];
+    f[5] = (float)ns * 0.05f;
+    f[6] = essf;
+    sv.cs[k * GG_MLP_H + u] = gg_mlp_c(w, u, f);
+  }
+  __syncthreads();
+}
+
+// Learned likelihood of the workgroup's tile: for every satellite the 1024
+// residuals go through LDS to the matrix-core layout (16 particles per pass
+// and wavefront), the pair log-likelihoods are summed per particle in LDS.
+// se and slw: GG_TILE floats of LDS each.
+template <int NISB>
+__device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
+                                                  const float (&p)[GG_ITEMS][4 + NISB],
+                                                  const gg_mlp_w& w, float* se, float* slw,
+                                                  float (&out)[GG_ITEMS]) {
+  const int t = threadIdx.x;
+  const int l = t & 63;
+  const int base = (t >> 6) * 256;
+  float pp[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
+    slw[t * GG_ITEMS + q] = 0.0f;
+  }
+  for (int k = 0; k < ns; ++k) {
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) se[t * GG_ITEMS + q] = gg_residual<NISB>(sv, k, p[q], pp[q]);
+    __syncthreads();
+    gg_bf16x8 a1[4];
+    gg_mlp_a1(w.w_e, sv.cs + k * GG_MLP_H, a1);
+    for (int cb = 0; cb < 16; ++cb) {
+      const int i = base + 16 * cb + (l & 15);
+      const float e = se[i];
+      gg_mlp_act act;
+      gg_mlp_fwd16(w, a1, gg_mlp_xe(e), act);
+      if (l < 16) slw[i] += gg_mlp_ll(e, act.z30, act.z31);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) out[q] = slw[t * GG_ITEMS + q];
 }
 
 // ---------------------------------------------------------------------------

@@ -108,11 +108,15 @@ __global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
 // ---------------------------------------------------------------------------
 // predict (optional) + likelihood + per-tile partials.
 // ---------------------------------------------------------------------------
-template <int NISB, bool PRED>
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args s) {
+// predict (PRED) + likelihood (learned when LEARN) + tile partials.  The step
+// frame is computed once per workgroup into LDS, so its binary64 values do not
+// occupy registers for the whole kernel.
+template <int NISB, bool PRED, bool LEARN>
+__device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_args& s,
+                                               char* smem) {
   constexpr int D = 4 + NISB;
-  extern __shared__ __align__(16) char smem[];
   __shared__ double lds[8 + GG_BLOCK];
+  __shared__ gg_frame sfr;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
@@ -122,37 +126,68 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight(gg_arena a, gg_step_args
   const int epoch = dy->epoch;
   const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
   const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
-  gg_frame fr;
-  if (PRED) {
-    const double* vg = gg_guide_ptr(a, s, f, epoch);
-    double vge[3] = {vg[0], vg[1], vg[2]};
-    gg_step_frame<D>(dy, s, vge, fr);
-  } else {
-    for (int k = 0; k < 3; ++k) fr.anchor[k] = dy->anchor[k];
-    fr.cb_anchor = dy->cb_anchor;
-    for (int k = 0; k < GG_MAX_ISB; ++k) fr.isb_anchor[k] = dy->isb_anchor[k];
-  }
   const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
   const int64_t base = F.p_off + i0;
   float p[GG_ITEMS][D];
   gg_load4<D>(a, cur, base, p);
+  if (threadIdx.x == 0) {
+    if (PRED) {
+      const double* vg = gg_guide_ptr(a, s, f, epoch);
+      double vge[3] = {vg[0], vg[1], vg[2]};
+      gg_step_frame<D>(dy, s, vge, sfr);
+    } else {
+      for (int k = 0; k < 3; ++k) sfr.anchor[k] = dy->anchor[k];
+      sfr.cb_anchor = dy->cb_anchor;
+      for (int k = 0; k < GG_MAX_ISB; ++k) sfr.isb_anchor[k] = dy->isb_anchor[k];
+    }
+  }
+  __syncthreads();
   gg_sat_view sv;
-  const int ns = gg_stage_sats(a, f, F, dy, fr.anchor[0], fr.anchor[1], fr.anchor[2], fr.cb_anchor,
-                               fr.isb_anchor[0], fr.isb_anchor[1], fr.isb_anchor[2], slot, sv, smem);
+  const int ns = gg_stage_sats(a, f, F, dy, sfr.anchor, &sfr.cb_anchor, sfr.isb_anchor, slot, sv,
+                               smem);
   bool valid[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
   if (PRED) {
-    gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, fr.pl, p);
-    gg_apply_shift<D>(fr, p);
+    gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, sfr.pl, p);
+    gg_apply_shift<D>(sfr, p);
     gg_store4<D>(a, cur, base, p);
   }
   float lw[GG_ITEMS];
-  gg_loglik4<NISB>(sv, ns, p, s, lw);
+  if constexpr (LEARN) {
+    __shared__ gg_mlp_w smlp;
+    __shared__ float se[GG_TILE];
+    __shared__ float slw[GG_TILE];
+    {
+      const uint4* src = reinterpret_cast<const uint4*>(a.mlp);
+      uint4* dst = reinterpret_cast<uint4*>(&smlp);
+      for (int i = threadIdx.x; i < (int)(sizeof(gg_mlp_w) / 16); i += GG_BLOCK) dst[i] = src[i];
+    }
+    __syncthreads();
+    gg_stage_learned(a, F, dy, slot, ns, smlp, sv);
+    gg_loglik_learned<NISB>(sv, ns, p, smlp, se, slw, lw);
+  } else {
+    gg_loglik4<NISB>(sv, ns, p, s, lw);
+  }
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? lw[q] : 0.0f;
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
   gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
+}
+
+// One round of workgroups covers 1M particles only at 4 waves per SIMD, so the
+// register budget of the plain likelihood is held to 128 VGPRs.
+template <int NISB, bool PRED>
+__global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void gg_k_weight(gg_arena a, gg_step_args s) {
+  extern __shared__ __align__(16) char smem[];
+  gg_weight_body<NISB, PRED, false>(a, s, smem);
+}
+
+template <int NISB, bool PRED>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_learned(gg_arena a, gg_step_args s) {
+  extern __shared__ __align__(16) char smem[];
+  gg_weight_body<NISB, PRED, true>(a, s, smem);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,9 @@ Additions (keyword-only, reference behaviour by default):
   megopolis_iterations   iterations per Megopolis resample (reference: 15)
   systematic_u0          "reference" (u0 = (seed % 1e6) / (1e6 N), identical
                          every call) or "fresh" (a new Philox u0 in [0,1))
+  set_learned_likelihood a gnss_gpu_amd.learned.LearnedLikelihood; updates
+                         given ``cn0=`` and ``los_flag=`` then use it in place
+                         of the Gaussian / Student-t term
 """
 
 from __future__ import annotations
@@ -152,9 +155,17 @@ class ParticleFilterDevice:
                                self.seed, self._step, sv, alpha, use_kf, qv,
                                sigma_isb=self.sigma_isb)
 
+    def set_learned_likelihood(self, model):
+        """Attach a LearnedLikelihood (None detaches).  The model must outlive
+        the updates that use it."""
+        self._learned = model
+        _dev.pf_device_set_mlp(self._state, model)
+
     def update(self, sat_ecef, pseudoranges, weights=None, sigma_pr=None, resample=True,
-               *, system_ids=None):
-        """Pseudorange weight update (overwrites the log-weights, like the reference)."""
+               *, system_ids=None, cn0=None, los_flag=None):
+        """Pseudorange weight update (overwrites the log-weights, like the reference).
+        With a learned likelihood attached, ``cn0`` (dB-Hz) and ``los_flag``
+        (line of sight at the particle mean) per satellite select it."""
         self._require_init()
         sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
         pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
@@ -163,6 +174,16 @@ class ParticleFilterDevice:
             weights = np.ones(n_sat, dtype=np.float64)
         else:
             weights = np.asarray(weights, dtype=np.float64).ravel()
+        if cn0 is not None or los_flag is not None:
+            if getattr(self, "_learned", None) is None:
+                raise RuntimeError("cn0/los_flag given but no learned likelihood is attached")
+            if cn0 is None or los_flag is None:
+                raise ValueError("the learned likelihood needs both cn0 and los_flag")
+            _dev.pf_device_weight_learned(self._state, sat.ravel(), pr, weights, n_sat, cn0,
+                                          los_flag, system_ids=system_ids)
+            if resample:
+                _ = self.resample_if_needed()
+            return
         sp = float(self.sigma_pr if sigma_pr is None else sigma_pr)
         _dev.pf_device_weight(
             self._state, sat.ravel(), pr, weights, n_sat, sp, float(self.nu),

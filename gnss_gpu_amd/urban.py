@@ -160,3 +160,49 @@ def training_set(rng, n_epochs, n_sat=16, n_canyons=None, mean_err=3.0):
     r0 = np.concatenate(rs)[:n_epochs]
     u = np.concatenate(us)[:n_epochs]
     return f, r0, u, np.full(n_epochs, n_sat, dtype=np.int32)
+
+
+@dataclass
+class Drive:
+    """A canyon drive in ECEF for the particle filter."""
+    canyon: Canyon
+    anchor: np.ndarray     # ECEF origin of the local frame
+    R: np.ndarray          # ECEF -> ENU rotation
+    truth_enu: np.ndarray  # [T, 3]
+    truth: np.ndarray      # [T, 3] ECEF
+    vel: np.ndarray        # [T, 3] ECEF velocity (truth[t] - truth[t-1])
+    sat: np.ndarray        # [T, S, 3] ECEF
+    pr: np.ndarray         # [T, S]
+    cn0: np.ndarray        # [T, S]
+    u: np.ndarray          # [T, S, 3] ENU unit vectors to the satellites
+    nlos: np.ndarray       # [T, S]
+    cb: float
+
+    def los_flags(self, est_ecef, t):
+        """Line-of-sight flags of epoch t's satellites seen from an ECEF point."""
+        e = self.R @ (np.asarray(est_ecef, dtype=np.float64) - self.anchor)
+        return (~blocked(self.canyon, e[None, :], self.u[t])).astype(np.float32)
+
+
+def drive_ecef(rng, n_epochs, n_sat=16, anchor=(-3957199.0, 3310205.0, 3737911.0), cb=30.0,
+               speed=5.0):
+    """A straight drive along a fresh canyon; satellites are placed 2.2e7 m from
+    the receiver along the simulated lines of sight."""
+    from .sim import enu_rotation
+
+    can, se = sat_epochs(rng, n_epochs, n_sat, speed=speed)
+    anchor = np.asarray(anchor, dtype=np.float64)
+    R = enu_rotation(anchor)
+    # keep the drive continuous: no wrap inside one drive
+    truth_enu = se.truth.copy()
+    truth_enu[:, 0] = -0.5 * speed * n_epochs + speed * np.arange(n_epochs)
+    truth = anchor[None, :] + truth_enu @ R
+    vel = np.vstack([np.zeros(3), np.diff(truth, axis=0)])
+    nl = blocked(can, truth_enu[:, None, :], se.u)
+    bias = np.where(nl, nlos_excess(can, truth_enu[:, None, :], se.el, se.az, rng), 0.0)
+    cn0 = rng.uniform(20.0, 50.0, nl.shape) - 8.0 * nl
+    sat = truth[:, None, :] + 2.2e7 * (se.u @ R)
+    rng_true = np.linalg.norm(sat - truth[:, None, :], axis=2)
+    pr = rng_true + cb + bias + se.noise
+    return Drive(can, anchor, R, truth_enu, truth, vel, sat, pr, cn0.astype(np.float32), se.u,
+                 nl, cb)

@@ -193,6 +193,20 @@ int gg_mlp_train(gg_mlp* h, int32_t n_steps, int32_t batch, float lr, uint64_t s
  * given candidates cand[n_epochs][64][4] (dx dy dz dcb); loss[n_epochs]. */
 int gg_mlp_loss_grad(gg_mlp* h, int32_t n_epochs, const float* cand, float* grad, float* loss);
 
+/* Attach a learned likelihood to a particle filter (NULL detaches).  The gg_mlp
+ * handle must outlive every weight launch that uses it; steps select it with
+ * gg_step_args.lik = GG_LIK_LEARNED. */
+int gg_pf_set_mlp(gg_pf* h, gg_mlp* m);
+/* Per-row satellite inputs of the learned likelihood for n_slots epoch slots:
+ * cn0[n_slots][rows] (dB-Hz) and los_flag[n_slots][rows] (1 = line of sight at
+ * the particle mean), rows as in gg_pfb_load_epochs. */
+int gg_pfb_load_features(gg_pf* h, int32_t n_slots, const float* cn0, const float* los_flag);
+/* pf_device_weight (pf_device.cu:1516-1574) with the learned likelihood in
+ * place of the Gaussian / Student-t term; cn0[n_sat], los_flag[n_sat]. */
+int gg_pf_device_weight_learned(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                                const double* weights_sat, const int32_t* sys, int32_t n_sat,
+                                const float* cn0, const float* los_flag);
+
 /* --- test hook (no reference counterpart) ----------------------------------- */
 /* Evaluates a deterministic device building block for n inputs so tests can
  * compare it bit for bit with the oracle: op 0 exp64, 1 exp32, 2 Philox words

@@ -42,6 +42,7 @@
 #define GG_LIK_GAUSS 0
 #define GG_LIK_STUDENT 1
 #define GG_LIK_HUBER 2
+#define GG_LIK_LEARNED 3  // per-pair MLP (gg_mlp), needs satellite C/N0 and LOS flags
 
 struct gg_filter {
   int64_t p_off;     // first particle slot in the arena
