@@ -44,6 +44,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-likelihood leg")
+    ap.add_argument("--learned-particles", type=int, default=1_000_000)
+    ap.add_argument("--learned-epochs", type=int, default=120)
+    ap.add_argument("--learned-steps", type=int, default=100)
+    ap.add_argument("--learned-train-epochs", type=int, default=60000)
+    ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -104,6 +110,98 @@ def cpu_steps(sc, init, N, S, T, threads, seconds, max_steps):
     used = int(lib.cpu_pf_threads())
     lib.cpu_pf_destroy(h)
     return float(np.median(times)), len(times), used
+
+
+def learned_leg(args):
+    """Learned vs plain (Cauchy) filter on a canyon drive, 1M particles x 16
+    satellites: P50 horizontal error of both (drop-in API, host-driven epochs,
+    line-of-sight flags from the previous estimate), then device ms per step of
+    both with graph replays of the same epochs, and the learned weight kernel's
+    matrix-core rate on its layer-2 contraction."""
+    import torch
+
+    from gnss_gpu_amd import ParticleFilterDevice, urban
+    from gnss_gpu_amd._lib import LIK_LEARNED, LIK_STUDENT
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    N, S, T = args.learned_particles, 16, args.learned_epochs
+    rng = np.random.default_rng(2026)
+    f, r0, u, ns = urban.training_set(rng, args.learned_train_epochs, n_sat=S)
+    model = LearnedLikelihood(seed=1)
+    model.load_data(f, r0, u, ns)
+    t0 = time.perf_counter()
+    loss = model.train(args.learned_train_steps, batch=256, lr=2e-3, seed=7)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    drive = urban.drive_ecef(np.random.default_rng(77), T, n_sat=S)
+
+    def run(kind):
+        pf = ParticleFilterDevice(n_particles=N, sigma_pos=1.0, sigma_cb=1.0, sigma_pr=5.0,
+                                  nu=1.0, seed=12345)
+        if kind == "learned":
+            pf.set_learned_likelihood(model)
+        pf.initialize(drive.truth[0], clock_bias=drive.cb, spread_pos=5.0, spread_cb=5.0)
+        est = drive.truth[0].copy()
+        errs, flags = [], np.zeros((T, S), np.float32)
+        for t in range(1, T):
+            pf.predict(velocity=drive.vel[t], dt=1.0)
+            flags[t] = drive.los_flags(est, t)
+            if kind == "learned":
+                pf.update(drive.sat[t], drive.pr[t], cn0=drive.cn0[t], los_flag=flags[t])
+            else:
+                pf.update(drive.sat[t], drive.pr[t])
+            est = pf.estimate()[:3]
+            errs.append(float(np.linalg.norm((drive.R @ (est - drive.truth[t]))[:2])))
+        return float(np.median(errs[10:])), flags
+
+    p50_plain, _ = run("plain")
+    p50_learned, flags = run("learned")
+
+    def timed(lik):
+        bf = BatchedParticleFilter([N], sat_cap=S, n_slots=T)
+        bf.load_epochs([[drive.sat[t]] for t in range(T)], [[drive.pr[t]] for t in range(T)],
+                       [[S]] * T, vel=drive.vel[:, None, :])
+        if lik == LIK_LEARNED:
+            bf.set_learned(model)
+            bf.load_features([[drive.cn0[t]] for t in range(T)], [[flags[t]] for t in range(T)])
+        bf.initialize(np.concatenate([drive.truth[0] - drive.vel[1], [drive.cb]])[None],
+                      [12345], spread_pos=5.0, spread_cb=5.0)
+        st = bf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_pr=5.0, nu=1.0, lik=lik, n_slots=T,
+                          ess_frac=0.5)
+        G = 10
+        bf.capture(st, "systematic", 32, steps=G)
+        bf.replay(2)
+        bf.sync()
+        reps = max(1, args.learned_steps // G)
+        ms = bf.time_replays_each(reps)
+        res, oth, cnt = bf.profile_steps(st, 20)
+        wk = (res["predict_weight"] * cnt[0] + oth["predict_weight"] * cnt[1]) / max(sum(cnt), 1)
+        bf.close()
+        return float(ms.sum()) / (reps * G), wk
+
+    ms_plain, wk_plain = timed(LIK_STUDENT)
+    ms_learned, wk_learned = timed(LIK_LEARNED)
+    flops_l2 = 2.0 * 64 * 64 * N * S            # the dense 64x64 contraction per pair
+    flops_all = 2.0 * (2 * 64 + 64 * 64 + 64 * 2) * N * S
+    return {
+        "workload": f"canyon_{N // 1000000}Mx{S}_learned_vs_cauchy",
+        "particles": N, "satellites": S, "epochs": T,
+        "nlos_fraction": round(float(drive.nlos.mean()), 3),
+        "p50_err_plain_m": round(p50_plain, 3), "p50_err_learned_m": round(p50_learned, 3),
+        "ms_per_step_plain": round(ms_plain, 5), "ms_per_step_learned": round(ms_learned, 5),
+        "step_time_ratio": round(ms_learned / ms_plain, 3),
+        "weight_kernel_ms_plain": round(wk_plain, 5),
+        "weight_kernel_ms_learned": round(wk_learned, 5),
+        "mfma": {"dtype": "bf16", "layer2_tflops": round(flops_l2 / (wk_learned * 1e-3) / 1e12, 1),
+                 "all_layers_tflops": round(flops_all / (wk_learned * 1e-3) / 1e12, 1),
+                 "peak_tflops": 2500.0,
+                 "frac_all_layers": round(flops_all / (wk_learned * 1e-3) / 2.5e15, 4)},
+        "training": {"epochs": len(ns), "steps": args.learned_train_steps, "batch": 256,
+                     "candidates": 64, "seconds": round(train_s, 2),
+                     "loss_first": round(float(loss[:10].mean()), 4),
+                     "loss_last": round(float(loss[-50:].mean()), 4)},
+    }
 
 
 def main():
@@ -219,6 +317,8 @@ def main():
                          f"C restatement of the reference step (oracle/cpu_pf.c, gcc -O3 "
                          f"-march=x86-64-v3 -fopenmp); CPU {cpu_info()}"}
 
+    learned = None if args.no_learned else learned_leg(args)
+
     out = {
         "metric": METRIC,
         "value": round(ms_step, 5),
@@ -247,6 +347,7 @@ def main():
         "resample_kernel_gbs": None if resample_gbs is None else round(resample_gbs, 1),
         "p50_err_m": None if p50 is None else round(p50, 3),
         "roofline": roofline,
+        "learned": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
