@@ -37,6 +37,8 @@ def compile_flags(arch: str) -> list[str]:
         "-I" + CSRC,
         # binary32 packed math is kept off (compiler-formed v_pk_* only)
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # matrix-core results straight into VGPRs (no accvgpr copies)
+        "-mllvm", "-amdgpu-mfma-vgpr-form",
         "-Wno-unused-command-line-argument",
     ]
 

@@ -90,9 +90,11 @@ __global__ __launch_bounds__(256) void gg_k_mlp_eval(const gg_mlp_w* __restrict_
   if (!live) return;
   gg_bf16x8 a1[4];
   gg_mlp_a1(w->w_e, sc[wave], a1);
+  gg_mlp_regs wr;
+  gg_mlp_load_regs(*w, wr);
   const float ev = (l < 16) ? e[(int64_t)b * 16 + l] : 0.0f;
   gg_mlp_act act;
-  gg_mlp_fwd16(*w, a1, gg_mlp_xe(ev), act);
+  gg_mlp_fwd16(wr, a1, gg_mlp_xe(ev), act);
   if (l < 16) {
     ll[(int64_t)b * 16 + l] = gg_mlp_ll(ev, act.z30, act.z31);
     z3[((int64_t)b * 16 + l) * 2 + 0] = act.z30;
@@ -178,6 +180,8 @@ __global__ __launch_bounds__(256) void gg_k_mlp_train(
   for (int i = t; i < ns * GG_MLP_H; i += 256)
     sc[i / GG_MLP_H][i % GG_MLP_H] = gg_mlp_c(*w, i % GG_MLP_H, sf[i / GG_MLP_H]);
   __syncthreads();
+  gg_mlp_regs wr;
+  gg_mlp_load_regs(*w, wr);
   // ---- forward
   for (int s = wave; s < ns; s += 4) {
     gg_bf16x8 a1[4];
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(256) void gg_k_mlp_train(
       const int k = 16 * cb + (l & 15);
       const float e = gg_cand_residual(sr0[s], su[s], scand[k]);
       gg_mlp_act act;
-      gg_mlp_fwd16(*w, a1, gg_mlp_xe(e), act);
+      gg_mlp_fwd16(wr, a1, gg_mlp_xe(e), act);
       if (l < 16) sll[s][k] = gg_mlp_ll(e, act.z30, act.z31);
     }
   }
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(256) void gg_k_mlp_train(
       const float e = gg_cand_residual(sr0[s], su[s], scand[k]);
       const float xe = gg_mlp_xe(e);
       gg_mlp_act act;
-      gg_mlp_fwd16(*w, a1, xe, act);
+      gg_mlp_fwd16(wr, a1, xe, act);
       float dz30 = 0.0f, dz31 = 0.0f;
       if (l < 16) {
         const float gl = sdL[k];

@@ -81,13 +81,40 @@ __device__ __forceinline__ float gg_mlp_xe(float e) {
   return fminf(fmaxf(e * 0.1f, -GG_MLP_X_CLAMP), GG_MLP_X_CLAMP);
 }
 
+typedef short gg_i16x2 __attribute__((ext_vector_type(2)));
+typedef float gg_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 gg_bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int gg_w32x4 __attribute__((ext_vector_type(4)));
+
+// Two binary32 values to a packed bf16 pair (round to nearest even), low half
+// from a: one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t gg_cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((gg_f32x2){a, b}, gg_bf16x2));
+}
+
+// ReLU of a packed bf16 pair: a signed 16-bit max with 0 on the bit patterns
+// (negative values have the sign bit set).  Rounding and ReLU commute, so
+// relu16(cvt(x)) = bf16(max(x, 0)).
+__device__ __forceinline__ uint32_t gg_relu_pk_bf16(uint32_t v) {
+  const gg_i16x2 zero = {0, 0};
+  return __builtin_bit_cast(uint32_t,
+                            __builtin_elementwise_max(__builtin_bit_cast(gg_i16x2, v), zero));
+}
+
 // ReLU of a lane's 16 units (C layout, c[mb][j]) packed as the two B
 // fragments of the next layer.
 __device__ __forceinline__ void gg_mlp_relu_pack(const gg_f32x4 (&c)[4], gg_bf16x8 (&b)[2]) {
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-    for (int s = 0; s < 8; ++s) b[ks][s] = (__bf16)fmaxf(c[2 * ks + (s >> 2)][s & 3], 0.0f);
+  for (int ks = 0; ks < 2; ++ks) {
+    const gg_f32x4 lo = c[2 * ks];
+    const gg_f32x4 hi = c[2 * ks + 1];
+    gg_w32x4 v;
+    v[0] = gg_relu_pk_bf16(gg_cvt_pk_bf16(lo[0], lo[1]));
+    v[1] = gg_relu_pk_bf16(gg_cvt_pk_bf16(lo[2], lo[3]));
+    v[2] = gg_relu_pk_bf16(gg_cvt_pk_bf16(hi[0], hi[1]));
+    v[3] = gg_relu_pk_bf16(gg_cvt_pk_bf16(hi[2], hi[3]));
+    b[ks] = __builtin_bit_cast(gg_bf16x8, v);
+  }
 }
 
 // One wavefront, 16 pairs (pair = lane % 16).  xe: the pair's scaled residual
@@ -102,8 +129,32 @@ struct gg_mlp_act {
   float z30, z31;
 };
 
-__device__ __forceinline__ void gg_mlp_fwd16(const gg_mlp_w& w, const gg_bf16x8 (&a1)[4], float xe,
-                                             gg_mlp_act& act) {
+// A lane's share of the layer-2/3 weights, held in registers across passes.
+struct gg_mlp_regs {
+  gg_bf16x8 A2[4][2];
+  gg_bf16x8 A3[2];
+  gg_f32x4 b2[4];  // C-layout bias of the lane's units 16 mb + 4 g + j
+  float b30, b31;
+};
+
+__device__ __forceinline__ void gg_mlp_load_regs(const gg_mlp_w& w, gg_mlp_regs& r) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    r.A2[mb][0] = w.A2[mb][0][l];
+    r.A2[mb][1] = w.A2[mb][1][l];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r.b2[mb][j] = w.b2[16 * mb + 4 * g + j];
+  }
+  r.A3[0] = w.A3[0][l];
+  r.A3[1] = w.A3[1][l];
+  r.b30 = (g == 0) ? w.b3[0] : 0.0f;
+  r.b31 = (g == 0) ? w.b3[1] : 0.0f;
+}
+
+__device__ __forceinline__ void gg_mlp_fwd16(const gg_mlp_regs& w, const gg_bf16x8 (&a1)[4],
+                                             float xe, gg_mlp_act& act) {
   const int l = threadIdx.x & 63;
   const int g = l >> 4;
   gg_bf16x8 b1;
@@ -119,21 +170,13 @@ __device__ __forceinline__ void gg_mlp_fwd16(const gg_mlp_w& w, const gg_bf16x8 
   gg_mlp_relu_pack(act.z1, act.h1);
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
-    gg_f32x4 c;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) c[j] = w.b2[16 * mb + 4 * g + j];
-    c = gg_mfma(w.A2[mb][0][l], act.h1[0], c);
-    c = gg_mfma(w.A2[mb][1][l], act.h1[1], c);
-    act.z2[mb] = c;
+    gg_f32x4 c = gg_mfma(w.A2[mb][0], act.h1[0], w.b2[mb]);
+    act.z2[mb] = gg_mfma(w.A2[mb][1], act.h1[1], c);
   }
   gg_mlp_relu_pack(act.z2, act.h2);
-  gg_f32x4 c3 = zero;
-  if (g == 0) {
-    c3[0] = w.b3[0];
-    c3[1] = w.b3[1];
-  }
-  c3 = gg_mfma(w.A3[0][l], act.h2[0], c3);
-  c3 = gg_mfma(w.A3[1][l], act.h2[1], c3);
+  gg_f32x4 c3 = {w.b30, w.b31, 0.0f, 0.0f};
+  c3 = gg_mfma(w.A3[0], act.h2[0], c3);
+  c3 = gg_mfma(w.A3[1], act.h2[1], c3);
   act.z30 = c3[0];
   act.z31 = c3[1];
 }

@@ -536,6 +536,8 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
     pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
     slw[t * GG_ITEMS + q] = 0.0f;
   }
+  gg_mlp_regs wr;
+  gg_mlp_load_regs(w, wr);
   for (int k = 0; k < ns; ++k) {
 #pragma unroll
     for (int q = 0; q < GG_ITEMS; ++q) se[t * GG_ITEMS + q] = gg_residual<NISB>(sv, k, p[q], pp[q]);
@@ -546,7 +548,7 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
       const int i = base + 16 * cb + (l & 15);
       const float e = se[i];
       gg_mlp_act act;
-      gg_mlp_fwd16(w, a1, gg_mlp_xe(e), act);
+      gg_mlp_fwd16(wr, a1, gg_mlp_xe(e), act);
       if (l < 16) slw[i] += gg_mlp_ll(e, act.z30, act.z31);
     }
     __syncthreads();
