@@ -158,7 +158,7 @@ def learned_leg(args):
     p50_plain, _ = run("plain")
     p50_learned, flags = run("learned")
 
-    def timed(lik):
+    def timed(lik, mode=0):
         bf = BatchedParticleFilter([N], sat_cap=S, n_slots=T)
         bf.load_epochs([[drive.sat[t]] for t in range(T)], [[drive.pr[t]] for t in range(T)],
                        [[S]] * T, vel=drive.vel[:, None, :])
@@ -168,7 +168,7 @@ def learned_leg(args):
         bf.initialize(np.concatenate([drive.truth[0] - drive.vel[1], [drive.cb]])[None],
                       [12345], spread_pos=5.0, spread_cb=5.0)
         st = bf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_pr=5.0, nu=1.0, lik=lik, n_slots=T,
-                          ess_frac=0.5)
+                          ess_frac=0.5, learned_mode=mode)
         G = 10
         bf.capture(st, "systematic", 32, steps=G)
         bf.replay(2)
@@ -181,7 +181,8 @@ def learned_leg(args):
         return float(ms.sum()) / (reps * G), wk
 
     ms_plain, wk_plain = timed(LIK_STUDENT)
-    ms_learned, wk_learned = timed(LIK_LEARNED)
+    ms_learned, wk_learned = timed(LIK_LEARNED, 0)
+    ms_table, wk_table = timed(LIK_LEARNED, 1)
     flops_l2 = 2.0 * 64 * 64 * N * S            # the dense 64x64 contraction per pair
     flops_all = 2.0 * (2 * 64 + 64 * 64 + 64 * 2) * N * S
     return {
@@ -191,8 +192,13 @@ def learned_leg(args):
         "p50_err_plain_m": round(p50_plain, 3), "p50_err_learned_m": round(p50_learned, 3),
         "ms_per_step_plain": round(ms_plain, 5), "ms_per_step_learned": round(ms_learned, 5),
         "step_time_ratio": round(ms_learned / ms_plain, 3),
+        "ms_per_step_learned_code_tables": round(ms_table, 5),
+        "step_time_ratio_code_tables": round(ms_table / ms_plain, 3),
         "weight_kernel_ms_plain": round(wk_plain, 5),
         "weight_kernel_ms_learned": round(wk_learned, 5),
+        "weight_ms_learned_code_tables": round(wk_table, 5),
+        "code_table_evals_per_step": 16 * 33602,
+        "per_pair_evals_per_step": N * S,
         "mfma": {"dtype": "bf16", "layer2_tflops": round(flops_l2 / (wk_learned * 1e-3) / 1e12, 1),
                  "all_layers_tflops": round(flops_all / (wk_learned * 1e-3) / 1e12, 1),
                  "peak_tflops": 2500.0,

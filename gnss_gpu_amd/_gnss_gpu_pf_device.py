@@ -104,7 +104,7 @@ def pf_device_weight(state, sat_ecef, pseudoranges, weights_sat, n_sat, sigma_pr
 
 
 def pf_device_weight_learned(state, sat_ecef, pseudoranges, weights_sat, n_sat, cn0, los_flag,
-                             system_ids=None):
+                             system_ids=None, table=False):
     """Weight update with the attached learned likelihood (no reference
     counterpart; replaces the Student-t term of pf_device.cu:262-266)."""
     n_sat = int(n_sat)
@@ -119,7 +119,7 @@ def pf_device_weight_learned(state, sat_ecef, pseudoranges, weights_sat, n_sat, 
     F = C.POINTER(C.c_float)
     call("gg_pf_device_weight_learned", state.handle, dptr(sat), dptr(pr), dptr(w),
          iptr(sys) if sys is not None else None, n_sat, c.ctypes.data_as(F),
-         fl.ctypes.data_as(F))
+         fl.ctypes.data_as(F), 1 if table else 0)
 
 
 def pf_device_set_mlp(state, model):

@@ -55,7 +55,7 @@ class StepArgs(C.Structure):
         ("auto_flag", C.c_int32),
         ("u0_mode", C.c_int32),
         ("recenter", C.c_int32),
-        ("pad1", C.c_int32),
+        ("learned_mode", C.c_int32),
         ("u0_value", C.c_double),
     ]
 
@@ -113,7 +113,7 @@ _SIGS = {
     "gg_mlp_loss_grad": [_P, _I, _FP, _FP, _FP],
     "gg_pf_set_mlp": [_P, _P],
     "gg_pfb_load_features": [_P, _I, _FP, _FP],
-    "gg_pf_device_weight_learned": [_P, _DP, _DP, _DP, _IP, _I, _FP, _FP],
+    "gg_pf_device_weight_learned": [_P, _DP, _DP, _DP, _IP, _I, _FP, _FP, _I],
     "gg_abi_version": [],
     "gg_struct_sizes": [_IP, _IP, _IP],
     "gg_device_count": [_IP],

@@ -38,6 +38,9 @@
 #define GG_MLP_LV_MAX (12.0f)
 #define GG_MLP_MU_SCALE 10.0f
 #define GG_MLP_X_CLAMP 20.0f
+// bf16 codes of |x| <= 20 (0x0000..0x41A0 and their negatives)
+#define GG_MLP_CODES_HALF 16801
+#define GG_MLP_CODES (2 * GG_MLP_CODES_HALF)
 
 typedef __bf16 gg_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float gg_f32x4 __attribute__((ext_vector_type(4)));
@@ -74,6 +77,34 @@ __device__ __forceinline__ float gg_mlp_c(const gg_mlp_w& w, int u, const float*
 #pragma unroll
   for (int j = 0; j < GG_MLP_NF - 1; ++j) c = c + w.W1[u][1 + j] * f[j];
   return c;
+}
+
+// Satellite features 1..7 from the unit vector to the satellite in the local
+// ENU frame: el / (pi/2), sin az, cos az, (cn0 - 35) / 10, LOS flag,
+// n_sat / 20, ESS / N.
+__device__ __forceinline__ void gg_mlp_features(float ux, float uy, float uz, float cn0, float flag,
+                                                int ns, float essf, float* f) {
+  const float ch = sqrtf(ux * ux + uy * uy);
+  f[0] = asinf(fminf(fmaxf(uz, -1.0f), 1.0f)) * 0.63661977236758134f;
+  f[1] = (ch > 0.0f) ? ux / ch : 0.0f;
+  f[2] = (ch > 0.0f) ? uy / ch : 1.0f;
+  f[3] = (cn0 - 35.0f) * 0.1f;
+  f[4] = flag;
+  f[5] = (float)ns * 0.05f;
+  f[6] = essf;
+}
+
+// Index of the bf16 code of a scaled residual (|x| <= 20) in a code table.
+__device__ __forceinline__ int gg_mlp_code_index(float x) {
+  const uint32_t b = (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x);
+  return (b & 0x8000u) ? GG_MLP_CODES_HALF + (int)(b & 0x7FFFu) : (int)b;
+}
+
+// The binary32 value of table entry idx (exactly representable in bf16).
+__device__ __forceinline__ float gg_mlp_code_value(int idx) {
+  const uint32_t b = (idx < GG_MLP_CODES_HALF) ? (uint32_t)idx
+                                                : (0x8000u | (uint32_t)(idx - GG_MLP_CODES_HALF));
+  return __builtin_bit_cast(float, b << 16);
 }
 
 __device__ __forceinline__ float gg_mlp_xe(float e) {

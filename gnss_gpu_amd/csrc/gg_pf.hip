@@ -46,6 +46,9 @@ struct gg_pf {
   double* vel_guide = nullptr;
   float* satf = nullptr;        // [n_slots][2][rows] C/N0 | LOS flag (learned likelihood)
   const void* mlp_w = nullptr;  // packed network of the attached gg_mlp
+  float2* mlp_table = nullptr;  // [table_rows][GG_MLP_CODES] (learned_mode 1)
+  int32_t* row_filter = nullptr;
+  int table_rows = 0;
   double* io_tmp = nullptr;
   size_t io_tmp_bytes = 0;
   // pinned host staging
@@ -96,6 +99,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.satd = h->satd;
   a.satf = h->satf;
   a.mlp = (const gg_mlp_w*)h->mlp_w;
+  a.mlp_table = h->mlp_table;
   a.sati = h->sati;
   a.vel_guide = h->vel_guide;
   a.P = h->P;
@@ -210,12 +214,46 @@ size_t gather_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(int32_
 size_t decide_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(double); }
 
 // likelihood (+ predict) over all tiles, then the per-filter decision
+// Code tables of the learned likelihood: one row per satellite row of the
+// arena (GG_MLP_CODES float2 each) and the row -> filter map.
+int ensure_table(gg_pf* h) {
+  if (h->table_rows == h->sat_rows && h->mlp_table) return 0;
+  GG_HIP(hipStreamSynchronize(h->stream));
+  if (h->mlp_table) GG_HIP(hipFree(h->mlp_table));
+  if (h->row_filter) GG_HIP(hipFree(h->row_filter));
+  h->mlp_table = nullptr;
+  h->row_filter = nullptr;
+  h->table_rows = 0;
+  if (dev_alloc(&h->mlp_table, (size_t)h->sat_rows * GG_MLP_CODES)) return -1;
+  std::vector<int32_t> rf((size_t)h->sat_rows, 0);
+  for (int f = 0; f < h->n_filters; ++f)
+    for (int k = 0; k < h->filt_h[f].sat_cap; ++k) rf[(size_t)h->filt_h[f].sat_off + k] = f;
+  if (dev_alloc(&h->row_filter, rf.size())) return -1;
+  GG_HIP(hipMemcpy(h->row_filter, rf.data(), sizeof(int32_t) * rf.size(), hipMemcpyHostToDevice));
+  h->table_rows = h->sat_rows;
+  return 0;
+}
+
 int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide = true) {
-  const gg_arena a = arena_of(h);
   const bool learned = s.lik == GG_LIK_LEARNED;
   if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
-  const size_t smem = sat_smem(h, learned);
-  if (learned) {
+  const bool table = learned && s.learned_mode == 1;
+  if (table && ensure_table(h)) return -1;
+  const gg_arena a = arena_of(h);
+  const size_t smem = sat_smem(h, learned && !table);
+  if (table) {
+    const dim3 tgrid((GG_MLP_CODES + 1023) / 1024, h->sat_rows);
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_mlp_table<NI>), tgrid, dim3(GG_BLOCK), 0,
+                                                  h->stream, a, s, pred ? 1 : 0, h->row_filter));
+    GG_HIP(hipGetLastError());
+    if (pred) {
+      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_table<NI, true>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+    } else {
+      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_table<NI, false>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+    }
+  } else if (learned) {
     if (pred) {
       GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, true>), dim3(h->n_tiles),
                                                     dim3(GG_BLOCK), smem, h->stream, a, s));
@@ -493,7 +531,7 @@ int gg_pf_destroy(gg_pf* h) {
   void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->g, h->Pb,
                   h->emax,  h->csum,  h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
-                  h->satf,
+                  h->satf,  h->mlp_table, h->row_filter,
                   h->io_tmp};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -717,12 +755,14 @@ int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudora
 
 int gg_pf_device_weight_learned(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
                                 const double* weights_sat, const int32_t* sys, int32_t n_sat,
-                                const float* cn0, const float* los_flag) {
+                                const float* cn0, const float* los_flag, int32_t learned_mode) {
   if (int rc = check_single(h)) return rc;
   if (!cn0 || !los_flag) return gg_fail(-2, "learned likelihood needs cn0 and los_flag");
   if (stage_single(h, sat_ecef, pseudoranges, weights_sat, sys, n_sat, cn0, los_flag)) return -1;
+  if (learned_mode < 0 || learned_mode > 1) return gg_fail(-2, "learned_mode must be 0 or 1");
   gg_step_args s = default_args();
   s.lik = GG_LIK_LEARNED;
+  s.learned_mode = learned_mode;
   s.auto_flag = 0;
   if (launch_weight(h, s, false)) return -1;
   h->decide_valid = true;
@@ -1113,6 +1153,8 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
     GG_HIP(hipGraphDestroy(h->graph));
     h->graph = nullptr;
   }
+  // allocations cannot happen inside the capture
+  if (args->lik == GG_LIK_LEARNED && args->learned_mode == 1 && ensure_table(h)) return -1;
   GG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter);

@@ -52,6 +52,7 @@ struct gg_arena {
   //   satf + s*2*sat_rows: cn0[sat_rows] | los_flag[sat_rows]   (binary32)
   const gg_mlp_w* mlp;
   const float* satf;
+  float2* mlp_table;  // [sat_rows][GG_MLP_CODES] network outputs per bf16 code
   int64_t P;
   int32_t ncomp;
   int32_t n_isb;
@@ -288,6 +289,17 @@ __device__ __forceinline__ void gg_predict4(const gg_arena& a, int cur, int64_t 
     if (i0 + q < n) gg_predict_one<NISB>(a, cur, base + q, seed, (uint32_t)(i0 + q), step, s, pl, p[q]);
 }
 
+// Satellite row r relative to the frame anchor in the filter's ENU frame
+// (binary64): d, |d| and 1/|d|.
+__device__ __forceinline__ void gg_sat_local(const double* sd, const gg_dyn* dy,
+                                             const double* anchor, int64_t r, double (&d)[3],
+                                             double& rho, double& inv_rho) {
+  double dx[3] = {sd[r * 3 + 0] - anchor[0], sd[r * 3 + 1] - anchor[1], sd[r * 3 + 2] - anchor[2]};
+  gg_rot_local(dy->R, dx, d);
+  rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  inv_rho = 1.0 / rho;
+}
+
 // Stage the filter's satellites for this epoch slot in LDS.
 // anchor[3], cb_anchor, isb_anchor[3]: the step frame's binary64 origins
 // (usually in LDS).
@@ -309,11 +321,8 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   sv.cs = (float*)(sv.isb + cap);
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
     const int64_t r = F.sat_off + k;
-    double dx[3] = {sd[r * 3 + 0] - anchor[0], sd[r * 3 + 1] - anchor[1], sd[r * 3 + 2] - anchor[2]};
-    double d[3];
-    gg_rot_local(dy->R, dx, d);
-    const double rho = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const double inv_rho = 1.0 / rho;
+    double d[3], rho, inv_rho;
+    gg_sat_local(sd, dy, anchor, r, d, rho, inv_rho);
     const int sys = si[r];
     const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
     double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
@@ -503,19 +512,39 @@ __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_fil
   for (int i = threadIdx.x; i < ns * GG_MLP_H; i += blockDim.x) {
     const int k = i / GG_MLP_H;
     const int u = i % GG_MLP_H;
-    const float ux = sv.ux[k], uy = sv.uy[k], uz = sv.uz[k];
-    const float ch = sqrtf(ux * ux + uy * uy);
     float f[GG_MLP_NF - 1];
-    f[0] = asinf(fminf(fmaxf(uz, -1.0f), 1.0f)) * 0.63661977236758134f;  // el / (pi/2)
-    f[1] = (ch > 0.0f) ? ux / ch : 0.0f;                                  // sin az
-    f[2] = (ch > 0.0f) ? uy / ch : 1.0f;                                  // cos az
-    f[3] = (sf[F.sat_off + k] - 35.0f) * 0.1f;
-    f[4] = sf[a.sat_rows + F.sat_off + k];
-    f[5] = (float)ns * 0.05f;
-    f[6] = essf;
+    gg_mlp_features(sv.ux[k], sv.uy[k], sv.uz[k], sf[F.sat_off + k],
+                    sf[a.sat_rows + F.sat_off + k], ns, essf, f);
     sv.cs[k * GG_MLP_H + u] = gg_mlp_c(w, u, f);
   }
   __syncthreads();
+}
+
+// Learned likelihood through the per-satellite code tables (gg_k_mlp_table):
+// the network output for a residual depends only on the bf16 code of its
+// scaled value, so a lookup returns exactly what the per-pair network returns.
+// The per-particle sum runs over the satellites in the same order as
+// gg_loglik_learned, so both give identical log-weights.
+template <int NISB>
+__device__ __forceinline__ void gg_loglik_table(const gg_sat_view& sv, int ns, int64_t row0,
+                                                const float2* __restrict__ tab,
+                                                const float (&p)[GG_ITEMS][4 + NISB],
+                                                float (&out)[GG_ITEMS]) {
+  float pp[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
+    out[q] = 0.0f;
+  }
+  for (int k = 0; k < ns; ++k) {
+    const float2* tk = tab + (row0 + k) * GG_MLP_CODES;
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
+      const float2 z = tk[gg_mlp_code_index(gg_mlp_xe(e))];
+      out[q] += gg_mlp_ll(e, z.x, z.y);
+    }
+  }
 }
 
 // Learned likelihood of the workgroup's tile: for every satellite the 1024

@@ -108,10 +108,12 @@ __global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
 // ---------------------------------------------------------------------------
 // predict (optional) + likelihood + per-tile partials.
 // ---------------------------------------------------------------------------
-// predict (PRED) + likelihood (learned when LEARN) + tile partials.  The step
-// frame is computed once per workgroup into LDS, so its binary64 values do not
-// occupy registers for the whole kernel.
-template <int NISB, bool PRED, bool LEARN>
+// predict (PRED) + likelihood + tile partials.  LEARN: 0 the analytic
+// likelihood of s.lik, 1 the learned network per pair, 2 the learned network
+// through its per-satellite code tables.  The step frame is computed once per
+// workgroup into LDS, so its binary64 values do not occupy registers for the
+// whole kernel.
+template <int NISB, bool PRED, int LEARN>
 __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_args& s,
                                                char* smem) {
   constexpr int D = 4 + NISB;
@@ -154,7 +156,9 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     gg_store4<D>(a, cur, base, p);
   }
   float lw[GG_ITEMS];
-  if constexpr (LEARN) {
+  if constexpr (LEARN == 2) {
+    gg_loglik_table<NISB>(sv, ns, F.sat_off, a.mlp_table, p, lw);
+  } else if constexpr (LEARN == 1) {
     __shared__ gg_mlp_w smlp;
     __shared__ float se[GG_TILE];
     __shared__ float slw[GG_TILE];
@@ -181,13 +185,82 @@ template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void gg_k_weight(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, false>(a, s, smem);
+  gg_weight_body<NISB, PRED, 0>(a, s, smem);
 }
 
 template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_learned(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, true>(a, s, smem);
+  gg_weight_body<NISB, PRED, 1>(a, s, smem);
+}
+
+template <int NISB, bool PRED>
+__global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void gg_k_weight_table(gg_arena a, gg_step_args s) {
+  extern __shared__ __align__(16) char smem[];
+  gg_weight_body<NISB, PRED, 2>(a, s, smem);
+}
+
+// Per-satellite code tables of the learned likelihood: for satellite row
+// blockIdx.y of its filter, the network outputs (z3_0, z3_1) for every bf16
+// code of the scaled residual, 1024 codes per workgroup (16 per wavefront
+// pass), with the features the weight kernel would derive for that satellite
+// in the same step (same frame, same staging arithmetic).
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_mlp_table(gg_arena a, gg_step_args s, int pred,
+                                                           const int32_t* __restrict__ row_filter) {
+  constexpr int D = 4 + NISB;
+  __shared__ gg_frame sfr;
+  __shared__ float sc[GG_MLP_H];
+  __shared__ float su[3];
+  const int row = blockIdx.y;
+  const int f = row_filter[row];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int k = row - F.sat_off;
+  const int epoch = dy->epoch;
+  const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
+  const int32_t* si = a.sati + (int64_t)slot * (a.sat_rows + a.n_filters);
+  const int ns = si[a.sat_rows + f];
+  if (k < 0 || k >= ns) return;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    if (pred) {
+      const double* vg = gg_guide_ptr(a, s, f, epoch);
+      double vge[3] = {vg[0], vg[1], vg[2]};
+      gg_step_frame<D>(dy, s, vge, sfr);
+    } else {
+      for (int q = 0; q < 3; ++q) sfr.anchor[q] = dy->anchor[q];
+    }
+    const double* sd = a.satd + (int64_t)slot * 5 * a.sat_rows;
+    double d[3], rho, inv_rho;
+    gg_sat_local(sd, dy, sfr.anchor, row, d, rho, inv_rho);
+    su[0] = (float)(d[0] * inv_rho);
+    su[1] = (float)(d[1] * inv_rho);
+    su[2] = (float)(d[2] * inv_rho);
+  }
+  __syncthreads();
+  const gg_mlp_w& w = *a.mlp;
+  if (t < GG_MLP_H) {
+    const float* sf = a.satf + (int64_t)slot * 2 * a.sat_rows;
+    const float essf = (F.n > 0) ? (float)(dy->ess / (double)F.n) : 1.0f;
+    float fe[GG_MLP_NF - 1];
+    gg_mlp_features(su[0], su[1], su[2], sf[row], sf[a.sat_rows + row], ns, essf, fe);
+    sc[t] = gg_mlp_c(w, t, fe);
+  }
+  __syncthreads();
+  gg_bf16x8 a1[4];
+  gg_mlp_a1(w.w_e, sc, a1);
+  gg_mlp_regs wr;
+  gg_mlp_load_regs(w, wr);
+  const int l = t & 63;
+  float2* out = a.mlp_table + (int64_t)row * GG_MLP_CODES;
+  for (int cb = 0; cb < 16; ++cb) {
+    const int idx = blockIdx.x * 1024 + (t >> 6) * 256 + cb * 16 + (l & 15);
+    gg_mlp_act act;
+    gg_mlp_fwd16(wr, a1, gg_mlp_code_value(idx < GG_MLP_CODES ? idx : 0), act);
+    if (l < 16 && idx < GG_MLP_CODES) out[idx] = make_float2(act.z30, act.z31);
+  }
 }
 
 // ---------------------------------------------------------------------------

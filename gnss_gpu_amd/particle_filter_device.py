@@ -155,10 +155,13 @@ class ParticleFilterDevice:
                                self.seed, self._step, sv, alpha, use_kf, qv,
                                sigma_isb=self.sigma_isb)
 
-    def set_learned_likelihood(self, model):
+    def set_learned_likelihood(self, model, table=None):
         """Attach a LearnedLikelihood (None detaches).  The model must outlive
-        the updates that use it."""
+        the updates that use it.  table: evaluate the network per satellite
+        over the bf16 codes of its input and look the particles up (same
+        values); None chooses it when the particles outnumber the codes."""
         self._learned = model
+        self._learned_table = (self.n_particles > 33602) if table is None else bool(table)
         _dev.pf_device_set_mlp(self._state, model)
 
     def update(self, sat_ecef, pseudoranges, weights=None, sigma_pr=None, resample=True,
@@ -180,7 +183,8 @@ class ParticleFilterDevice:
             if cn0 is None or los_flag is None:
                 raise ValueError("the learned likelihood needs both cn0 and los_flag")
             _dev.pf_device_weight_learned(self._state, sat.ravel(), pr, weights, n_sat, cn0,
-                                          los_flag, system_ids=system_ids)
+                                          los_flag, system_ids=system_ids,
+                                          table=self._learned_table)
             if resample:
                 _ = self.resample_if_needed()
             return

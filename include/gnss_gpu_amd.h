@@ -202,10 +202,14 @@ int gg_pf_set_mlp(gg_pf* h, gg_mlp* m);
  * the particle mean), rows as in gg_pfb_load_epochs. */
 int gg_pfb_load_features(gg_pf* h, int32_t n_slots, const float* cn0, const float* los_flag);
 /* pf_device_weight (pf_device.cu:1516-1574) with the learned likelihood in
- * place of the Gaussian / Student-t term; cn0[n_sat], los_flag[n_sat]. */
+ * place of the Gaussian / Student-t term; cn0[n_sat], los_flag[n_sat].
+ * learned_mode 0 evaluates the network per (particle, satellite) pair; 1 first
+ * tabulates it per satellite over the bf16 codes of its scaled-residual input
+ * (GG_MLP_CODES entries) and looks the pairs up: the same values, fewer
+ * network evaluations when particles outnumber the codes. */
 int gg_pf_device_weight_learned(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
                                 const double* weights_sat, const int32_t* sys, int32_t n_sat,
-                                const float* cn0, const float* los_flag);
+                                const float* cn0, const float* los_flag, int32_t learned_mode);
 
 /* --- test hook (no reference counterpart) ----------------------------------- */
 /* Evaluates a deterministic device building block for n inputs so tests can

@@ -113,6 +113,6 @@ struct gg_step_args {
   int32_t u0_mode;       // 0: reference (seed % 1e6) / (1e6 * n); 1: fresh Philox;
                          // 2: u0_value
   int32_t recenter;      // gather re-centres offsets on the weighted mean
-  int32_t pad1;
+  int32_t learned_mode;  // GG_LIK_LEARNED: 0 network per pair, 1 per-satellite code table
   double u0_value;
 };

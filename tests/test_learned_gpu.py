@@ -165,3 +165,57 @@ def test_learned_filter_beats_cauchy_in_canyons(gpu):
             p50[kind].append(float(np.median(errs[10:])))
     print("P50 horizontal error [m]", p50)
     assert np.mean(p50["learned"]) <= np.mean(p50["cauchy"])
+
+
+def test_learned_code_tables_equal_per_pair_network(gpu):
+    """The per-satellite code tables return exactly the per-pair network's
+    outputs: identical log-weights (bit for bit) from both modes."""
+    from gnss_gpu_amd import ParticleFilterDevice, urban
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    rng = np.random.default_rng(21)
+    d = urban.drive_ecef(rng, 4, n_sat=14)
+    m = LearnedLikelihood(_params(13))
+    out = []
+    for table in (False, True):
+        pf = ParticleFilterDevice(n_particles=70_000, seed=5)
+        pf.set_learned_likelihood(m, table=table)
+        pf.initialize(d.truth[1], clock_bias=d.cb, spread_pos=30.0, spread_cb=30.0)
+        pf.predict(velocity=d.vel[2], dt=1.0)
+        pf.update(d.sat[2], d.pr[2], cn0=d.cn0[2], los_flag=d.los_flags(d.truth[2], 2),
+                  resample=False)
+        out.append(pf.get_log_weights())
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_batched_learned_modes_match(gpu):
+    """Graph-free batched steps with the learned likelihood: the direct and the
+    table mode advance two ragged filters identically."""
+    from gnss_gpu_amd import urban
+    from gnss_gpu_amd._lib import LIK_LEARNED
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    m = LearnedLikelihood(_params(17))
+    T = 6
+    drives = [urban.drive_ecef(np.random.default_rng(40 + f), T, n_sat=s)
+              for f, s in enumerate((12, 9))]
+    ns = [20_000, 5_000]
+    est = []
+    for mode in (0, 1):
+        b = BatchedParticleFilter(ns, sat_cap=[12, 9], n_slots=T)
+        b.load_epochs([[dr.sat[t] for dr in drives] for t in range(T)],
+                      [[dr.pr[t] for dr in drives] for t in range(T)], [[12, 9]] * T,
+                      vel=np.array([[dr.vel[t] for dr in drives] for t in range(T)]))
+        b.set_learned(m)
+        b.load_features([[dr.cn0[t] for dr in drives] for t in range(T)],
+                        [[dr.los_flags(dr.truth[t], t) for dr in drives] for t in range(T)])
+        b.initialize(np.array([np.concatenate([dr.truth[0], [dr.cb]]) for dr in drives]), [3, 4],
+                     5.0, 5.0)
+        st = b.step_args(lik=LIK_LEARNED, learned_mode=mode, sigma_pos=1.0, sigma_cb=1.0)
+        e = []
+        for t in range(T):
+            b.step(st)
+            e.append(b.estimates())
+        est.append(np.array(e))
+    np.testing.assert_array_equal(est[0], est[1])
