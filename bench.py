@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-likelihood leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-filter and WLS legs")
     ap.add_argument("--learned-particles", type=int, default=1_000_000)
     ap.add_argument("--learned-epochs", type=int, default=120)
     ap.add_argument("--learned-steps", type=int, default=100)
@@ -210,6 +211,111 @@ def learned_leg(args):
     }
 
 
+def batched_leg(args):
+    """256 independent filters (particle counts log-uniform in [1e3, 1e5],
+    satellite counts U{6..20}, Cauchy likelihood, systematic resampling) in one
+    hipGraph of 10 steps; 200 epochs."""
+    import torch
+
+    from gnss_gpu_amd import sim
+    from gnss_gpu_amd._lib import LIK_STUDENT
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+
+    F, T = 256, 200
+    n, s = sim.batched_filters(F, seed=1)
+    sc = sim.batched_scenarios(s, T + 20, seed=2)
+    bf = BatchedParticleFilter(n, sat_cap=s, n_slots=T + 20)
+    bf.load_epochs([[sc[f].sat for f in range(F)] for _ in range(T + 20)],
+                   [[sc[f].pr[t] for f in range(F)] for t in range(T + 20)],
+                   [s] * (T + 20), vel=np.tile(np.array([x.vel for x in sc])[None], (T + 20, 1, 1)))
+    init = np.array([np.concatenate([x.truth[0] - x.vel, [x.cb]]) for x in sc])
+    bf.initialize(init, np.arange(F) + 100, spread_pos=10.0, spread_cb=10.0)
+    st = bf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_pr=5.0, nu=1.0, lik=LIK_STUDENT,
+                      n_slots=T + 20, ess_frac=0.5)
+    G = 10
+    bf.capture(st, "systematic", 32, steps=G)
+    bf.replay(2)
+    bf.sync()
+    torch.cuda.synchronize()
+    ms = bf.time_replays_each(T // G)
+    bf.sync()
+    # accuracy after the timed epochs (epoch counter at 20 + T)
+    est = bf.estimates()
+    err = np.array([np.linalg.norm(est[f, :3] - sc[f].truth[20 + T - 1]) for f in range(F)])
+    ms_step = float(ms.sum()) / T
+    P = int(n.sum())
+    res, oth, cnt = bf.profile_steps(st, 20)
+    bf.close()
+    return {"workload": "batched_256_filters_1e3_1e5_particles_6_20_sats", "filters": F,
+            "epochs": T, "particles_total": P, "sat_rows": int(s.sum()),
+            "ms_per_step": round(ms_step, 5), "particles_per_s": round(P / (ms_step * 1e-3), 1),
+            "median_err_last_epoch_m": round(float(np.median(err)), 3),
+            "stage_ms": {k: round((res[k] * cnt[0] + oth[k] * cnt[1]) / max(sum(cnt), 1), 5)
+                         for k in res}}
+
+
+def wls_leg(args):
+    """Batched Gauss-Newton WLS on 10,000 epochs with U{6..20} satellites
+    (Sagnac range model, tol 1e-4 m, binary64), device-resident inputs."""
+    import ctypes as C
+
+    import torch
+
+    from gnss_gpu_amd import sim
+    from gnss_gpu_amd._gnss_gpu import wls_batch_device
+
+    E = 10_000
+    sat, pr, w, ns, truth = sim.wls_epochs(E, seed=4)
+    dev = torch.device("cuda")
+    ts = torch.tensor(sat, device=dev)
+    tp = torch.tensor(pr, device=dev)
+    tw = torch.tensor(w, device=dev)
+    tn = torch.tensor(ns.astype(np.int32), device=dev)
+    out = torch.zeros((E, 4), dtype=torch.float64, device=dev)
+    it = torch.zeros(E, dtype=torch.int32, device=dev)
+    for _ in range(5):
+        wls_batch_device(ts, tp, tw, out, it, n_sat=tn)
+    torch.cuda.synchronize()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        wls_batch_device(ts, tp, tw, out, it, n_sat=tn)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    res = out.cpu().numpy()
+    err = np.linalg.norm(res[:, :3] - truth, axis=1)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import build_cpu
+
+        lib = build_cpu.load()
+        DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        a = [np.ascontiguousarray(x) for x in (sat, pr, w)]
+        nsi = np.ascontiguousarray(ns.astype(np.int32))
+        o = np.zeros((E, 4))
+        io = np.zeros(E, np.int32)
+        rows = {}
+        for thr in (int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 1):
+            t0 = time.perf_counter()
+            n_rep = 0
+            while n_rep < 3 or time.perf_counter() - t0 < 1.0:
+                lib.cpu_wls_batch(a[0].ctypes.data_as(DP), a[1].ctypes.data_as(DP),
+                                  a[2].ctypes.data_as(DP), nsi.ctypes.data_as(IP), E, sat.shape[1],
+                                  10, 1e-4, 1, thr, o.ctypes.data_as(DP), io.ctypes.data_as(IP))
+                n_rep += 1
+            rows[thr] = (time.perf_counter() - t0) * 1e3 / n_rep
+        cpu = {"kind": "port", "ms_per_batch": {str(k): round(v, 3) for k, v in rows.items()},
+               "sample": "C restatement of wls.cu (oracle/cpu_wls.c), OpenMP over epochs"}
+    return {"workload": "wls_batch_10000_epochs_6_20_sats", "epochs": E,
+            "ms_per_batch": round(ms, 4), "epochs_per_s": round(E / (ms * 1e-3), 1),
+            "median_err_m": round(float(np.median(err)), 3),
+            "algorithmic_bytes": int(40 * ns.sum() + 36 * E),
+            "achieved_gbs": round((40 * ns.sum() + 36 * E) / (ms * 1e-3) / 1e9, 1),
+            "cpu_baseline": cpu}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -324,6 +430,8 @@ def main():
                          f"-march=x86-64-v3 -fopenmp); CPU {cpu_info()}"}
 
     learned = None if args.no_learned else learned_leg(args)
+    batched = None if args.no_batched else batched_leg(args)
+    wls = None if args.no_batched else wls_leg(args)
 
     out = {
         "metric": METRIC,
@@ -354,6 +462,8 @@ def main():
         "p50_err_m": None if p50 is None else round(p50, 3),
         "roofline": roofline,
         "learned": learned,
+        "batched": batched,
+        "wls": wls,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

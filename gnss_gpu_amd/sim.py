@@ -149,3 +149,27 @@ def wls_epochs(n_epoch=10_000, s_min=6, s_max=20, seed=0, noise=3.0, cb=3000.0,
         pr[e, :k] = rr + cb + rng.normal(0.0, noise, k)
         w[e, :k] = 1.0
     return sat, pr, w, counts, truth
+
+
+def batched_scenarios(n_sat, n_epochs, seed=0, nlos_frac=0.3, nlos_mean=20.0, los_sigma=3.0,
+                      speed=10.0, cb=100.0):
+    """One GPS-only scenario per filter with n_sat[f] satellites: the anchor of
+    configuration 1 moved by up to 5 km, el U(15,85), az U(0,360), a straight
+    trajectory at `speed` m/s in a random direction, and the NLOS generator of
+    configuration 2 (nlos_frac of the satellites per epoch, bias Exp(nlos_mean))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in np.asarray(n_sat):
+        a0 = ANCHOR_CFG1 + rng.uniform(-5000.0, 5000.0, 3)
+        el = rng.uniform(15.0, 85.0, int(s))
+        az = rng.uniform(0.0, 360.0, int(s))
+        sat = place_satellites(a0, el, az)
+        R = enu_rotation(a0)
+        h = rng.uniform(0.0, 2 * np.pi)
+        vel = np.array([speed * np.cos(h), speed * np.sin(h), 0.0]) @ R
+        truth = a0[None, :] + np.arange(n_epochs)[:, None] * vel[None, :]
+        sys = np.zeros(int(s), dtype=np.int32)
+        pr, nlos = _pseudoranges(rng, sat, sys, truth, cb, np.zeros(0), los_sigma, nlos_frac,
+                                 nlos_mean)
+        out.append(Scenario(sat, sys, truth, cb, np.zeros(0), vel, pr, nlos, el, az))
+    return out

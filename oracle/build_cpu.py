@@ -1,4 +1,5 @@
-"""Build oracle/_ref-free CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP).
+"""Build the CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP): the C
+restatements of the particle-filter step (cpu_pf.c) and of the WLS (cpu_wls.c).
 
 Measurement infrastructure only; the output lands in oracle/_build/.
 """
@@ -15,10 +16,10 @@ LIB = os.path.join(OUT_DIR, "liboracle_cpu_pf.so")
 
 def build() -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    src = os.path.join(HERE, "cpu_pf.c")
+    srcs = [os.path.join(HERE, "cpu_pf.c"), os.path.join(HERE, "cpu_wls.c")]
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11",
-           "-D_GNU_SOURCE", src, "-o", tmp, "-lm"]
+           "-D_GNU_SOURCE", *srcs, "-o", tmp, "-lm"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(tmp, LIB)
     return LIB
@@ -40,6 +41,9 @@ def load():
     lib.cpu_pf_initialize.argtypes = [P, DP, D, D, D]
     lib.cpu_pf_step.restype = I
     lib.cpu_pf_step.argtypes = [P, DP, D, D, D, D, I, DP, DP, IP, I, D, D, D, DP]
+    lib.cpu_wls_threads.restype = I
+    lib.cpu_wls_batch.restype = None
+    lib.cpu_wls_batch.argtypes = [DP, DP, DP, IP, I, I, I, D, I, I, DP, IP]
     return lib
 
 

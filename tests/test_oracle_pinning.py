@@ -461,3 +461,26 @@ def test_contrastive_gradient_finite_differences():
         num = (ol.contrastive_loss_and_grad(pp, r0, u, f, dpos, dcb)[0]
                - ol.contrastive_loss_and_grad(pm, r0, u, f, dpos, dcb)[0]) / (2 * h)
         assert num == pytest.approx(g[i], abs=1e-7, rel=1e-5)
+
+
+def test_cpu_wls_baseline_matches_oracle():
+    """The C WLS restatement (timed CPU baseline) against the NumPy oracle."""
+    import ctypes as C
+
+    from gnss_gpu_amd import sim
+    from oracle import build_cpu
+
+    lib = build_cpu.load()
+    sat, pr, w, ns, truth = sim.wls_epochs(200, seed=3)
+    E, S = pr.shape
+    out = np.zeros((E, 4))
+    it = np.zeros(E, np.int32)
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    a = [np.ascontiguousarray(x) for x in (sat, pr, w)]
+    nsi = np.ascontiguousarray(ns.astype(np.int32))
+    lib.cpu_wls_batch(a[0].ctypes.data_as(DP), a[1].ctypes.data_as(DP), a[2].ctypes.data_as(DP),
+                      nsi.ctypes.data_as(IP), E, S, 10, 1e-4, 1, 2, out.ctypes.data_as(DP),
+                      it.ctypes.data_as(IP))
+    ref, rit = owls.wls_batch_ref(sat, pr, w, n_sat=ns)
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(it, rit)
