@@ -211,6 +211,39 @@ def learned_leg(args):
     }
 
 
+def megopolis_leg(args, sc, init, T, S):
+    """The 1M x 12 workload with Megopolis (B = 32) in place of systematic
+    resampling, both the published form (Chesser et al.) and the reference's
+    rule, graph replays of 10 steps."""
+    import torch
+
+    from gnss_gpu_amd._lib import LIK_STUDENT
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+
+    out = {}
+    for name in ("megopolis", "megopolis_ref"):
+        pf = BatchedParticleFilter([args.particles], n_isb=3, sat_cap=S, n_slots=T)
+        pf.load_epochs([[sc.sat] for _ in range(T)], [[sc.pr[t]] for t in range(T)],
+                       n_sat=[[S]] * T, sys=[[sc.sys] for _ in range(T)],
+                       vel=np.tile(sc.vel, (T, 1, 1)))
+        pf.initialize(init, [12345], spread_pos=10.0, spread_cb=10.0, spread_isb=5.0)
+        st = pf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_isb=0.1, sigma_pr=5.0, nu=1.0,
+                          lik=LIK_STUDENT, n_slots=T, ess_frac=0.5)
+        G = 10
+        pf.capture(st, name, 32, steps=G)
+        pf.replay(2)
+        pf.sync()
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 100) // G)
+        ms = pf.time_replays_each(reps)
+        res, oth, cnt = pf.profile_steps(st, 20, name, 32)
+        pf.close()
+        out[name] = {"iterations": 32, "ms_per_step": round(float(ms.sum()) / (reps * G), 5),
+                     "stage_ms_resampling_steps": {k: round(v, 5) for k, v in res.items()},
+                     "resample_fraction": round(cnt[0] / max(sum(cnt), 1), 3)}
+    return out
+
+
 def batched_leg(args):
     """256 independent filters (particle counts log-uniform in [1e3, 1e5],
     satellite counts U{6..20}, Cauchy likelihood, systematic resampling) in one
@@ -429,6 +462,7 @@ def main():
                          f"C restatement of the reference step (oracle/cpu_pf.c, gcc -O3 "
                          f"-march=x86-64-v3 -fopenmp); CPU {cpu_info()}"}
 
+    mego = None if args.no_batched else megopolis_leg(args, sc, init, T, S)
     learned = None if args.no_learned else learned_leg(args)
     batched = None if args.no_batched else batched_leg(args)
     wls = None if args.no_batched else wls_leg(args)
@@ -461,6 +495,7 @@ def main():
         "resample_kernel_gbs": None if resample_gbs is None else round(resample_gbs, 1),
         "p50_err_m": None if p50 is None else round(p50, 3),
         "roofline": roofline,
+        "megopolis": mego,
         "learned": learned,
         "batched": batched,
         "wls": wls,

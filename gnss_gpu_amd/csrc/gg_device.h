@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include "gnss_gpu_amd_layout.h"
 
 // ---------------------------------------------------------------------------
@@ -196,50 +198,129 @@ __device__ __forceinline__ double gg_block_scan256(double v, double* lds, double
   return x;
 }
 
+// ---------------------------------------------------------------------------
+// Cross-lane exchanges on the DPP network (register to register, no LDS
+// traffic).  dpp_ctrl values of the gfx9 encoding:
+//   0xB1  quad_perm [1,0,3,2]   lane ^ 1
+//   0x4E  quad_perm [2,3,0,1]   lane ^ 2
+//   0x141 row_half_mirror       lane i <-> 7 - i inside each group of 8
+//   0x140 row_mirror            lane i <-> 15 - i inside each row of 16
+//   0x111, 0x112, 0x114, 0x118  row_shr 1, 2, 4, 8 (inside rows)
+//   0x142, 0x143                row_bcast 15, 31
+// After the lane^1 and lane^2 steps every lane of a quad holds the same
+// value, so the half-mirror partner stands in for lane ^ 4; likewise the
+// mirror partner for lane ^ 8.  ds_swizzle in bit mode (and 0x1f, xor 0x10)
+// gives lane ^ 16 inside each half of the wavefront.
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int gg_dpp_i32(int x, int old = 0) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, ROW_MASK, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float gg_dpp_f32(float x) {
+  return __builtin_bit_cast(float, gg_dpp_i32<CTRL>(__builtin_bit_cast(int, x)));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double gg_dpp_f64(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)gg_dpp_i32<CTRL>((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)gg_dpp_i32<CTRL>((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+__device__ __forceinline__ int gg_swz16_i32(int x) { return __builtin_amdgcn_ds_swizzle(x, 0x401F); }
+
+__device__ __forceinline__ float gg_swz16_f32(float x) {
+  return __builtin_bit_cast(float, gg_swz16_i32(__builtin_bit_cast(int, x)));
+}
+
+__device__ __forceinline__ double gg_swz16_f64(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)gg_swz16_i32((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)gg_swz16_i32((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+// Butterfly over 32 lanes: afterwards lanes 0..31 hold op over lanes 0..31
+// and lanes 32..63 op over lanes 32..63, each in the balanced pairwise order.
+#define GG_BUTTERFLY32(T, x, OP)          \
+  do {                                    \
+    x = OP(x, gg_dpp_##T<0xB1>(x));       \
+    x = OP(x, gg_dpp_##T<0x4E>(x));       \
+    x = OP(x, gg_dpp_##T<0x141>(x));      \
+    x = OP(x, gg_dpp_##T<0x140>(x));      \
+    x = OP(x, gg_swz16_##T(x));           \
+  } while (0)
+
+__device__ __forceinline__ double gg_add_f64(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double gg_max_f64(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float gg_max_f32(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ int gg_add_i32(int a, int b) { return a + b; }
+__device__ __forceinline__ int gg_max_i32(int a, int b) { return max(a, b); }
+
 // Block-wide max of a double (exact; any order gives the same result).
 __device__ __forceinline__ double gg_block_max256(double v, double* lds) {
   const int t = threadIdx.x;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
-  if ((t & 63) == 0) lds[t >> 6] = v;
+  GG_BUTTERFLY32(f64, v, gg_max_f64);
+  if ((t & 31) == 0) lds[t >> 5] = v;
   __syncthreads();
-  const double m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  const double m = fmax(fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3])),
+                        fmax(fmax(lds[4], lds[5]), fmax(lds[6], lds[7])));
+  __syncthreads();
+  return m;
+}
+
+// Block-wide max of a float (exact, NaN ignored as fmaxf does).
+__device__ __forceinline__ float gg_block_max256f(float v, double* lds) {
+  const int t = threadIdx.x;
+  float* l = reinterpret_cast<float*>(lds);
+  GG_BUTTERFLY32(f32, v, gg_max_f32);
+  if ((t & 31) == 0) l[t >> 5] = v;
+  __syncthreads();
+  const float m = fmaxf(fmaxf(fmaxf(l[0], l[1]), fmaxf(l[2], l[3])),
+                        fmaxf(fmaxf(l[4], l[5]), fmaxf(l[6], l[7])));
   __syncthreads();
   return m;
 }
 
 __device__ __forceinline__ int gg_block_max256_i(int v, int* lds) {
   const int t = threadIdx.x;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
-  if ((t & 63) == 0) lds[t >> 6] = v;
+  GG_BUTTERFLY32(i32, v, gg_max_i32);
+  if ((t & 31) == 0) lds[t >> 5] = v;
   __syncthreads();
-  const int m = max(max(lds[0], lds[1]), max(lds[2], lds[3]));
+  const int m = max(max(max(lds[0], lds[1]), max(lds[2], lds[3])),
+                    max(max(lds[4], lds[5]), max(lds[6], lds[7])));
   __syncthreads();
   return m;
 }
 
 __device__ __forceinline__ int gg_block_sum256_i(int v, int* lds) {
   const int t = threadIdx.x;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((t & 63) == 0) lds[t >> 6] = v;
+  GG_BUTTERFLY32(i32, v, gg_add_i32);
+  if ((t & 31) == 0) lds[t >> 5] = v;
   __syncthreads();
-  const int m = lds[0] + lds[1] + lds[2] + lds[3];
+  const int m = ((lds[0] + lds[1]) + (lds[2] + lds[3])) + ((lds[4] + lds[5]) + (lds[6] + lds[7]));
   __syncthreads();
   return m;
 }
 
-// Inclusive prefix max of ints over the block (exact, order-free).
+// Inclusive prefix max of ints over the block (exact, order-free): row scan
+// with row_shr 1, 2, 4, 8, then row_bcast 15 / 31 across the rows.
 __device__ __forceinline__ int gg_block_prefix_max256_i(int v, int* lds) {
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(v, d, 64);
-    if (lane >= d) v = max(v, y);
-  }
+  v = max(v, gg_dpp_i32<0x111>(v, INT_MIN));
+  v = max(v, gg_dpp_i32<0x112>(v, INT_MIN));
+  v = max(v, gg_dpp_i32<0x114>(v, INT_MIN));
+  v = max(v, gg_dpp_i32<0x118>(v, INT_MIN));
+  v = max(v, gg_dpp_i32<0x142, 0xA>(v, INT_MIN));
+  v = max(v, gg_dpp_i32<0x143, 0xC>(v, INT_MIN));
   if (lane == 63) lds[w] = v;
   __syncthreads();
   int pre = INT_MIN;

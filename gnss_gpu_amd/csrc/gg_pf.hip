@@ -112,12 +112,12 @@ gg_arena arena_of(const gg_pf* h) {
   return a;
 }
 
-// LDS of the staged satellites: 7 binary32 arrays (+ 64 first-layer biases
-// per satellite for the learned likelihood)
+// LDS of the staged satellites: a 32-byte record and a group index per
+// satellite (+ 64 first-layer biases per satellite for the learned likelihood)
 size_t sat_smem(const gg_pf* h, bool learned = false) {
   int cap = 0;
   for (const auto& f : h->filt_h) cap = std::max(cap, f.sat_cap);
-  return (size_t)std::max(cap, 1) * (7 + (learned ? GG_MLP_H : 0)) * sizeof(float);
+  return (size_t)std::max(cap, 1) * (9 + (learned ? GG_MLP_H : 0)) * sizeof(float);
 }
 
 int ensure_pinned(gg_pf* h, size_t bytes) {

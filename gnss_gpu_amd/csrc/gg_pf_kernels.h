@@ -66,16 +66,24 @@ __device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) 
   return a.state + ((int64_t)buf * a.ncomp + comp) * a.P;
 }
 
-// Satellite geometry staged in LDS as binary32 values relative to the anchor.
+// Satellite geometry staged in LDS as binary32 values relative to the frame
+// anchor, grouped by inter-system-bias component: the reference system first,
+// then ISB 0, 1, 2, each group in the epoch row's order.  Record k is two
+// float4:
+//   rec[2k]     = (-2 ux, -2 uy, -2 uz, 1/rho)   u: unit vector anchor -> sat,
+//                                                rho = |sat - anchor|
+//   rec[2k + 1] = (c, wl, row index bits, isb bits)
+//     c  = pr - rho - cb_anchor - isb_anchor of the satellite's system
+//     wl = the likelihood's weight factor: -w / (2 sigma^2) (Gaussian),
+//          w / (nu sigma^2) (Student-t), w (Huber, learned)
+//     row index: position of the satellite in its epoch row; isb: ISB index,
+//     -1 for the reference system.
+// gend[g]: end of group g (g = 0 the reference system, g = 1 + ISB index).
 struct gg_sat_view {
-  float* ux;
-  float* uy;
-  float* uz;
-  float* irho;  // 1 / |sat - anchor|
-  float* c;     // pr - |sat - anchor| - cb_anchor - isb_anchor
-  float* w;
-  int* isb;     // index into the ISB offsets, -1 for the reference system
-  float* cs;    // learned likelihood: first-layer bias per satellite [cap][64]
+  const float4* rec;
+  float* cs;     // learned likelihood: first-layer bias per record [cap][64]
+  int gend[GG_MAX_ISB + 1];
+  float irmax;   // largest 1/rho of the staged satellites
 };
 
 // ---------------------------------------------------------------------------
@@ -300,63 +308,141 @@ __device__ __forceinline__ void gg_sat_local(const double* sd, const gg_dyn* dy,
   inv_rho = 1.0 / rho;
 }
 
-// Stage the filter's satellites for this epoch slot in LDS.
-// anchor[3], cb_anchor, isb_anchor[3]: the step frame's binary64 origins
-// (usually in LDS).
+// Stage the filter's satellites for this epoch slot in LDS (records grouped by
+// ISB component, see gg_sat_view).  anchor[3], cb_anchor, isb_anchor[3]: the
+// step frame's binary64 origins (usually in LDS).  LDS: 36 bytes per
+// satellite of the filter's capacity (records + group scratch), then the
+// learned likelihood's first-layer biases.
 __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_filter& F,
                                              const gg_dyn* dy, const double* anchor,
                                              const double* cb_anchor, const double* isb_anchor,
-                                             int slot, gg_sat_view& sv, char* smem) {
+                                             int slot, const gg_step_args& s, gg_sat_view& sv,
+                                             char* smem) {
+  __shared__ int sgend[GG_MAX_ISB + 1];
+  __shared__ float sirmax;
   const double* sd = a.satd + (int64_t)slot * 5 * a.sat_rows;
   const int32_t* si = a.sati + (int64_t)slot * (a.sat_rows + a.n_filters);
   const int ns = si[a.sat_rows + f];
   const int cap = F.sat_cap;
-  sv.ux = (float*)smem;
-  sv.uy = sv.ux + cap;
-  sv.uz = sv.uy + cap;
-  sv.irho = sv.uz + cap;
-  sv.c = sv.irho + cap;
-  sv.w = sv.c + cap;
-  sv.isb = (int*)(sv.w + cap);
-  sv.cs = (float*)(sv.isb + cap);
-  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+  const int t = threadIdx.x;
+  float4* rec = reinterpret_cast<float4*>(smem);
+  int* grp = reinterpret_cast<int*>(rec + 2 * cap);
+  for (int k = t; k < ns; k += blockDim.x) {
+    const int sys = si[F.sat_off + k];
+    grp[k] = (sys >= 1 && sys <= a.n_isb) ? sys : 0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int cnt[GG_MAX_ISB + 1] = {0, 0, 0, 0};
+    for (int k = 0; k < ns; ++k) cnt[grp[k]] += 1;
+    int run = 0;
+    for (int g = 0; g <= GG_MAX_ISB; ++g) {
+      run += cnt[g];
+      sgend[g] = run;
+    }
+  }
+  // weight factor of the likelihood (binary64, rounded once)
+  const double sig2 = (double)s.sigma_pr * (double)s.sigma_pr;
+  const double wfac = (s.lik == GG_LIK_GAUSS) ? -0.5 / sig2
+                      : (s.lik == GG_LIK_STUDENT) ? 1.0 / ((double)s.nu * sig2) : 1.0;
+  float irm = 0.0f;
+  for (int k = t; k < ns; k += blockDim.x) {
+    const int gk = grp[k];
+    int dest = 0;
+    for (int j = 0; j < ns; ++j) {
+      const int gj = grp[j];
+      dest += (gj < gk || (gj == gk && j < k)) ? 1 : 0;
+    }
     const int64_t r = F.sat_off + k;
     double d[3], rho, inv_rho;
     gg_sat_local(sd, dy, anchor, r, d, rho, inv_rho);
-    const int sys = si[r];
-    const int ib = (sys >= 1 && sys <= a.n_isb) ? sys - 1 : -1;
+    const int ib = gk - 1;
     double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
     if (ib >= 0) c -= isb_anchor[ib];
-    sv.ux[k] = (float)(d[0] * inv_rho);
-    sv.uy[k] = (float)(d[1] * inv_rho);
-    sv.uz[k] = (float)(d[2] * inv_rho);
-    sv.irho[k] = (float)inv_rho;
-    sv.c[k] = (float)c;
-    sv.w[k] = (float)sd[4 * a.sat_rows + r];
-    sv.isb[k] = ib;
+    const float ir = (float)inv_rho;
+    irm = fmaxf(irm, ir);
+    rec[2 * dest] = make_float4(-2.0f * (float)(d[0] * inv_rho), -2.0f * (float)(d[1] * inv_rho),
+                                -2.0f * (float)(d[2] * inv_rho), ir);
+    rec[2 * dest + 1] = make_float4((float)c, (float)(sd[4 * a.sat_rows + r] * wfac),
+                                    __int_as_float(k), __int_as_float(ib));
   }
+  if (t == 0) sirmax = 0.0f;
   __syncthreads();
+  if (irm > 0.0f) atomicMax(reinterpret_cast<int*>(&sirmax), __float_as_int(irm));
+  __syncthreads();
+  sv.rec = rec;
+  sv.cs = reinterpret_cast<float*>(grp + cap);
+#pragma unroll
+  for (int g = 0; g <= GG_MAX_ISB; ++g) sv.gend[g] = __builtin_amdgcn_readfirstlane(sgend[g]);
+  sv.irmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sirmax)));
   return ns;
 }
 
-// Pseudorange residual pr - (|sat - x| + cb) for a binary32 offset particle,
-// using |sat-x| - rho = q / (1 + sqrt(1 + q/rho)), q = |p|^2/rho - 2 u.p,
-// which never forms the 2e7 m range in binary32.
-template <int NISB>
-__device__ __forceinline__ float gg_residual(const gg_sat_view& sv, int k, const float* p,
-                                             float pp) {
-  const float ud = sv.ux[k] * p[0] + sv.uy[k] * p[1] + sv.uz[k] * p[2];
-  const float ir = sv.irho[k];
-  const float q = pp * ir - 2.0f * ud;
-  const float dr = q * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f + q * ir));
-  float e = sv.c[k] - dr - p[3];
-  if (NISB > 0) {
-    const int ib = sv.isb[k];
+// Pseudorange residual pr - (|sat - x| + bias) of a binary32 offset particle x
+// (pp = |x|^2; bias = clock bias + the ISB of the satellite's system).
+// |sat - x| - rho = q h with q = pp/rho - 2 u.x, t = q/rho and
+// h = 1 / (1 + sqrt(1 + t)); the 2e7 m range is never formed in binary32.
+// SERIES evaluates h = 1/2 - t/8 + t^2/16, whose truncation error
+// (5/128) t^3 is below 1e-10 relative for |t| <= 2^-10; gg_series_ok tells
+// when every (particle, satellite) pair of a thread is in that range.
+template <bool SERIES>
+__device__ __forceinline__ float gg_residual(const float4& g, const float4& m, const float* p,
+                                             float pp, float bias) {
+  const float ud2 = fmaf(g.z, p[2], fmaf(g.y, p[1], g.x * p[0]));  // -2 u.x
+  const float q = fmaf(pp, g.w, ud2);
+  const float t = q * g.w;
+  float h;
+  if (SERIES)
+    h = fmaf(fmaf(t, 0.0625f, -0.125f), t, 0.5f);
+  else
+    h = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f + t));
+  return fmaf(-q, h, m.x) - bias;
+}
+
+// |t| <= (pp/rho + 2 |x|)/rho <= 2^-10 for every staged satellite when
+// pp (1/rho)^2 <= 2^-25 (|x| within about 3.5 km of the anchor for GNSS
+// ranges); the margin covers the binary32 rounding of q and t.
+__device__ __forceinline__ bool gg_series_ok(float ppmax, float irmax) {
+  return ppmax * irmax * irmax <= 2.98023223876953125e-08f;  // 2^-25
+}
+
+// |x|^2 of the thread's particles and whether the series range form holds
+// for all of them (gg_series_ok).
+template <int D>
+__device__ __forceinline__ bool gg_pp4(const gg_sat_view& sv, const float (&p)[GG_ITEMS][D],
+                                       float (&pp)[GG_ITEMS]) {
+  float ppmax = 0.0f;
 #pragma unroll
-    for (int j = 0; j < NISB; ++j)
-      if (ib == j) e -= p[4 + j];
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    pp[q] = fmaf(p[q][2], p[q][2], fmaf(p[q][1], p[q][1], p[q][0] * p[q][0]));
+    ppmax = fmaxf(ppmax, pp[q]);
   }
-  return e;
+  return gg_series_ok(ppmax, sv.irmax);
+}
+
+// Record k's bias term for group G (-1: reference system) of particle p.
+template <int NISB, int G>
+__device__ __forceinline__ float gg_group_bias(const float* p) {
+  if constexpr (G < 0 || G >= NISB)
+    return p[3];
+  else
+    return p[3] + p[4 + G];
+}
+
+// Calls fn(group tag, k0, k1) for the record ranges of the reference system
+// and of ISB components 0 .. NISB-1, in that order (the group index is a
+// compile-time constant inside fn).
+template <int G>
+struct gg_grp {
+  static constexpr int value = G;
+};
+
+template <int NISB, typename Fn>
+__device__ __forceinline__ void gg_for_groups(const gg_sat_view& sv, Fn&& fn) {
+  fn(gg_grp<-1>{}, 0, sv.gend[0]);
+  if constexpr (NISB > 0) fn(gg_grp<0>{}, sv.gend[0], sv.gend[1]);
+  if constexpr (NISB > 1) fn(gg_grp<1>{}, sv.gend[1], sv.gend[2]);
+  if constexpr (NISB > 2) fn(gg_grp<2>{}, sv.gend[2], sv.gend[3]);
 }
 
 // natural log on the hardware log2 unit (relative error ~1e-7 for the
@@ -365,66 +451,156 @@ __device__ __forceinline__ float gg_lnf(float x) {
   return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
 }
 
+// Likelihood of one particle with the NLOS gate (residuals beyond gate_thr
+// are dropped unless fewer than min(ns, 4) would remain).  Satellites in
+// record order; the exact range form.
 template <int NISB>
 __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const float* p,
                                            const gg_step_args& s) {
-  const float pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+  const float pp = fmaf(p[2], p[2], fmaf(p[1], p[1], p[0] * p[0]));
   const float sig = s.sigma_pr;
-  const float inv_s2 = 1.0f / (sig * sig);
   bool gate = s.gate_thr > 0.0f;
   if (gate) {
     int kept = 0;
-    for (int k = 0; k < ns; ++k) {
-      const float e = gg_residual<NISB>(sv, k, p, pp);
-      kept += (fabsf(e) <= s.gate_thr) ? 1 : 0;
-    }
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
+      for (int k = k0; k < k1; ++k) {
+        const float e = gg_residual<false>(sv.rec[2 * k], sv.rec[2 * k + 1], p, pp, bias);
+        kept += (fabsf(e) <= s.gate_thr) ? 1 : 0;
+      }
+    });
     const int min_kept = ns < 4 ? ns : 4;
     if (kept < min_kept) gate = false;
   }
   float lw = 0.0f;
   if (s.lik == GG_LIK_STUDENT) {
-    const float half_nup1 = 0.5f * (s.nu + 1.0f);
-    const float inv_nu_s2 = 1.0f / (s.nu * sig * sig);
     // sum of logs = log of the product; the product is taken over groups of
     // three factors so that it stays far from binary32 overflow.
-    float acc = 0.0f;
-    float prod = 1.0f;
+    float acc = 0.0f, prod = 1.0f;
     int cnt = 0;
-    for (int k = 0; k < ns; ++k) {
-      const float e = gg_residual<NISB>(sv, k, p, pp);
-      if (gate && fabsf(e) > s.gate_thr) continue;
-      prod *= 1.0f + sv.w[k] * e * e * inv_nu_s2;
-      if (++cnt == 3) {
-        acc += gg_lnf(prod);
-        prod = 1.0f;
-        cnt = 0;
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
+      for (int k = k0; k < k1; ++k) {
+        const float4 m = sv.rec[2 * k + 1];
+        const float e = gg_residual<false>(sv.rec[2 * k], m, p, pp, bias);
+        if (gate && fabsf(e) > s.gate_thr) continue;
+        prod *= fmaf(m.y * e, e, 1.0f);
+        if (++cnt == 3) {
+          acc += gg_lnf(prod);
+          prod = 1.0f;
+          cnt = 0;
+        }
       }
-    }
+    });
     acc += gg_lnf(prod);
-    lw = -half_nup1 * acc;
-  } else if (s.lik == GG_LIK_HUBER) {
-    const float hk = s.huber_k;
-    for (int k = 0; k < ns; ++k) {
-      const float e = gg_residual<NISB>(sv, k, p, pp);
-      if (gate && fabsf(e) > s.gate_thr) continue;
-      const float z = fabsf(e) / sig;
-      const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
-      lw -= sv.w[k] * rho;
-    }
+    lw = -0.5f * (s.nu + 1.0f) * acc;
   } else {
-    for (int k = 0; k < ns; ++k) {
-      const float e = gg_residual<NISB>(sv, k, p, pp);
-      if (gate && fabsf(e) > s.gate_thr) continue;
-      lw += -0.5f * sv.w[k] * e * e * inv_s2;
-    }
+    const float hk = s.huber_k;
+    const float isig = 1.0f / sig;
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
+      for (int k = k0; k < k1; ++k) {
+        const float4 m = sv.rec[2 * k + 1];
+        const float e = gg_residual<false>(sv.rec[2 * k], m, p, pp, bias);
+        if (gate && fabsf(e) > s.gate_thr) continue;
+        if (s.lik == GG_LIK_HUBER) {
+          const float z = fabsf(e) * isig;
+          const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
+          lw -= m.y * rho;
+        } else {
+          lw = fmaf(m.y * e, e, lw);
+        }
+      }
+    });
   }
   return lw;
 }
 
 // Likelihood of the thread's four particles with the satellites in the outer
-// loop, so that the four residual chains interleave.  Per particle the
-// arithmetic and its order are those of gg_loglik; with the NLOS gate on the
-// kept set differs per particle and gg_loglik is used as is.
+// loop, so that the four residual chains interleave; a group's bias is formed
+// once per particle.  With the NLOS gate on, the kept set differs per
+// particle and gg_loglik is used.
+template <int NISB, bool SERIES>
+__device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
+                                                const float (&p)[GG_ITEMS][4 + NISB],
+                                                const float (&pp)[GG_ITEMS],
+                                                const gg_step_args& s, float (&out)[GG_ITEMS]) {
+  if (s.lik == GG_LIK_STUDENT) {
+    float acc[GG_ITEMS], prod[GG_ITEMS];
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      acc[q] = 0.0f;
+      prod[q] = 1.0f;
+    }
+    int cnt = 0;
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      float bias[GG_ITEMS];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+      for (int k = k0; k < k1; ++k) {
+        const float4 g = sv.rec[2 * k];
+        const float4 m = sv.rec[2 * k + 1];
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          prod[q] *= fmaf(m.y * e, e, 1.0f);
+        }
+        if (++cnt == 3) {
+#pragma unroll
+          for (int q = 0; q < GG_ITEMS; ++q) {
+            acc[q] += gg_lnf(prod[q]);
+            prod[q] = 1.0f;
+          }
+          cnt = 0;
+        }
+      }
+    });
+    const float hn = -0.5f * (s.nu + 1.0f);
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = hn * (acc[q] + gg_lnf(prod[q]));
+  } else if (s.lik == GG_LIK_HUBER) {
+    const float hk = s.huber_k;
+    const float isig = 1.0f / s.sigma_pr;
+    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      float bias[GG_ITEMS];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+      for (int k = k0; k < k1; ++k) {
+        const float4 g = sv.rec[2 * k];
+        const float4 m = sv.rec[2 * k + 1];
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          const float z = fabsf(e) * isig;
+          const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
+          lw[q] -= m.y * rho;
+        }
+      }
+    });
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
+  } else {
+    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      float bias[GG_ITEMS];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+      for (int k = k0; k < k1; ++k) {
+        const float4 g = sv.rec[2 * k];
+        const float4 m = sv.rec[2 * k + 1];
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          lw[q] = fmaf(m.y * e, e, lw[q]);
+        }
+      }
+    });
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
+  }
+}
+
 template <int NISB>
 __device__ __forceinline__ void gg_loglik4(const gg_sat_view& sv, int ns,
                                            const float (&p)[GG_ITEMS][4 + NISB],
@@ -435,75 +611,16 @@ __device__ __forceinline__ void gg_loglik4(const gg_sat_view& sv, int ns,
     return;
   }
   float pp[GG_ITEMS];
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
-  const float sig = s.sigma_pr;
-  if (s.lik == GG_LIK_STUDENT) {
-    const float half_nup1 = 0.5f * (s.nu + 1.0f);
-    const float inv_nu_s2 = 1.0f / (s.nu * sig * sig);
-    float acc[GG_ITEMS], prod[GG_ITEMS];
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) {
-      acc[q] = 0.0f;
-      prod[q] = 1.0f;
-    }
-    int cnt = 0;
-    for (int k = 0; k < ns; ++k) {
-      const float wk = sv.w[k];
-#pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
-        prod[q] *= 1.0f + wk * e * e * inv_nu_s2;
-      }
-      if (++cnt == 3) {
-#pragma unroll
-        for (int q = 0; q < GG_ITEMS; ++q) {
-          acc[q] += gg_lnf(prod[q]);
-          prod[q] = 1.0f;
-        }
-        cnt = 0;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) {
-      acc[q] += gg_lnf(prod[q]);
-      out[q] = -half_nup1 * acc[q];
-    }
-  } else if (s.lik == GG_LIK_HUBER) {
-    const float hk = s.huber_k;
-    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k = 0; k < ns; ++k) {
-      const float wk = sv.w[k];
-#pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
-        const float z = fabsf(e) / sig;
-        const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
-        lw[q] -= wk * rho;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
-  } else {
-    const float inv_s2 = 1.0f / (sig * sig);
-    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k = 0; k < ns; ++k) {
-      const float wk = sv.w[k];
-#pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
-        lw[q] += -0.5f * wk * e * e * inv_s2;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
-  }
+  if (gg_pp4(sv, p, pp))
+    gg_loglik4_form<NISB, true>(sv, p, pp, s, out);
+  else
+    gg_loglik4_form<NISB, false>(sv, p, pp, s, out);
 }
 
 // Learned likelihood, per-satellite part: features (elevation and azimuth
 // from the staged unit vector, C/N0 and line-of-sight flag of the epoch slot,
 // satellite count, ESS fraction of the last decision) and the first-layer
-// bias c_s = b1 + W1[:, 1:] f of every satellite, into sv.cs.
+// bias c_s = b1 + W1[:, 1:] f of every record, into sv.cs.
 __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_filter& F,
                                                  const gg_dyn* dy, int slot, int ns,
                                                  const gg_mlp_w& w, gg_sat_view& sv) {
@@ -512,9 +629,11 @@ __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_fil
   for (int i = threadIdx.x; i < ns * GG_MLP_H; i += blockDim.x) {
     const int k = i / GG_MLP_H;
     const int u = i % GG_MLP_H;
+    const float4 g = sv.rec[2 * k];
+    const int64_t r = F.sat_off + __float_as_int(sv.rec[2 * k + 1].z);
     float f[GG_MLP_NF - 1];
-    gg_mlp_features(sv.ux[k], sv.uy[k], sv.uz[k], sf[F.sat_off + k],
-                    sf[a.sat_rows + F.sat_off + k], ns, essf, f);
+    gg_mlp_features(-0.5f * g.x, -0.5f * g.y, -0.5f * g.z, sf[r], sf[a.sat_rows + r], ns, essf,
+                    f);
     sv.cs[k * GG_MLP_H + u] = gg_mlp_c(w, u, f);
   }
   __syncthreads();
@@ -523,36 +642,53 @@ __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_fil
 // Learned likelihood through the per-satellite code tables (gg_k_mlp_table):
 // the network output for a residual depends only on the bf16 code of its
 // scaled value, so a lookup returns exactly what the per-pair network returns.
-// The per-particle sum runs over the satellites in the same order as
-// gg_loglik_learned, so both give identical log-weights.
+// The residuals and the per-particle sum follow gg_loglik_learned (record
+// order, same range form), so both give identical log-weights.
+template <int NISB, bool SERIES>
+__device__ __forceinline__ void gg_loglik_table_form(const gg_sat_view& sv, int64_t row0,
+                                                     const float2* __restrict__ tab,
+                                                     const float (&p)[GG_ITEMS][4 + NISB],
+                                                     const float (&pp)[GG_ITEMS],
+                                                     float (&out)[GG_ITEMS]) {
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) out[q] = 0.0f;
+  gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+    float bias[GG_ITEMS];
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+    for (int k = k0; k < k1; ++k) {
+      const float4 g = sv.rec[2 * k];
+      const float4 m = sv.rec[2 * k + 1];
+      const float2* tk = tab + (row0 + __float_as_int(m.z)) * GG_MLP_CODES;
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+        const float2 z = tk[gg_mlp_code_index(gg_mlp_xe(e))];
+        out[q] += gg_mlp_ll(e, z.x, z.y);
+      }
+    }
+  });
+}
+
 template <int NISB>
-__device__ __forceinline__ void gg_loglik_table(const gg_sat_view& sv, int ns, int64_t row0,
+__device__ __forceinline__ void gg_loglik_table(const gg_sat_view& sv, int64_t row0,
                                                 const float2* __restrict__ tab,
                                                 const float (&p)[GG_ITEMS][4 + NISB],
                                                 float (&out)[GG_ITEMS]) {
   float pp[GG_ITEMS];
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
-    out[q] = 0.0f;
-  }
-  for (int k = 0; k < ns; ++k) {
-    const float2* tk = tab + (row0 + k) * GG_MLP_CODES;
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) {
-      const float e = gg_residual<NISB>(sv, k, p[q], pp[q]);
-      const float2 z = tk[gg_mlp_code_index(gg_mlp_xe(e))];
-      out[q] += gg_mlp_ll(e, z.x, z.y);
-    }
-  }
+  if (gg_pp4(sv, p, pp))
+    gg_loglik_table_form<NISB, true>(sv, row0, tab, p, pp, out);
+  else
+    gg_loglik_table_form<NISB, false>(sv, row0, tab, p, pp, out);
 }
 
 // Learned likelihood of the workgroup's tile: for every satellite the 1024
 // residuals go through LDS to the matrix-core layout (16 particles per pass
 // and wavefront), the pair log-likelihoods are summed per particle in LDS.
-// se and slw: GG_TILE floats of LDS each.
+// se and slw: GG_TILE floats of LDS each.  The loop bounds are uniform over
+// the workgroup (the barriers inside); the range form is chosen per thread.
 template <int NISB>
-__device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
+__device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv,
                                                   const float (&p)[GG_ITEMS][4 + NISB],
                                                   const gg_mlp_w& w, float* se, float* slw,
                                                   float (&out)[GG_ITEMS]) {
@@ -560,28 +696,35 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
   const int l = t & 63;
   const int base = (t >> 6) * 256;
   float pp[GG_ITEMS];
+  const bool series = gg_pp4(sv, p, pp);
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    pp[q] = p[q][0] * p[q][0] + p[q][1] * p[q][1] + p[q][2] * p[q][2];
-    slw[t * GG_ITEMS + q] = 0.0f;
-  }
+  for (int q = 0; q < GG_ITEMS; ++q) slw[t * GG_ITEMS + q] = 0.0f;
   gg_mlp_regs wr;
   gg_mlp_load_regs(w, wr);
-  for (int k = 0; k < ns; ++k) {
+  gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+    float bias[GG_ITEMS];
 #pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) se[t * GG_ITEMS + q] = gg_residual<NISB>(sv, k, p[q], pp[q]);
-    __syncthreads();
-    gg_bf16x8 a1[4];
-    gg_mlp_a1(w.w_e, sv.cs + k * GG_MLP_H, a1);
-    for (int cb = 0; cb < 16; ++cb) {
-      const int i = base + 16 * cb + (l & 15);
-      const float e = se[i];
-      gg_mlp_act act;
-      gg_mlp_fwd16(wr, a1, gg_mlp_xe(e), act);
-      if (l < 16) slw[i] += gg_mlp_ll(e, act.z30, act.z31);
+    for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+    for (int k = k0; k < k1; ++k) {
+      const float4 g = sv.rec[2 * k];
+      const float4 m = sv.rec[2 * k + 1];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q)
+        se[t * GG_ITEMS + q] = series ? gg_residual<true>(g, m, p[q], pp[q], bias[q])
+                                      : gg_residual<false>(g, m, p[q], pp[q], bias[q]);
+      __syncthreads();
+      gg_bf16x8 a1[4];
+      gg_mlp_a1(w.w_e, sv.cs + k * GG_MLP_H, a1);
+      for (int cb = 0; cb < 16; ++cb) {
+        const int i = base + 16 * cb + (l & 15);
+        const float e = se[i];
+        gg_mlp_act act;
+        gg_mlp_fwd16(wr, a1, gg_mlp_xe(e), act);
+        if (l < 16) slw[i] += gg_mlp_ll(e, act.z30, act.z31);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
+  });
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) out[q] = slw[t * GG_ITEMS + q];
 }
@@ -592,6 +735,14 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv, int ns,
 // each sum in the fixed order: per-thread left to
 // right over its 4 particles, then gg_block_scan256's total.
 // ---------------------------------------------------------------------------
+// Totals of NV doubles per thread over the block, each in the order of
+// gg_block_scan256's total: inside a wavefront the pairwise tree that the
+// Hillis-Steele scan leaves in lane 63 (lanes 2i and 2i+1, then pairs of
+// pairs, ...), formed here by butterfly exchanges (addition is commutative,
+// so both partners compute the same bits), then ((T0 + T1) + T2) + T3 over
+// the wavefront totals T_w = (lanes 0..31) + (lanes 32..63).  The totals
+// are formed in thread 0 only (the other threads get zeros); the call
+// synchronises the block on entry to the combine and on return.
 template <int NV>
 __device__ __forceinline__ void gg_block_totals_n(const double (&v)[NV], double* lds, double* out) {
 #pragma clang fp contract(off)
@@ -601,17 +752,19 @@ __device__ __forceinline__ void gg_block_totals_n(const double (&v)[NV], double*
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double x = v[q];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const double y = __shfl_up(x, d, 64);
-      if (lane >= d) x = y + x;
-    }
-    if (lane == 63) lds[q * 4 + w] = x;
+    GG_BUTTERFLY32(f64, x, gg_add_f64);
+    if ((lane & 31) == 0) lds[q * 8 + w * 2 + (lane >> 5)] = x;
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < NV; ++q)
-    out[q] = ((lds[q * 4 + 0] + lds[q * 4 + 1]) + lds[q * 4 + 2]) + lds[q * 4 + 3];
+  for (int q = 0; q < NV; ++q) out[q] = 0.0;
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double* l = lds + q * 8;
+      out[q] = (((l[0] + l[1]) + (l[2] + l[3])) + (l[4] + l[5])) + (l[6] + l[7]);
+    }
+  }
   __syncthreads();
 }
 
@@ -631,7 +784,7 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
     if (valid[q]) m = fmaxf(m, lw[q]);
-  const float mb = (float)gg_block_max256((double)m, lds);
+  const float mb = gg_block_max256f(m, lds);
   double e[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)

@@ -145,8 +145,8 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
   }
   __syncthreads();
   gg_sat_view sv;
-  const int ns = gg_stage_sats(a, f, F, dy, sfr.anchor, &sfr.cb_anchor, sfr.isb_anchor, slot, sv,
-                               smem);
+  const int ns = gg_stage_sats(a, f, F, dy, sfr.anchor, &sfr.cb_anchor, sfr.isb_anchor, slot, s,
+                               sv, smem);
   bool valid[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
@@ -157,7 +157,7 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
   }
   float lw[GG_ITEMS];
   if constexpr (LEARN == 2) {
-    gg_loglik_table<NISB>(sv, ns, F.sat_off, a.mlp_table, p, lw);
+    gg_loglik_table<NISB>(sv, F.sat_off, a.mlp_table, p, lw);
   } else if constexpr (LEARN == 1) {
     __shared__ gg_mlp_w smlp;
     __shared__ float se[GG_TILE];
@@ -169,7 +169,7 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     }
     __syncthreads();
     gg_stage_learned(a, F, dy, slot, ns, smlp, sv);
-    gg_loglik_learned<NISB>(sv, ns, p, smlp, se, slw, lw);
+    gg_loglik_learned<NISB>(sv, p, smlp, se, slw, lw);
   } else {
     gg_loglik4<NISB>(sv, ns, p, s, lw);
   }
@@ -735,8 +735,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, in
     const gg_u32x4 r = gg_draw(seed, (uint32_t)i, (uint32_t)it, GG_PURPOSE_MEGO_REF, 0u);
     const double u1 = gg_u01d(r.v[0], r.v[1]);
     const double u2 = gg_u01d(r.v[2], r.v[3]);
-    const int off = (int)(u1 * (double)(n - 1)) + 1;
-    const int j = (int)(((int64_t)i + off) % n);
+    const int off = (int)(u1 * (double)(n - 1)) + 1;  // 1 <= off <= n - 1
+    int j = i + off;
+    if (j >= n) j -= n;
     const double lwi = lwz ? 0.0 : (double)lw[i];
     const double lwj = lwz ? 0.0 : (double)lw[j];
     const double alpha = fmin(1.0, gg_exp64(lwj - lwi));
@@ -783,26 +784,36 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
       myO = (uint32_t)(((uint64_t)r.v[0] * (uint64_t)(uint32_t)n) >> 32);
     }
     const int bend = min(B - b0, 64);
-    for (int bb = 0; bb < bend; ++bb) {
-      const int b = b0 + bb;
-      const uint32_t O = (uint32_t)__shfl((int)myO, bb, 64);
+    // one Philox block per particle serves four iterations (b0 is a multiple
+    // of 64, so the groups of four start at multiples of four)
+    for (int g0 = 0; g0 < bend; g0 += 4) {
+      gg_u32x4 r[GG_ITEMS];
 #pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const int i = i0 + q;
-        if (i >= n) continue;
-        const gg_u32x4 r = gg_draw(F.seed, (uint32_t)i, call, GG_PURPOSE_MEGO_U, (uint32_t)(b >> 2));
-        const float u = gg_u01f(r.v[b & 3]);
-        int j = i + (int)O;
-        if (j >= n) j -= n;
-        const float lj = lwz ? 0.0f : lw[j];
-        const float dl = lj - lk[q];
-        // accept iff dl >= 0, or -18 <= dl < 0 and u < exp(dl); u >= 2^-24
-        // so exp(dl) < u for every dl < -18.  Evaluated without branches:
-        // the exp core runs on the clamped argument and the move is a select.
-        const float e = gg_exp32_core(fminf(fmaxf(dl, -18.0f), 0.0f));
-        const bool acc = (dl >= 0.0f) | ((dl >= -18.0f) & (u < e));
-        k[q] = acc ? j : k[q];
-        lk[q] = acc ? lj : lk[q];
+      for (int q = 0; q < GG_ITEMS; ++q)
+        r[q] = gg_draw(F.seed, (uint32_t)(i0 + q), call, GG_PURPOSE_MEGO_U,
+                       (uint32_t)((b0 + g0) >> 2));
+      const int gend = min(bend - g0, 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= gend) break;
+        const uint32_t O = (uint32_t)__shfl((int)myO, g0 + c, 64);
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const int i = i0 + q;
+          if (i >= n) continue;
+          const float u = gg_u01f(r[q].v[c]);
+          int j = i + (int)O;
+          if (j >= n) j -= n;
+          const float lj = lwz ? 0.0f : lw[j];
+          const float dl = lj - lk[q];
+          // accept iff dl >= 0, or -18 <= dl < 0 and u < exp(dl); u >= 2^-24
+          // so exp(dl) < u for every dl < -18.  Evaluated without branches:
+          // the exp core runs on the clamped argument and the move is a select.
+          const float e = gg_exp32_core(fminf(fmaxf(dl, -18.0f), 0.0f));
+          const bool acc = (dl >= 0.0f) | ((dl >= -18.0f) & (u < e));
+          k[q] = acc ? j : k[q];
+          lk[q] = acc ? lj : lk[q];
+        }
       }
     }
   }
