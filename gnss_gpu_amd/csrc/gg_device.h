@@ -258,6 +258,10 @@ __device__ __forceinline__ double gg_add_f64(double a, double b) {
 #pragma clang fp contract(off)
   return a + b;
 }
+__device__ __forceinline__ float gg_add_f32(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
 __device__ __forceinline__ double gg_max_f64(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float gg_max_f32(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ int gg_add_i32(int a, int b) { return a + b; }

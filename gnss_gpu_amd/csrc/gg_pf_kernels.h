@@ -308,11 +308,35 @@ __device__ __forceinline__ void gg_sat_local(const double* sd, const gg_dyn* dy,
   inv_rho = 1.0 / rho;
 }
 
+__device__ __forceinline__ int gg_sat_group(const gg_arena& a, int sys) {
+  return (sys >= 1 && sys <= a.n_isb) ? sys : 0;
+}
+
+// Record of epoch-row satellite k (row r = sat_off + k) at position dest.
+__device__ __forceinline__ float gg_stage_one(const gg_arena& a, const double* sd, const gg_dyn* dy,
+                                              const double* anchor, const double* cb_anchor,
+                                              const double* isb_anchor, int64_t r, int k, int gk,
+                                              double wfac, float4* rec, int dest) {
+  double d[3], rho, inv_rho;
+  gg_sat_local(sd, dy, anchor, r, d, rho, inv_rho);
+  const int ib = gk - 1;
+  double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
+  if (ib >= 0) c -= isb_anchor[ib];
+  const float ir = (float)inv_rho;
+  rec[2 * dest] = make_float4(-2.0f * (float)(d[0] * inv_rho), -2.0f * (float)(d[1] * inv_rho),
+                              -2.0f * (float)(d[2] * inv_rho), ir);
+  rec[2 * dest + 1] = make_float4((float)c, (float)(sd[4 * a.sat_rows + r] * wfac),
+                                  __int_as_float(k), __int_as_float(ib));
+  return ir;
+}
+
 // Stage the filter's satellites for this epoch slot in LDS (records grouped by
 // ISB component, see gg_sat_view).  anchor[3], cb_anchor, isb_anchor[3]: the
 // step frame's binary64 origins (usually in LDS).  LDS: 36 bytes per
 // satellite of the filter's capacity (records + group scratch), then the
-// learned likelihood's first-layer biases.
+// learned likelihood's first-layer biases.  Up to 64 satellites are placed by
+// the first wavefront alone (ballots give each satellite its position and the
+// group ends); larger rows go through LDS group indices.
 __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_filter& F,
                                              const gg_dyn* dy, const double* anchor,
                                              const double* cb_anchor, const double* isb_anchor,
@@ -327,48 +351,63 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   const int t = threadIdx.x;
   float4* rec = reinterpret_cast<float4*>(smem);
   int* grp = reinterpret_cast<int*>(rec + 2 * cap);
-  for (int k = t; k < ns; k += blockDim.x) {
-    const int sys = si[F.sat_off + k];
-    grp[k] = (sys >= 1 && sys <= a.n_isb) ? sys : 0;
-  }
-  __syncthreads();
-  if (t == 0) {
-    int cnt[GG_MAX_ISB + 1] = {0, 0, 0, 0};
-    for (int k = 0; k < ns; ++k) cnt[grp[k]] += 1;
-    int run = 0;
-    for (int g = 0; g <= GG_MAX_ISB; ++g) {
-      run += cnt[g];
-      sgend[g] = run;
-    }
-  }
   // weight factor of the likelihood (binary64, rounded once)
   const double sig2 = (double)s.sigma_pr * (double)s.sigma_pr;
   const double wfac = (s.lik == GG_LIK_GAUSS) ? -0.5 / sig2
                       : (s.lik == GG_LIK_STUDENT) ? 1.0 / ((double)s.nu * sig2) : 1.0;
-  float irm = 0.0f;
-  for (int k = t; k < ns; k += blockDim.x) {
-    const int gk = grp[k];
-    int dest = 0;
-    for (int j = 0; j < ns; ++j) {
-      const int gj = grp[j];
-      dest += (gj < gk || (gj == gk && j < k)) ? 1 : 0;
+  if (ns <= 64) {
+    if (t < 64) {
+      int gk = GG_MAX_ISB + 1;  // lanes without a satellite: after every group
+      if (t < ns) gk = gg_sat_group(a, si[F.sat_off + t]);
+      int base = 0, below = 0, run = 0;
+#pragma unroll
+      for (int g = 0; g <= GG_MAX_ISB; ++g) {
+        const uint64_t m = __builtin_amdgcn_ballot_w64(gk == g);
+        const int c = __builtin_popcountll(m);
+        if (g == gk) {
+          base = run;
+          below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        }
+        run += c;
+        if (t == 0) sgend[g] = run;
+      }
+      float ir = 0.0f;
+      if (t < ns)
+        ir = gg_stage_one(a, sd, dy, anchor, cb_anchor, isb_anchor, F.sat_off + t, t, gk, wfac, rec,
+                          base + below);
+      GG_BUTTERFLY32(f32, ir, gg_max_f32);
+      if (t == 0)
+        sirmax = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(ir), 0)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ir), 32)));
     }
-    const int64_t r = F.sat_off + k;
-    double d[3], rho, inv_rho;
-    gg_sat_local(sd, dy, anchor, r, d, rho, inv_rho);
-    const int ib = gk - 1;
-    double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
-    if (ib >= 0) c -= isb_anchor[ib];
-    const float ir = (float)inv_rho;
-    irm = fmaxf(irm, ir);
-    rec[2 * dest] = make_float4(-2.0f * (float)(d[0] * inv_rho), -2.0f * (float)(d[1] * inv_rho),
-                                -2.0f * (float)(d[2] * inv_rho), ir);
-    rec[2 * dest + 1] = make_float4((float)c, (float)(sd[4 * a.sat_rows + r] * wfac),
-                                    __int_as_float(k), __int_as_float(ib));
+  } else {
+    for (int k = t; k < ns; k += blockDim.x) grp[k] = gg_sat_group(a, si[F.sat_off + k]);
+    __syncthreads();
+    if (t == 0) {
+      int cnt[GG_MAX_ISB + 1] = {0, 0, 0, 0};
+      for (int k = 0; k < ns; ++k) cnt[grp[k]] += 1;
+      int run = 0;
+      for (int g = 0; g <= GG_MAX_ISB; ++g) {
+        run += cnt[g];
+        sgend[g] = run;
+      }
+      sirmax = 0.0f;
+    }
+    float irm = 0.0f;
+    for (int k = t; k < ns; k += blockDim.x) {
+      const int gk = grp[k];
+      int dest = 0;
+      for (int j = 0; j < ns; ++j) {
+        const int gj = grp[j];
+        dest += (gj < gk || (gj == gk && j < k)) ? 1 : 0;
+      }
+      irm = fmaxf(irm, gg_stage_one(a, sd, dy, anchor, cb_anchor, isb_anchor, F.sat_off + k, k, gk,
+                                    wfac, rec, dest));
+    }
+    __syncthreads();
+    if (irm > 0.0f) atomicMax(reinterpret_cast<int*>(&sirmax), __float_as_int(irm));
   }
-  if (t == 0) sirmax = 0.0f;
-  __syncthreads();
-  if (irm > 0.0f) atomicMax(reinterpret_cast<int*>(&sirmax), __float_as_int(irm));
   __syncthreads();
   sv.rec = rec;
   sv.cs = reinterpret_cast<float*>(grp + cap);
@@ -731,9 +770,11 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv,
 
 // ---------------------------------------------------------------------------
 // Tile partials: m_b = max lw, S_b = sum e, S2_b = sum e^2, sum_d = sum e p_d
-// with e = exp32(lw - m_b) (binary32, like lw itself) accumulated in binary64,
-// each sum in the fixed order: per-thread left to
-// right over its 4 particles, then gg_block_scan256's total.
+// with e = exp32(lw - m_b) (binary32, like lw itself).  S and S2 accumulate
+// in binary64 in a fixed order: per thread left to right over its 4
+// particles, then gg_block_scan256's total.  The weighted state sums (used
+// only for the estimate) accumulate in binary32 inside each half-wavefront
+// and in binary64 across the 8 half-wavefronts, also in a fixed order.
 // ---------------------------------------------------------------------------
 // Totals of NV doubles per thread over the block, each in the order of
 // gg_block_scan256's total: inside a wavefront the pairwise tree that the
@@ -785,28 +826,49 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
   for (int q = 0; q < GG_ITEMS; ++q)
     if (valid[q]) m = fmaxf(m, lw[q]);
   const float mb = gg_block_max256f(m, lds);
-  double e[GG_ITEMS];
+  float e[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
-    e[q] = (valid[q] && mb > -__builtin_huge_valf()) ? (double)gg_exp32(lw[q] - mb) : 0.0;
-  double v[D + 2];
-  v[0] = e[0];
-  v[1] = e[0] * e[0];
-#pragma unroll
-  for (int d = 0; d < D; ++d) v[2 + d] = e[0] * (double)p[0][d];
+    e[q] = (valid[q] && mb > -__builtin_huge_valf()) ? gg_exp32(lw[q] - mb) : 0.0f;
+  // S and S2 in binary64 (they set the CDF tile offsets and the ESS)
+  double v[2];
+  v[0] = (double)e[0];
+  v[1] = (double)e[0] * (double)e[0];
 #pragma unroll
   for (int q = 1; q < GG_ITEMS; ++q) {
-    v[0] = v[0] + e[q];
-    v[1] = v[1] + e[q] * e[q];
-#pragma unroll
-    for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + e[q] * (double)p[q][d];
+    v[0] = v[0] + (double)e[q];
+    v[1] = v[1] + (double)e[q] * (double)e[q];
   }
-  double tot[D + 2];
-  gg_block_totals<D>(v, lds, tot);
-  if (threadIdx.x == 0) {
-    part_out[0] = (double)mb;
+  // weighted state sums (the estimate): binary32 inside the wavefront,
+  // binary64 across the 8 half-wavefront partials
+  float ms[D];
 #pragma unroll
-    for (int q = 0; q < D + 2; ++q) part_out[1 + q] = tot[q];
+  for (int d = 0; d < D; ++d) {
+    ms[d] = e[0] * p[0][d];
+#pragma unroll
+    for (int q = 1; q < GG_ITEMS; ++q) ms[d] = ms[d] + e[q] * p[q][d];
+  }
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  float* lf = reinterpret_cast<float*>(lds + 16);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float x = ms[d];
+    GG_BUTTERFLY32(f32, x, gg_add_f32);
+    if ((lane & 31) == 0) lf[d * 8 + (t >> 5)] = x;
+  }
+  double tot[2];
+  gg_block_totals_n<2>(v, lds, tot);  // synchronises: lf is complete here
+  if (t == 0) {
+    part_out[0] = (double)mb;
+    part_out[1] = tot[0];
+    part_out[2] = tot[1];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float* l = lf + d * 8;
+      part_out[3 + d] = (((double)l[0] + (double)l[1]) + ((double)l[2] + (double)l[3])) +
+                        (((double)l[4] + (double)l[5]) + ((double)l[6] + (double)l[7]));
+    }
   }
 }
 
@@ -922,23 +984,50 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
   const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
-  double m = -__builtin_huge_val();
-  for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
-  const double M = gg_block_max256(m, lds);
   double vv[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) vv[d] = 0.0;
-  for (int b = t; b < nt; b += GG_BLOCK) {
-    const double* pb = part + (int64_t)b * GG_NPART;
-    const double mb = pb[0];
+  // pass 2 for tile b given M (m_b, S_b, S2_b and the sums read from pb)
+  auto pass2 = [&](int b, double mb, double Sb, double S2b, const double* pb, double M) {
     const double gb = (mb > -__builtin_huge_val()) ? gg_exp64(mb - M) : 0.0;
     if (ALL) a.g[F.tile_off + b] = gb;
     if (!ALL && b == own) sx[1] = gb;
-    sT[b] = pb[1] * gb;
-    vv[0] = vv[0] + pb[2] * (gb * gb);
+    sT[b] = Sb * gb;
+    vv[0] = vv[0] + S2b * (gb * gb);
     if (FULL) {
 #pragma unroll
       for (int d = 0; d < D; ++d) vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + pb[3 + d] * gb;
+    }
+  };
+  constexpr int R = 4;  // tiles per thread held in registers across the max
+  double M;
+  if (nt <= R * GG_BLOCK) {
+    double pm[R], pS[R], pS2[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int b = t + k * GG_BLOCK;
+      const double* pb = part + (int64_t)min(b, nt - 1) * GG_NPART;
+      pm[k] = pb[0];
+      pS[k] = pb[1];
+      pS2[k] = pb[2];
+    }
+    double m = -__builtin_huge_val();
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (t + k * GG_BLOCK < nt) m = fmax(m, pm[k]);
+    M = gg_block_max256(m, lds);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int b = t + k * GG_BLOCK;
+      if (b < nt) pass2(b, pm[k], pS[k], pS2[k], part + (int64_t)b * GG_NPART, M);
+    }
+  } else {
+    double m = -__builtin_huge_val();
+    for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
+    M = gg_block_max256(m, lds);
+    for (int b = t; b < nt; b += GG_BLOCK) {
+      const double* pb = part + (int64_t)b * GG_NPART;
+      pass2(b, pb[0], pb[1], pb[2], pb, M);
     }
   }
   __syncthreads();

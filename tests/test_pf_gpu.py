@@ -269,6 +269,29 @@ def test_state_roundtrip_and_frame_shifts(gpu):
     np.testing.assert_allclose(p[:, 3], s[:, 3] + 12.5, atol=1e-4)
 
 
+@pytest.mark.parametrize("spread", [3000.0, 50000.0])
+@pytest.mark.parametrize("kind", ["gauss", "cauchy"])
+def test_weight_far_particles_match_oracle(gpu, kind, spread):
+    """Particles kilometres from the anchor: the range is evaluated in its
+    exact form (3 km mixes series and exact threads in one wavefront, 50 km is
+    exact everywhere); both against the binary64 likelihood."""
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    pr = np.linalg.norm(sat - tp, axis=1) + g["true_cb"] + np.random.default_rng(2).normal(0, 3, 8)
+    pf, orc = _setup(20000, spread=spread)
+    if kind == "cauchy":
+        pf.nu = 1.0
+    pf.update(sat, pr, resample=False)
+    lw = pf.get_log_weights()
+    st = pf.get_particles()
+    far = np.linalg.norm(st[:, :3] - tp, axis=1) > 3500.0
+    assert far.any() and (spread > 10000.0 or (~far).any())
+    ref = opf.log_likelihood(st[:, :3], st[:, 3], np.zeros((len(st), 0)), sat, pr, np.ones(8), None,
+                             5.0, nu=1.0 if kind == "cauchy" else 0.0)
+    np.testing.assert_allclose(lw, ref, rtol=1e-5, atol=1e-4)
+
+
 def test_many_satellites_grow_capacity(gpu):
     sat, pr, w = _stream_scenario(80)
     dev = _dev()
