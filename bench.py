@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=100)
     ap.add_argument("--learned-train-epochs", type=int, default=60000)
     ap.add_argument("--learned-train-steps", type=int, default=2000)
+    ap.add_argument("--no-scale", action="store_true", help="skip the 4M x 32 scale leg")
+    ap.add_argument("--scale-particles", type=int, default=4_000_000)
+    ap.add_argument("--scale-epochs", type=int, default=1000)
+    ap.add_argument("--scale-train-epochs", type=int, default=15000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -209,6 +213,126 @@ def learned_leg(args):
                      "loss_first": round(float(loss[:10].mean()), 4),
                      "loss_last": round(float(loss[-50:].mean()), 4)},
     }
+
+
+def scale_leg(args):
+    """4M particles x 32 satellites (8 each of 4 systems, state [e,n,u,cb,isb x3]),
+    a canyon drive with about 40% NLOS, Megopolis B = 64, 1000 epochs as 100
+    replays of a 10-step graph (no host synchronisation inside): the learned
+    likelihood through its code tables and the Cauchy likelihood.  Accuracy
+    passes run the same epochs eagerly, the learned one with line-of-sight
+    flags taken at its own previous estimate."""
+    import torch
+
+    from gnss_gpu_amd import urban
+    from gnss_gpu_amd._lib import LIK_LEARNED, LIK_STUDENT
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    N, S, G, B = args.scale_particles, 32, 10, 64
+    T = args.scale_epochs + G + 1
+    ckw = dict(height=(20.0, 150.0))
+    rng = np.random.default_rng(2027)
+    # 32-satellite training epochs enter the loss as two 16-satellite halves;
+    # the satellite-count feature keeps the deployed count
+    f, r0, u, _ = urban.training_set(rng, args.scale_train_epochs, n_sat=S, canyon_kw=ckw)
+    E = f.shape[0]
+    print(f"scale leg: training set of {E} epochs ready", file=sys.stderr, flush=True)
+    model = LearnedLikelihood(seed=3)
+    model.load_data(f.reshape(2 * E, S // 2, 7), r0.reshape(2 * E, S // 2),
+                    u.reshape(2 * E, S // 2, 3), np.full(2 * E, S // 2, np.int32))
+    t0 = time.perf_counter()
+    loss = model.train(args.learned_train_steps, batch=256, lr=2e-3, seed=11)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    drive = urban.drive_ecef(np.random.default_rng(92), T, n_sat=S, speed=1.5, n_sys=4,
+                             canyon_kw=ckw)
+    # the step for epoch t predicts with vel[t] = truth[t] - truth[t-1] (vel[0] = 0)
+    init = np.concatenate([drive.truth[0], [drive.cb], drive.isb])[None]
+
+    def make(lik):
+        bf = BatchedParticleFilter([N], n_isb=3, sat_cap=S, n_slots=T)
+        bf.load_epochs([[drive.sat[t]] for t in range(T)], [[drive.pr[t]] for t in range(T)],
+                       [[S]] * T, sys=[[drive.sys] for _ in range(T)],
+                       vel=drive.vel[:, None, :])
+        if lik == LIK_LEARNED:
+            bf.set_learned(model)
+        st = bf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_isb=0.05, sigma_pr=5.0, nu=1.0,
+                          lik=lik, n_slots=T, ess_frac=0.5, learned_mode=1)
+        return bf, st
+
+    def start(bf):
+        bf.initialize(init, [4242], spread_pos=5.0, spread_cb=5.0, spread_isb=2.0)
+        bf.set_epoch(0)
+
+    out = {"workload": f"canyon_{N // 1000000}Mx{S}_4sys_megopolis{B}_{args.scale_epochs}_epochs",
+           "particles": N, "satellites": S, "systems": 4, "megopolis_iterations": B,
+           "epochs": args.scale_epochs, "nlos_fraction": round(float(drive.nlos.mean()), 3)}
+    flags = np.ones((T, S), np.float32)
+    for name, lik in (("cauchy", LIK_STUDENT), ("learned", LIK_LEARNED)):
+        print(f"scale leg: {name} filter", file=sys.stderr, flush=True)
+        bf, st = make(lik)
+        start(bf)
+        # accuracy: eager steps, estimate read after each one
+        errs = []
+        est = drive.truth[0].copy()
+        for t in range(T):
+            if lik == LIK_LEARNED:
+                flags[t] = drive.los_flags(est, t)
+                bf.load_features([[drive.cn0[k]] for k in range(T)],
+                                 [[flags[k]] for k in range(T)])
+            bf.step(st, "megopolis", B)
+            est = bf.estimates()[0][:3]
+            errs.append(float(np.linalg.norm((drive.R @ (est - drive.truth[t]))[:2])))
+        e = np.asarray(errs[10:])
+        # timed: graph replays over the same epochs
+        start(bf)
+        bf.capture(st, "megopolis", B, steps=G)
+        bf.replay(1)
+        bf.sync()
+        torch.cuda.synchronize()
+        rc0 = bf.read_dyn()[0].resample_count
+        reps = args.scale_epochs // G
+        ms = bf.time_replays_each(reps)
+        bf.sync()
+        rc1 = bf.read_dyn()[0].resample_count
+        res, oth, cnt = bf.profile_steps(st, 20, "megopolis", B)
+        tot = max(sum(cnt), 1)
+        stage = {k: round((res[k] * cnt[0] + oth[k] * cnt[1]) / tot, 5) for k in res}
+        wk = stage["predict_weight"]
+        rec = {"ms_per_step": round(float(ms.sum()) / (reps * G), 5),
+               "p50_err_m": round(float(np.median(e)), 3),
+               "rms_err_m": round(float(np.sqrt(np.mean(e * e))), 3),
+               "resample_fraction": round((rc1 - rc0) / (reps * G), 3),
+               "stage_ms": stage,
+               "stage_ms_resampling_steps": {k: round(v, 5) for k, v in res.items()},
+               "stage_ms_other_steps": {k: round(v, 5) for k, v in oth.items()},
+               "weight_kernel_gbs": round(60.0 * N / (wk * 1e-3) / 1e9, 1),
+               "weight_kernel_hbm_frac": round(60.0 * N / (wk * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "megopolis_gather_gbs": round((64.0 + 4.0 * B) * N /
+                                            (res["megopolis_gather"] * 1e-3) / 1e9, 1)
+               if cnt[0] else None}
+        if lik == LIK_LEARNED:
+            rec["code_table_evals_per_step"] = S * 33602
+            # the per-pair network on the matrix cores, eager steps, same epochs
+            bf.close()
+            bf, st2 = make(LIK_LEARNED)
+            st2.learned_mode = 0
+            start(bf)
+            bf.load_features([[drive.cn0[k]] for k in range(T)], [[flags[k]] for k in range(T)])
+            r2, o2, c2 = bf.profile_steps(st2, 10, "megopolis", B)
+            wpp = (r2["predict_weight"] * c2[0] + o2["predict_weight"] * c2[1]) / max(sum(c2), 1)
+            flops = 2.0 * (2 * 64 + 64 * 64 + 64 * 2) * N * S
+            rec["per_pair_weight_ms"] = round(wpp, 5)
+            rec["per_pair_mfma"] = {"dtype": "bf16", "tflops": round(flops / (wpp * 1e-3) / 1e12, 1),
+                                    "peak_tflops": 2500.0,
+                                    "frac": round(flops / (wpp * 1e-3) / 2.5e15, 4)}
+        bf.close()
+        out[name] = rec
+    out["training"] = {"epochs": E, "steps": args.learned_train_steps, "seconds": round(train_s, 2),
+                       "loss_first": round(float(loss[:10].mean()), 4),
+                       "loss_last": round(float(loss[-50:].mean()), 4)}
+    return out
 
 
 def megopolis_leg(args, sc, init, T, S):
@@ -466,6 +590,7 @@ def main():
     learned = None if args.no_learned else learned_leg(args)
     batched = None if args.no_batched else batched_leg(args)
     wls = None if args.no_batched else wls_leg(args)
+    scale = None if args.no_scale else scale_leg(args)
 
     out = {
         "metric": METRIC,
@@ -499,6 +624,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "wls": wls,
+        "scale": scale,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -136,11 +136,12 @@ def sat_features(el, az, cn0, los_at_mean, n_sat, ess_frac):
     return f.astype(np.float32)
 
 
-def training_set(rng, n_epochs, n_sat=16, n_canyons=None, mean_err=3.0):
+def training_set(rng, n_epochs, n_sat=16, n_canyons=None, mean_err=3.0, canyon_kw=None):
     """Satellite-epochs for the learned likelihood from fresh canyons.
 
     Returns (f [E, S, 7] float32 features, r0 [E, S] residual at the true
     position [m], u [E, S, 3] unit vectors to the satellites, n_sat [E] int32).
+    canyon_kw goes to make_canyon.
     The line-of-sight feature is tested from a stand-in for the particle mean:
     the true position plus a N(0, mean_err^2) horizontal error; the ESS
     fraction is drawn from U(0.05, 1)."""
@@ -148,7 +149,8 @@ def training_set(rng, n_epochs, n_sat=16, n_canyons=None, mean_err=3.0):
     per = -(-n_epochs // n_canyons)
     fs, rs, us = [], [], []
     for _ in range(n_canyons):
-        can, se = sat_epochs(rng, per, n_sat)
+        can = make_canyon(rng, **(canyon_kw or {}))
+        can, se = sat_epochs(rng, per, n_sat, canyon=can)
         est = se.truth.copy()
         est[:, :2] += rng.normal(0.0, mean_err, (per, 2))
         flag = ~blocked(can, est[:, None, :], se.u)
@@ -177,6 +179,8 @@ class Drive:
     u: np.ndarray          # [T, S, 3] ENU unit vectors to the satellites
     nlos: np.ndarray       # [T, S]
     cb: float
+    sys: np.ndarray = None  # [S] system index (0 = reference system)
+    isb: np.ndarray = None  # [n_sys - 1] inter-system biases [m] added to pr
 
     def los_flags(self, est_ecef, t):
         """Line-of-sight flags of epoch t's satellites seen from an ECEF point."""
@@ -185,12 +189,18 @@ class Drive:
 
 
 def drive_ecef(rng, n_epochs, n_sat=16, anchor=(-3957199.0, 3310205.0, 3737911.0), cb=30.0,
-               speed=5.0):
+               speed=5.0, n_sys=1, isb=(12.0, -7.5, 20.0), canyon_kw=None):
     """A straight drive along a fresh canyon; satellites are placed 2.2e7 m from
-    the receiver along the simulated lines of sight."""
+    the receiver along the simulated lines of sight.  With n_sys > 1 the
+    satellites are split evenly over n_sys systems (row order), system k > 0
+    carrying the inter-system bias isb[k-1]; canyon_kw goes to make_canyon
+    (its length is stretched to cover the drive)."""
     from .sim import enu_rotation
 
-    can, se = sat_epochs(rng, n_epochs, n_sat, speed=speed)
+    kw = dict(canyon_kw or {})
+    kw["length"] = max(float(kw.get("length", 2000.0)), speed * n_epochs + 400.0)
+    can = make_canyon(rng, **kw)
+    can, se = sat_epochs(rng, n_epochs, n_sat, canyon=can, speed=speed)
     anchor = np.asarray(anchor, dtype=np.float64)
     R = enu_rotation(anchor)
     # keep the drive continuous: no wrap inside one drive
@@ -203,6 +213,9 @@ def drive_ecef(rng, n_epochs, n_sat=16, anchor=(-3957199.0, 3310205.0, 3737911.0
     cn0 = rng.uniform(20.0, 50.0, nl.shape) - 8.0 * nl
     sat = truth[:, None, :] + 2.2e7 * (se.u @ R)
     rng_true = np.linalg.norm(sat - truth[:, None, :], axis=2)
-    pr = rng_true + cb + bias + se.noise
+    sys_ids = (np.arange(n_sat) * n_sys // n_sat).astype(np.int32)
+    isb_v = np.asarray(isb, dtype=np.float64)[: max(n_sys - 1, 0)]
+    offs = np.concatenate([[0.0], isb_v])[sys_ids]
+    pr = rng_true + cb + offs[None, :] + bias + se.noise
     return Drive(can, anchor, R, truth_enu, truth, vel, sat, pr, cn0.astype(np.float32), se.u,
-                 nl, cb)
+                 nl, cb, sys_ids, isb_v)
