@@ -245,18 +245,25 @@ def scale_leg(args):
     loss = model.train(args.learned_train_steps, batch=256, lr=2e-3, seed=11)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    # FP8 E4M3 weights (per-tensor power-of-two scales): the trained network,
+    # quantised, then fine-tuned through the quantised passes
+    model8 = LearnedLikelihood(model.get_params())
+    model8.load_data(f.reshape(2 * E, S // 2, 7), r0.reshape(2 * E, S // 2),
+                     u.reshape(2 * E, S // 2, 3), np.full(2 * E, S // 2, np.int32))
+    model8.set_fp8(True)
+    loss8 = model8.train(args.learned_train_steps // 4, batch=256, lr=5e-4, seed=12)
     drive = urban.drive_ecef(np.random.default_rng(92), T, n_sat=S, speed=1.5, n_sys=4,
                              canyon_kw=ckw)
     # the step for epoch t predicts with vel[t] = truth[t] - truth[t-1] (vel[0] = 0)
     init = np.concatenate([drive.truth[0], [drive.cb], drive.isb])[None]
 
-    def make(lik):
+    def make(lik, net=None):
         bf = BatchedParticleFilter([N], n_isb=3, sat_cap=S, n_slots=T)
         bf.load_epochs([[drive.sat[t]] for t in range(T)], [[drive.pr[t]] for t in range(T)],
                        [[S]] * T, sys=[[drive.sys] for _ in range(T)],
                        vel=drive.vel[:, None, :])
         if lik == LIK_LEARNED:
-            bf.set_learned(model)
+            bf.set_learned(net)
         st = bf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_isb=0.05, sigma_pr=5.0, nu=1.0,
                           lik=lik, n_slots=T, ess_frac=0.5, learned_mode=1)
         return bf, st
@@ -269,9 +276,10 @@ def scale_leg(args):
            "particles": N, "satellites": S, "systems": 4, "megopolis_iterations": B,
            "epochs": args.scale_epochs, "nlos_fraction": round(float(drive.nlos.mean()), 3)}
     flags = np.ones((T, S), np.float32)
-    for name, lik in (("cauchy", LIK_STUDENT), ("learned", LIK_LEARNED)):
+    for name, lik, net in (("cauchy", LIK_STUDENT, None), ("learned", LIK_LEARNED, model),
+                           ("learned_fp8", LIK_LEARNED, model8)):
         print(f"scale leg: {name} filter", file=sys.stderr, flush=True)
-        bf, st = make(lik)
+        bf, st = make(lik, net)
         start(bf)
         # accuracy: eager steps, estimate read after each one
         errs = []
@@ -313,10 +321,12 @@ def scale_leg(args):
                                             (res["megopolis_gather"] * 1e-3) / 1e9, 1)
                if cnt[0] else None}
         if lik == LIK_LEARNED:
+            rec["weights"] = "fp8_e4m3_per_tensor_pow2_scale" if net.fp8 else "bf16"
             rec["code_table_evals_per_step"] = S * 33602
+        if net is model:
             # the per-pair network on the matrix cores, eager steps, same epochs
             bf.close()
-            bf, st2 = make(LIK_LEARNED)
+            bf, st2 = make(LIK_LEARNED, model)
             st2.learned_mode = 0
             start(bf)
             bf.load_features([[drive.cn0[k]] for k in range(T)], [[flags[k]] for k in range(T)])
@@ -331,7 +341,10 @@ def scale_leg(args):
         out[name] = rec
     out["training"] = {"epochs": E, "steps": args.learned_train_steps, "seconds": round(train_s, 2),
                        "loss_first": round(float(loss[:10].mean()), 4),
-                       "loss_last": round(float(loss[-50:].mean()), 4)}
+                       "loss_last": round(float(loss[-50:].mean()), 4),
+                       "fp8_finetune_steps": args.learned_train_steps // 4,
+                       "fp8_loss_first": round(float(loss8[:10].mean()), 4),
+                       "fp8_loss_last": round(float(loss8[-50:].mean()), 4)}
     return out
 
 

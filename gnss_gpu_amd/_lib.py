@@ -107,6 +107,7 @@ _SIGS = {
     "gg_mlp_destroy": [_P],
     "gg_mlp_set_params": [_P, _FP],
     "gg_mlp_get_params": [_P, _FP],
+    "gg_mlp_set_fp8": [_P, _I],
     "gg_mlp_eval": [_P, _I, _FP, _FP, _FP, _FP],
     "gg_mlp_load_data": [_P, _I, _I, _FP, _FP, _FP, _IP],
     "gg_mlp_train": [_P, _I, _I, C.c_float, _U64, C.c_float, C.c_float, C.c_float, _FP],

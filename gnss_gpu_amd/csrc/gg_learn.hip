@@ -30,23 +30,63 @@ struct gg_mlp {
   int cand_cap = 0;
   int32_t* batch_idx = nullptr;
   int adam_t = 0;
+  int fp8 = 0;              // FP8 E4M3 weight mode (gg_k_mlp_pack)
   hipStream_t stream = nullptr;
 };
 
 namespace {
 
-__global__ void gg_k_mlp_pack(const float* p, gg_mlp_w* w) {
+// Per-tensor scale 2^k of the FP8 (OCP E4M3FN) weight mode: the smallest
+// power of two with max|W| / 2^k <= 448 (1 for an all-zero tensor).
+__device__ float gg_e4m3_scale(const float* W, int n, float* lds) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, fabsf(W[i]));
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float mm = 0.0f;
+    for (int i = 0; i < (int)blockDim.x; ++i) mm = fmaxf(mm, lds[i]);
+    float sc = 1.0f;
+    if (mm > 0.0f) {
+      int e;
+      frexpf(mm / 448.0f, &e);  // mm / 448 in [2^(e-1), 2^e)
+      sc = ldexpf(1.0f, e);
+      if (ldexpf(1.0f, e - 1) >= mm / 448.0f) sc = ldexpf(1.0f, e - 1);
+    }
+    lds[blockDim.x] = sc;
+  }
+  __syncthreads();
+  const float sc = lds[blockDim.x];
+  __syncthreads();
+  return sc;
+}
+
+// Packs the binary32 master parameters into the matrix-core fragments.  With
+// fp8 set, W1, W2 and W3 are first replaced by their FP8 E4M3 values times a
+// per-tensor power-of-two scale (gg_q_e4m3); those values are exact in bf16,
+// so the bf16 matrix-core path computes with the FP8 weights unchanged.
+__global__ void gg_k_mlp_pack(const float* p, gg_mlp_w* w, int fp8) {
+  __shared__ float lds[257];
   const int t = threadIdx.x;
   const float* W1 = p + GG_MLP_OFF_W1;
   const float* W2 = p + GG_MLP_OFF_W2;
   const float* W3 = p + GG_MLP_OFF_W3;
+  float s1 = 1.0f, s2 = 1.0f, s3 = 1.0f;
+  if (fp8) {
+    s1 = gg_e4m3_scale(W1, GG_MLP_H * GG_MLP_NF, lds);
+    s2 = gg_e4m3_scale(W2, GG_MLP_H * GG_MLP_H, lds);
+    s3 = gg_e4m3_scale(W3, 2 * GG_MLP_H, lds);
+  }
+  auto q1 = [&](int i) { return fp8 ? gg_q_e4m3(W1[i], s1) : W1[i]; };
+  auto q2 = [&](int i) { return fp8 ? gg_q_e4m3(W2[i], s2) : W2[i]; };
+  auto q3 = [&](int i) { return fp8 ? gg_q_e4m3(W3[i], s3) : W3[i]; };
   for (int i = t; i < 4 * 2 * 64; i += blockDim.x) {
     const int mb = i / 128, ks = (i / 64) % 2, l = i % 64;
     gg_bf16x8 a, at;
     for (int s = 0; s < 8; ++s) {
       const int u = gg_mlp_unit(ks, l >> 4, s);
-      a[s] = (__bf16)W2[(16 * mb + (l & 15)) * GG_MLP_H + u];
-      at[s] = (__bf16)W2[u * GG_MLP_H + 16 * mb + (l & 15)];
+      a[s] = (__bf16)q2((16 * mb + (l & 15)) * GG_MLP_H + u);
+      at[s] = (__bf16)q2(u * GG_MLP_H + 16 * mb + (l & 15));
     }
     w->A2[mb][ks][l] = a;
     w->A2t[mb][ks][l] = at;
@@ -55,17 +95,17 @@ __global__ void gg_k_mlp_pack(const float* p, gg_mlp_w* w) {
     const int ks = i / 64, l = i % 64;
     gg_bf16x8 a;
     for (int s = 0; s < 8; ++s)
-      a[s] = ((l & 15) < 2) ? (__bf16)W3[(l & 15) * GG_MLP_H + gg_mlp_unit(ks, l >> 4, s)]
+      a[s] = ((l & 15) < 2) ? (__bf16)q3((l & 15) * GG_MLP_H + gg_mlp_unit(ks, l >> 4, s))
                             : (__bf16)0.0f;
     w->A3[ks][l] = a;
   }
   for (int u = t; u < GG_MLP_H; u += blockDim.x) {
     w->b2[u] = p[GG_MLP_OFF_B2 + u];
-    w->w_e[u] = (float)(__bf16)W1[u * GG_MLP_NF];
-    for (int j = 0; j < GG_MLP_NF; ++j) w->W1[u][j] = W1[u * GG_MLP_NF + j];
+    w->w_e[u] = (float)(__bf16)q1(u * GG_MLP_NF);
+    for (int j = 0; j < GG_MLP_NF; ++j) w->W1[u][j] = q1(u * GG_MLP_NF + j);
     w->b1[u] = p[GG_MLP_OFF_B1 + u];
-    w->W3b[0][u] = (float)(__bf16)W3[u];
-    w->W3b[1][u] = (float)(__bf16)W3[GG_MLP_H + u];
+    w->W3b[0][u] = (float)(__bf16)q3(u);
+    w->W3b[1][u] = (float)(__bf16)q3(GG_MLP_H + u);
   }
   if (t < 4) w->b3[t] = (t < 2) ? p[GG_MLP_OFF_B3 + t] : 0.0f;
 }
@@ -413,7 +453,7 @@ __global__ void gg_k_mlp_reduce(const float* __restrict__ partial, int rows,
 }
 
 int pack(gg_mlp* h) {
-  hipLaunchKernelGGL(gg_k_mlp_pack, dim3(1), dim3(256), 0, h->stream, h->params, h->w);
+  hipLaunchKernelGGL(gg_k_mlp_pack, dim3(1), dim3(256), 0, h->stream, h->params, h->w, h->fp8);
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -472,6 +512,14 @@ int gg_mlp_set_params(gg_mlp* h, const float* p) {
   GG_HIP(hipMemsetAsync(h->adam_m, 0, sizeof(float) * GG_MLP_NPARAMS, h->stream));
   GG_HIP(hipMemsetAsync(h->adam_v, 0, sizeof(float) * GG_MLP_NPARAMS, h->stream));
   h->adam_t = 0;
+  if (pack(h)) return -1;
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_mlp_set_fp8(gg_mlp* h, int32_t enable) {
+  if (!h) return gg_fail(-2, "null handle");
+  h->fp8 = enable ? 1 : 0;
   if (pack(h)) return -1;
   GG_HIP(hipStreamSynchronize(h->stream));
   return 0;

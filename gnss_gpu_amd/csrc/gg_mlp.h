@@ -45,6 +45,22 @@
 typedef __bf16 gg_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float gg_f32x4 __attribute__((ext_vector_type(4)));
 
+// FP8 E4M3 (OCP E4M3FN: 3 mantissa bits, exponents -6..8, subnormals down to
+// 2^-9, largest finite 448) value nearest to x / sc, ties to even, saturated at
+// +-448, times the power-of-two scale sc; the result is exact in bf16.
+__host__ __device__ inline float gg_q_e4m3(float x, float sc) {
+  const float y = x / sc;
+  const float a = fabsf(y);
+  if (!(a > 0.0f)) return x * 0.0f;  // zero keeps its sign, NaN stays NaN
+  if (a >= 448.0f) return copysignf(448.0f, y) * sc;
+  int e;
+  frexpf(a, &e);                               // a in [2^(e-1), 2^e)
+  const int lead = (e - 1 < -6) ? -6 : e - 1;  // exponent of the quantum's binade
+  const float step = ldexpf(1.0f, lead - 3);
+  const float q = fminf(rintf(a / step) * step, 448.0f);
+  return copysignf(q, y) * sc;
+}
+
 // Packed weights (built by gg_k_mlp_pack from the binary32 parameters).
 struct gg_mlp_w {
   gg_bf16x8 A2[4][2][64];   // W2 as A fragments: [mb][ks][lane]

@@ -40,6 +40,7 @@ class LearnedLikelihood:
         h = C.c_void_p()
         call("gg_mlp_create", C.byref(h))
         self._h = h
+        self.fp8 = False
         self.set_params(init_params(seed) if params is None else params)
 
     def close(self):
@@ -62,6 +63,12 @@ class LearnedLikelihood:
         if p.size != N_PARAMS:
             raise ValueError(f"expected {N_PARAMS} parameters, got {p.size}")
         call("gg_mlp_set_params", self._h, fptr(p))
+
+    def set_fp8(self, enable: bool = True):
+        """FP8 E4M3 weights with per-tensor power-of-two scales (see
+        gg_mlp_set_fp8); the master parameters stay binary32."""
+        call("gg_mlp_set_fp8", self._h, 1 if enable else 0)
+        self.fp8 = bool(enable)
 
     def get_params(self) -> np.ndarray:
         out = np.empty(N_PARAMS, dtype=np.float32)

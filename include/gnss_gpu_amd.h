@@ -174,6 +174,12 @@ int gg_mlp_create(gg_mlp** out);
 int gg_mlp_destroy(gg_mlp* h);
 int gg_mlp_set_params(gg_mlp* h, const float* params);
 int gg_mlp_get_params(gg_mlp* h, float* params);
+/* FP8 weight mode (enable != 0): W1, W2 and W3 are used as OCP FP8 E4M3 values
+ * (round to nearest even, saturating) times one power-of-two scale per tensor,
+ * the smallest with max|W| / scale <= 448; the binary32 master parameters are
+ * kept, so training in this mode updates them through the quantised forward
+ * and backward passes.  Off (bf16 weights) by default. */
+int gg_mlp_set_fp8(gg_mlp* h, int32_t enable);
 /* Pair log-likelihoods for n_groups groups of 16 pairs sharing one satellite:
  * e[n_groups*16] residuals [m], f[n_groups*7] satellite features; outputs
  * ll[n_groups*16] and z3[n_groups*16*2] (network outputs). */

@@ -70,6 +70,49 @@ def bf16(x):
     return np.where(np.isnan(a), np.nan, out)
 
 
+def e4m3_scale(W):
+    """Per-tensor scale of the FP8 weight mode (twin of gg_e4m3_scale): the
+    smallest power of two 2^k with max|W| / 2^k <= 448 (binary32 quotient),
+    1 for an all-zero tensor."""
+    mm = np.float32(np.max(np.abs(np.asarray(W, dtype=np.float32)))) if np.size(W) else 0.0
+    if not mm > 0:
+        return np.float32(1.0)
+    r = np.float32(mm / np.float32(448.0))
+    _, e = np.frexp(r)
+    sc = np.float32(np.ldexp(1.0, int(e)))
+    if np.float32(np.ldexp(1.0, int(e) - 1)) >= r:
+        sc = np.float32(np.ldexp(1.0, int(e) - 1))
+    return sc
+
+
+def q_e4m3(x, sc):
+    """OCP FP8 E4M3FN value nearest to x / sc (ties to even, saturated at
+    +-448), times sc (twin of gg_q_e4m3)."""
+    x = np.asarray(x, dtype=np.float32)
+    y = (x / np.float32(sc)).astype(np.float32)
+    a = np.abs(y)
+    _, e = np.frexp(a)
+    lead = np.maximum(e.astype(np.int32) - 1, -6)
+    step = np.ldexp(np.float32(1.0), lead - 3).astype(np.float32)
+    with np.errstate(invalid="ignore"):
+        q = np.minimum(np.rint(a / step) * step, np.float32(448.0)).astype(np.float32)
+    q = np.where(a >= 448.0, np.float32(448.0), q)
+    out = (np.copysign(q, y) * np.float32(sc)).astype(np.float32)
+    out = np.where(a > 0, out, x * np.float32(0.0))
+    return out.astype(np.float32)
+
+
+def quantised(p, mode):
+    """Parameters as the device uses them: in "fp8" mode W1, W2 and W3 are
+    replaced by their per-tensor scaled E4M3 values (binary64 arrays)."""
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    if mode == "fp8":
+        W1 = q_e4m3(W1, e4m3_scale(W1)).astype(np.float64)
+        W2 = q_e4m3(W2, e4m3_scale(W2)).astype(np.float64)
+        W3 = q_e4m3(W3, e4m3_scale(W3)).astype(np.float64)
+    return W1, b1, W2, b2, W3, b3
+
+
 def first_layer_bias(W1, b1, f):
     """c = b1 + sum_j W1[:, 1+j] f_j in binary32, j left to right (device order).
     f [..., 7] -> [..., 64]."""
@@ -85,11 +128,13 @@ def scaled_residual(e):
 
 
 def mlp_forward(p, e, f, mode="exact"):
-    """e [...], f [..., 7] -> dict with ll [...] and the intermediates."""
-    W1, b1, W2, b2, W3, b3 = unpack(p)
+    """e [...], f [..., 7] -> dict with ll [...] and the intermediates.
+    mode: "exact" (binary64), "bf16" (device rounding), "fp8" (device
+    rounding with the FP8 weight mode)."""
+    W1, b1, W2, b2, W3, b3 = quantised(p, mode)
     e64 = np.asarray(e, dtype=np.float64)
     xe = scaled_residual(e).astype(np.float64)
-    if mode == "bf16":
+    if mode in ("bf16", "fp8"):
         c = first_layer_bias(W1, b1, f).astype(np.float64)
         z1 = (bf16(W1[:, 0]) * bf16(xe)[..., None] + bf16(c)).astype(np.float32).astype(np.float64)
         h1 = bf16(np.maximum(z1, 0.0))
@@ -120,7 +165,7 @@ def pair_loglik(e, mu, lv):
 def mlp_backward(p, f, fw, g_ll, mode="exact"):
     """Gradient of sum(g_ll * ll) with respect to the parameters, from the
     intermediates of mlp_forward.  Returns a flat binary64 vector."""
-    W1, b1, W2, b2, W3, b3 = unpack(p)
+    W1, b1, W2, b2, W3, b3 = quantised(p, mode)
     z3 = fw["z3"]
     g = np.asarray(g_ll, dtype=np.float64)
     r = fw["e"] - fw["mu"]
@@ -128,11 +173,12 @@ def mlp_backward(p, f, fw, g_ll, mode="exact"):
     inside = (z3[..., 1] > LV_MIN) & (z3[..., 1] < LV_MAX)
     dz3 = np.stack([MU_SCALE * r * ex * g, np.where(inside, (0.5 * r * r * ex - 0.5) * g, 0.0)],
                    axis=-1)
-    W3u = bf16(W3) if mode == "bf16" else W3
-    W2u = bf16(W2) if mode == "bf16" else W2
+    dev = mode in ("bf16", "fp8")
+    W3u = bf16(W3) if dev else W3
+    W2u = bf16(W2) if dev else W2
     dh2 = dz3 @ W3u
     dz2 = dh2 * (fw["z2"] > 0)
-    dz2u = bf16(dz2) if mode == "bf16" else dz2
+    dz2u = bf16(dz2) if dev else dz2
     dh1 = dz2u @ W2u
     dz1 = dh1 * (fw["z1"] > 0)
     lead = int(np.prod(g.shape))
@@ -142,7 +188,7 @@ def mlp_backward(p, f, fw, g_ll, mode="exact"):
     db3 = dz3.reshape(lead, 2).sum(0)
     dW2 = dz2u.reshape(lead, H).T @ h1
     db2 = dz2.reshape(lead, H).sum(0)
-    xe = (bf16(fw["xe"]) if mode == "bf16" else fw["xe"]).reshape(lead)
+    xe = (bf16(fw["xe"]) if dev else fw["xe"]).reshape(lead)
     dz1f = dz1.reshape(lead, H)
     fl = np.broadcast_to(np.asarray(f, dtype=np.float64), g.shape + (NF - 1,)).reshape(lead, NF - 1)
     dW1 = np.concatenate([(dz1f * xe[:, None]).sum(0)[:, None], dz1f.T @ fl], axis=1)

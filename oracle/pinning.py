@@ -77,6 +77,13 @@ PINS = {
     "learned.init_params": ("pinned", "shapes and the b3 = [0, ln 25] start; no numerics"),
     "learned.bf16": ("pinned", "round-to-nearest-even on hand-checked bit patterns "
                      "(test_bf16_rounding_known_values)"),
+    "learned.q_e4m3": ("pinned", "every finite OCP E4M3FN value is a fixed point, midpoints go "
+                       "to the even neighbour, saturation at 448, subnormal quantum 2^-9 "
+                       "(test_e4m3_quantiser_known_values)"),
+    "learned.e4m3_scale": ("pinned", "smallest power of two with max|W|/s <= 448 "
+                           "(test_e4m3_quantiser_known_values)"),
+    "learned.quantised": ("pinned", "fp8 mode applies q_e4m3 with e4m3_scale per tensor; "
+                          "other modes return the parameters (test_e4m3_quantiser_known_values)"),
     "learned.first_layer_bias": ("pinned", "binary32 left-to-right sum against a binary64 sum"),
     "learned.scaled_residual": ("pinned", "clamp bounds"),
     "learned.mlp_forward": ("pinned", "exact mode: a ReLU network is affine on a fixed activation "

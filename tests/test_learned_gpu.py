@@ -219,3 +219,50 @@ def test_batched_learned_modes_match(gpu):
             e.append(b.estimates())
         est.append(np.array(e))
     np.testing.assert_array_equal(est[0], est[1])
+
+
+def test_fp8_weights_eval_matches_oracle(gpu):
+    """FP8 E4M3 weight mode: the device network against the oracle's fp8 mode
+    (same per-tensor scales and roundings), tolerances of the bf16 test; the
+    quantised network differs from the bf16 one."""
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    rng = np.random.default_rng(8)
+    G = 64
+    p = _params(11)
+    e = rng.normal(0.0, 25.0, (G, 16)).astype(np.float32)
+    f = rng.normal(0.0, 1.0, (G, 7)).astype(np.float32)
+    m = LearnedLikelihood(p)
+    _, z_bf16 = m.eval_groups(e, f)
+    m.set_fp8(True)
+    ll, z3 = m.eval_groups(e, f)
+    fb = np.broadcast_to(f[:, None, :], (G, 16, 7))
+    fw = ol.mlp_forward(p, e, fb, mode="fp8")
+    np.testing.assert_allclose(z3, fw["z3"], rtol=0, atol=2e-3)
+    np.testing.assert_allclose(ll, fw["ll"], rtol=5e-3, atol=5e-3)
+    assert np.max(np.abs(z3 - z_bf16)) > 1e-3
+    # the master parameters are untouched by the mode
+    np.testing.assert_array_equal(m.get_params(), p)
+    m.set_fp8(False)
+    _, z_back = m.eval_groups(e, f)
+    np.testing.assert_array_equal(z_back, z_bf16)
+
+
+def test_fp8_code_tables_equal_per_pair_network(gpu):
+    """Code tables built from the FP8 network equal its per-pair evaluation."""
+    from gnss_gpu_amd import ParticleFilterDevice, urban
+    from gnss_gpu_amd.learned import LearnedLikelihood
+
+    d = urban.drive_ecef(np.random.default_rng(23), 4, n_sat=12)
+    m = LearnedLikelihood(_params(19))
+    m.set_fp8(True)
+    out = []
+    for table in (False, True):
+        pf = ParticleFilterDevice(n_particles=50_000, seed=6)
+        pf.set_learned_likelihood(m, table=table)
+        pf.initialize(d.truth[1], clock_bias=d.cb, spread_pos=30.0, spread_cb=30.0)
+        pf.predict(velocity=d.vel[2], dt=1.0)
+        pf.update(d.sat[2], d.pr[2], cn0=d.cn0[2], los_flag=d.los_flags(d.truth[2], 2),
+                  resample=False)
+        out.append(pf.get_log_weights())
+    np.testing.assert_array_equal(out[0], out[1])

@@ -404,6 +404,50 @@ def test_bf16_rounding_known_values():
     assert np.isnan(ol.bf16(np.float32(np.nan)))
 
 
+def test_e4m3_quantiser_known_values():
+    """OCP FP8 E4M3FN: values m/8 * 2^-6 (m < 8, subnormal) and (1 + m/8) 2^e
+    for e in -6..8, the top code 1.875 * 2^8 being NaN, so 448 is the largest."""
+    from oracle import learned as ol
+
+    finite = [m / 8 * 2.0 ** -6 for m in range(8)]
+    finite += [(1 + m / 8) * 2.0 ** e for e in range(-6, 9) for m in range(8)]
+    finite = np.array(sorted(set(v for v in finite if v <= 448.0)), np.float32)
+    assert finite[-1] == 448.0 and finite[1] == 2.0 ** -9 and finite.size == 127
+    for sc in (1.0, 2.0 ** -5, 2.0 ** 3):
+        np.testing.assert_array_equal(ol.q_e4m3(finite * np.float32(sc), sc), finite * sc)
+        np.testing.assert_array_equal(ol.q_e4m3(-finite * np.float32(sc), sc), -finite * sc)
+    # midpoints go to the neighbour with the even mantissa
+    mid = ((finite[1:] + finite[:-1]) / 2).astype(np.float32)
+    q = ol.q_e4m3(mid, 1.0)
+    # 8 values per binade (and 8 subnormals), so a value's position in the
+    # sorted list has the parity of its mantissa field
+    idx = np.searchsorted(finite, q)
+    np.testing.assert_array_equal(finite[idx], q)
+    assert np.all(idx % 2 == 0)
+    # anything just off a midpoint goes to the nearer value
+    above = np.nextafter(mid, np.float32(np.inf))
+    np.testing.assert_array_equal(ol.q_e4m3(above, 1.0), finite[1:])
+    below = np.nextafter(mid, np.float32(-np.inf))
+    np.testing.assert_array_equal(ol.q_e4m3(below, 1.0), finite[:-1])
+    # saturation, subnormal quantum, signed zero
+    assert ol.q_e4m3(np.float32(1e6), 1.0) == 448.0 and ol.q_e4m3(np.float32(-500.0), 1.0) == -448.0
+    assert ol.q_e4m3(np.float32(2.0 ** -10), 1.0) == 0.0
+    assert ol.q_e4m3(np.float32(3 * 2.0 ** -11), 1.0) == 2.0 ** -9
+    assert np.signbit(ol.q_e4m3(np.float32(-0.0), 1.0))
+    # per-tensor scale: the smallest power of two with max|W| / s <= 448
+    for mx in (448.0, 449.0, 1.0, 0.3, 3e-4):
+        W = np.array([mx, -0.5 * mx], np.float32)
+        s = float(ol.e4m3_scale(W))
+        assert s == 2.0 ** np.round(np.log2(s))
+        assert mx / s <= 448.0 and mx / (s / 2) > 448.0
+    assert ol.e4m3_scale(np.zeros(4)) == 1.0
+    p = ol.init_params(3)
+    assert all(np.array_equal(a, b) for a, b in zip(ol.quantised(p, "bf16"), ol.unpack(p)))
+    W2q = ol.quantised(p, "fp8")[2]
+    s2 = ol.e4m3_scale(ol.unpack(p)[2])
+    assert np.all(np.isin(np.abs(W2q / s2).astype(np.float32), finite))
+
+
 def test_first_layer_bias_close_to_binary64():
     from oracle import learned as ol
 
