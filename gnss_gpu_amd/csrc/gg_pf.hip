@@ -90,6 +90,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.Pb = h->Pb;
   a.emax = h->emax;
   a.E = h->E;
+  a.mw = reinterpret_cast<float*>(h->E);  // Megopolis steps do not use the ends
   a.csum = h->csum;
   a.scratch_f = h->scratch_f;
   a.filt = h->filt;
@@ -343,6 +344,8 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
 
 int launch_mego(gg_pf* h, int n_iter, int force) {
   const gg_arena a = arena_of(h);
+  hipLaunchKernelGGL(gg_k_mego_w, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, force);
+  GG_HIP(hipGetLastError());
   hipLaunchKernelGGL(gg_k_mego, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
                      force);
   GG_HIP(hipGetLastError());

@@ -31,6 +31,7 @@ struct gg_arena {
   float* state;           // [2][ncomp][P]
   float* lw;              // [P]
   int32_t* E;             // [P] tile-local prefix max of the output ends
+  float* mw;              // [P] Megopolis weights exp32(lw - M) (same memory as E)
   int32_t* anc;           // [2][P]
   double* part;           // [n_tiles_total][GG_NPART]
   double* g;              // [n_tiles_total]

@@ -752,9 +752,33 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, in
 
 // ---------------------------------------------------------------------------
 // Megopolis (Chesser et al. 2021): Metropolis chains on the original weights
-// with one proposal offset per 256-particle segment and iteration, so the
-// wavefront's reads of lw[(i + O) mod n] are contiguous.  Writes anc[1].
+// with one proposal offset per 256-particle segment and iteration.
+// gg_k_mego_w first writes w_i = exp32(lw_i - M) (M: the decision's maximum;
+// 1 for zeroed weights); a chain at k then moves to proposal j when
+// u * w_k < w_j, u in (0, 1), i.e. with probability min(1, w_j / w_k) (a chain
+// on a zero weight leaves for any positive one).  Wavefront w of a tile owns
+// segment 4 * tile + w; lane l holds its particles l + 64 q, so every load and
+// store of a wavefront is 64 consecutive values.  Writes anc[1].
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int lt = tile - F.tile_off;
+  const bool lwz = dy->lw_zero != 0;
+  const float M = (float)dy->M;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const float4 lv = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
+  const float l4[GG_ITEMS] = {lv.x, lv.y, lv.z, lv.w};
+  float w[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    w[q] = (i0 + q >= F.n) ? 0.0f : (lwz ? 1.0f : gg_exp32(l4[q] - M));
+  *reinterpret_cast<float4*>(a.mw + F.p_off + i0) = make_float4(w[0], w[1], w[2], w[3]);
+}
+
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int force) {
 #pragma clang fp contract(off)
   const int tile = blockIdx.x;
@@ -765,17 +789,17 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
   const int lt = tile - F.tile_off;
   const int n = F.n;
   const uint32_t call = (uint32_t)dy->resample_count;
-  const bool lwz = dy->lw_zero != 0;
-  const float* lw = a.lw + F.p_off;
+  const float* w = a.mw + F.p_off;
   const int lane = threadIdx.x & 63;
   const uint32_t seg = (uint32_t)(lt * (GG_TILE / 256) + (threadIdx.x >> 6));
-  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int ib = (int)seg * 256 + lane;  // particle of item q: ib + 64 q
   int k[GG_ITEMS];
-  float lk[GG_ITEMS];
+  float wk[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) {
-    k[q] = i0 + q;
-    lk[q] = (lwz || i0 + q >= n) ? 0.0f : lw[i0 + q];
+    const int i = ib + 64 * q;
+    k[q] = i;
+    wk[q] = (i < n) ? w[i] : 0.0f;
   }
   for (int b0 = 0; b0 < B; b0 += 64) {
     uint32_t myO = 0;
@@ -790,34 +814,31 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
       gg_u32x4 r[GG_ITEMS];
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q)
-        r[q] = gg_draw(F.seed, (uint32_t)(i0 + q), call, GG_PURPOSE_MEGO_U,
+        r[q] = gg_draw(F.seed, (uint32_t)(ib + 64 * q), call, GG_PURPOSE_MEGO_U,
                        (uint32_t)((b0 + g0) >> 2));
       const int gend = min(bend - g0, 4);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (c >= gend) break;
-        const uint32_t O = (uint32_t)__shfl((int)myO, g0 + c, 64);
+        const int O = __builtin_amdgcn_readlane((int)myO, g0 + c);
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const int i = i0 + q;
+          const int i = ib + 64 * q;
           if (i >= n) continue;
-          const float u = gg_u01f(r[q].v[c]);
-          int j = i + (int)O;
+          int j = i + O;
           if (j >= n) j -= n;
-          const float lj = lwz ? 0.0f : lw[j];
-          const float dl = lj - lk[q];
-          // accept iff dl >= 0, or -18 <= dl < 0 and u < exp(dl); u >= 2^-24
-          // so exp(dl) < u for every dl < -18.  Evaluated without branches:
-          // the exp core runs on the clamped argument and the move is a select.
-          const float e = gg_exp32_core(fminf(fmaxf(dl, -18.0f), 0.0f));
-          const bool acc = (dl >= 0.0f) | ((dl >= -18.0f) & (u < e));
+          const float wj = w[j];
+          const bool acc = gg_u01f(r[q].v[c]) * wk[q] < wj;
           k[q] = acc ? j : k[q];
-          lk[q] = acc ? lj : lk[q];
+          wk[q] = acc ? wj : wk[q];
         }
       }
     }
   }
-  *reinterpret_cast<int4*>(a.anc + a.P + F.p_off + i0) = make_int4(k[0], k[1], k[2], k[3]);
+  int32_t* out = a.anc + a.P + F.p_off;
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    if (ib + 64 * q < n) out[ib + 64 * q] = k[q];
 }
 
 // ---------------------------------------------------------------------------

@@ -245,14 +245,26 @@ def megopolis_ref_ancestors(lw, n, n_iter, seed):
     return a.astype(np.int32)
 
 
-def megopolis_ancestors(lw, n, n_iter, seed, call):
-    """Megopolis of Chesser et al. (2021) as implemented by gg_k_mego: Metropolis
-    chains on the original weights, one offset per 256-particle segment."""
+def megopolis_weights(lw, n, lw_zero=False):
+    """w_i = exp32(lw_i - M), M = max lw (binary32; NaN ignored), or 1 for
+    zeroed weights (twin of gg_k_mego_w)."""
     lw32 = np.asarray(lw, dtype=np.float32)[:n]
+    if lw_zero:
+        return np.ones(n, dtype=np.float32)
+    M = np.float32(np.nanmax(lw32)) if n else np.float32(0.0)
+    return exp32((lw32 - M).astype(np.float32))
+
+
+def megopolis_ancestors(lw, n, n_iter, seed, call, lw_zero=False):
+    """Megopolis of Chesser et al. (2021) as implemented by gg_k_mego: Metropolis
+    chains on the original weights, one offset per 256-particle segment; a
+    chain at k moves to proposal j when u * w_k < w_j (binary32 product),
+    w = megopolis_weights."""
+    w = megopolis_weights(lw, n, lw_zero)
     i = np.arange(n, dtype=np.int64)
     seg = i // 256
     k = i.copy()
-    lk = lw32.copy()
+    wk = w.copy()
     nseg = int(seg.max()) + 1 if n else 0
     segs = np.arange(nseg)
     for b in range(n_iter):
@@ -262,13 +274,11 @@ def megopolis_ancestors(lw, n, n_iter, seed, call):
         u = philox.u01f(ru[b & 3])
         j = i + O[seg]
         j = np.where(j >= n, j - n, j)
-        lj = lw32[j]
-        dl = (lj - lk).astype(np.float32)
-        acc = dl >= 0
-        mid = (~acc) & (dl >= _F(-18.0))
-        acc[mid] = u[mid] < exp32(dl[mid])
+        wj = w[j]
+        with np.errstate(invalid="ignore"):
+            acc = (u * wk).astype(np.float32) < wj
         k = np.where(acc, j, k)
-        lk = np.where(acc, lj, lk)
+        wk = np.where(acc, wj, wk)
     return k.astype(np.int32)
 
 

@@ -45,7 +45,11 @@ PINS = {
                                         "proposal, slot-weight rule of pf_device.cu:1262-1270 "
                                         "on a hand-computed case"),
     "detmath.megopolis_ancestors": ("pinned", "chi-square against the target distribution "
-                                    "(test_megopolis_chesser_matches_target_distribution)"),
+                                    "(test_megopolis_chesser_matches_target_distribution); "
+                                    "acceptance probability min(1, w_j/w_k) measured "
+                                    "(test_megopolis_acceptance_rule)"),
+    "detmath.megopolis_weights": ("pinned", "max weight 1, ratios exp(lw_j - lw_k) to binary32 "
+                                  "(test_megopolis_acceptance_rule)"),
     "detmath.sequential_systematic": ("pinned", "textbook searchsorted(side='left') on tiny N"),
     # pf.py
     "pf.enu_rotation": ("pinned", "orthonormality and the up axis along the ellipsoid normal "

@@ -207,6 +207,31 @@ def test_megopolis_ref_slot_weight_rule():
     assert np.mean(np.isin(a, heavy)) > len(heavy) / n
 
 
+def test_megopolis_acceptance_rule():
+    """One Megopolis iteration: a chain on a heavy particle moves to a proposal
+    four times lighter with probability 1/4, and a light chain always moves
+    to a heavy proposal (Metropolis rule min(1, w_j / w_k))."""
+    n = 1 << 16
+    rng = np.random.default_rng(31)
+    heavy = rng.random(n) < 0.5
+    lw = np.where(heavy, 0.0, np.log(0.25)).astype(np.float32)
+    w = detmath.megopolis_weights(lw, n)
+    assert w.max() == 1.0
+    np.testing.assert_allclose(w[~heavy], 0.25, rtol=1e-6)
+    a = detmath.megopolis_ancestors(lw, n, 1, 17, 3)
+    i = np.arange(n)
+    r = philox.draw(17, i // 256, 3, philox.PURPOSE_MEGO, 0)
+    prop = (i + ((r[0] * np.uint64(n)) >> np.uint64(32)).astype(np.int64)) % n
+    assert np.all((a == i) | (a == prop))
+    hl = heavy & ~heavy[prop]
+    lh = ~heavy & heavy[prop]
+    moved = a != i
+    frac = moved[hl].mean()
+    assert abs(frac - 0.25) < 4 * np.sqrt(0.25 * 0.75 / hl.sum())
+    assert np.all(moved[lh])
+    assert np.all(moved[heavy == heavy[prop]])  # equal weights: always accepted
+
+
 def test_megopolis_chesser_matches_target_distribution():
     n = 8192
     rng = np.random.default_rng(8)
