@@ -6,8 +6,8 @@ implemented over the C ABI of libgnss_gpu_amd.so.  Particle state lives on the
 GPU as binary32 offsets in a local ENU frame around a binary64 anchor; every
 getter returns binary64 absolute (ECEF) values like the reference.
 
-Update kernels that are not part of the pseudorange particle-filter step
-(double-differenced, carrier, Doppler, GMM) raise NotImplementedError.
+The double-differenced pseudorange and carrier-phase AFV updates are here;
+the Doppler and GMM updates raise NotImplementedError.
 """
 
 from __future__ import annotations
@@ -101,6 +101,56 @@ def pf_device_weight(state, sat_ecef, pseudoranges, weights_sat, n_sat, sigma_pr
          iptr(sys) if sys is not None else None, n_sat, float(sigma_pr), float(nu),
          float(per_particle_nlos_threshold_m), 1 if per_particle_huber else 0,
          float(per_particle_huber_k))
+
+
+def _dd_arrays(n, sat_k, ref, obs, brk, brr, w, wl=None):
+    n = int(n)
+    arrs = [f64(sat_k).ravel(), f64(ref).ravel(), f64(obs).ravel(), f64(brk).ravel(),
+            f64(brr).ravel(), f64(w).ravel()] + ([] if wl is None else [f64(wl).ravel()])
+    need = [3 * n, 3 * n, n, n, n, n] + ([] if wl is None else [n])
+    if any(a.size < k for a, k in zip(arrs, need)):
+        raise ValueError("double-difference arrays are shorter than n_dd")
+    return n, arrs
+
+
+def pf_device_weight_dd_pseudorange(state, sat_ecef_k, ref_ecef, dd_pseudorange, base_range_k,
+                                    base_range_ref, weights_dd, n_dd, sigma_pr,
+                                    per_particle_nlos_threshold_m=0.0, per_particle_huber=False,
+                                    per_particle_huber_k=1.5):
+    """DD pseudorange weight update; overwrites the log-weights
+    (_pf_device_bindings.cpp:111-143, pf_device.cu:1576-1620)."""
+    n, a = _dd_arrays(n_dd, sat_ecef_k, ref_ecef, dd_pseudorange, base_range_k, base_range_ref,
+                      weights_dd)
+    call("gg_pf_device_weight_dd_pseudorange", state.handle, *[dptr(x) for x in a], n,
+         float(sigma_pr), float(per_particle_nlos_threshold_m), 1 if per_particle_huber else 0,
+         float(per_particle_huber_k))
+
+
+def pf_device_weight_carrier_afv(state, sat_ecef, carrier_phase, weights_sat, n_sat,
+                                 wavelength=0.190293673, sigma_cycles=0.05):
+    """Undifferenced carrier-phase AFV update; adds to the log-weights
+    (_pf_device_bindings.cpp:167-183, pf_device.cu:2062-2116)."""
+    n = int(n_sat)
+    sat = f64(sat_ecef).ravel()
+    cp = f64(carrier_phase).ravel()
+    w = f64(weights_sat).ravel()
+    if sat.size < 3 * n or cp.size < n or w.size < n:
+        raise ValueError("satellite arrays are shorter than n_sat")
+    call("gg_pf_device_weight_carrier_afv", state.handle, dptr(sat), dptr(cp), dptr(w), n,
+         float(wavelength), float(sigma_cycles))
+
+
+def pf_device_weight_dd_carrier_afv(state, sat_ecef_k, ref_ecef, dd_carrier, base_range_k,
+                                    base_range_ref, weights_dd, wavelengths_m, n_dd,
+                                    sigma_cycles=0.05, per_particle_nlos_threshold_cycles=0.0,
+                                    per_particle_huber=False, per_particle_huber_k=1.5):
+    """DD carrier-phase AFV update; adds to the log-weights
+    (_pf_device_bindings.cpp:185-220, pf_device.cu:2122-2170)."""
+    n, a = _dd_arrays(n_dd, sat_ecef_k, ref_ecef, dd_carrier, base_range_k, base_range_ref,
+                      weights_dd, wavelengths_m)
+    call("gg_pf_device_weight_dd_carrier_afv", state.handle, *[dptr(x) for x in a], n,
+         float(sigma_cycles), float(per_particle_nlos_threshold_cycles),
+         1 if per_particle_huber else 0, float(per_particle_huber_k))
 
 
 def pf_device_weight_learned(state, sat_ecef, pseudoranges, weights_sat, n_sat, cn0, los_flag,
@@ -235,16 +285,12 @@ def pf_device_sync(state):
 def _not_in_scope(name):
     def fn(*args, **kwargs):
         raise NotImplementedError(
-            f"{name}: only the pseudorange particle-filter step is implemented on this "
-            "backend (double-differenced, carrier-phase, Doppler and GMM updates are not)")
+            f"{name}: the Doppler and GMM updates are not implemented on this backend")
 
     fn.__name__ = name
     return fn
 
 
-pf_device_weight_dd_pseudorange = _not_in_scope("pf_device_weight_dd_pseudorange")
 pf_device_weight_gmm = _not_in_scope("pf_device_weight_gmm")
-pf_device_weight_carrier_afv = _not_in_scope("pf_device_weight_carrier_afv")
-pf_device_weight_dd_carrier_afv = _not_in_scope("pf_device_weight_dd_carrier_afv")
 pf_device_weight_doppler = _not_in_scope("pf_device_weight_doppler")
 pf_device_doppler_kf_update = _not_in_scope("pf_device_doppler_kf_update")

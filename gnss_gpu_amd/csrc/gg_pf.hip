@@ -51,6 +51,8 @@ struct gg_pf {
   int table_rows = 0;
   double* io_tmp = nullptr;
   size_t io_tmp_bytes = 0;
+  double* obsd = nullptr;       // [obs_cap][11] DD / carrier observations (single filter)
+  int obs_cap = 0;
   // pinned host staging
   char* pin = nullptr;
   size_t pin_bytes = 0;
@@ -535,7 +537,7 @@ int gg_pf_destroy(gg_pf* h) {
                   h->emax,  h->csum,  h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,
-                  h->io_tmp};
+                  h->io_tmp, h->obsd};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (h->pin) hipHostFree(h->pin);
@@ -728,9 +730,118 @@ int stage_single(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
   return 0;
 }
 
+// Observations of a DD / carrier update, 11 binary64 per observation (see
+// gg_k_weight_obs), through the pinned buffer to h->obsd.
+constexpr int GG_MAX_OBS = 1024;
+
+int stage_obs(gg_pf* h, int n, const double* sk, const double* sr, const double* val,
+              const double* brk, const double* brr, const double* w, const double* wl,
+              double wl_all) {
+  if (n < 0 || n > GG_MAX_OBS) return gg_fail(-2, "observation count must be in [0, 1024]");
+  if (n > h->obs_cap) {
+    GG_HIP(hipStreamSynchronize(h->stream));
+    if (h->obsd) GG_HIP(hipFree(h->obsd));
+    h->obsd = nullptr;
+    h->obs_cap = 0;
+    if (dev_alloc(&h->obsd, (size_t)std::max(n, 16) * 11)) return -1;
+    h->obs_cap = std::max(n, 16);
+  }
+  const size_t bytes = sizeof(double) * 11 * (size_t)std::max(n, 1);
+  if (ensure_pinned(h, std::max(h->pin_bytes, bytes))) return -1;
+  double* o = (double*)h->pin;
+  for (int k = 0; k < n; ++k) {
+    double* r = o + (size_t)k * 11;
+    for (int c = 0; c < 3; ++c) {
+      r[c] = sk[k * 3 + c];
+      r[3 + c] = sr ? sr[k * 3 + c] : 0.0;
+    }
+    r[6] = val[k];
+    r[7] = brk ? brk[k] : 0.0;
+    r[8] = brr ? brr[k] : 0.0;
+    r[9] = w ? w[k] : 1.0;
+    r[10] = wl ? wl[k] : wl_all;
+  }
+  if (n > 0)
+    GG_HIP(hipMemcpyAsync(h->obsd, o, sizeof(double) * 11 * n, hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  return 0;
+}
+
+int launch_obs(gg_pf* h, int kind, int n, double sigma, double thr, int huber, double huber_k) {
+  const gg_arena a = arena_of(h);
+  const size_t smem = (size_t)std::max(n, 1) * 48;
+  const float sg = (float)sigma, th = (float)thr, hk = (float)huber_k;
+#define GG_OBS_LAUNCH(K)                                                                       \
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_obs<K, NI>), dim3(h->n_tiles),      \
+                                                dim3(GG_BLOCK), smem, h->stream, a, h->obsd, n, \
+                                                sg, th, huber, hk))
+  if (kind == 0) {
+    GG_OBS_LAUNCH(0);
+  } else if (kind == 1) {
+    GG_OBS_LAUNCH(1);
+  } else {
+    GG_OBS_LAUNCH(2);
+  }
+#undef GG_OBS_LAUNCH
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_clear_lw_zero, dim3(1), dim3(64), 0, h->stream, h->dyn, 0);
+  GG_HIP(hipGetLastError());
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gg_pf_device_weight_dd_pseudorange(gg_pf* h, const double* sat_ecef_k, const double* ref_ecef,
+                                       const double* dd_pseudorange, const double* base_range_k,
+                                       const double* base_range_ref, const double* weights_dd,
+                                       int32_t n_dd, double sigma_pr,
+                                       double per_particle_nlos_threshold_m,
+                                       int32_t per_particle_huber, double per_particle_huber_k) {
+  if (int rc = check_single(h)) return rc;
+  if (n_dd > 0 && (!sat_ecef_k || !ref_ecef || !dd_pseudorange || !base_range_k ||
+                   !base_range_ref || !weights_dd))
+    return gg_fail(-2, "null argument");
+  if (stage_obs(h, n_dd, sat_ecef_k, ref_ecef, dd_pseudorange, base_range_k, base_range_ref,
+                weights_dd, nullptr, 1.0))
+    return -1;
+  return launch_obs(h, 0, n_dd, sigma_pr, per_particle_nlos_threshold_m, per_particle_huber,
+                    per_particle_huber_k);
+}
+
+int gg_pf_device_weight_carrier_afv(gg_pf* h, const double* sat_ecef, const double* carrier_phase,
+                                    const double* weights_sat, int32_t n_sat, double wavelength,
+                                    double sigma_cycles) {
+  if (int rc = check_single(h)) return rc;
+  if (n_sat > 0 && (!sat_ecef || !carrier_phase || !weights_sat))
+    return gg_fail(-2, "null argument");
+  if (stage_obs(h, n_sat, sat_ecef, nullptr, carrier_phase, nullptr, nullptr, weights_sat,
+                nullptr, wavelength))
+    return -1;
+  return launch_obs(h, 1, n_sat, sigma_cycles, 0.0, 0, 1.5);
+}
+
+int gg_pf_device_weight_dd_carrier_afv(gg_pf* h, const double* sat_ecef_k, const double* ref_ecef,
+                                       const double* dd_carrier, const double* base_range_k,
+                                       const double* base_range_ref, const double* weights_dd,
+                                       const double* wavelengths_m, int32_t n_dd,
+                                       double sigma_cycles,
+                                       double per_particle_nlos_threshold_cycles,
+                                       int32_t per_particle_huber, double per_particle_huber_k) {
+  if (int rc = check_single(h)) return rc;
+  if (n_dd > 0 && (!sat_ecef_k || !ref_ecef || !dd_carrier || !base_range_k ||
+                   !base_range_ref || !weights_dd || !wavelengths_m))
+    return gg_fail(-2, "null argument");
+  if (stage_obs(h, n_dd, sat_ecef_k, ref_ecef, dd_carrier, base_range_k, base_range_ref,
+                weights_dd, wavelengths_m, 1.0))
+    return -1;
+  return launch_obs(h, 2, n_dd, sigma_cycles, per_particle_nlos_threshold_cycles,
+                    per_particle_huber, per_particle_huber_k);
+}
 
 int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
                         const double* weights_sat, const int32_t* sys, int32_t n_sat,

@@ -1064,3 +1064,113 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_position_update(gg_arena a, int
 __global__ void gg_k_clear_lw_zero(gg_dyn* dyn, int f) {
   if (threadIdx.x == 0 && blockIdx.x == 0) dyn[f].lw_zero = 0;
 }
+
+// ---------------------------------------------------------------------------
+// Double-differenced pseudorange and carrier-phase AFV likelihoods (single
+// filter).  KIND 0: DD pseudorange (pf_device.cu:282-368), overwrites lw.
+// KIND 1: undifferenced carrier AFV with the clock bias (:439-494), adds.
+// KIND 2: DD carrier AFV with per-pair wavelengths (:502-604), adds.
+// obs holds 11 binary64 per observation: sat_k xyz, ref xyz, observation
+// (m or cycles), base_range_k, base_range_ref, weight, wavelength.
+// Each workgroup stages per observation, in the filter's ENU frame, the unit
+// vectors and inverse ranges from the anchor to both satellites and the
+// binary64 part of the residual; ranges enter per particle only through
+// r - rho = q / (1 + sqrt(1 + q / rho)), q = |p|^2 / rho - 2 u.p.  For the
+// carrier kinds the binary64 part is reduced to its fractional cycle, which
+// leaves x - rint(x) unchanged.  Dynamic LDS: 48 bytes per observation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gg_obs_delta(const float4& g, const float* p, float pp) {
+  const float ud2 = fmaf(g.z, p[2], fmaf(g.y, p[1], g.x * p[0]));  // -2 u.p
+  const float q = fmaf(pp, g.w, ud2);
+  return q * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(fmaf(q, g.w, 1.0f)));
+}
+
+template <int KIND, int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_obs(gg_arena a, const double* __restrict__ obs,
+                                                            int n_obs, float sigma, float thr,
+                                                            int huber, float huber_k) {
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
+  float4* rec = reinterpret_cast<float4*>(smem);  // [n_obs][3]
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int cur = dy->cur;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, cur, base, p);
+  for (int k = threadIdx.x; k < n_obs; k += blockDim.x) {
+    const double* o = obs + (int64_t)k * 11;
+    double dk[3], dr[3], rk, irk, rr, irr;
+    gg_sat_local(o, dy, dy->anchor, 0, dk, rk, irk);
+    gg_sat_local(o + 3, dy, dy->anchor, 0, dr, rr, irr);
+    const double wl = o[10];
+    double c;
+    if (KIND == 1) {
+      c = o[6] - (rk + dy->cb_anchor) * (1.0 / wl);
+      c = c - rint(c);
+    } else {
+      c = o[6] - (((rk - rr) - o[7]) + o[8]) * (KIND == 2 ? 1.0 / wl : 1.0);
+      if (KIND == 2) c = c - rint(c);
+    }
+    rec[3 * k + 0] = make_float4(-2.0f * (float)(dk[0] * irk), -2.0f * (float)(dk[1] * irk),
+                                 -2.0f * (float)(dk[2] * irk), (float)irk);
+    rec[3 * k + 1] = make_float4(-2.0f * (float)(dr[0] * irr), -2.0f * (float)(dr[1] * irr),
+                                 -2.0f * (float)(dr[2] * irr), (float)irr);
+    rec[3 * k + 2] = make_float4((float)c, (float)o[9], (float)(1.0 / wl), 0.0f);
+  }
+  __syncthreads();
+  const float isig = 1.0f / sigma;
+  const float inv_s2 = isig * isig;
+  const bool use_huber = huber != 0 && huber_k > 0.0f;
+  float lw[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    const float pp = fmaf(p[q][2], p[q][2], fmaf(p[q][1], p[q][1], p[q][0] * p[q][0]));
+    // residual of observation k: metres (KIND 0) or the AFV fraction of a cycle
+    auto resid = [&](int k) {
+      const float4 m = rec[3 * k + 2];
+      const float dk = gg_obs_delta(rec[3 * k + 0], p[q], pp);
+      if (KIND == 1) {
+        const float x = m.x - (dk + p[q][3]) * m.z;
+        return x - rintf(x);
+      }
+      const float dd = dk - gg_obs_delta(rec[3 * k + 1], p[q], pp);
+      if (KIND == 0) return m.x - dd;
+      const float x = m.x - dd * m.z;
+      return x - rintf(x);
+    };
+    bool gate = KIND != 1 && thr > 0.0f;
+    if (gate) {
+      int kept = 0;
+      for (int k = 0; k < n_obs; ++k) kept += (fabsf(resid(k)) <= thr) ? 1 : 0;
+      if (kept < (n_obs < 3 ? n_obs : 3)) gate = false;
+    }
+    float acc = 0.0f;
+    for (int k = 0; k < n_obs; ++k) {
+      const float r = resid(k);
+      if (gate && fabsf(r) > thr) continue;
+      const float w = rec[3 * k + 2].y;
+      if (KIND != 1 && use_huber) {
+        const float z = fabsf(r) * isig;
+        acc -= w * ((z < huber_k) ? 0.5f * z * z : huber_k * z - 0.5f * huber_k * huber_k);
+      } else {
+        acc -= 0.5f * w * r * r * inv_s2;
+      }
+    }
+    lw[q] = acc;
+  }
+  if (KIND != 0) {
+    const bool lwz = dy->lw_zero != 0;
+    const float4 old = *reinterpret_cast<const float4*>(a.lw + base);
+    const float o4[GG_ITEMS] = {old.x, old.y, old.z, old.w};
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) lw[q] = (lwz ? 0.0f : o4[q]) + lw[q];
+  }
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) lw[q] = (i0 + q < F.n) ? lw[q] : 0.0f;
+  *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
+}

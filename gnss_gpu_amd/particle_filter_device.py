@@ -201,14 +201,59 @@ class ParticleFilterDevice:
     def update_gmm(self, *args, **kwargs):
         return _dev.pf_device_weight_gmm(self._state, *args, **kwargs)
 
-    def update_dd_pseudorange(self, *args, **kwargs):
-        return _dev.pf_device_weight_dd_pseudorange(self._state, *args, **kwargs)
+    def update_dd_pseudorange(self, dd_result, sigma_pr=0.75, resample=True):
+        """DD pseudorange likelihood (no clock bias); overwrites the log-weights
+        (particle_filter_device.py:354-388)."""
+        self._require_init()
+        _dev.pf_device_weight_dd_pseudorange(
+            self._state,
+            np.asarray(dd_result.sat_ecef_k, dtype=np.float64).ravel(),
+            np.asarray(dd_result.sat_ecef_ref, dtype=np.float64).ravel(),
+            dd_result.dd_pseudorange_m, dd_result.base_range_k, dd_result.base_range_ref,
+            dd_result.dd_weights, dd_result.n_dd, float(sigma_pr),
+            self._per_particle_threshold(self.per_particle_nlos_dd_pr_threshold_m),
+            self.per_particle_huber,
+            self._per_particle_huber_k(self.per_particle_huber_dd_pr_k))
+        if resample:
+            _ = self.resample_if_needed()
 
-    def update_carrier_afv(self, *args, **kwargs):
-        return _dev.pf_device_weight_carrier_afv(self._state, *args, **kwargs)
+    def update_carrier_afv(self, sat_ecef, carrier_phase_cycles, weights=None,
+                           wavelength=0.190293673, sigma_cycles=0.05, resample=True):
+        """Undifferenced carrier-phase AFV likelihood, added to the log-weights
+        (particle_filter_device.py:390-432)."""
+        self._require_init()
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        cp = np.asarray(carrier_phase_cycles, dtype=np.float64).ravel()
+        n_sat = len(cp)
+        w = (np.ones(n_sat, dtype=np.float64) if weights is None
+             else np.asarray(weights, dtype=np.float64).ravel())
+        _dev.pf_device_weight_carrier_afv(self._state, sat.ravel(), cp, w, n_sat,
+                                          float(wavelength), float(sigma_cycles))
+        if resample:
+            _ = self.resample_if_needed()
 
-    def update_dd_carrier_afv(self, *args, **kwargs):
-        return _dev.pf_device_weight_dd_carrier_afv(self._state, *args, **kwargs)
+    def update_dd_carrier_afv(self, dd_result, wavelength=None, sigma_cycles=0.05,
+                              resample=True):
+        """DD carrier-phase AFV likelihood, added to the log-weights; per-pair
+        wavelengths from dd_result unless overridden (particle_filter_device.py:434-480)."""
+        self._require_init()
+        if wavelength is None:
+            wl = getattr(dd_result, "wavelengths_m", None)
+            wl = (np.full(dd_result.n_dd, 0.190293673, dtype=np.float64) if wl is None
+                  else np.asarray(wl, dtype=np.float64).ravel())
+        else:
+            wl = np.full(dd_result.n_dd, float(wavelength), dtype=np.float64)
+        _dev.pf_device_weight_dd_carrier_afv(
+            self._state,
+            np.asarray(dd_result.sat_ecef_k, dtype=np.float64).ravel(),
+            np.asarray(dd_result.sat_ecef_ref, dtype=np.float64).ravel(),
+            dd_result.dd_carrier_cycles, dd_result.base_range_k, dd_result.base_range_ref,
+            dd_result.dd_weights, wl, dd_result.n_dd, float(sigma_cycles),
+            self._per_particle_threshold(self.per_particle_nlos_dd_carrier_threshold_cycles),
+            self.per_particle_huber,
+            self._per_particle_huber_k(self.per_particle_huber_dd_carrier_k))
+        if resample:
+            _ = self.resample_if_needed()
 
     def update_doppler(self, *args, **kwargs):
         return _dev.pf_device_weight_doppler(self._state, *args, **kwargs)

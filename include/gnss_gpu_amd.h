@@ -163,6 +163,37 @@ int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const do
                     int32_t n_sat, int32_t max_iter, double tol, double* result,
                     int32_t* iters);
 
+/* --- double-differenced and carrier-phase updates (single filter) ---------- */
+/* reference: pf_device_weight_dd_pseudorange, src/particle_filter/pf_device.cu:1576-1620
+ * (kernel :282-368).  Overwrites the log-weights with the DD pseudorange
+ * likelihood over n_dd pairs (sat_ecef_k / ref_ecef [n_dd][3] ECEF, dd_pseudorange,
+ * base_range_k, base_range_ref, weights_dd [n_dd]); no clock bias.  Gate: a
+ * threshold > 0 drops |residual| > threshold unless fewer than min(n_dd, 3)
+ * remain; per_particle_huber selects the weighted Huber cost.  n_dd <= 1024. */
+int gg_pf_device_weight_dd_pseudorange(gg_pf* h, const double* sat_ecef_k, const double* ref_ecef,
+                                       const double* dd_pseudorange, const double* base_range_k,
+                                       const double* base_range_ref, const double* weights_dd,
+                                       int32_t n_dd, double sigma_pr,
+                                       double per_particle_nlos_threshold_m,
+                                       int32_t per_particle_huber, double per_particle_huber_k);
+/* reference: pf_device_weight_carrier_afv, pf_device.cu:2062-2116 (kernel :439-494).
+ * Adds the undifferenced carrier-phase AFV log-likelihood
+ * -w afv^2 / (2 sigma^2), afv = x - rint(x), x = phase - (range + cb) / wavelength,
+ * to the log-weights (carrier_phase in cycles). */
+int gg_pf_device_weight_carrier_afv(gg_pf* h, const double* sat_ecef, const double* carrier_phase,
+                                    const double* weights_sat, int32_t n_sat, double wavelength,
+                                    double sigma_cycles);
+/* reference: pf_device_weight_dd_carrier_afv, pf_device.cu:2122-2170 (kernel :502-604).
+ * Adds the DD carrier-phase AFV log-likelihood with per-pair wavelengths; gate in
+ * cycles and Huber as for the DD pseudorange update. */
+int gg_pf_device_weight_dd_carrier_afv(gg_pf* h, const double* sat_ecef_k, const double* ref_ecef,
+                                       const double* dd_carrier, const double* base_range_k,
+                                       const double* base_range_ref, const double* weights_dd,
+                                       const double* wavelengths_m, int32_t n_dd,
+                                       double sigma_cycles,
+                                       double per_particle_nlos_threshold_cycles,
+                                       int32_t per_particle_huber, double per_particle_huber_k);
+
 /* --- learned likelihood (no reference counterpart) -------------------------- */
 /* An 8 -> 64 -> 64 -> 2 MLP per (particle, satellite) pair predicting the NLOS
  * bias mean and log-variance of the pseudorange residual; it replaces the

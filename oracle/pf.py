@@ -6,6 +6,7 @@ Follows src/particle_filter/pf_device.cu:
   ESS       :924-965 + :1722-1762          estimate :968-1023 + :1926-1969
   spread    :1025-1076 + :1764-1807        systematic :1813-1878
   Megopolis :1231-1298 + :1884-1920
+  DD pseudorange :282-368   carrier AFV :439-494   DD carrier AFV :502-604
 and python/gnss_gpu/particle_filter_device.py:198-303,679-714 for the
 orchestration.  State is absolute ECEF binary64 ([x y z cb isb...]), like the
 reference.  Random draws: the reference's cuRAND streams cannot be reproduced
@@ -68,6 +69,59 @@ def log_likelihood(pos, cb, isb, sat, pr, w, sys, sigma_pr, nu=0.0, gate_thr=0.0
     else:
         term = -0.5 * w[None, :] * resid * resid / (sigma_pr * sigma_pr)
     return np.sum(np.where(keep, term, 0.0), axis=1)
+
+
+def _gated_sum(resid, w, sigma, thr, huber, huber_k):
+    """Sum over observations of the Gaussian or weighted-Huber cost with the
+    per-particle gate of the DD kernels (min(n, 3) kept, pf_device.cu:317-335)."""
+    n = resid.shape[1]
+    keep = np.ones(resid.shape, dtype=bool)
+    if thr > 0.0:
+        keep = np.abs(resid) <= thr
+        gate_on = keep.sum(axis=1) >= min(n, 3)
+        keep = np.where(gate_on[:, None], keep, True)
+    if huber and huber_k > 0.0:
+        term = -huber_cost(resid, sigma, w[None, :], huber_k)
+    else:
+        term = -0.5 * w[None, :] * resid * resid / (sigma * sigma)
+    return np.sum(np.where(keep, term, 0.0), axis=1)
+
+
+def dd_pseudorange_loglik(pos, sat_k, sat_ref, dd, br_k, br_ref, w, sigma, thr=0.0,
+                          huber=False, huber_k=1.5):
+    """DD pseudorange log-weight, pf_device.cu:282-368 (overwrites):
+    residual = dd - (r_k - r_ref - br_k + br_ref)."""
+    rk = np.linalg.norm(pos[:, None, :] - np.asarray(sat_k)[None, :, :], axis=2)
+    rr = np.linalg.norm(pos[:, None, :] - np.asarray(sat_ref)[None, :, :], axis=2)
+    resid = np.asarray(dd)[None, :] - (((rk - rr) - np.asarray(br_k)[None, :]) +
+                                       np.asarray(br_ref)[None, :])
+    return _gated_sum(resid, np.asarray(w, dtype=np.float64), sigma, thr, huber, huber_k)
+
+
+def afv(x):
+    """Distance to the nearest integer cycle, x - rint(x)."""
+    x = np.asarray(x, dtype=np.float64)
+    return x - np.rint(x)
+
+
+def carrier_afv_loglik(pos, cb, sat, cp, w, wavelength, sigma_cycles):
+    """Undifferenced carrier AFV term, pf_device.cu:439-494 (added to lw):
+    -w afv^2 / (2 sigma^2), afv of cp - (r + cb) / wavelength."""
+    r = np.linalg.norm(pos[:, None, :] - np.asarray(sat)[None, :, :], axis=2)
+    a = afv(np.asarray(cp)[None, :] - (r + np.asarray(cb)[:, None]) * (1.0 / wavelength))
+    return np.sum(-0.5 * np.asarray(w)[None, :] * a * a / (sigma_cycles * sigma_cycles), axis=1)
+
+
+def dd_carrier_afv_loglik(pos, sat_k, sat_ref, ddc, br_k, br_ref, w, wl, sigma_cycles, thr=0.0,
+                          huber=False, huber_k=1.5):
+    """DD carrier AFV term, pf_device.cu:502-604 (added to lw), per-pair
+    wavelengths; gate and Huber on the AFV in cycles."""
+    rk = np.linalg.norm(pos[:, None, :] - np.asarray(sat_k)[None, :, :], axis=2)
+    rr = np.linalg.norm(pos[:, None, :] - np.asarray(sat_ref)[None, :, :], axis=2)
+    exp_c = (((rk - rr) - np.asarray(br_k)[None, :]) + np.asarray(br_ref)[None, :]) * \
+        (1.0 / np.asarray(wl, dtype=np.float64))[None, :]
+    a = afv(np.asarray(ddc)[None, :] - exp_c)
+    return _gated_sum(a, np.asarray(w, dtype=np.float64), sigma_cycles, thr, huber, huber_k)
 
 
 def ess(lw):

@@ -66,6 +66,14 @@ PINS = {
                             "noise values parity unpinned (cuRAND)"),
     "pf.OraclePF.weight": ("pinned", "as log_likelihood"),
     "pf.OraclePF.resample_systematic": ("pinned", "as systematic_ancestors"),
+    "pf._gated_sum": ("pinned", "inside the DD closed forms (gate: all kept / too few kept)"),
+    "pf.dd_pseudorange_loglik": ("pinned", "reference closed forms -5000 / -148.875 "
+                                 "(test_cuda_streams.py:696-737; test_dd_closed_forms)"),
+    "pf.afv": ("pinned", "integer cycles map to 0, x+k to x, |afv| <= 1/2 (test_dd_closed_forms)"),
+    "pf.carrier_afv_loglik": ("pinned", "zero at the truth, integer-cycle invariance, a 0.3-cycle "
+                              "offset gives -0.5 (0.3/sigma)^2 (test_dd_closed_forms)"),
+    "pf.dd_carrier_afv_loglik": ("pinned", "reference closed forms -40.5 / -12.375 "
+                                 "(test_cuda_streams.py:739-771; test_dd_closed_forms)"),
     "pf.OraclePF.resample_megopolis": ("pinned", "as megopolis_ref_ancestors"),
     "pf.OraclePF.resample_megopolis_chesser": ("pinned", "as megopolis_ancestors"),
     # wls.py

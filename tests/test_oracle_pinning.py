@@ -553,3 +553,69 @@ def test_cpu_wls_baseline_matches_oracle():
     ref, rit = owls.wls_batch_ref(sat, pr, w, n_sat=ns)
     np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6)
     np.testing.assert_array_equal(it, rit)
+
+
+# ------------------------------------------- DD pseudorange / carrier AFV ----
+def _dd_scenario():
+    d = GOLD["dd_closed_forms"]
+    rng = np.random.RandomState(123)
+    dirs = rng.randn(5, 3)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    sat = dirs * 20_000_000.0
+    rover = np.zeros(3)
+    base = np.array(d["base_pos"])
+    rr = np.linalg.norm(sat - rover, axis=1)
+    br = np.linalg.norm(sat - base, axis=1)
+    dd = (rr[1:] - rr[0]) - (br[1:] - br[0])
+    return d, sat, rover, br, dd
+
+
+def test_dd_closed_forms():
+    """The reference's own DD constants (test_cuda_streams.py:604-771) from the
+    oracle's binary64 DD pseudorange and DD carrier AFV terms."""
+    d, sat, rover, br, dd = _dd_scenario()
+    n = len(dd)
+    sk, sr = sat[1:], np.repeat(sat[[0]], n, axis=0)
+    brk, brr, w = br[1:], np.repeat(br[0], n), np.ones(n)
+    pos = rover[None, :]
+    k = d["huber_k"]
+    ddo = dd.copy()
+    ddo[0] += d["dd_pr_outlier_m"]
+    s = d["dd_pr_sigma_m"]
+    assert opf.dd_pseudorange_loglik(pos, sk, sr, ddo, brk, brr, w, s)[0] == pytest.approx(
+        d["dd_pr_gaussian"])
+    assert opf.dd_pseudorange_loglik(pos, sk, sr, ddo, brk, brr, w, s, huber=True,
+                                     huber_k=k)[0] == pytest.approx(d["dd_pr_huber"])
+    gated = opf.dd_pseudorange_loglik(pos, sk, sr, ddo, brk, brr, w, s, thr=d["dd_pr_gate_m"])
+    assert abs(gated[0]) < d["gated_abs_max"]
+    wl = np.full(n, d["wavelength_m"])
+    ddc = dd / wl
+    ddc[0] += d["dd_cp_outlier_cycles"]
+    sc = d["dd_cp_sigma_cycles"]
+    assert opf.dd_carrier_afv_loglik(pos, sk, sr, ddc, brk, brr, w, wl, sc)[0] == pytest.approx(
+        d["dd_cp_gaussian"], abs=1e-9)
+    assert opf.dd_carrier_afv_loglik(pos, sk, sr, ddc, brk, brr, w, wl, sc, huber=True,
+                                     huber_k=k)[0] == pytest.approx(d["dd_cp_huber"], abs=1e-9)
+    g2 = opf.dd_carrier_afv_loglik(pos, sk, sr, ddc, brk, brr, w, wl, sc,
+                                   thr=d["dd_cp_gate_cycles"])
+    assert abs(g2[0]) < d["gated_abs_max"]
+    # every pair beyond the gate: fewer than min(n, 3) kept, so the gate is off
+    ddo2 = dd + 20.0
+    full = opf.dd_pseudorange_loglik(pos, sk, sr, ddo2, brk, brr, w, s, thr=10.0)
+    assert full[0] == pytest.approx(-0.5 * 400.0 / (s * s) * n)
+    # AFV: integer cycles vanish, x + k maps to x, |afv| <= 1/2
+    x = np.array([0.0, 3.0, -7.0, 0.25, 2.25, -0.49, 0.51])
+    np.testing.assert_allclose(opf.afv(x), [0, 0, 0, 0.25, 0.25, -0.49, -0.49], atol=1e-12)
+    # undifferenced carrier AFV: zero at the truth, whole cycles invisible,
+    # a 0.3-cycle offset costs -0.5 (0.3 / sigma)^2 per satellite
+    cb = np.array([100.0])
+    r = np.linalg.norm(sat - rover, axis=1)
+    cp = (r + 100.0) / d["wavelength_m"]
+    w5 = np.ones(5)
+    assert abs(opf.carrier_afv_loglik(pos, cb, sat, cp, w5, d["wavelength_m"], 0.05)[0]) < 1e-9
+    assert abs(opf.carrier_afv_loglik(pos, cb, sat, cp + 17.0, w5, d["wavelength_m"],
+                                      0.05)[0]) < 1e-9
+    cp3 = cp.copy()
+    cp3[2] += 0.3
+    assert opf.carrier_afv_loglik(pos, cb, sat, cp3, w5, d["wavelength_m"], 0.05)[0] == \
+        pytest.approx(-0.5 * (0.3 / 0.05) ** 2, rel=1e-6)

@@ -394,3 +394,132 @@ def test_end_to_end_plain_filter_matches_oracle(gpu):
     assert mism == 0
     assert max(diffs) < 1e-3
     assert np.median(errs) < 10.0
+
+
+def _dd_case():
+    d = GOLD["dd_closed_forms"]
+    rng = np.random.RandomState(123)
+    dirs = rng.randn(5, 3)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    sat = dirs * 20_000_000.0
+    base = np.array(d["base_pos"])
+    rr = np.linalg.norm(sat, axis=1)
+    br = np.linalg.norm(sat - base, axis=1)
+    dd = (rr[1:] - rr[0]) - (br[1:] - br[0])
+    n = len(dd)
+    from types import SimpleNamespace
+
+    def res(**kw):
+        return SimpleNamespace(sat_ecef_k=sat[1:], sat_ecef_ref=np.repeat(sat[[0]], n, axis=0),
+                               base_range_k=br[1:], base_range_ref=np.repeat(br[0], n),
+                               dd_weights=np.ones(n), n_dd=n, **kw)
+    return d, dd, res
+
+
+@pytest.mark.parametrize("huber", [False, True])
+def test_dd_closed_forms_through_device(gpu, huber):
+    """The reference's DD pseudorange and DD carrier AFV constants
+    (test_cuda_streams.py:604-771) through ParticleFilterDevice; log-weights
+    are binary32 here, so the comparison is at 1e-6 relative."""
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    d, dd, res = _dd_case()
+
+    def lw(update, gate=False, **kw):
+        pf = ParticleFilterDevice(n_particles=1024, sigma_pos=1.0, sigma_cb=300.0, sigma_pr=5.0,
+                                  resampling="systematic", seed=SEED, per_particle_huber=huber,
+                                  per_particle_nlos_gate=gate, **kw)
+        pf.initialize(np.zeros(3), clock_bias=0.0, spread_pos=0.0, spread_cb=0.0)
+        update(pf)
+        return pf.get_log_weights()
+
+    ddo = dd.copy()
+    ddo[0] += d["dd_pr_outlier_m"]
+    out = lw(lambda pf: pf.update_dd_pseudorange(res(dd_pseudorange_m=ddo),
+                                                 sigma_pr=d["dd_pr_sigma_m"], resample=False))
+    want = d["dd_pr_huber"] if huber else d["dd_pr_gaussian"]
+    np.testing.assert_allclose(out, want, rtol=1e-6)
+    gated = lw(lambda pf: pf.update_dd_pseudorange(res(dd_pseudorange_m=ddo),
+                                                   sigma_pr=d["dd_pr_sigma_m"], resample=False),
+               gate=True, per_particle_nlos_dd_pr_threshold_m=d["dd_pr_gate_m"])
+    assert float(np.max(np.abs(gated))) < d["gated_abs_max"]
+    wl = np.full(len(dd), d["wavelength_m"])
+    ddc = dd / wl
+    ddc[0] += d["dd_cp_outlier_cycles"]
+    out = lw(lambda pf: pf.update_dd_carrier_afv(res(dd_carrier_cycles=ddc, wavelengths_m=wl),
+                                                 sigma_cycles=d["dd_cp_sigma_cycles"],
+                                                 resample=False))
+    want = d["dd_cp_huber"] if huber else d["dd_cp_gaussian"]
+    np.testing.assert_allclose(out, want, rtol=1e-6)
+    gated = lw(lambda pf: pf.update_dd_carrier_afv(res(dd_carrier_cycles=ddc, wavelengths_m=wl),
+                                                   sigma_cycles=d["dd_cp_sigma_cycles"],
+                                                   resample=False),
+               gate=True, per_particle_nlos_dd_carrier_threshold_cycles=d["dd_cp_gate_cycles"])
+    assert float(np.max(np.abs(gated))) < d["gated_abs_max"]
+
+
+def test_dd_and_carrier_afv_match_oracle(gpu):
+    """A particle cloud around the Tokyo receiver with a base 800 m away:
+    DD pseudorange (Gaussian, Huber, gate), undifferenced carrier AFV and DD
+    carrier AFV against the binary64 terms of the oracle.  The carrier terms
+    add to the log-weights, so they are checked as differences."""
+    from gnss_gpu_amd import ParticleFilterDevice
+
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    base = tp + np.array([500.0, -600.0, 120.0])
+    rng = np.random.default_rng(9)
+    S = len(sat)
+    rr = np.linalg.norm(sat - tp, axis=1)
+    br = np.linalg.norm(sat - base, axis=1)
+    ref = 3
+    ks = [k for k in range(S) if k != ref]
+    dd = (rr[ks] - rr[ref]) - (br[ks] - br[ref]) + rng.normal(0, 0.3, len(ks))
+    dd[1] += 25.0
+    wl = np.full(len(ks), 0.190293673)
+    wl[::2] = 0.2442102134
+    ddc = dd / wl + rng.normal(0, 0.02, len(ks))
+    w = rng.uniform(0.5, 1.5, len(ks))
+    sk, sr = sat[ks], np.repeat(sat[[ref]], len(ks), axis=0)
+    brk, brr = br[ks], np.repeat(br[ref], len(ks))
+    from types import SimpleNamespace
+
+    def res(**kw):
+        return SimpleNamespace(sat_ecef_k=sk, sat_ecef_ref=sr, base_range_k=brk,
+                               base_range_ref=brr, dd_weights=w, n_dd=len(ks), **kw)
+
+    for huber, gate in ((False, False), (True, False), (False, True)):
+        pf = ParticleFilterDevice(n_particles=20_000, resampling="systematic", seed=SEED,
+                                  per_particle_huber=huber, per_particle_nlos_gate=gate,
+                                  per_particle_nlos_dd_pr_threshold_m=3.0)
+        pf.initialize(tp, clock_bias=50.0, spread_pos=4.0, spread_cb=2.0)
+        pf.update_dd_pseudorange(res(dd_pseudorange_m=dd), sigma_pr=0.8, resample=False)
+        st = pf.get_particles()
+        ref_lw = opf.dd_pseudorange_loglik(st[:, :3], sk, sr, dd, brk, brr, w, 0.8,
+                                           thr=3.0 if gate else 0.0, huber=huber)
+        np.testing.assert_allclose(pf.get_log_weights(), ref_lw, rtol=1e-5, atol=2e-3)
+    # carrier terms on top of a pseudorange update (and after a resample)
+    pf = ParticleFilterDevice(n_particles=20_000, resampling="systematic", seed=SEED,
+                              sigma_pr=5.0)
+    pf.initialize(tp, clock_bias=50.0, spread_pos=1.5, spread_cb=0.3)
+    cp = (rr + 50.0) / 0.190293673 + rng.normal(0, 0.02, S) + 1234.0
+    pf.update(sat, rr + 50.0 + rng.normal(0, 2.0, S), resample=False)
+    lw0 = pf.get_log_weights()
+    pf.update_carrier_afv(sat, cp, sigma_cycles=0.05, resample=False)
+    st = pf.get_particles()
+    add = opf.carrier_afv_loglik(st[:, :3], st[:, 3], sat, cp, np.ones(S), 0.190293673, 0.05)
+    np.testing.assert_allclose(pf.get_log_weights() - lw0, add, rtol=1e-4, atol=0.05)
+    lw1 = pf.get_log_weights()
+    pf.update_dd_carrier_afv(res(dd_carrier_cycles=ddc, wavelengths_m=wl), sigma_cycles=0.05,
+                             resample=False)
+    add2 = opf.dd_carrier_afv_loglik(st[:, :3], sk, sr, ddc, brk, brr, w, wl, 0.05)
+    np.testing.assert_allclose(pf.get_log_weights() - lw1, add2, rtol=1e-4, atol=0.05)
+    # after a resample the weights restart from zero
+    from gnss_gpu_amd import _gnss_gpu_pf_device as dev
+
+    dev.pf_device_resample_systematic(pf._state, 7)
+    pf.update_carrier_afv(sat, cp, sigma_cycles=0.05, resample=False)
+    st = pf.get_particles()
+    add3 = opf.carrier_afv_loglik(st[:, :3], st[:, 3], sat, cp, np.ones(S), 0.190293673, 0.05)
+    np.testing.assert_allclose(pf.get_log_weights(), add3, rtol=1e-4, atol=0.05)
