@@ -6,8 +6,8 @@ implemented over the C ABI of libgnss_gpu_amd.so.  Particle state lives on the
 GPU as binary32 offsets in a local ENU frame around a binary64 anchor; every
 getter returns binary64 absolute (ECEF) values like the reference.
 
-The double-differenced pseudorange and carrier-phase AFV updates are here;
-the Doppler and GMM updates raise NotImplementedError.
+The double-differenced pseudorange, carrier-phase AFV and Doppler updates are
+here; the GMM update raises NotImplementedError.
 """
 
 from __future__ import annotations
@@ -153,6 +153,35 @@ def pf_device_weight_dd_carrier_afv(state, sat_ecef_k, ref_ecef, dd_carrier, bas
          1 if per_particle_huber else 0, float(per_particle_huber_k))
 
 
+def _doppler_arrays(sat_ecef, sat_vel, doppler_hz, weights_sat, n_sat):
+    n = int(n_sat)
+    sat, vel = f64(sat_ecef).ravel(), f64(sat_vel).ravel()
+    dop, w = f64(doppler_hz).ravel(), f64(weights_sat).ravel()
+    if sat.size < 3 * n or vel.size < 3 * n or dop.size < n or w.size < n:
+        raise ValueError("Doppler arrays are shorter than n_sat")
+    return n, sat, vel, dop, w
+
+
+def pf_device_weight_doppler(state, sat_ecef, sat_vel, doppler_hz, weights_sat, n_sat,
+                             wavelength_m=0.19029367279836488, sigma_mps=0.5,
+                             velocity_update_gain=0.25, max_velocity_update_mps=10.0):
+    """Doppler likelihood and per-particle velocity nudge
+    (_pf_device_bindings.cpp:225-253, pf_device.cu:2183-2226)."""
+    n, sat, vel, dop, w = _doppler_arrays(sat_ecef, sat_vel, doppler_hz, weights_sat, n_sat)
+    call("gg_pf_device_weight_doppler", state.handle, dptr(sat), dptr(vel), dptr(dop), dptr(w),
+         n, float(wavelength_m), float(sigma_mps), float(velocity_update_gain),
+         float(max_velocity_update_mps))
+
+
+def pf_device_doppler_kf_update(state, sat_ecef, sat_vel, doppler_hz, weights_sat, n_sat,
+                                wavelength_m=0.19029367279836488, sigma_mps=0.5):
+    """Per-particle velocity Kalman update from Doppler rows (RBPF)
+    (_pf_device_bindings.cpp:255-278, pf_device.cu:2228-2270)."""
+    n, sat, vel, dop, w = _doppler_arrays(sat_ecef, sat_vel, doppler_hz, weights_sat, n_sat)
+    call("gg_pf_device_doppler_kf_update", state.handle, dptr(sat), dptr(vel), dptr(dop),
+         dptr(w), n, float(wavelength_m), float(sigma_mps))
+
+
 def pf_device_weight_learned(state, sat_ecef, pseudoranges, weights_sat, n_sat, cn0, los_flag,
                              system_ids=None, table=False):
     """Weight update with the attached learned likelihood (no reference
@@ -285,12 +314,10 @@ def pf_device_sync(state):
 def _not_in_scope(name):
     def fn(*args, **kwargs):
         raise NotImplementedError(
-            f"{name}: the Doppler and GMM updates are not implemented on this backend")
+            f"{name}: the GMM update is not implemented on this backend")
 
     fn.__name__ = name
     return fn
 
 
 pf_device_weight_gmm = _not_in_scope("pf_device_weight_gmm")
-pf_device_weight_doppler = _not_in_scope("pf_device_weight_doppler")
-pf_device_doppler_kf_update = _not_in_scope("pf_device_doppler_kf_update")

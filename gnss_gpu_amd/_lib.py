@@ -131,6 +131,8 @@ _SIGS = {
     "gg_pf_device_weight_carrier_afv": [_P, _DP, _DP, _DP, _I, _D, _D],
     "gg_pf_device_weight_dd_carrier_afv": [_P, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _I, _D, _D, _I,
                                            _D],
+    "gg_pf_device_weight_doppler": [_P, _DP, _DP, _DP, _DP, _I, _D, _D, _D, _D],
+    "gg_pf_device_doppler_kf_update": [_P, _DP, _DP, _DP, _DP, _I, _D, _D],
     "gg_pf_device_ess": [_P, _DP],
     "gg_pf_device_position_spread": [_P, _D, _D, _D, _DP],
     "gg_pf_device_resample_systematic": [_P, _U64, _I],

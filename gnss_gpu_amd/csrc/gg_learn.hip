@@ -462,7 +462,9 @@ template <typename T>
 int alloc(T** p, size_t n) {
   *p = nullptr;
   GG_HIP(hipMalloc((void**)p, (n ? n : 1) * sizeof(T)));
-  GG_HIP(hipMemset(*p, 0, (n ? n : 1) * sizeof(T)));
+  // the handle's stream does not synchronise with the null stream
+  GG_HIP(hipMemsetAsync(*p, 0, (n ? n : 1) * sizeof(T), nullptr));
+  GG_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
 

@@ -78,7 +78,10 @@ int dev_alloc(T** p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
   GG_HIP(hipMalloc((void**)p, count * sizeof(T)));
-  GG_HIP(hipMemset(*p, 0, count * sizeof(T)));
+  // the handle's streams do not synchronise with the null stream, so the
+  // clear must be complete before any of them touches the buffer
+  GG_HIP(hipMemsetAsync(*p, 0, count * sizeof(T), nullptr));
+  GG_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
 
@@ -792,9 +795,87 @@ int launch_obs(gg_pf* h, int kind, int n, double sigma, double thr, int huber, d
   return 0;
 }
 
+// Doppler rows, 8 binary64 per satellite (see gg_k_doppler), to h->obsd.
+int stage_doppler(gg_pf* h, int n, const double* sat, const double* vel, const double* dop,
+                  const double* w) {
+  if (n > 512) return gg_fail(-2, "at most 512 Doppler satellites");
+  if (n > h->obs_cap) {
+    GG_HIP(hipStreamSynchronize(h->stream));
+    if (h->obsd) GG_HIP(hipFree(h->obsd));
+    h->obsd = nullptr;
+    h->obs_cap = 0;
+    if (dev_alloc(&h->obsd, (size_t)std::max(n, 16) * 11)) return -1;
+    h->obs_cap = std::max(n, 16);
+  }
+  const size_t bytes = sizeof(double) * 8 * (size_t)n;
+  if (ensure_pinned(h, std::max(h->pin_bytes, bytes))) return -1;
+  double* o = (double*)h->pin;
+  for (int k = 0; k < n; ++k) {
+    double* r = o + (size_t)k * 8;
+    for (int c = 0; c < 3; ++c) {
+      r[c] = sat[k * 3 + c];
+      r[3 + c] = vel[k * 3 + c];
+    }
+    r[6] = dop[k];
+    r[7] = w ? w[k] : 1.0;
+  }
+  GG_HIP(hipMemcpyAsync(h->obsd, o, bytes, hipMemcpyHostToDevice, h->stream));
+  GG_HIP(hipEventRecord(h->pin_done, h->stream));
+  return 0;
+}
+
+int launch_doppler(gg_pf* h, bool kf, int n, double wl, double sigma, double gain,
+                   double max_dv) {
+  const gg_arena a = arena_of(h);
+  const int grid = (h->filt_h[0].n + GG_BLOCK - 1) / GG_BLOCK;
+  const size_t smem = (size_t)n * 64;
+  if (kf) {
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_doppler<NI, true>), dim3(grid),
+                                                  dim3(GG_BLOCK), smem, h->stream, a, h->obsd, n,
+                                                  wl, sigma, gain, max_dv));
+  } else {
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_doppler<NI, false>), dim3(grid),
+                                                  dim3(GG_BLOCK), smem, h->stream, a, h->obsd, n,
+                                                  wl, sigma, gain, max_dv));
+  }
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_set_vel_mode, dim3(1), dim3(64), 0, h->stream, h->dyn, 0, kf ? 2 : 1);
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_clear_lw_zero, dim3(1), dim3(64), 0, h->stream, h->dyn, 0);
+  GG_HIP(hipGetLastError());
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gg_pf_device_weight_doppler(gg_pf* h, const double* sat_ecef, const double* sat_vel,
+                                const double* doppler_hz, const double* weights_sat,
+                                int32_t n_sat, double wavelength_m, double sigma_mps,
+                                double velocity_update_gain, double max_velocity_update_mps) {
+  if (int rc = check_single(h)) return rc;
+  if (n_sat <= 0 || !(sigma_mps > 0.0) || !(wavelength_m > 0.0)) return 0;
+  if (!h->has_vel) return gg_fail(-2, "Doppler updates need velocity storage");
+  if (!sat_ecef || !sat_vel || !doppler_hz) return gg_fail(-2, "null argument");
+  if (stage_doppler(h, n_sat, sat_ecef, sat_vel, doppler_hz, weights_sat)) return -1;
+  return launch_doppler(h, false, n_sat, wavelength_m, sigma_mps, velocity_update_gain,
+                        max_velocity_update_mps);
+}
+
+int gg_pf_device_doppler_kf_update(gg_pf* h, const double* sat_ecef, const double* sat_vel,
+                                   const double* doppler_hz, const double* weights_sat,
+                                   int32_t n_sat, double wavelength_m, double sigma_mps) {
+  if (int rc = check_single(h)) return rc;
+  if (n_sat <= 0 || !(sigma_mps > 0.0) || !(wavelength_m > 0.0)) return 0;
+  if (!h->has_vel) return gg_fail(-2, "Doppler updates need velocity storage");
+  if (!sat_ecef || !sat_vel || !doppler_hz) return gg_fail(-2, "null argument");
+  if (stage_doppler(h, n_sat, sat_ecef, sat_vel, doppler_hz, weights_sat)) return -1;
+  return launch_doppler(h, true, n_sat, wavelength_m, sigma_mps, 0.0, 0.0);
+}
 
 int gg_pf_device_weight_dd_pseudorange(gg_pf* h, const double* sat_ecef_k, const double* ref_ecef,
                                        const double* dd_pseudorange, const double* base_range_k,
@@ -1238,7 +1319,7 @@ int gg_pfb_load_epochs(gg_pf* h, int32_t n_slots, const double* sat_xyz, const d
   if (vel) {
     GG_HIP(hipMemcpy(h->vel_guide, vel, sizeof(double) * 3 * F * n_slots, hipMemcpyHostToDevice));
   } else {
-    GG_HIP(hipMemset(h->vel_guide, 0, sizeof(double) * 3 * F * n_slots));
+    GG_HIP(hipMemsetAsync(h->vel_guide, 0, sizeof(double) * 3 * F * n_slots, h->stream));
   }
   return 0;
 }

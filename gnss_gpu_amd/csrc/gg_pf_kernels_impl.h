@@ -1174,3 +1174,209 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_obs(gg_arena a, const do
   for (int q = 0; q < GG_ITEMS; ++q) lw[q] = (i0 + q < F.n) ? lw[q] : 0.0f;
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
 }
+
+// ---------------------------------------------------------------------------
+// Doppler velocity updates (single filter), binary64 per particle in the
+// filter's ENU frame (range rates and the Kalman update are invariant under
+// the rotation).  obs: 8 binary64 per satellite: ECEF position, ECEF
+// velocity, Doppler [Hz], weight.  Observed range rate = -doppler * lambda.
+// KF = false: pfd_weight_doppler_kernel (pf_device.cu:610-740): clock drift
+//   as the weighted mean misfit, Gaussian log-likelihood added to lw, then
+//   (gain > 0, >= 4 satellites) the velocity moves toward the particle's
+//   4-state Doppler WLS solution, the step capped at max_dv.
+// KF = true: pfd_doppler_kf_update_kernel (:742-880): sequential scalar
+//   Kalman updates of the particle's velocity Gaussian with the line-of-sight
+//   rows and observations centred on their weighted means (clock drift
+//   eliminated); adds sum of -(innov^2 / S + log S) / 2 to lw.
+// Velocities and covariances are read per particle when the filter holds
+// them (vel_mode >= 1 / 2), otherwise from the filter record, and are written
+// per particle; gg_k_set_vel_mode raises vel_mode afterwards.
+// Dynamic LDS: 64 bytes per satellite.
+// ---------------------------------------------------------------------------
+__device__ bool gg_solve4(double (&A)[4][5], double (&x)[4]) {
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 4; ++r)
+      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+    if (piv != col)
+      for (int k = 0; k < 5; ++k) {
+        const double t = A[col][k];
+        A[col][k] = A[piv][k];
+        A[piv][k] = t;
+      }
+    if (fabs(A[col][col]) < 1e-12) return false;
+    for (int r = col + 1; r < 4; ++r) {
+      const double fct = A[r][col] / A[col][col];
+      for (int k = col; k < 5; ++k) A[r][k] -= fct * A[col][k];
+    }
+  }
+  for (int r = 3; r >= 0; --r) {
+    double v = A[r][4];
+    for (int c = r + 1; c < 4; ++c) v -= A[r][c] * x[c];
+    x[r] = v / A[r][r];
+  }
+  return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(x[3]);
+}
+
+template <int NISB, bool KF>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_doppler(gg_arena a, const double* __restrict__ obs,
+                                                         int n_sat, double wl, double sigma,
+                                                         double gain, double max_dv) {
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
+  double* sd = reinterpret_cast<double*>(smem);  // [n_sat][8]
+  const gg_filter F = a.filt[0];
+  const gg_dyn* dy = a.dyn;
+  for (int k = threadIdx.x; k < n_sat; k += blockDim.x) {
+    const double* o = obs + (int64_t)k * 8;
+    const double dx[3] = {o[0] - dy->anchor[0], o[1] - dy->anchor[1], o[2] - dy->anchor[2]};
+    gg_rot_local(dy->R, dx, sd + k * 8);
+    gg_rot_local(dy->R, o + 3, sd + k * 8 + 3);
+    sd[k * 8 + 6] = -o[6] * wl;
+    sd[k * 8 + 7] = fmax(o[7], 0.0);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  const int vm = dy->vel_mode;
+  const double x[3] = {(double)gg_comp(a, cur, 0)[s], (double)gg_comp(a, cur, 1)[s],
+                       (double)gg_comp(a, cur, 2)[s]};
+  double v[3];
+  for (int k = 0; k < 3; ++k) v[k] = (vm >= 1) ? (double)gg_comp(a, cur, D + k)[s] : dy->vel_u[k];
+  // line of sight from the particle to satellite k (unit), false below 1 m
+  auto los = [&](int k, double (&l)[3]) {
+    const double d0 = sd[k * 8 + 0] - x[0], d1 = sd[k * 8 + 1] - x[1], d2 = sd[k * 8 + 2] - x[2];
+    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (r < 1.0) return false;
+    l[0] = d0 / r;
+    l[1] = d1 / r;
+    l[2] = d2 / r;
+    return true;
+  };
+  auto radial = [&](int k, const double (&l)[3]) {
+    return sd[k * 8 + 3] * l[0] + sd[k * 8 + 4] * l[1] + sd[k * 8 + 5] * l[2];
+  };
+  double lw_add = 0.0;
+  if (!KF) {
+    double wcd = 0.0, wsum = 0.0;
+    for (int k = 0; k < n_sat; ++k) {
+      double l[3];
+      if (!los(k, l)) continue;
+      const double w = sd[k * 8 + 7];
+      wcd += w * (sd[k * 8 + 6] - radial(k, l) + (v[0] * l[0] + v[1] * l[1] + v[2] * l[2]));
+      wsum += w;
+    }
+    const bool any = wsum > 0.0;  // otherwise the particle is left as it is
+    const double cd = any ? wcd / wsum : 0.0;
+    const double inv_s2 = 1.0 / (sigma * sigma);
+    for (int k = 0; any && k < n_sat; ++k) {
+      double l[3];
+      if (!los(k, l)) continue;
+      const double pred = (sd[k * 8 + 3] - v[0]) * l[0] + (sd[k * 8 + 4] - v[1]) * l[1] +
+                          (sd[k * 8 + 5] - v[2]) * l[2] + cd;
+      const double r = sd[k * 8 + 6] - pred;
+      lw_add += -0.5 * sd[k * 8 + 7] * r * r * inv_s2;
+    }
+    const double g = fmin(1.0, fmax(0.0, gain));
+    if (any && g > 0.0 && n_sat >= 4) {
+      double A[4][5];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 5; ++c) A[r][c] = 0.0;
+      for (int k = 0; k < n_sat; ++k) {
+        double l[3];
+        if (!los(k, l)) continue;
+        const double h[4] = {-l[0], -l[1], -l[2], 1.0};
+        const double rhs = sd[k * 8 + 6] - radial(k, l);
+        const double w = sd[k * 8 + 7];
+        for (int r = 0; r < 4; ++r) {
+          A[r][4] += h[r] * w * rhs;
+          for (int c = 0; c < 4; ++c) A[r][c] += h[r] * w * h[c];
+        }
+      }
+      for (int d = 0; d < 4; ++d) A[d][d] += 1e-9;
+      double sol[4] = {0.0, 0.0, 0.0, 0.0};
+      if (gg_solve4(A, sol)) {
+        double dv[3] = {sol[0] - v[0], sol[1] - v[1], sol[2] - v[2]};
+        const double nrm = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        if (max_dv > 0.0 && nrm > max_dv) {
+          const double sc = max_dv / fmax(nrm, 1e-9);
+          for (int k = 0; k < 3; ++k) dv[k] *= sc;
+        }
+        for (int k = 0; k < 3; ++k) v[k] += g * dv[k];
+      }
+    }
+  } else {
+    double P[9];
+    {
+      double c6[6];
+      for (int k = 0; k < 6; ++k)
+        c6[k] = (vm >= 2) ? (double)gg_comp(a, cur, D + 3 + k)[s] : dy->vcov_u[k];
+      P[0] = c6[0]; P[1] = c6[1]; P[2] = c6[2];
+      P[3] = c6[1]; P[4] = c6[3]; P[5] = c6[4];
+      P[6] = c6[2]; P[7] = c6[4]; P[8] = c6[5];
+    }
+    double hm[3] = {0.0, 0.0, 0.0}, ym = 0.0, wsum = 0.0;
+    for (int k = 0; k < n_sat; ++k) {
+      const double w = sd[k * 8 + 7];
+      if (w <= 0.0) continue;
+      double l[3];
+      if (!los(k, l)) continue;
+      hm[0] += w * -l[0];
+      hm[1] += w * -l[1];
+      hm[2] += w * -l[2];
+      ym += w * (sd[k * 8 + 6] - radial(k, l));
+      wsum += w;
+    }
+    const bool any = wsum > 0.0;  // otherwise the particle is left as it is
+    if (any) {
+      for (int k = 0; k < 3; ++k) hm[k] /= wsum;
+      ym /= wsum;
+    }
+    const double s2 = sigma * sigma;
+    for (int k = 0; any && k < n_sat; ++k) {
+      const double w = sd[k * 8 + 7];
+      if (w <= 0.0) continue;
+      double l[3];
+      if (!los(k, l)) continue;
+      const double h[3] = {-l[0] - hm[0], -l[1] - hm[1], -l[2] - hm[2]};
+      const double yc = (sd[k * 8 + 6] - radial(k, l)) - ym;
+      const double ph0 = P[0] * h[0] + P[1] * h[1] + P[2] * h[2];
+      const double ph1 = P[3] * h[0] + P[4] * h[1] + P[5] * h[2];
+      const double ph2 = P[6] * h[0] + P[7] * h[1] + P[8] * h[2];
+      const double S = h[0] * ph0 + h[1] * ph1 + h[2] * ph2 + s2 / w;
+      if (!(S > 1e-18) || !isfinite(S)) continue;
+      const double innov = yc - (h[0] * v[0] + h[1] * v[1] + h[2] * v[2]);
+      const double K[3] = {ph0 / S, ph1 / S, ph2 / S};
+      for (int q = 0; q < 3; ++q) v[q] += K[q] * innov;
+      const double hp0 = h[0] * P[0] + h[1] * P[3] + h[2] * P[6];
+      const double hp1 = h[0] * P[1] + h[1] * P[4] + h[2] * P[7];
+      const double hp2 = h[0] * P[2] + h[1] * P[5] + h[2] * P[8];
+      for (int q = 0; q < 3; ++q) {
+        P[q * 3 + 0] -= K[q] * hp0;
+        P[q * 3 + 1] -= K[q] * hp1;
+        P[q * 3 + 2] -= K[q] * hp2;
+      }
+      const double p01 = 0.5 * (P[1] + P[3]);
+      const double p02 = 0.5 * (P[2] + P[6]);
+      const double p12 = 0.5 * (P[5] + P[7]);
+      P[1] = P[3] = isfinite(p01) ? p01 : 0.0;
+      P[2] = P[6] = isfinite(p02) ? p02 : 0.0;
+      P[5] = P[7] = isfinite(p12) ? p12 : 0.0;
+      P[0] = (isfinite(P[0]) && P[0] > 1e-12) ? P[0] : 1e-12;
+      P[4] = (isfinite(P[4]) && P[4] > 1e-12) ? P[4] : 1e-12;
+      P[8] = (isfinite(P[8]) && P[8] > 1e-12) ? P[8] : 1e-12;
+      lw_add += -0.5 * (innov * innov / S + log(S));
+    }
+    const double c6[6] = {P[0], P[1], P[2], P[4], P[5], P[8]};
+    for (int k = 0; k < 6; ++k) gg_comp(a, cur, D + 3 + k)[s] = (float)c6[k];
+  }
+  for (int k = 0; k < 3; ++k) gg_comp(a, cur, D + k)[s] = (float)v[k];
+  const float base = dy->lw_zero ? 0.0f : a.lw[s];
+  a.lw[s] = base + (float)lw_add;
+}
+
+__global__ void gg_k_set_vel_mode(gg_dyn* dyn, int f, int mode) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && dyn[f].vel_mode < mode) dyn[f].vel_mode = mode;
+}

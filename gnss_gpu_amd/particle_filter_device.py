@@ -255,11 +255,45 @@ class ParticleFilterDevice:
         if resample:
             _ = self.resample_if_needed()
 
-    def update_doppler(self, *args, **kwargs):
-        return _dev.pf_device_weight_doppler(self._state, *args, **kwargs)
+    def _doppler_rows(self, sat_ecef, sat_vel, doppler_hz, weights):
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        sv = np.asarray(sat_vel, dtype=np.float64).reshape(-1, 3)
+        dop = np.asarray(doppler_hz, dtype=np.float64).ravel()
+        if sat.shape != sv.shape:
+            raise ValueError("sat_ecef and sat_vel must have the same shape")
+        if sat.shape[0] != dop.size:
+            raise ValueError("doppler_hz length must match sat_ecef rows")
+        n_sat = int(dop.size)
+        w = (np.ones(n_sat, dtype=np.float64) if weights is None
+             else np.asarray(weights, dtype=np.float64).ravel())
+        if w.size != n_sat:
+            raise ValueError("weights length must match doppler_hz")
+        return sat, sv, dop, w, n_sat
 
-    def update_doppler_kf(self, *args, **kwargs):
-        return _dev.pf_device_doppler_kf_update(self._state, *args, **kwargs)
+    def update_doppler(self, sat_ecef, sat_vel, doppler_hz, weights=None,
+                       wavelength=0.19029367279836488, sigma_mps=0.5,
+                       velocity_update_gain=0.25, max_velocity_update_mps=10.0, resample=True):
+        """Doppler likelihood and per-particle velocity nudge toward each
+        particle's Doppler WLS solution (particle_filter_device.py:484-544)."""
+        self._require_init()
+        sat, sv, dop, w, n_sat = self._doppler_rows(sat_ecef, sat_vel, doppler_hz, weights)
+        _dev.pf_device_weight_doppler(self._state, sat.ravel(), sv.ravel(), dop, w, n_sat,
+                                      float(wavelength), float(sigma_mps),
+                                      float(velocity_update_gain),
+                                      float(max_velocity_update_mps))
+        if resample:
+            _ = self.resample_if_needed()
+
+    def update_doppler_kf(self, sat_ecef, sat_vel, doppler_hz, weights=None,
+                          wavelength=0.19029367279836488, sigma_mps=0.5, resample=True):
+        """Per-particle velocity Kalman update (RBPF) from Doppler rows
+        (particle_filter_device.py:546-582)."""
+        self._require_init()
+        sat, sv, dop, w, n_sat = self._doppler_rows(sat_ecef, sat_vel, doppler_hz, weights)
+        _dev.pf_device_doppler_kf_update(self._state, sat.ravel(), sv.ravel(), dop, w, n_sat,
+                                         float(wavelength), float(sigma_mps))
+        if resample:
+            _ = self.resample_if_needed()
 
     def position_update(self, ref_ecef, sigma_pos=30.0):
         """Gaussian position-domain likelihood added to the log-weights."""

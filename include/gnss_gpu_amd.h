@@ -194,6 +194,24 @@ int gg_pf_device_weight_dd_carrier_afv(gg_pf* h, const double* sat_ecef_k, const
                                        double per_particle_nlos_threshold_cycles,
                                        int32_t per_particle_huber, double per_particle_huber_k);
 
+/* reference: pf_device_weight_doppler, pf_device.cu:2183-2226 (kernel :610-740).
+ * Doppler rows (sat_ecef, sat_vel [n_sat][3] ECEF, doppler_hz, weights_sat):
+ * adds the range-rate log-likelihood (clock drift = weighted mean misfit) and,
+ * for velocity_update_gain > 0 and n_sat >= 4, moves each particle's velocity
+ * toward its Doppler WLS solution (step capped at max_velocity_update_mps).
+ * A no-op for n_sat <= 0, sigma_mps <= 0 or wavelength_m <= 0, as in the
+ * reference.  Needs a handle with velocity storage; n_sat <= 512. */
+int gg_pf_device_weight_doppler(gg_pf* h, const double* sat_ecef, const double* sat_vel,
+                                const double* doppler_hz, const double* weights_sat,
+                                int32_t n_sat, double wavelength_m, double sigma_mps,
+                                double velocity_update_gain, double max_velocity_update_mps);
+/* reference: pf_device_doppler_kf_update, pf_device.cu:2228-2270 (kernel :742-880).
+ * Per-particle Kalman update of the velocity Gaussian (RBPF), clock drift
+ * eliminated by centring; adds the innovation log-likelihood. */
+int gg_pf_device_doppler_kf_update(gg_pf* h, const double* sat_ecef, const double* sat_vel,
+                                   const double* doppler_hz, const double* weights_sat,
+                                   int32_t n_sat, double wavelength_m, double sigma_mps);
+
 /* --- learned likelihood (no reference counterpart) -------------------------- */
 /* An 8 -> 64 -> 64 -> 2 MLP per (particle, satellite) pair predicting the NLOS
  * bias mean and log-variance of the pseudorange residual; it replaces the

@@ -7,6 +7,7 @@ Follows src/particle_filter/pf_device.cu:
   spread    :1025-1076 + :1764-1807        systematic :1813-1878
   Megopolis :1231-1298 + :1884-1920
   DD pseudorange :282-368   carrier AFV :439-494   DD carrier AFV :502-604
+  Doppler :610-740          Doppler KF :742-880
 and python/gnss_gpu/particle_filter_device.py:198-303,679-714 for the
 orchestration.  State is absolute ECEF binary64 ([x y z cb isb...]), like the
 reference.  Random draws: the reference's cuRAND streams cannot be reproduced
@@ -122,6 +123,91 @@ def dd_carrier_afv_loglik(pos, sat_k, sat_ref, ddc, br_k, br_ref, w, wl, sigma_c
         (1.0 / np.asarray(wl, dtype=np.float64))[None, :]
     a = afv(np.asarray(ddc)[None, :] - exp_c)
     return _gated_sum(a, np.asarray(w, dtype=np.float64), sigma_cycles, thr, huber, huber_k)
+
+
+def _los(pos, sat):
+    d = np.asarray(sat)[None, :, :] - pos[:, None, :]
+    r = np.linalg.norm(d, axis=2)
+    ok = r >= 1.0
+    return d / np.where(ok, r, 1.0)[..., None], ok
+
+
+def doppler_update(pos, vel, sat, sv, dop, w, wl, sigma, gain, max_dv):
+    """pfd_weight_doppler_kernel, pf_device.cu:610-740, binary64, per particle:
+    returns (log-weight increment [N], new velocity [N, 3])."""
+    pos = np.asarray(pos, dtype=np.float64)
+    vel = np.array(vel, dtype=np.float64, copy=True)
+    l, ok = _los(pos, sat)
+    obs = -np.asarray(dop) * wl
+    svr = np.einsum("nsk,sk->ns", l, np.asarray(sv))
+    w = np.maximum(np.asarray(w, dtype=np.float64), 0.0)[None, :] * ok
+    rxr = np.einsum("nsk,nk->ns", l, vel)
+    wsum = w.sum(axis=1)
+    anyw = wsum > 0
+    cd = np.where(anyw, (w * (obs[None, :] - svr + rxr)).sum(axis=1) / np.where(anyw, wsum, 1), 0)
+    pred = svr - rxr + cd[:, None]
+    res = obs[None, :] - pred
+    add = np.where(anyw, np.sum(-0.5 * w * res * res / (sigma * sigma), axis=1), 0.0)
+    g = min(1.0, max(0.0, gain))
+    if g > 0.0 and len(obs) >= 4:
+        for i in np.nonzero(anyw)[0]:
+            H = np.concatenate([-l[i], np.ones((len(obs), 1))], axis=1)
+            rhs = obs - svr[i]
+            A = np.zeros((4, 5))
+            A[:, :4] = (H * w[i][:, None]).T @ H + 1e-9 * np.eye(4)
+            A[:, 4] = (H * w[i][:, None]).T @ rhs
+            try:
+                sol = np.linalg.solve(A[:, :4], A[:, 4])
+            except np.linalg.LinAlgError:
+                continue
+            if not np.all(np.isfinite(sol)):
+                continue
+            dv = sol[:3] - vel[i]
+            nrm = np.linalg.norm(dv)
+            if max_dv > 0.0 and nrm > max_dv:
+                dv *= max_dv / max(nrm, 1e-9)
+            vel[i] = vel[i] + g * dv
+    return add, vel
+
+
+def doppler_kf_update(pos, vel, cov, sat, sv, dop, w, wl, sigma):
+    """pfd_doppler_kf_update_kernel, pf_device.cu:742-880, binary64: sequential
+    scalar Kalman updates with centred rows; returns (log-weight increment [N],
+    velocity [N, 3], covariance [N, 3, 3])."""
+    pos = np.asarray(pos, dtype=np.float64)
+    mu = np.array(vel, dtype=np.float64, copy=True)
+    P = np.array(cov, dtype=np.float64, copy=True).reshape(-1, 3, 3)
+    l, ok = _los(pos, sat)
+    obs = -np.asarray(dop) * wl
+    svr = np.einsum("nsk,sk->ns", l, np.asarray(sv))
+    w = np.maximum(np.asarray(w, dtype=np.float64), 0.0)
+    use = ok & (w[None, :] > 0)
+    wm = np.where(use, w[None, :], 0.0)
+    wsum = wm.sum(axis=1)
+    add = np.zeros(pos.shape[0])
+    for i in range(pos.shape[0]):
+        if not wsum[i] > 0:
+            continue
+        hm = (wm[i][:, None] * -l[i]).sum(axis=0) / wsum[i]
+        ym = (wm[i] * (obs - svr[i])).sum() / wsum[i]
+        for k in range(len(obs)):
+            if not use[i, k]:
+                continue
+            h = -l[i, k] - hm
+            yc = (obs[k] - svr[i, k]) - ym
+            ph = P[i] @ h
+            S = h @ ph + sigma * sigma / w[k]
+            if not (S > 1e-18 and np.isfinite(S)):
+                continue
+            innov = yc - h @ mu[i]
+            K = ph / S
+            mu[i] = mu[i] + K * innov
+            P[i] = P[i] - np.outer(K, h @ P[i])
+            P[i] = 0.5 * (P[i] + P[i].T)
+            for d in range(3):
+                P[i, d, d] = P[i, d, d] if (np.isfinite(P[i, d, d]) and P[i, d, d] > 1e-12) else 1e-12
+            add[i] += -0.5 * (innov * innov / S + np.log(S))
+    return add, mu, P
 
 
 def ess(lw):

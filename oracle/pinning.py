@@ -74,6 +74,13 @@ PINS = {
                               "offset gives -0.5 (0.3/sigma)^2 (test_dd_closed_forms)"),
     "pf.dd_carrier_afv_loglik": ("pinned", "reference closed forms -40.5 / -12.375 "
                                  "(test_cuda_streams.py:739-771; test_dd_closed_forms)"),
+    "pf._los": ("pinned", "inside the Doppler checks"),
+    "pf.doppler_update": ("pinned", "noise-free rows: gain 1 returns the true velocity, "
+                          "zero misfit at the truth for any clock drift, the step cap "
+                          "(test_doppler_oracle_known_answers; test_cuda_streams.py:285-309)"),
+    "pf.doppler_kf_update": ("pinned", "a vague prior converges to the true velocity and the "
+                             "covariance shrinks; one row against the closed-form scalar update "
+                             "(test_doppler_oracle_known_answers; test_cuda_streams.py:311-335)"),
     "pf.OraclePF.resample_megopolis": ("pinned", "as megopolis_ref_ancestors"),
     "pf.OraclePF.resample_megopolis_chesser": ("pinned", "as megopolis_ancestors"),
     # wls.py

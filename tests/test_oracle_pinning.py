@@ -619,3 +619,58 @@ def test_dd_closed_forms():
     cp3[2] += 0.3
     assert opf.carrier_afv_loglik(pos, cb, sat, cp3, w5, d["wavelength_m"], 0.05)[0] == \
         pytest.approx(-0.5 * (0.3 / 0.05) ** 2, rel=1e-6)
+
+
+# ------------------------------------------------------------------ Doppler ----
+def _doppler_rows():
+    d = GOLD["doppler_rows"]
+    rx = np.array(d["rx_pos"])
+    sat = np.array(d["sat_ecef"])
+    sv = np.array(d["sat_vel"])
+    tv = np.array(d["true_vel"])
+    los = (sat - rx) / np.linalg.norm(sat - rx, axis=1)[:, None]
+    rr = np.einsum("sk,sk->s", sv - tv[None, :], los) + d["true_cd"]
+    dop = -rr / d["wavelength_m"]
+    return d, rx, sat, sv, tv, dop
+
+
+def test_doppler_oracle_known_answers():
+    """Noise-free Doppler rows of the reference's tests (_make_doppler_data):
+    the WLS nudge with gain 1 lands on the true velocity, the likelihood is
+    zero at the truth for any clock drift, the step cap holds, and the KF
+    update from a vague prior converges to the truth with a shrinking
+    covariance; a single row matches the closed-form scalar Kalman update."""
+    d, rx, sat, sv, tv, dop = _doppler_rows()
+    wl, sig = d["wavelength_m"], d["sigma_mps"]
+    w = np.ones(len(dop))
+    pos = rx[None, :]
+    add, v = opf.doppler_update(pos, np.zeros((1, 3)), sat, sv, dop, w, wl, sig, 1.0, 100.0)
+    np.testing.assert_allclose(v[0], tv, atol=1e-9)
+    add0, v0 = opf.doppler_update(pos, tv[None, :], sat, sv, dop, w, wl, sig, 0.0, 100.0)
+    assert abs(add0[0]) < 1e-12 and np.array_equal(v0[0], tv)
+    assert add[0] < 0.0
+    _, vc = opf.doppler_update(pos, np.zeros((1, 3)), sat, sv, dop, w, wl, sig, 1.0, 0.25)
+    assert np.linalg.norm(vc[0]) == pytest.approx(0.25, rel=1e-9)
+    s0 = d["kf_init_sigma"] ** 2
+    addk, mu, P = opf.doppler_kf_update(pos, np.zeros((1, 3)), s0 * np.eye(3)[None], sat, sv, dop,
+                                        w, wl, sig)
+    np.testing.assert_allclose(mu[0], tv, atol=d["kf_atol"])
+    assert np.all(np.diag(P[0]) < s0) and np.all(np.diag(P[0]) > 0)
+    # one satellite pair: two rows centred on their mean are +-(l1 - l0)/2
+    sel = [0, 1]
+    _, mu1, P1 = opf.doppler_kf_update(pos, np.zeros((1, 3)), s0 * np.eye(3)[None], sat[sel],
+                                       sv[sel], dop[sel], w[sel], wl, sig)
+    l = (sat[sel] - rx) / np.linalg.norm(sat[sel] - rx, axis=1)[:, None]
+    y = -dop[sel] * wl - np.einsum("sk,sk->s", sv[sel], l)
+    h = -(l[0] - l.mean(axis=0))
+    yc = y[0] - y.mean()
+    S = s0 * h @ h + sig * sig
+    K = s0 * h / S
+    mu_a = K * yc
+    P_a = s0 * np.eye(3) - np.outer(K, h * s0)
+    h2 = -(l[1] - l.mean(axis=0))
+    yc2 = y[1] - y.mean()
+    S2 = h2 @ P_a @ h2 + sig * sig
+    K2 = P_a @ h2 / S2
+    np.testing.assert_allclose(mu1[0], mu_a + K2 * (yc2 - h2 @ mu_a), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(P1[0], P_a - np.outer(K2, h2 @ P_a), rtol=1e-7, atol=1e-9)
