@@ -486,6 +486,71 @@ def wls_leg(args):
             "cpu_baseline": cpu}
 
 
+def multi_gnss_leg(args):
+    """Multi-GNSS WLS (position + 4 system clocks, 7 states) on 10,000 epochs of
+    24 satellites, 6 each of four systems with ISBs 12 / -7.5 / 20 m (Sagnac
+    range model, tol 1e-4 m, binary64), device-resident inputs."""
+    import ctypes as C
+
+    import torch
+
+    from gnss_gpu_amd import sim
+    from gnss_gpu_amd._gnss_gpu_multi_gnss import wls_multi_gnss_batch_device
+
+    E, S, NS = 10_000, 24, 4
+    sat, pr, w, ns, truth = sim.wls_epochs(E, s_min=S, s_max=S, seed=5)
+    ids = np.ascontiguousarray(np.tile(np.arange(S, dtype=np.int32) % NS, (E, 1)))
+    isb = np.array([0.0, 12.0, -7.5, 20.0])
+    pr = np.ascontiguousarray(pr + isb[ids])
+    dev = torch.device("cuda")
+    t = [torch.tensor(a, device=dev) for a in (sat, pr, w, ids)]
+    out = torch.zeros((E, 3 + NS), dtype=torch.float64, device=dev)
+    it = torch.zeros(E, dtype=torch.int32, device=dev)
+    for _ in range(5):
+        wls_multi_gnss_batch_device(*t, out, it, NS)
+    torch.cuda.synchronize()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        wls_multi_gnss_batch_device(*t, out, it, NS)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    res = out.cpu().numpy()
+    err = np.linalg.norm(res[:, :3] - truth, axis=1)
+    isb_err = np.abs((res[:, 4:] - res[:, 3:4]) - isb[1:])
+    cpu = None
+    if not args.no_cpu:
+        from oracle import build_cpu
+
+        lib = build_cpu.load()
+        DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        o = np.zeros((E, 3 + NS))
+        io = np.zeros(E, np.int32)
+        rows = {}
+        for thr in (int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 1):
+            t0 = time.perf_counter()
+            n_rep = 0
+            while n_rep < 3 or time.perf_counter() - t0 < 1.0:
+                lib.cpu_wls_mg_batch(sat.ctypes.data_as(DP), pr.ctypes.data_as(DP),
+                                     w.ctypes.data_as(DP), ids.ctypes.data_as(IP), E, S, NS, 10,
+                                     1e-4, 1, 0, thr, o.ctypes.data_as(DP), io.ctypes.data_as(IP))
+                n_rep += 1
+            rows[thr] = (time.perf_counter() - t0) * 1e3 / n_rep
+        cpu = {"kind": "port", "ms_per_batch": {str(k): round(v, 3) for k, v in rows.items()},
+               "sample": "C restatement of multi_gnss.cu (oracle/cpu_wls.c), OpenMP over epochs"}
+    nbytes = E * S * 44 + E * ((3 + NS) * 8 + 4)
+    return {"workload": "wls_multi_gnss_10000_epochs_24_sats_4_systems", "epochs": E,
+            "ms_per_batch": round(ms, 4), "epochs_per_s": round(E / (ms * 1e-3), 1),
+            "median_err_m": round(float(np.median(err)), 3),
+            "median_isb_err_m": round(float(np.median(isb_err)), 3),
+            "mean_iters": round(float(it.float().mean().item()), 3),
+            "algorithmic_bytes": int(nbytes),
+            "achieved_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "cpu_baseline": cpu}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -603,6 +668,7 @@ def main():
     learned = None if args.no_learned else learned_leg(args)
     batched = None if args.no_batched else batched_leg(args)
     wls = None if args.no_batched else wls_leg(args)
+    mgnss = None if args.no_batched else multi_gnss_leg(args)
     scale = None if args.no_scale else scale_leg(args)
 
     out = {
@@ -637,6 +703,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "wls": wls,
+        "multi_gnss_wls": mgnss,
         "scale": scale,
         "cpu_baseline": cpu,
     }

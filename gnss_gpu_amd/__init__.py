@@ -3,7 +3,7 @@
 Public names follow python/gnss_gpu/__init__.py:36-92 for the in-scope part of
 that package:
 
-    ParticleFilter, ParticleFilterDevice, wls_position, wls_batch
+    ParticleFilter, ParticleFilterDevice, wls_position, wls_batch, MultiGNSSSolver
 
 plus the batched multi-filter engine ``BatchedParticleFilter``.  The compiled
 HIP library (libgnss_gpu_amd.so) is loaded on first use of any of them; there
@@ -19,6 +19,7 @@ _LAZY = {
     "wls_position": ("_gnss_gpu", "wls_position"),
     "wls_batch": ("_gnss_gpu", "wls_batch"),
     "wls_batch_device": ("_gnss_gpu", "wls_batch_device"),
+    "MultiGNSSSolver": ("multi_gnss", "MultiGNSSSolver"),
 }
 
 __all__ = list(_LAZY)

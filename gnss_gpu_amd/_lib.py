@@ -167,6 +167,9 @@ _SIGS = {
     "gg_wls_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_batch_device": [_P, _P, _P, _P, _I, _I, _I, _D, _P, _P, _I, _P],
     "gg_wls_position": [_DP, _DP, _DP, _I, _I, _D, _DP, _IP],
+    "gg_wls_multi_gnss_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _I, _D, _DP, _IP],
+    "gg_wls_multi_gnss": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
+    "gg_wls_multi_gnss_batch_device": [_P, _P, _P, _P, _I, _I, _I, _I, _D, _P, _P, _I, _P],
     "gg_debug_math": [_I, _DP, _DP, _I],
 }
 

@@ -163,6 +163,36 @@ int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const do
                     int32_t n_sat, int32_t max_iter, double tol, double* result,
                     int32_t* iters);
 
+/* --- multi-GNSS WLS: position + one clock bias per system ------------------ */
+#define GG_MAX_SYSTEMS 5 /* GPS, GLONASS, Galileo, BeiDou, QZSS (multi_gnss.h:5) */
+
+/* reference: wls_multi_gnss_batch, src/positioning/multi_gnss.cu:307-350 (kernel
+ * :211-305; binding python/gnss_gpu/_multi_gnss_bindings.cpp:36-66).  Host arrays
+ * sat_ecef [n_epoch][n_sat][3], pseudoranges / weights [n_epoch][n_sat], int32
+ * system_ids [n_epoch][n_sat] in [0, n_systems) (others add no clock column and
+ * use clock 0's bias); results [n_epoch][3 + n_systems] = x, y, z, cb_0..;
+ * iters [n_epoch] (may be NULL).  Sagnac range model.  n_systems in [0, 5];
+ * n_sat < 3 + n_systems is an error, as in the reference binding.  weights NULL
+ * means all ones. */
+int gg_wls_multi_gnss_batch(const double* sat_ecef, const double* pseudoranges,
+                            const double* weights, const int32_t* system_ids, int32_t n_epoch,
+                            int32_t n_sat, int32_t n_systems, int32_t max_iter, double tol,
+                            double* results, int32_t* iters);
+/* reference: wls_multi_gnss, src/positioning/multi_gnss.cu:50-153 (binding
+ * _multi_gnss_bindings.cpp:10-34).  One epoch, result [3 + n_systems]; plain
+ * ranges (no Sagnac term, as the reference's CPU solver); fewer than 3 + n_systems
+ * satellites or a satellite centroid below 1e-6 m gives zeros and *iters = -1. */
+int gg_wls_multi_gnss(const double* sat_ecef, const double* pseudoranges, const double* weights,
+                      const int32_t* system_ids, int32_t n_sat, int32_t n_systems,
+                      int32_t max_iter, double tol, double* result, int32_t* iters);
+/* Device-pointer variant of both, enqueued on hip_stream without synchronising;
+ * mode 0 = wls_multi_gnss_batch semantics, 1 = wls_multi_gnss semantics. */
+int gg_wls_multi_gnss_batch_device(const double* sat_ecef, const double* pseudoranges,
+                                   const double* weights, const int32_t* system_ids,
+                                   int32_t n_epoch, int32_t n_sat, int32_t n_systems,
+                                   int32_t max_iter, double tol, double* results, int32_t* iters,
+                                   int32_t mode, void* hip_stream);
+
 /* --- double-differenced and carrier-phase updates (single filter) ---------- */
 /* reference: pf_device_weight_dd_pseudorange, src/particle_filter/pf_device.cu:1576-1620
  * (kernel :282-368).  Overwrites the log-weights with the DD pseudorange

@@ -1,5 +1,6 @@
 """Build the CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP): the C
-restatements of the particle-filter step (cpu_pf.c) and of the WLS (cpu_wls.c).
+restatements of the particle-filter step (cpu_pf.c) and of the WLS solvers,
+single-clock and multi-GNSS (cpu_wls.c).
 
 Measurement infrastructure only; the output lands in oracle/_build/.
 """
@@ -44,6 +45,8 @@ def load():
     lib.cpu_wls_threads.restype = I
     lib.cpu_wls_batch.restype = None
     lib.cpu_wls_batch.argtypes = [DP, DP, DP, IP, I, I, I, D, I, I, DP, IP]
+    lib.cpu_wls_mg_batch.restype = None
+    lib.cpu_wls_mg_batch.argtypes = [DP, DP, DP, IP, I, I, I, I, D, I, I, I, DP, IP]
     return lib
 
 

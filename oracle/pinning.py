@@ -90,6 +90,16 @@ PINS = {
     "wls.wls_batch_ref": ("pinned", "test_positioning.py:49-75 truth (Sagnac)"),
     "wls._los": ("pinned", "Sagnac closed form of range_model.py:20-29 (test_sagnac_rotation_closed_form)"),
     "wls._solve4": ("pinned", "numpy.linalg.solve on random SPD systems; singular pivot rule"),
+    "wls._solve_n": ("pinned", "numpy.linalg.solve on random SPD systems n = 3..8; a zero "
+                     "row/column gives 0 for its unknown and the reduced solve for the rest "
+                     "(test_solve_n_against_numpy_and_zero_rows)"),
+    "wls.wls_multi_gnss_solve": ("pinned", "test_multi_gnss.py:109-151 ISB recoveries (Sagnac); "
+                                 "one system equals the pinned 4-state WLS; the noisy optimum "
+                                 "equals scipy least_squares; empty system and foreign id "
+                                 "(test_multi_gnss_*)"),
+    "wls.wls_multi_gnss_ref": ("pinned", "plain ranges recovered, Sagnac ranges not "
+                               "(multi_gnss.cu:101-112); guards of test_multi_gnss.py:188-197"),
+    "wls.wls_multi_gnss_batch_ref": ("pinned", "test_multi_gnss.py:100-172 truths and bounds"),
     # learned.py
     "learned.unpack": ("pinned", "inverse of pack (round trip in the GPU parameter test)"),
     "learned.pack": ("pinned", "inverse of unpack"),

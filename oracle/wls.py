@@ -2,7 +2,9 @@
 
 wls_position_ref : wls.cu:10-123   (CPU, no Sagnac, singular -> current estimate)
 wls_batch_ref    : wls.cu:126-241  (GPU kernel, Sagnac, pivot < 1e-15 skipped)
-Both keep the reference's sequential summation over satellites.  The batch
+wls_multi_gnss_ref       : multi_gnss.cu:50-153  (CPU, no Sagnac, 3 + n_systems states)
+wls_multi_gnss_batch_ref : multi_gnss.cu:211-305 (GPU kernel, Sagnac)
+All keep the reference's sequential summation over satellites.  The batch
 version is vectorised over epochs; the loop over satellites and the Gaussian
 elimination run in the reference's order.
 """
@@ -148,3 +150,141 @@ def wls_position_ref(sat_ecef, pr, w, max_iter=10, tol=1e-4):
 def wls_batch_ref(sat, pr, w, max_iter=10, tol=1e-4, n_sat=None):
     """wls.cu:126-241."""
     return wls_solve(sat, pr, w, n_sat, max_iter, tol, sagnac=True, cpu_rule=False)
+
+
+# ----------------------------------------------------------- multi-GNSS ----
+MAX_SYSTEMS = 5  # include/gnss_gpu/multi_gnss.h:5
+
+
+def _solve_n(A):
+    """solve_NxN_cpu / solve_NxN_device (multi_gnss.cu:15-47, 157-187) on
+    A [E, n, n+1]: partial pivoting, a pivot below 1e-15 skips its column,
+    back substitution sets the unknown of a pivot <= 1e-15 to zero."""
+    A = A.copy()
+    E, n = A.shape[0], A.shape[1]
+    rows = np.arange(E)
+    for col in range(n):
+        piv = np.full(E, col)
+        for row in range(col + 1, n):
+            piv = np.where(np.abs(A[rows, row, col]) > np.abs(A[rows, piv, col]), row, piv)
+        tmp = A[rows, col, :].copy()
+        A[rows, col, :] = A[rows, piv, :]
+        A[rows, piv, :] = tmp
+        keep = ~(np.abs(A[:, col, col]) < 1e-15)
+        for row in range(col + 1, n):
+            with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+                fac = A[:, row, col] / A[:, col, col]
+                upd = A[:, row, col:] - fac[:, None] * A[:, col, col:]
+            A[:, row, col:] = np.where(keep[:, None], upd, A[:, row, col:])
+    x = np.zeros((E, n))
+    for row in range(n - 1, -1, -1):
+        v = A[:, row, n].copy()
+        for c in range(row + 1, n):
+            v = v - A[:, row, c] * x[:, c]
+        p = A[:, row, row]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x[:, row] = np.where(np.abs(p) > 1e-15, v / p, 0.0)
+    return x
+
+
+def wls_multi_gnss_solve(sat, pr, w, sys_ids, n_systems, max_iter=10, tol=1e-4,
+                         sagnac=True, cpu_rule=False):
+    """Multi-GNSS Gauss-Newton WLS with one clock per system.
+
+    sat [E,S,3], pr/w [E,S], sys_ids int [E,S]; returns (results [E, 3+n_systems],
+    iters [E]).  sagnac/cpu_rule=(True, False) is wls_multi_gnss_batch_kernel
+    (multi_gnss.cu:211-305): Sagnac ranges with r >= 1e-12 (:189-209), no
+    guards.  (False, True) is the CPU wls_multi_gnss (multi_gnss.cu:50-153):
+    plain ranges, zeros and -1 for S < 3 + n_systems or a centroid below 1e-6 m.
+    A system id outside [0, n_systems) adds no clock column and is predicted
+    with clock 0's bias (:111, :120); a system with no satellites keeps a
+    zero column, whose unknown the solver sets to 0.
+    """
+    sat = np.asarray(sat, dtype=np.float64)
+    pr = np.asarray(pr, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    sid = np.asarray(sys_ids, dtype=np.int64)
+    E, S = pr.shape
+    ns = int(n_systems)
+    n = 3 + ns
+    res = np.zeros((E, n))
+    iters = np.full(E, -1, dtype=np.int32)
+    if cpu_rule and S < n:
+        return res, iters
+    valid = (sid >= 0) & (sid < ns)
+    cx = np.zeros(E); cy = np.zeros(E); cz = np.zeros(E)
+    for s in range(S):
+        cx = cx + sat[:, s, 0]
+        cy = cy + sat[:, s, 1]
+        cz = cz + sat[:, s, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        cx, cy, cz = cx / S, cy / S, cz / S
+        cn = np.sqrt(cx * cx + cy * cy + cz * cz)
+        scale = WGS84_A / cn
+        x, y, z = cx * scale, cy * scale, cz * scale
+    live = cn >= 1e-6 if cpu_rule else np.ones(E, dtype=bool)
+    cb = np.zeros((E, max(ns, 1)))
+    cnt = np.zeros((E, max(ns, 1)), dtype=np.int64)
+    for s in range(S):
+        dx, dy, dz, r = _los(sat[:, s, 0], sat[:, s, 1], sat[:, s, 2], x, y, z, sagnac)
+        for k in range(ns):
+            m = valid[:, s] & (sid[:, s] == k)
+            cb[:, k] = np.where(m, cb[:, k] + (pr[:, s] - r), cb[:, k])
+            cnt[:, k] += m
+    cb = np.where(cnt > 0, cb / np.maximum(cnt, 1), cb)
+    it = np.full(E, max_iter if max_iter > 0 else 0, dtype=np.int32)
+    active = live.copy()
+    for k in range(max_iter):
+        if not active.any():
+            break
+        HtWH = np.zeros((E, n, n))
+        HtWy = np.zeros((E, n))
+        for s in range(S):
+            dx, dy, dz, r = _los(sat[:, s, 0], sat[:, s, 1], sat[:, s, 2], x, y, z, sagnac)
+            ss = sid[:, s]
+            vs = valid[:, s]
+            cbs = np.where(vs, cb[np.arange(E), np.clip(ss, 0, max(ns - 1, 0))],
+                           cb[:, 0] if ns > 0 else 0.0)
+            resid = pr[:, s] - (r + cbs)
+            ws = w[:, s]
+            H = np.zeros((E, n))
+            H[:, 0], H[:, 1], H[:, 2] = dx / r, dy / r, dz / r
+            for j in range(ns):
+                H[:, 3 + j] = np.where(vs & (ss == j), 1.0, 0.0)
+            for a in range(n):
+                HtWy[:, a] += H[:, a] * ws * resid
+                for b in range(n):
+                    HtWH[:, a, b] += H[:, a] * ws * H[:, b]
+        d = _solve_n(np.concatenate([HtWH, HtWy[:, :, None]], axis=2))
+        upd = active
+        x = np.where(upd, x + d[:, 0], x)
+        y = np.where(upd, y + d[:, 1], y)
+        z = np.where(upd, z + d[:, 2], z)
+        for j in range(ns):
+            cb[:, j] = np.where(upd, cb[:, j] + d[:, 3 + j], cb[:, j])
+        nrm = np.sqrt(np.sum(d * d, axis=1))
+        conv = upd & (nrm < tol)
+        it = np.where(conv, k + 1, it)
+        active = upd & ~conv
+    res[:, 0], res[:, 1], res[:, 2] = x, y, z
+    res[:, 3:] = cb[:, :ns]
+    iters[live] = it[live]
+    res[~live] = 0.0
+    return res, iters
+
+
+def wls_multi_gnss_ref(sat_ecef, pr, w, sys_ids, n_systems, max_iter=10, tol=1e-4):
+    """multi_gnss.cu:50-153 (CPU single epoch): returns (result[3+n_systems], iters)."""
+    pr = np.asarray(pr, dtype=np.float64).ravel()
+    S = pr.size
+    sat = np.asarray(sat_ecef, dtype=np.float64).reshape(1, S, 3)
+    res, it = wls_multi_gnss_solve(sat, pr[None, :], np.asarray(w, dtype=np.float64).reshape(1, S),
+                                   np.asarray(sys_ids).reshape(1, S), n_systems, max_iter, tol,
+                                   sagnac=False, cpu_rule=True)
+    return res[0], int(it[0])
+
+
+def wls_multi_gnss_batch_ref(sat, pr, w, sys_ids, n_systems, max_iter=10, tol=1e-4):
+    """multi_gnss.cu:211-305 (GPU batch kernel)."""
+    return wls_multi_gnss_solve(sat, pr, w, sys_ids, n_systems, max_iter, tol,
+                                sagnac=True, cpu_rule=False)

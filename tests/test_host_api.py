@@ -73,3 +73,46 @@ def test_wls_epochs_ragged():
     assert counts.min() >= 6 and counts.max() <= 20
     for e in range(50):
         assert np.all(w[e, counts[e]:] == 0) and np.all(w[e, :counts[e]] == 1)
+
+
+def test_multi_gnss_solver_host_logic():
+    """multi_gnss.py:38-53, 218-240 and test_multi_gnss.py:174-197: PRN parsing,
+    names, the sorted system numbering and the early -1 for too few satellites
+    (no device call)."""
+    import json
+    import os
+
+    from gnss_gpu_amd.multi_gnss import (SYSTEM_BEIDOU, SYSTEM_GALILEO, SYSTEM_GPS,
+                                         MultiGNSSSolver)
+
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "reference_cases.json")))["multi_gnss_scenario"]
+    for prn, sys_id, num in g["prn_to_system"]:
+        assert MultiGNSSSolver.prn_to_system(prn) == (sys_id, num)
+    assert MultiGNSSSolver.prn_to_system("X09") == (SYSTEM_GPS, 9)
+    for sys_id, name in g["system_names"]:
+        assert MultiGNSSSolver.system_name(sys_id) == name
+    assert MultiGNSSSolver.system_name(9) == "Unknown(9)"
+    solver = MultiGNSSSolver(systems=[SYSTEM_BEIDOU, SYSTEM_GPS, SYSTEM_GALILEO])
+    assert solver.systems == [SYSTEM_GPS, SYSTEM_GALILEO, SYSTEM_BEIDOU] and solver.n_systems == 3
+    np.testing.assert_array_equal(solver._map_ids([3, 0, 2, 1, 4, 3]), [2, 0, 1, 0, 0, 2])
+    ins = g["insufficient"]
+    pos, biases, it = MultiGNSSSolver(systems=ins["systems"]).solve(
+        np.array(ins["sat_ecef"]), np.array(ins["pr"]), np.array(ins["systems"]))
+    assert it == -1 and np.all(pos == 0) and biases == {0: 0.0, 2: 0.0}
+    pos, cbs, its = MultiGNSSSolver(systems=[0, 1, 2]).solve_batch(
+        np.zeros((3, 5, 3)), np.zeros((3, 5)), np.zeros((3, 5), dtype=np.int32))
+    assert pos.shape == (3, 3) and cbs.shape == (3, 3) and np.all(its == -1)
+
+
+def test_multi_gnss_argument_checks():
+    from gnss_gpu_amd._gnss_gpu_multi_gnss import wls_multi_gnss, wls_multi_gnss_batch
+
+    with pytest.raises(RuntimeError, match="3 \\+ n_systems"):
+        wls_multi_gnss_batch(np.zeros((2, 5, 3)), np.zeros((2, 5)), np.ones((2, 5)),
+                             np.zeros((2, 5), dtype=np.int32), 3)
+    with pytest.raises(ValueError, match="n_systems"):
+        wls_multi_gnss(np.zeros(30), np.zeros(10), np.ones(10), np.zeros(10, np.int32), 6)
+    with pytest.raises(ValueError):
+        wls_multi_gnss_batch(np.zeros((2, 9)), np.zeros((2, 9)), np.ones((2, 9)),
+                             np.zeros((2, 9), dtype=np.int32), 1)

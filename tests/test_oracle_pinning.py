@@ -381,6 +381,155 @@ def test_solve4_against_numpy_and_pivot_rule():
     assert not ok0[0]
 
 
+# ------------------------------------------------------------ multi-GNSS ----
+def _mg_scenario(isb_gal=0.0, isb_glo=0.0, sagnac=True):
+    """test_multi_gnss.py:47-92: GPS / Galileo / GLONASS, four satellites each."""
+    from gnss_gpu_amd.range_model import geometric_ranges_sagnac
+
+    g = GOLD["multi_gnss_scenario"]
+    sat = np.vstack([g["gps_sats"], g["galileo_sats"], g["glonass_sats"]])
+    tp = np.array(g["true_pos"])
+    sid = np.array([0] * 4 + [2] * 4 + [1] * 4)
+    rng = geometric_ranges_sagnac(tp, sat) if sagnac else np.linalg.norm(sat - tp, axis=1)
+    pr = rng + g["true_cb_gps"] + np.repeat([0.0, isb_gal, isb_glo], 4)
+    return sat, pr, sid, tp
+
+
+def test_multi_gnss_batch_recovers_reference_isbs():
+    """test_multi_gnss.py:109-151: noise-free Sagnac ranges; the batch kernel's
+    model recovers the position and every ISB (systems numbered in sorted order)."""
+    g = GOLD["multi_gnss_scenario"]
+    for case in g["isb_cases"]:
+        sat, pr, sid, tp = _mg_scenario(case["isb_galileo"], case["isb_glonass"])
+        n = case["n_sat"]
+        systems = case["systems"]
+        mapped = np.array([systems.index(s) for s in sid[:n]])
+        res, it = owls.wls_multi_gnss_batch_ref(sat[None, :n], pr[None, :n], np.ones((1, n)),
+                                                mapped[None], len(systems))
+        assert np.linalg.norm(res[0, :3] - tp) < g["position_error_m"]
+        cb = dict(zip(systems, res[0, 3:]))
+        assert abs(cb[2] - cb[0] - case["isb_galileo"]) < g["isb_error_m"]
+        if 1 in cb:
+            assert abs(cb[1] - cb[0] - case["isb_glonass"]) < g["isb_error_m"]
+        assert abs(cb[0] - g["true_cb_gps"]) < 1e-4
+        assert 1 <= it[0] <= 10
+    # GPS only, eight satellites (test_multi_gnss.py:23-44, 100-107)
+    sat = np.array(GOLD["tokyo_constellation"]["sat_ecef"])
+    from gnss_gpu_amd.range_model import geometric_ranges_sagnac
+    pr = geometric_ranges_sagnac(np.array(g["true_pos"]), sat) + 3000.0
+    res, it = owls.wls_multi_gnss_batch_ref(sat[None], pr[None], np.ones((1, 8)),
+                                            np.zeros((1, 8), int), 1)
+    assert np.linalg.norm(res[0, :3] - g["true_pos"]) < g["gps_only_error_m"]
+
+
+def test_multi_gnss_single_epoch_has_no_sagnac_term():
+    """multi_gnss.cu:101-112: the single-epoch solver takes plain ranges.  Plain
+    ranges are recovered to the reference's 0.1 m bound; Sagnac ranges are not
+    (the Earth-rotation term moves ranges by up to ~30 m here)."""
+    g = GOLD["multi_gnss_scenario"]
+    sat, pr, sid, tp = _mg_scenario(15.0, -8.0, sagnac=False)
+    mapped = np.array([[0, 1, 2].index(s) for s in sid])
+    res, it = owls.wls_multi_gnss_ref(sat, pr, np.ones(12), mapped, 3)
+    assert np.linalg.norm(res[:3] - tp) < 1e-6
+    np.testing.assert_allclose(res[3:], [3000.0, 2992.0, 3015.0], atol=1e-6)
+    sat, pr, sid, tp = _mg_scenario(15.0, -8.0, sagnac=True)
+    res, it = owls.wls_multi_gnss_ref(sat, pr, np.ones(12), mapped, 3)
+    assert np.linalg.norm(res[:3] - tp) > 1.0
+
+
+def test_multi_gnss_guards_and_one_system_equals_wls():
+    g = GOLD["multi_gnss_scenario"]
+    ins = g["insufficient"]
+    res, it = owls.wls_multi_gnss_ref(np.array(ins["sat_ecef"]), ins["pr"], np.ones(2),
+                                      [0, 1], 2)
+    assert it == -1 and res.shape == (5,) and np.all(res == 0)
+    res, it = owls.wls_multi_gnss_ref(np.zeros((6, 3)), np.full(6, 2e7), np.ones(6),
+                                      np.zeros(6, int), 1)
+    assert it == -1 and np.all(res == 0)
+    # one system with every satellite in it is the pinned 4-state WLS
+    from gnss_gpu_amd import sim
+
+    sat, pr, w, ns, truth = sim.wls_epochs(n_epoch=40, s_min=8, s_max=8, seed=11)
+    a, ia = owls.wls_multi_gnss_batch_ref(sat, pr, w, np.zeros((40, 8), int), 1)
+    b, ib = owls.wls_batch_ref(sat, pr, w)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-7)
+    np.testing.assert_array_equal(ia, ib)
+    a, ia = owls.wls_multi_gnss_ref(sat[0], pr[0], w[0], np.zeros(8, int), 1)
+    b, ib = owls.wls_position_ref(sat[0], pr[0], w[0])
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-7)
+
+
+def test_multi_gnss_noisy_optimum_matches_least_squares():
+    """With noise the Gauss-Newton fixed point is the weighted least-squares
+    optimum: scipy's trust-region solver on the same Sagnac model and weights
+    lands on it, and the gradient H^T W r vanishes there."""
+    from scipy.optimize import least_squares
+
+    from gnss_gpu_amd.range_model import geometric_ranges_sagnac
+
+    sat, pr, sid, tp = _mg_scenario(15.0, -8.0)
+    mapped = np.array([[0, 1, 2].index(s) for s in sid])
+    rng = np.random.default_rng(5)
+    E = 6
+    prn = pr[None, :] + rng.normal(0, 2.0, (E, 12))
+    w = rng.uniform(0.5, 2.0, (E, 12))
+    res, it = owls.wls_multi_gnss_batch_ref(np.tile(sat, (E, 1, 1)), prn, w,
+                                            np.tile(mapped, (E, 1)), 3, max_iter=20, tol=1e-9)
+    for e in range(E):
+        def f(s):
+            return np.sqrt(w[e]) * (prn[e] - geometric_ranges_sagnac(s[:3], sat) - s[3:][mapped])
+        x0 = np.concatenate([tp + 50.0, [2900.0, 2900.0, 2900.0]])
+        ls = least_squares(f, x0, xtol=1e-15, ftol=1e-15, gtol=1e-15)
+        np.testing.assert_allclose(res[e], ls.x, rtol=0, atol=1e-5)
+        assert np.linalg.norm(res[e, :3] - tp) < GOLD["multi_gnss_scenario"]["batch_error_m"]
+
+
+def test_multi_gnss_empty_system_and_foreign_id():
+    """A system with no satellites leaves a zero column: its clock stays at 0
+    and the others are unaffected; a system id outside [0, n_systems) uses
+    clock 0's bias without a clock column (multi_gnss.cu:111, :120)."""
+    sat, pr, sid, tp = _mg_scenario(15.0, 0.0)
+    mapped = np.array([{0: 0, 2: 2, 1: 0}[s] for s in sid])  # GLONASS folded into GPS
+    full, it = owls.wls_multi_gnss_batch_ref(sat[None], pr[None], np.ones((1, 12)),
+                                             mapped[None], 3)
+    assert full[0, 4] == 0.0
+    assert np.linalg.norm(full[0, :3] - tp) < 1e-6
+    np.testing.assert_allclose(full[0, [3, 5]], [3000.0, 3015.0], atol=1e-6)
+    # the truth is still the fixed point, but the row's missing clock column
+    # makes the Jacobian inexact: convergence turns linear (~48 iterations
+    # instead of the 5 of the labelled case)
+    foreign = mapped.copy()
+    foreign[0] = 7
+    res, it = owls.wls_multi_gnss_batch_ref(sat[None], pr[None], np.ones((1, 12)),
+                                            foreign[None], 3, max_iter=100)
+    assert np.linalg.norm(res[0, :3] - tp) < 1e-3
+    assert 20 < it[0] < 100
+
+
+def test_solve_n_against_numpy_and_zero_rows():
+    rng = np.random.default_rng(8)
+    for n in range(3, 9):
+        B = rng.normal(size=(32, n, n))
+        A = np.einsum("eij,ekj->eik", B, B) + 0.1 * np.eye(n)
+        b = rng.normal(size=(32, n))
+        x = owls._solve_n(np.concatenate([A, b[:, :, None]], axis=2))
+        np.testing.assert_allclose(x, np.linalg.solve(A, b[:, :, None])[:, :, 0], rtol=1e-8,
+                                   atol=1e-10)
+    # zero row and column k: unknown k is 0, the rest solve the reduced system
+    n, k = 6, 4
+    B = rng.normal(size=(1, n, n))
+    A = np.einsum("eij,ekj->eik", B, B) + 0.1 * np.eye(n)
+    A[:, k, :] = 0.0
+    A[:, :, k] = 0.0
+    b = rng.normal(size=(1, n))
+    b[:, k] = 0.0
+    x = owls._solve_n(np.concatenate([A, b[:, :, None]], axis=2))
+    keep = [i for i in range(n) if i != k]
+    red = np.linalg.solve(A[0][np.ix_(keep, keep)], b[0, keep])
+    assert x[0, k] == 0.0
+    np.testing.assert_allclose(x[0, keep], red, rtol=1e-9)
+
+
 def test_oracle_predict_deterministic_closed_form():
     """sigma = 0: every particle moves by v*dt exactly (test_cuda_streams.py:239-255)."""
     pf = opf.OraclePF(100, seed=1)
@@ -553,6 +702,36 @@ def test_cpu_wls_baseline_matches_oracle():
     ref, rit = owls.wls_batch_ref(sat, pr, w, n_sat=ns)
     np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6)
     np.testing.assert_array_equal(it, rit)
+
+
+def test_cpu_multi_gnss_baseline_matches_oracle():
+    """The C multi-GNSS restatement (timed CPU baseline) against the NumPy
+    oracle, batch rule and single-epoch rule."""
+    import ctypes as C
+
+    from gnss_gpu_amd import sim
+    from oracle import build_cpu
+
+    lib = build_cpu.load()
+    sat, pr, w, ns, truth = sim.wls_epochs(120, s_min=14, s_max=14, seed=13)
+    E, S = pr.shape
+    ids = np.ascontiguousarray(np.tile(np.arange(S, dtype=np.int32) % 3, (E, 1)))
+    ids[::4, 2] = -1
+    pr = np.ascontiguousarray(pr + np.array([0.0, 9.0, -4.0])[np.clip(ids, 0, 2)])
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    for sagnac, guards, ref_fn in ((1, 0, owls.wls_multi_gnss_batch_ref), (0, 1, None)):
+        out = np.zeros((E, 6))
+        it = np.zeros(E, np.int32)
+        lib.cpu_wls_mg_batch(sat.ctypes.data_as(DP), pr.ctypes.data_as(DP), w.ctypes.data_as(DP),
+                             ids.ctypes.data_as(IP), E, S, 3, 40, 1e-4, sagnac, guards, 2,
+                             out.ctypes.data_as(DP), it.ctypes.data_as(IP))
+        if ref_fn is not None:
+            ref, rit = ref_fn(sat, pr, w, ids, 3, max_iter=40)
+        else:
+            ref, rit = owls.wls_multi_gnss_solve(sat, pr, w, ids, 3, 40, 1e-4, sagnac=False,
+                                                 cpu_rule=True)
+        np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6)
+        np.testing.assert_array_equal(it, rit)
 
 
 # ------------------------------------------- DD pseudorange / carrier AFV ----
