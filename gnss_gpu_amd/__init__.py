@@ -3,7 +3,8 @@
 Public names follow python/gnss_gpu/__init__.py:36-92 for the in-scope part of
 that package:
 
-    ParticleFilter, ParticleFilterDevice, wls_position, wls_batch, MultiGNSSSolver
+    ParticleFilter, ParticleFilterDevice, wls_position, wls_batch, MultiGNSSSolver,
+    BuildingModel, BVHAccelerator, ParticleFilter3D, ParticleFilter3DBVH
 
 plus the batched multi-filter engine ``BatchedParticleFilter``.  The compiled
 HIP library (libgnss_gpu_amd.so) is loaded on first use of any of them; there
@@ -20,6 +21,10 @@ _LAZY = {
     "wls_batch": ("_gnss_gpu", "wls_batch"),
     "wls_batch_device": ("_gnss_gpu", "wls_batch_device"),
     "MultiGNSSSolver": ("multi_gnss", "MultiGNSSSolver"),
+    "BuildingModel": ("raytrace", "BuildingModel"),
+    "BVHAccelerator": ("bvh", "BVHAccelerator"),
+    "ParticleFilter3D": ("particle_filter_3d", "ParticleFilter3D"),
+    "ParticleFilter3DBVH": ("particle_filter_3d", "ParticleFilter3DBVH"),
 }
 
 __all__ = list(_LAZY)

@@ -170,6 +170,21 @@ _SIGS = {
     "gg_wls_multi_gnss_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_multi_gnss": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_multi_gnss_batch_device": [_P, _P, _P, _P, _I, _I, _I, _I, _D, _P, _P, _I, _P],
+    "gg_bvh_build": [_DP, _I, _DP, _IP, _IP],
+    "gg_bvh_create": [_DP, _I, _DP, _I, _P],
+    "gg_bvh_destroy": [_P],
+    "gg_raytrace_los_check_bvh": [_DP, _DP, _DP, _I, _DP, _I, _IP, _I],
+    "gg_raytrace_los_check_bvh_batch": [_DP, _DP, _DP, _I, _DP, _I, _IP, _I, _I],
+    "gg_raytrace_multipath_bvh": [_DP, _DP, _DP, _I, _DP, _I, _DP, _DP, _I],
+    "gg_raytrace_multipath_bvh_batch": [_DP, _DP, _DP, _I, _DP, _I, _DP, _DP, _I, _I],
+    "gg_raytrace_los_check": [_DP, _DP, _DP, _I, _IP, _I],
+    "gg_raytrace_multipath": [_DP, _DP, _DP, _I, _DP, _DP, _I],
+    "gg_pf_weight_3d_bvh": [_DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _I, _DP, _I, _DP, _I, _I,
+                            _D, _D, _D, _D, _D],
+    "gg_pf_weight_3d": [_DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _I, _DP, _I, _I, _D, _D, _D, _D,
+                        _D],
+    "gg_pf_weight_3d_bvh_device": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D,
+                                   _P],
     "gg_debug_math": [_I, _DP, _DP, _I],
 }
 

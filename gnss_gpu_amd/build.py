@@ -17,7 +17,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_NAME = "libgnss_gpu_amd.so"
-SOURCES = ["gg_common.hip", "gg_pf.hip", "gg_wls.hip", "gg_learn.hip"]
+SOURCES = ["gg_common.hip", "gg_pf.hip", "gg_wls.hip", "gg_learn.hip", "gg_bvh.hip"]
 
 
 def hipcc_path() -> str:

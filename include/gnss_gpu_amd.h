@@ -193,6 +193,94 @@ int gg_wls_multi_gnss_batch_device(const double* sat_ecef, const double* pseudor
                                    int32_t max_iter, double tol, double* results, int32_t* iters,
                                    int32_t mode, void* hip_stream);
 
+/* --- ray casting through building meshes (binary64 ECEF) ------------------- */
+/* Trees use the reference's flat node rows [n_nodes][10]: min_xyz, max_xyz, left,
+ * right, tri_start, tri_count (left == -1: leaf; node 0 is the root), with the
+ * triangles [n_tri][3][3] in the tree's order.  Every entry point that takes a
+ * tree checks it first (a proper tree from node 0, leaf ranges inside the
+ * triangle array, depth <= 62) and returns an error otherwise.  An empty tree
+ * (n_nodes == 0) blocks nothing. */
+typedef struct gg_bvh gg_bvh;
+
+/* reference: bvh_build, src/raytrace/bvh.cu:221-243 (binding _bvh_bindings.cpp:12-49).
+ * Binned-SAH build on the host; nodes_flat holds up to 2 * n_tri rows, *n_nodes is
+ * the count written, sorted_tri_indices [n_tri] the triangle order of the leaves.
+ * The tree may differ from the reference's; every query result depends only on the
+ * triangles. */
+int gg_bvh_build(const double* triangles, int32_t n_tri, double* nodes_flat, int32_t* n_nodes,
+                 int32_t* sorted_tri_indices);
+/* Device-resident copy of a tree for repeated queries (no reference counterpart:
+ * the reference uploads the tree on every call). */
+int gg_bvh_create(const double* nodes_flat, int32_t n_nodes, const double* sorted_tris,
+                  int32_t n_tri, gg_bvh** out);
+int gg_bvh_destroy(gg_bvh* bvh);
+
+/* reference: raytrace_los_check_bvh, src/raytrace/bvh.cu:798-846 (kernel :293-354).
+ * is_los [n_sat]: 1 when no triangle lies strictly between receiver and satellite. */
+int gg_raytrace_los_check_bvh(const double* rx_ecef, const double* sat_ecef,
+                              const double* nodes_flat, int32_t n_nodes,
+                              const double* sorted_tris, int32_t n_tri, int32_t* is_los,
+                              int32_t n_sat);
+/* reference: raytrace_los_check_bvh_batch, bvh.cu:693-741 (kernel :517-585).
+ * rx_ecef [n_epoch][3], sat_ecef [n_epoch][n_sat][3], is_los [n_epoch][n_sat];
+ * a non-finite coordinate gives 0. */
+int gg_raytrace_los_check_bvh_batch(const double* rx_ecef, const double* sat_ecef,
+                                    const double* nodes_flat, int32_t n_nodes,
+                                    const double* sorted_tris, int32_t n_tri, int32_t* is_los,
+                                    int32_t n_epoch, int32_t n_sat);
+/* reference: raytrace_multipath_bvh, bvh.cu:464-515 (kernel :362-462).  Smallest
+ * positive excess delay of a first-order specular reflection per satellite
+ * (0 and a zero point when there is none); reflection_points [n_sat][3]. */
+int gg_raytrace_multipath_bvh(const double* rx_ecef, const double* sat_ecef,
+                              const double* nodes_flat, int32_t n_nodes,
+                              const double* sorted_tris, int32_t n_tri,
+                              double* reflection_points, double* excess_delays, int32_t n_sat);
+/* reference: raytrace_multipath_bvh_batch, bvh.cu:743-796 (kernel :587-691); a
+ * non-finite coordinate gives 0 and a zero point. */
+int gg_raytrace_multipath_bvh_batch(const double* rx_ecef, const double* sat_ecef,
+                                    const double* nodes_flat, int32_t n_nodes,
+                                    const double* sorted_tris, int32_t n_tri,
+                                    double* reflection_points, double* excess_delays,
+                                    int32_t n_epoch, int32_t n_sat);
+/* reference: raytrace_los_check / raytrace_multipath, src/raytrace/raytrace.cu:195-277
+ * (linear scans); same results through a tree built on the call. */
+int gg_raytrace_los_check(const double* rx_ecef, const double* sat_ecef, const double* triangles,
+                          int32_t n_tri, int32_t* is_los, int32_t n_sat);
+int gg_raytrace_multipath(const double* rx_ecef, const double* sat_ecef, const double* triangles,
+                          int32_t n_tri, double* reflection_points, double* excess_delays,
+                          int32_t n_sat);
+
+/* reference: pf_weight_3d_bvh, src/particle_filter/weight_3d_bvh.cu:225-300 (kernel
+ * :127-220; binding _pf3d_bvh_bindings.cpp:18-92).  Host arrays px/py/pz/pcb [N];
+ * overwrites log_weights [N] with the sum over satellites of the LOS / NLOS mixture:
+ * LOS term -0.5 w r^2 / sigma_los^2, NLOS term with r - nlos_bias for r > 0 and
+ * sigma_nlos, mixed with P(NLOS) = blocked_nlos_prob when a triangle lies on the
+ * particle-satellite ray (1e-6 m < t < range), clear_nlos_prob otherwise. */
+int gg_pf_weight_3d_bvh(const double* px, const double* py, const double* pz, const double* pcb,
+                        const double* sat_ecef, const double* pseudoranges,
+                        const double* weights_sat, const double* nodes_flat, int32_t n_nodes,
+                        const double* sorted_tris, int32_t n_tri, double* log_weights,
+                        int32_t n_particles, int32_t n_sat, double sigma_pr_los,
+                        double sigma_pr_nlos, double nlos_bias, double blocked_nlos_prob,
+                        double clear_nlos_prob);
+/* reference: pf_weight_3d, src/particle_filter/weight_3d.cu:184-253 (linear scan); same
+ * result through a tree built on the call. */
+int gg_pf_weight_3d(const double* px, const double* py, const double* pz, const double* pcb,
+                    const double* sat_ecef, const double* pseudoranges, const double* weights_sat,
+                    const double* triangles, int32_t n_tri, double* log_weights,
+                    int32_t n_particles, int32_t n_sat, double sigma_pr_los,
+                    double sigma_pr_nlos, double nlos_bias, double blocked_nlos_prob,
+                    double clear_nlos_prob);
+/* Device-pointer variant of gg_pf_weight_3d_bvh on a gg_bvh handle, enqueued on
+ * hip_stream without synchronising (all arrays device-resident binary64). */
+int gg_pf_weight_3d_bvh_device(const gg_bvh* bvh, const double* px, const double* py,
+                               const double* pz, const double* pcb, const double* sat_ecef,
+                               const double* pseudoranges, const double* weights_sat,
+                               double* log_weights, int32_t n_particles, int32_t n_sat,
+                               double sigma_pr_los, double sigma_pr_nlos, double nlos_bias,
+                               double blocked_nlos_prob, double clear_nlos_prob,
+                               void* hip_stream);
+
 /* --- double-differenced and carrier-phase updates (single filter) ---------- */
 /* reference: pf_device_weight_dd_pseudorange, src/particle_filter/pf_device.cu:1576-1620
  * (kernel :282-368).  Overwrites the log-weights with the DD pseudorange

@@ -100,6 +100,15 @@ PINS = {
     "wls.wls_multi_gnss_ref": ("pinned", "plain ranges recovered, Sagnac ranges not "
                                "(multi_gnss.cu:101-112); guards of test_multi_gnss.py:188-197"),
     "wls.wls_multi_gnss_batch_ref": ("pinned", "test_multi_gnss.py:100-172 truths and bounds"),
+    # raytrace.py
+    "raytrace.mt_hit": ("pinned", "plane parameter and barycentric 3x3 solve on random rays "
+                        "(test_mt_hit_against_plane_and_barycentric_solve)"),
+    "raytrace.los_blocked": ("pinned", "box cases of test_bvh.py:26-39, empty mesh"),
+    "raytrace.los_check": ("pinned", "test_bvh.py:26-39, 178-191 (NaN padding)"),
+    "raytrace.multipath": ("pinned", "wall reflection closed form "
+                           "(test_multipath_wall_closed_form)"),
+    "raytrace.mixture": ("pinned", "log((1-p)e^los + p e^nlos) closed form"),
+    "raytrace.weight_3d": ("pinned", "closed forms of test_pf3d_bvh.py:94-194"),
     # learned.py
     "learned.unpack": ("pinned", "inverse of pack (round trip in the GPU parameter test)"),
     "learned.pack": ("pinned", "inverse of unpack"),

@@ -116,3 +116,110 @@ def test_multi_gnss_argument_checks():
     with pytest.raises(ValueError):
         wls_multi_gnss_batch(np.zeros((2, 9)), np.zeros((2, 9)), np.ones((2, 9)),
                              np.zeros((2, 9), dtype=np.int32), 1)
+
+
+def _random_boxes(n_boxes, seed):
+    """test_bvh.py:64-77: a mesh of random boxes, 12 triangles each."""
+    from gnss_gpu_amd.raytrace import BuildingModel
+
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n_boxes):
+        c = [rng.uniform(-500, 500), rng.uniform(-500, 500), rng.uniform(0, 100)]
+        out.append(BuildingModel.create_box(c, rng.uniform(5, 30), rng.uniform(5, 30),
+                                            rng.uniform(10, 60)).triangles)
+    return np.concatenate(out, axis=0)
+
+
+def test_bvh_build_is_a_valid_tree():
+    """Host build (no GPU): every triangle in exactly one leaf, leaves of at most
+    4, each box contains its children's boxes or its triangles, depth <= 62."""
+    from gnss_gpu_amd.bvh import BVHAccelerator
+
+    for tris in (_random_boxes(100, 42), _random_boxes(1, 0), np.zeros((0, 3, 3))):
+        b = BVHAccelerator(tris)
+        assert b.n_triangles == tris.shape[0]
+        if tris.shape[0] == 0:
+            assert b.n_nodes == 0
+            continue
+        nodes = b._nodes_flat
+        assert sorted(b._sorted_indices.tolist()) == list(range(tris.shape[0]))
+        np.testing.assert_array_equal(b._sorted_tris, tris[b._sorted_indices])
+        covered = np.zeros(tris.shape[0], dtype=int)
+        stack = [(0, 0)]
+        while stack:
+            i, depth = stack.pop()
+            assert depth <= 62
+            lo, hi = nodes[i, :3], nodes[i, 3:6]
+            left, right, start, count = (int(v) for v in nodes[i, 6:])
+            if left == -1:
+                assert 1 <= count <= 4
+                covered[start:start + count] += 1
+                pts = b._sorted_tris[start:start + count].reshape(-1, 3)
+                assert np.all(pts >= lo - 1e-9) and np.all(pts <= hi + 1e-9)
+            else:
+                for ch in (left, right):
+                    assert np.all(nodes[ch, :3] >= lo) and np.all(nodes[ch, 3:6] <= hi)
+                    stack.append((ch, depth + 1))
+        assert np.all(covered == 1)
+
+
+def test_malformed_trees_are_rejected_before_any_device_work():
+    from gnss_gpu_amd._bvh import raytrace_los_check_bvh
+
+    tris = _random_boxes(2, 1)
+    from gnss_gpu_amd.bvh import BVHAccelerator
+
+    b = BVHAccelerator(tris)
+    sat = np.array([[0.0, 0.0, 2e7]])
+    bad = b._nodes_flat.copy()
+    bad[0, 6] = 0  # root is its own child
+    with pytest.raises(RuntimeError, match="child index"):
+        raytrace_los_check_bvh(np.zeros(3), sat, bad, b._sorted_tris)
+    bad = b._nodes_flat.copy()
+    leaf = int(np.nonzero(bad[:, 6] == -1)[0][0])
+    bad[leaf, 8] = tris.shape[0]  # leaf range past the triangle array
+    with pytest.raises(RuntimeError, match="leaf range"):
+        raytrace_los_check_bvh(np.zeros(3), sat, bad, b._sorted_tris)
+    bad = b._nodes_flat.copy()
+    bad[0, 7] = 0.5
+    with pytest.raises(RuntimeError, match="integers"):
+        raytrace_los_check_bvh(np.zeros(3), sat, bad, b._sorted_tris)
+    # a chain deeper than 62 levels
+    chain = np.zeros((2 * 70 + 1, 10))
+    for i in range(70):
+        chain[2 * i, 6:] = [2 * i + 1, 2 * i + 2, 0, 0]
+        chain[2 * i + 1, 6:] = [-1, -1, 0, 1]
+    chain[140, 6:] = [-1, -1, 0, 1]
+    with pytest.raises(RuntimeError, match="deeper"):
+        raytrace_los_check_bvh(np.zeros(3), sat, chain, tris[:1])
+
+
+def test_bvh_and_pf3d_argument_checks():
+    from gnss_gpu_amd import ParticleFilter3D, ParticleFilter3DBVH
+    from gnss_gpu_amd._gnss_gpu_pf3d_bvh import pf_weight_3d_bvh
+    from gnss_gpu_amd.bvh import BVHAccelerator
+
+    with pytest.raises(ValueError):
+        BVHAccelerator(np.zeros((10, 2, 3)))
+    b = BVHAccelerator.create_box([0, 0, 0], 10, 10, 10)
+    assert b.n_triangles == 12 and b.n_nodes >= 1
+    with pytest.raises(ValueError):
+        b.check_los_batch(np.zeros((4, 3)), np.zeros((4, 3)))
+    with pytest.raises(ValueError):
+        b.check_los_batch(np.zeros((4, 2)), np.zeros((4, 2, 3)))
+    with pytest.raises(ValueError):
+        b.check_los_batch(np.zeros((3, 3)), np.zeros((4, 2, 3)))
+    with pytest.raises(TypeError):
+        ParticleFilter3DBVH(bvh="not_a_bvh", n_particles=100)
+    with pytest.raises(TypeError):
+        ParticleFilter3D(building_model="x", n_particles=100)
+    pf = ParticleFilter3DBVH(bvh=b, n_particles=500, sigma_los=4.0, sigma_nlos=25.0,
+                             nlos_bias=15.0)
+    assert (pf.sigma_los, pf.sigma_nlos, pf.nlos_bias, pf.n_particles) == (4.0, 25.0, 15.0, 500)
+    with pytest.raises(RuntimeError, match="not initialized"):
+        pf.update(np.zeros((1, 3)), np.zeros(1))
+    with pytest.raises(ValueError, match="log_weights"):
+        pf_weight_3d_bvh(np.zeros(4), np.zeros(4), np.zeros(4), np.zeros(4), np.zeros(3),
+                         np.zeros(1), np.ones(1), b._nodes_flat, b._sorted_tris,
+                         np.zeros(4, dtype=np.float32), 4, 1, 3.0, 30.0, 20.0)

@@ -530,6 +530,107 @@ def test_solve_n_against_numpy_and_zero_rows():
     np.testing.assert_allclose(x[0, keep], red, rtol=1e-9)
 
 
+# ------------------------------------------------------------ ray casting ----
+def _box(center=(100.0, 0.0, 25.0), w=20.0, d=20.0, h=50.0):
+    from gnss_gpu_amd.raytrace import BuildingModel
+
+    return BuildingModel.create_box(list(center), w, d, h).triangles
+
+
+def test_mt_hit_against_plane_and_barycentric_solve():
+    """Moller-Trumbore against an independent form: the plane parameter
+    t = n.(v0 - o) / n.d and barycentric coordinates from a 3x3 solve of
+    o + t d = v0 + u e1 + v e2; agreement away from edges (margin 1e-9)."""
+    from oracle import raytrace as oray
+
+    rng = np.random.default_rng(21)
+    tris = rng.normal(0, 10, (60, 3, 3))
+    o = rng.normal(0, 30, (200, 3))
+    d = rng.normal(0, 1, (200, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    hit, t = oray.mt_hit(o, d, tris)
+    checked = 0
+    for r in range(200):
+        for k in range(60):
+            v0, e1, e2 = tris[k, 0], tris[k, 1] - tris[k, 0], tris[k, 2] - tris[k, 0]
+            M = np.stack([-d[r], e1, e2], axis=1)
+            tt, u, v = np.linalg.solve(M, o[r] - v0)
+            margin = min(u, v, 1 - u - v)
+            if abs(margin) < 1e-9:
+                continue
+            assert hit[r, k] == (margin > 0), (r, k)
+            if hit[r, k]:
+                assert t[r, k] == pytest.approx(tt, rel=1e-9, abs=1e-9)
+                checked += 1
+    assert checked > 50
+
+
+def test_los_box_cases_and_empty_mesh():
+    """test_bvh.py:26-39: satellite overhead LOS, behind the box NLOS; an
+    empty mesh blocks nothing (test_pf3d_bvh.py:196-214); NaN padding in the
+    batch query is not LOS (test_bvh.py:178-191)."""
+    from oracle import raytrace as oray
+
+    tris = _box()
+    los = oray.los_check([0.0, 0.0, 0.0], [[0.0, 0.0, 2e7], [200.0, 0.0, 25.0]], tris)
+    assert los.tolist() == [[True, False]]
+    assert oray.los_check([0.0, 0.0, 0.0], [[200.0, 0.0, 25.0]], np.zeros((0, 3, 3))).all()
+    rx = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])
+    sat = np.array([[[0.0, 0.0, 2e7], [200.0, 0.0, 25.0]], [[0.0, 0.0, 2e7], [np.nan] * 3]])
+    assert oray.los_check(rx, sat, tris, batch=True).tolist() == [[True, False], [True, False]]
+
+
+def test_weight_3d_reference_closed_forms():
+    """test_pf3d_bvh.py:94-194: zero LOS residual -> 0; NLOS residual equal to
+    the bias -> 0; a negative NLOS residual keeps its value (-0.5 b^2 / s^2);
+    the NLOS sigma penalises less than the LOS one; the 0 < p < 1 mixture is
+    log((1-p) e^los + p e^nlos)."""
+    from oracle import raytrace as oray
+
+    tris = _box()
+    z = np.zeros(8)
+    cb = np.full(8, 100.0)
+    up = np.array([[0.0, 0.0, 20000.0]])
+    lw = oray.weight_3d(z, z, z, cb, up, [20000.0 + 100.0], [1.0], tris, 3.0, 30.0, 20.0)
+    np.testing.assert_allclose(lw, 0.0, atol=1e-10)
+    behind = np.array([[200.0, 0.0, 25.0]])
+    dist = float(np.linalg.norm(behind))
+    lw = oray.weight_3d(z, z, z, cb, behind, [dist + 120.0], [1.0], tris, 3.0, 30.0, 20.0)
+    np.testing.assert_allclose(lw, 0.0, atol=1e-10)
+    lw = oray.weight_3d(z, z, z, cb, behind, [dist + 80.0], [1.0], tris, 3.0, 30.0, 20.0)
+    np.testing.assert_allclose(lw, -0.5 * 400.0 / 900.0, atol=1e-10)
+    lw = oray.weight_3d(z, z, z, cb, behind, [dist + 150.0], [1.0], tris, 3.0, 30.0, 20.0)
+    assert np.all(lw > -0.5 * 50.0 ** 2 / 9.0)
+    p = 0.3
+    lw = oray.weight_3d(z, z, z, cb, behind, [dist + 150.0], [2.0], tris, 3.0, 30.0, 20.0,
+                        blocked_nlos_prob=p)
+    los, nlos = -0.5 * 2.0 * 2500.0 / 9.0, -0.5 * 2.0 * 900.0 / 900.0
+    np.testing.assert_allclose(lw, math.log((1 - p) * math.exp(los) + p * math.exp(nlos)),
+                               rtol=1e-12)
+
+
+def test_multipath_wall_closed_form():
+    """A vertical wall x = 10 (two triangles): the receiver's mirror image is
+    (20 - x, y, z); the reflection point is where the image-to-satellite line
+    crosses the wall; the excess delay is |image - sat| - |rx - sat|."""
+    from oracle import raytrace as oray
+
+    wall = np.array([[[10.0, -100.0, -50.0], [10.0, 100.0, -50.0], [10.0, 100.0, 150.0]],
+                     [[10.0, -100.0, -50.0], [10.0, 100.0, 150.0], [10.0, -100.0, 150.0]]])
+    rx = np.array([0.0, 0.0, 1.5])
+    sat = np.array([-2.0e7, 3.0e6, 1.5e7])
+    refl, delay = oray.multipath(rx, sat[None], wall)
+    img = np.array([20.0, 0.0, 1.5])
+    t = (10.0 - img[0]) / (sat[0] - img[0])
+    pt = img + t * (sat - img)
+    np.testing.assert_allclose(refl[0, 0], pt, atol=1e-6)
+    np.testing.assert_allclose(delay[0, 0], np.linalg.norm(img - sat) - np.linalg.norm(rx - sat),
+                               rtol=1e-9)
+    # a satellite on the receiver's side of the wall plane behind it has no reflection
+    refl, delay = oray.multipath(rx, np.array([[2.0e7, 0.0, 1.0e7]]), wall)
+    assert delay[0, 0] == 0.0 and np.all(refl == 0.0)
+
+
 def test_oracle_predict_deterministic_closed_form():
     """sigma = 0: every particle moves by v*dt exactly (test_cuda_streams.py:239-255)."""
     pf = opf.OraclePF(100, seed=1)
@@ -548,7 +649,10 @@ def test_every_oracle_function_has_a_pinning_entry():
 
     from oracle import learned as olearned
 
-    mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls, "learned": olearned}
+    from oracle import raytrace as oray
+
+    mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls, "learned": olearned,
+            "raytrace": oray}
     missing = []
     for name, mod in mods.items():
         for fn, obj in vars(mod).items():
