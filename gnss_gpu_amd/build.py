@@ -17,7 +17,8 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_NAME = "libgnss_gpu_amd.so"
-SOURCES = ["gg_common.hip", "gg_pf.hip", "gg_wls.hip", "gg_learn.hip", "gg_bvh.hip"]
+SOURCES = ["gg_common.hip", "gg_pf.hip", "gg_wls.hip", "gg_learn.hip", "gg_bvh.hip",
+           "gg_smooth.hip"]
 
 
 def hipcc_path() -> str:

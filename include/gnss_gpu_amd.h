@@ -281,6 +281,29 @@ int gg_pf_weight_3d_bvh_device(const gg_bvh* bvh, const double* px, const double
                                double blocked_nlos_prob, double clear_nlos_prob,
                                void* hip_stream);
 
+/* --- backward trajectory sampling over a particle history ----------------- */
+/* reference: ffbsi_sample_indices / genealogy_smooth_indices,
+ * python/gnss_gpu/particle_ffbsi.py:32-104, 121-151 (NumPy).  n_traj trajectories,
+ * one workgroup each, steps t = T-1 .. 0 in one launch.  log_weights [T][N];
+ * FFBSi (ancestors == NULL): states [T][N][4] (x, y, z, cb), vel [T][3], dt [T],
+ * sigma_pos [T] (row t+1 drives the t -> t+1 transition), sigma_cb; genealogy
+ * (ancestors [T][N] given, values in [0, N)): only the last step samples.
+ * u [n_traj][T]: the uniform used at step t; forced [n_traj][T]: an index >= 0
+ * replaces step t's draw (-1: sample).  indices [n_traj][T] (int64) out.
+ * status [n_traj]: -1 done; -2 the last step is degenerate (indices not written);
+ * t >= 0 the first degenerate step met (a NaN or +inf log-weight, or all -inf),
+ * where the walk stopped -- the reference draws an integer there, which the caller
+ * passes through forced[] in a second call. */
+int gg_ffbsi_sample(const double* log_weights, const double* states, const double* vel,
+                    const double* dt, const double* sigma_pos, double sigma_cb,
+                    const int32_t* ancestors, int32_t T, int32_t N, int32_t n_traj,
+                    const double* u, const int32_t* forced, int64_t* indices, int32_t* status);
+/* reference: transition_logpdf, particle_ffbsi.py:14-29.  out [N] = log p(x_next | x_t[i])
+ * of one predict step (Gaussian position noise sigma_pos per axis after v*dt, clock
+ * noise sigma_cb), up to the normalising constant. */
+int gg_transition_logpdf(const double* x_next, const double* x_t, int32_t N, const double* vel,
+                         double dt, double sigma_pos, double sigma_cb, double* out);
+
 /* --- double-differenced and carrier-phase updates (single filter) ---------- */
 /* reference: pf_device_weight_dd_pseudorange, src/particle_filter/pf_device.cu:1576-1620
  * (kernel :282-368).  Overwrites the log-weights with the DD pseudorange

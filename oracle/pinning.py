@@ -109,6 +109,15 @@ PINS = {
                            "(test_multipath_wall_closed_form)"),
     "raytrace.mixture": ("pinned", "log((1-p)e^los + p e^nlos) closed form"),
     "raytrace.weight_3d": ("pinned", "closed forms of test_pf3d_bvh.py:94-194"),
+    # ffbsi.py
+    "ffbsi.choice_index": ("pinned", "numpy Generator.choice on 40 seeds "
+                           "(test_choice_index_is_generator_choice)"),
+    "ffbsi.transition_logpdf": ("pinned", "scipy.stats.norm.logpdf sums up to a constant"),
+    "ffbsi._normalised": ("pinned", "through ffbsi_indices"),
+    "ffbsi.ffbsi_indices": ("pinned", "test_ffbsi.py:37-50 bounds; chi-square of the backward "
+                            "draw against w0 p(x1 | x0); generator consumption incl. "
+                            "degenerate steps"),
+    "ffbsi.genealogy_indices": ("pinned", "test_ffbsi.py:23-35 trace invariant"),
     # learned.py
     "learned.unpack": ("pinned", "inverse of pack (round trip in the GPU parameter test)"),
     "learned.pack": ("pinned", "inverse of unpack"),

@@ -1,0 +1,104 @@
+"""GPU backward trajectory sampling (FFBSi, genealogy) against the NumPy
+restatement of python/gnss_gpu/particle_ffbsi.py, driven by the same
+generators: indices must be identical and the generators left in the same
+state."""
+
+import numpy as np
+import pytest
+
+from oracle import ffbsi as of
+
+pytestmark = pytest.mark.gpu
+
+
+def _history(T, N, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    X = np.cumsum(rng.normal(0, spread, (T, N, 4)), axis=0) * np.array([1.0, 1.0, 1.0, 3.0])
+    X[..., :3] += np.array([-3957199.0, 3310205.0, 3737911.0])
+    lw = rng.normal(0, 2.0, (T, N))
+    vel = rng.normal(0, 0.1, (T, 3))
+    return lw, X, vel, np.full(T, 1.0), rng.uniform(1.0, 2.0, T)
+
+
+def _same_state(a, b):
+    return a.random() == b.random()
+
+
+def test_ffbsi_matches_oracle_and_batch(gpu):
+    from gnss_gpu_amd.particle_ffbsi import (ffbsi_sample_indices, ffbsi_sample_indices_batch,
+                                             ffbsi_smooth_sample)
+
+    lw, X, vel, dt, sp = _history(60, 3000, 1)
+    for seed in range(4):
+        g, r = np.random.default_rng(seed), np.random.default_rng(seed)
+        got = ffbsi_sample_indices(lw, X, vel, dt, sp, 8.0, g)
+        ref = of.ffbsi_indices(lw, X, vel, dt, sp, 8.0, r)
+        np.testing.assert_array_equal(got, ref)
+        assert got.dtype == np.int64 and _same_state(g, r)
+    gens = [np.random.default_rng(100 + k) for k in range(8)]
+    batch = ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 8.0, gens)
+    for k in range(8):
+        np.testing.assert_array_equal(
+            batch[k], of.ffbsi_indices(lw, X, vel, dt, sp, 8.0, np.random.default_rng(100 + k)))
+    path = ffbsi_smooth_sample(lw, X, vel, dt, sp, 8.0, np.random.default_rng(0))
+    ref = of.ffbsi_indices(lw, X, vel, dt, sp, 8.0, np.random.default_rng(0))
+    np.testing.assert_array_equal(path, X[np.arange(60), ref, :3])
+
+
+def test_ffbsi_large_population(gpu):
+    """N = 200,000 (782 particles per thread range), T = 12."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
+
+    lw, X, vel, dt, sp = _history(12, 200_000, 2, spread=0.3)
+    got = ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 5.0,
+                                     [np.random.default_rng(k) for k in range(3)])
+    for k in range(3):
+        np.testing.assert_array_equal(
+            got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 5.0, np.random.default_rng(k)))
+
+
+def test_ffbsi_degenerate_steps(gpu):
+    """A NaN and an all -inf row mid-history redraw integers(0, N) at those
+    steps (two extra launches); a degenerate last step gives zeros and leaves
+    the generator untouched (particle_ffbsi.py:73-75, 98-100)."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices
+
+    lw, X, vel, dt, sp = _history(30, 500, 3)
+    lw[17, 40] = np.nan
+    lw[6] = -np.inf
+    lw[3, 2] = np.inf
+    for seed in range(3):
+        g, r = np.random.default_rng(seed), np.random.default_rng(seed)
+        got = ffbsi_sample_indices(lw, X, vel, dt, sp, 8.0, g)
+        np.testing.assert_array_equal(got, of.ffbsi_indices(lw, X, vel, dt, sp, 8.0, r))
+        assert _same_state(g, r)
+    lw[29] = np.nan
+    g, r = np.random.default_rng(9), np.random.default_rng(9)
+    assert np.all(ffbsi_sample_indices(lw, X, vel, dt, sp, 8.0, g) == 0)
+    assert _same_state(g, r)
+
+
+def test_genealogy_and_transition(gpu):
+    from gnss_gpu_amd.particle_ffbsi import (genealogy_smooth_indices, genealogy_smooth_sample,
+                                             transition_logpdf)
+
+    rng = np.random.default_rng(5)
+    T, N = 40, 2048
+    lw = rng.standard_normal((T, N)) * 2.0
+    anc = np.vstack([rng.integers(0, N, (T - 1, N)), np.arange(N)[None]])
+    for seed in range(3):
+        g, r = np.random.default_rng(seed), np.random.default_rng(seed)
+        got = genealogy_smooth_indices(lw, anc, g)
+        np.testing.assert_array_equal(got, of.genealogy_indices(lw, anc, r))
+        assert _same_state(g, r)
+    X = rng.normal(0, 1, (T, N, 4))
+    p = genealogy_smooth_sample(lw, X, anc, np.random.default_rng(1))
+    np.testing.assert_array_equal(
+        p, X[np.arange(T), of.genealogy_indices(lw, anc, np.random.default_rng(1)), :3])
+    xt = rng.normal(0, 5, (1000, 4))
+    xn = np.array([1.0, -2.0, 0.5, 40.0])
+    got = transition_logpdf(xn, xt, [0.3, -0.1, 0.2], 0.7, 1.2, 30.0)
+    np.testing.assert_allclose(got, of.transition_logpdf(xn, xt, [0.3, -0.1, 0.2], 0.7, 1.2, 30.0),
+                               rtol=1e-14, atol=1e-12)
+    with pytest.raises(ValueError):
+        genealogy_smooth_indices(lw, np.full((T, N), N), g)

@@ -631,6 +631,98 @@ def test_multipath_wall_closed_form():
     assert delay[0, 0] == 0.0 and np.all(refl == 0.0)
 
 
+# ------------------------------------------------------------------ FFBSi ----
+def test_choice_index_is_generator_choice():
+    """numpy's Generator.choice(N, p) restated as one random() and a search of
+    the normalised cumulative sum: same index for the same generator state."""
+    from oracle import ffbsi as of
+
+    for seed in range(40):
+        w = np.random.default_rng(1000 + seed).random(257) ** 3
+        a, b = np.random.default_rng(seed), np.random.default_rng(seed)
+        assert of.choice_index(w, a.random()) == int(b.choice(257, p=w / w.sum()))
+
+
+def test_transition_logpdf_is_gaussian_up_to_a_constant():
+    from scipy.stats import norm
+
+    from oracle import ffbsi as of
+
+    rng = np.random.default_rng(2)
+    xt = rng.normal(0, 5, (30, 4))
+    xn = np.array([1.0, -2.0, 0.5, 40.0])
+    v, dt, sp, sc = np.array([0.3, -0.1, 0.2]), 0.7, 1.3, 25.0
+    lp = of.transition_logpdf(xn, xt, v, dt, sp, sc)
+    ref = (norm.logpdf(xn[:3], xt[:, :3] + v * dt, sp).sum(axis=1) + norm.logpdf(xn[3], xt[:, 3], sc))
+    np.testing.assert_allclose(lp - lp[0], ref - ref[0], rtol=1e-12, atol=1e-12)
+
+
+def _history(T, N, seed):
+    rng = np.random.default_rng(seed)
+    X = np.cumsum(rng.normal(0, 1.0, (T, N, 4)), axis=0) * np.array([1.0, 1.0, 1.0, 3.0])
+    lw = rng.normal(0, 2.0, (T, N))
+    vel = rng.normal(0, 0.1, (T, 3))
+    return lw, X, vel, np.full(T, 1.0), np.full(T, 1.5)
+
+
+def test_ffbsi_backward_distribution_and_reference_shapes():
+    """test_ffbsi.py:23-56 (bounds, genealogy trace); with T = 2 the backward
+    draw of idx[0] follows w0_i p(x1[idx1] | x0_i) normalised (chi-square over
+    20,000 trajectories); the generator is consumed one random() per step."""
+    from scipy.stats import chisquare
+
+    from oracle import ffbsi as of
+
+    lw, X, vel, dt, sp = _history(5, 32, 0)
+    I = of.ffbsi_indices(lw, X, vel, dt, sp, 10.0, np.random.default_rng(0))
+    assert I.shape == (5,) and I.dtype == np.int64 and np.all((I >= 0) & (I < 32))
+    rng = np.random.default_rng(1)
+    anc = np.vstack([rng.integers(0, 16, (3, 16)), np.arange(16)[None]])
+    lwg = rng.standard_normal((4, 16)) * 2.0
+    J = of.genealogy_indices(lwg, anc, rng)
+    for t in range(2, -1, -1):
+        assert J[t] == anc[t, J[t + 1]]
+    lw, X, vel, dt, sp = _history(2, 6, 3)
+    X[1] = X[0] + rng.normal(0, 0.5, (6, 4))
+    counts = np.zeros((6, 6))
+    g = np.random.default_rng(7)
+    for _ in range(20000):
+        i = of.ffbsi_indices(lw, X, vel, dt, sp, 3.0, g)
+        counts[i[1], i[0]] += 1
+    for j in range(6):
+        if counts[j].sum() < 500:
+            continue
+        logp = lw[0] + of.transition_logpdf(X[1, j], X[0], vel[1], dt[1], sp[1], 3.0)
+        p = np.exp(logp - logp.max())
+        p /= p.sum()
+        keep = p * counts[j].sum() > 5
+        exp_ = p[keep] * counts[j].sum()
+        assert chisquare(counts[j][keep], exp_ * counts[j][keep].sum() / exp_.sum()).pvalue > 1e-4
+    a, b = np.random.default_rng(11), np.random.default_rng(11)
+    of.ffbsi_indices(lw, X, vel, dt, sp, 3.0, a)
+    b.random(2)
+    assert a.random() == b.random()
+
+
+def test_ffbsi_degenerate_steps_consume_like_the_reference():
+    """particle_ffbsi.py:73-75, 98-100: a degenerate step draws integers(0, N),
+    a degenerate last step draws nothing and returns zeros."""
+    from oracle import ffbsi as of
+
+    lw, X, vel, dt, sp = _history(6, 20, 4)
+    lw[2, 5] = np.nan
+    a, b = np.random.default_rng(3), np.random.default_rng(3)
+    I = of.ffbsi_indices(lw, X, vel, dt, sp, 10.0, a)
+    b.random(3)
+    assert I[2] == int(b.integers(0, 20))
+    b.random(2)
+    assert a.random() == b.random()
+    lw[5] = -np.inf
+    a, b = np.random.default_rng(3), np.random.default_rng(3)
+    assert np.all(of.ffbsi_indices(lw, X, vel, dt, sp, 10.0, a) == 0)
+    assert a.random() == b.random()
+
+
 def test_oracle_predict_deterministic_closed_form():
     """sigma = 0: every particle moves by v*dt exactly (test_cuda_streams.py:239-255)."""
     pf = opf.OraclePF(100, seed=1)
@@ -649,10 +741,11 @@ def test_every_oracle_function_has_a_pinning_entry():
 
     from oracle import learned as olearned
 
+    from oracle import ffbsi as offbsi
     from oracle import raytrace as oray
 
     mods = {"philox": philox, "detmath": detmath, "pf": opf, "wls": owls, "learned": olearned,
-            "raytrace": oray}
+            "raytrace": oray, "ffbsi": offbsi}
     missing = []
     for name, mod in mods.items():
         for fn, obj in vars(mod).items():
