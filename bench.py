@@ -551,6 +551,122 @@ def multi_gnss_leg(args):
             "cpu_baseline": cpu}
 
 
+def _box_city(n_boxes, span, seed):
+    """Random axis-aligned buildings (12 triangles each) over a square of
+    side 2 * span metres around the origin."""
+    from gnss_gpu_amd.raytrace import BuildingModel
+
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n_boxes):
+        c = [rng.uniform(-span, span), rng.uniform(-span, span), rng.uniform(0, 100)]
+        out.append(BuildingModel.create_box(c, rng.uniform(5, 30), rng.uniform(5, 30),
+                                            rng.uniform(10, 60)).triangles)
+    return np.concatenate(out)
+
+
+def raytrace_leg(args):
+    """Ray-traced NLOS weight (one ray per particle and satellite through a BVH,
+    binary64) on 1M particles x 12 satellites, device-resident, for meshes of
+    1,200 / 12,000 / 120,000 triangles."""
+    import ctypes as C
+
+    import torch
+
+    from gnss_gpu_amd import _lib
+    from gnss_gpu_amd.bvh import BVHAccelerator
+
+    dev = torch.device("cuda")
+    n, ns = 1 << 20, 12
+    rng = np.random.RandomState(4)
+    p = [rng.uniform(-300, 300, n), rng.uniform(-300, 300, n), rng.uniform(0, 30, n),
+         rng.uniform(90, 110, n)]
+    th = rng.uniform(0, 2 * np.pi, ns)
+    ph = rng.uniform(np.radians(10), np.pi / 2, ns)
+    sat = 2.2e7 * np.stack([np.cos(ph) * np.cos(th), np.cos(ph) * np.sin(th), np.sin(ph)], 1)
+    pr = np.sqrt((sat ** 2).sum(1)) + 100.0
+    tp = [torch.from_numpy(a).to(dev) for a in p]
+    ts, tr, tw = (torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                  for a in (sat, pr, np.ones(ns)))
+    lw = torch.empty(n, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream(dev).cuda_stream
+    rows = []
+    for nb, span in ((100, 500.0), (1000, 1500.0), (10000, 5000.0)):
+        tris = _box_city(nb, span, 1)
+        t0 = time.perf_counter()
+        b = BVHAccelerator(tris)
+        build_ms = (time.perf_counter() - t0) * 1e3
+        h = C.c_void_p()
+        _lib.call("gg_bvh_create", _lib.dptr(np.ascontiguousarray(b._nodes_flat)), b.n_nodes,
+                  _lib.dptr(np.ascontiguousarray(b._sorted_tris)), b.n_triangles, C.byref(h))
+        a = [h, *[C.c_void_p(t.data_ptr()) for t in tp], C.c_void_p(ts.data_ptr()),
+             C.c_void_p(tr.data_ptr()), C.c_void_p(tw.data_ptr()), C.c_void_p(lw.data_ptr()), n,
+             ns, 3.0, 30.0, 20.0, 1.0, 0.0, C.c_void_p(st)]
+        for _ in range(2):
+            _lib.call("gg_pf_weight_3d_bvh_device", *a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            _lib.call("gg_pf_weight_3d_bvh_device", *a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        _lib.call("gg_bvh_destroy", h)
+        rows.append({"triangles": int(len(tris)), "nodes": int(b.n_nodes),
+                     "build_ms": round(build_ms, 2), "weight_ms": round(ms, 4),
+                     "rays_per_s": round(n * ns / (ms * 1e-3), 1)})
+    cpu = None
+    if not args.no_cpu:
+        from oracle import raytrace as oray
+
+        k = 2000
+        tris = _box_city(100, 500.0, 1)
+        t0 = time.perf_counter()
+        oray.weight_3d(*(a[:k] for a in p), sat, pr, np.ones(ns), tris, 3.0, 30.0, 20.0)
+        sec = time.perf_counter() - t0
+        cpu = {"kind": "port", "rays_per_s": round(k * ns / sec, 1), "cores": 1,
+               "sample": "NumPy brute-force oracle (oracle/raytrace.py), 2,000 particles x 12 "
+                         "satellites x 1,200 triangles"}
+    return {"workload": "raytrace_weight_1Mx12", "particles": n, "satellites": ns,
+            "meshes": rows, "cpu_baseline": cpu}
+
+
+def ffbsi_leg(args):
+    """FFBSi backward sampling: 8 trajectories over T = 1000 epochs of
+    N = 8192 particles (the reference experiment's defaults), host arrays in,
+    one launch for all trajectories."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
+
+    T, N, M = 1000, 8192, 8
+    rng = np.random.default_rng(12)
+    X = np.cumsum(rng.normal(0, 1.0, (T, N, 4)), axis=0)
+    lw = rng.normal(0, 2.0, (T, N))
+    vel = rng.normal(0, 0.1, (T, 3))
+    dt, sp = np.ones(T), np.full(T, 1.5)
+    ffbsi_sample_indices_batch(lw[:10], X[:10], vel[:10], dt[:10], sp[:10], 10.0,
+                               [np.random.default_rng(0)])
+    reps = 3
+    t0 = time.perf_counter()
+    for r in range(reps):
+        ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 10.0,
+                                   [np.random.default_rng(100 * r + k) for k in range(M)])
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ffbsi as of
+
+        t0 = time.perf_counter()
+        of.ffbsi_indices(lw, X, vel, dt, sp, 10.0, np.random.default_rng(0))
+        one = (time.perf_counter() - t0) * 1e3
+        cpu = {"kind": "port", "ms_per_call": round(one * M, 1), "cores": 1,
+               "sample": "NumPy restatement of particle_ffbsi.py (oracle/ffbsi.py), one "
+                         "trajectory timed, times 8"}
+    return {"workload": "ffbsi_T1000_N8192_8paths", "ms_per_call": round(ms, 2),
+            "history_bytes": int(X.nbytes + lw.nbytes),
+            "note": "host arrays in (PCIe upload of the history included)", "cpu_baseline": cpu}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -669,6 +785,8 @@ def main():
     batched = None if args.no_batched else batched_leg(args)
     wls = None if args.no_batched else wls_leg(args)
     mgnss = None if args.no_batched else multi_gnss_leg(args)
+    rtrace = None if args.no_batched else raytrace_leg(args)
+    ffbsi = None if args.no_batched else ffbsi_leg(args)
     scale = None if args.no_scale else scale_leg(args)
 
     out = {
@@ -704,6 +822,8 @@ def main():
         "batched": batched,
         "wls": wls,
         "multi_gnss_wls": mgnss,
+        "raytrace": rtrace,
+        "ffbsi": ffbsi,
         "scale": scale,
         "cpu_baseline": cpu,
     }
