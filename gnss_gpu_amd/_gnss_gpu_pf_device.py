@@ -140,6 +140,28 @@ def pf_device_weight_carrier_afv(state, sat_ecef, carrier_phase, weights_sat, n_
          float(wavelength), float(sigma_cycles))
 
 
+def pf_device_weight_3d_bvh(state, bvh, sat_ecef, pseudoranges, weights_sat, n_sat,
+                            sigma_pr_los=3.0, sigma_pr_nlos=30.0, nlos_bias=20.0,
+                            blocked_nlos_prob=1.0, clear_nlos_prob=0.0, system_ids=None):
+    """Ray-traced NLOS weight on the device particles (no reference binding: the
+    device-filter counterpart of pf_weight_3d_bvh, weight_3d_bvh.cu:127-220);
+    overwrites the log-weights.  ``bvh`` is a ``BVHAccelerator``."""
+    n = int(n_sat)
+    sat = f64(sat_ecef).ravel()
+    pr = f64(pseudoranges).ravel()
+    w = f64(weights_sat).ravel()
+    if sat.size < 3 * n or pr.size < n or w.size < n:
+        raise ValueError("satellite arrays are shorter than n_sat")
+    sys = None
+    if system_ids is not None:
+        sys = np.ascontiguousarray(np.asarray(system_ids, dtype=np.int32).ravel())
+        if sys.size < n:
+            raise ValueError("system_ids shorter than n_sat")
+    call("gg_pf_device_weight_3d_bvh", state.handle, bvh.device_handle(), dptr(sat), dptr(pr),
+         dptr(w), iptr(sys) if sys is not None else None, n, float(sigma_pr_los),
+         float(sigma_pr_nlos), float(nlos_bias), float(blocked_nlos_prob), float(clear_nlos_prob))
+
+
 def pf_device_weight_dd_carrier_afv(state, sat_ecef_k, ref_ecef, dd_carrier, base_range_k,
                                     base_range_ref, weights_dd, wavelengths_m, n_dd,
                                     sigma_cycles=0.05, per_particle_nlos_threshold_cycles=0.0,

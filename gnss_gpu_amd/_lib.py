@@ -171,6 +171,7 @@ _SIGS = {
     "gg_wls_multi_gnss": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_multi_gnss_batch_device": [_P, _P, _P, _P, _I, _I, _I, _I, _D, _P, _P, _I, _P],
     "gg_bvh_build": [_DP, _I, _DP, _IP, _IP],
+    "gg_pf_device_weight_3d_bvh": [_P, _P, _DP, _DP, _DP, _IP, _I, _D, _D, _D, _D, _D],
     "gg_ffbsi_sample": [_DP, _DP, _DP, _DP, _DP, _D, _IP, _I, _I, _I, _DP, _IP, _P, _IP],
     "gg_transition_logpdf": [_DP, _DP, _I, _DP, _D, _D, _D, _DP],
     "gg_bvh_create": [_DP, _I, _DP, _I, _P],

@@ -19,6 +19,25 @@ class BVHAccelerator:
         self._nodes_flat, self._sorted_indices = bvh_build(self.triangles)
         self._sorted_tris = self.triangles[self._sorted_indices].copy()
 
+    def device_handle(self):
+        """Device-resident copy of the tree (created on first use, freed with the
+        object) for the device filter's ray-traced update."""
+        h = getattr(self, "_device", None)
+        if h is None:
+            import ctypes as C
+            import weakref
+
+            from . import _lib
+
+            h = C.c_void_p()
+            nodes = np.ascontiguousarray(self._nodes_flat, dtype=np.float64)
+            tris = np.ascontiguousarray(self._sorted_tris, dtype=np.float64)
+            _lib.call("gg_bvh_create", _lib.dptr(nodes), nodes.shape[0], _lib.dptr(tris),
+                      tris.shape[0], C.byref(h))
+            self._device = h
+            weakref.finalize(self, _lib.lib.gg_bvh_destroy, h)
+        return h
+
     @property
     def n_nodes(self):
         return self._nodes_flat.shape[0]

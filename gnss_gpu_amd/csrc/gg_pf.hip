@@ -795,6 +795,22 @@ int launch_obs(gg_pf* h, int kind, int n, double sigma, double thr, int huber, d
   return 0;
 }
 
+// Ray-traced NLOS weight on a single filter's particles (gg_k_weight_3d_pf).
+int launch_weight_3d(gg_pf* h, const gg_bvh* bvh, int n, const gg_rt::MixArgs& mx) {
+  const gg_arena a = arena_of(h);
+  const size_t smem = (size_t)std::max(n, 1) * 48;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_3d_pf<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), smem, h->stream, a, h->obsd, n,
+                                                bvh->nodes, bvh->tris, bvh->n_nodes, mx));
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_clear_lw_zero, dim3(1), dim3(64), 0, h->stream, h->dyn, 0);
+  GG_HIP(hipGetLastError());
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
 // Doppler rows, 8 binary64 per satellite (see gg_k_doppler), to h->obsd.
 int stage_doppler(gg_pf* h, int n, const double* sat, const double* vel, const double* dop,
                   const double* w) {
@@ -892,6 +908,27 @@ int gg_pf_device_weight_dd_pseudorange(gg_pf* h, const double* sat_ecef_k, const
     return -1;
   return launch_obs(h, 0, n_dd, sigma_pr, per_particle_nlos_threshold_m, per_particle_huber,
                     per_particle_huber_k);
+}
+
+int gg_pf_device_weight_3d_bvh(gg_pf* h, const gg_bvh* bvh, const double* sat_ecef,
+                               const double* pseudoranges, const double* weights_sat,
+                               const int32_t* system_ids, int32_t n_sat, double sigma_pr_los,
+                               double sigma_pr_nlos, double nlos_bias, double blocked_nlos_prob,
+                               double clear_nlos_prob) {
+  if (int rc = check_single(h)) return rc;
+  if (!bvh) return gg_fail(-2, "null BVH handle");
+  if (n_sat > 0 && (!sat_ecef || !pseudoranges)) return gg_fail(-2, "null argument");
+  std::vector<double> grp((size_t)std::max(n_sat, 1), 0.0);
+  for (int k = 0; k < n_sat; ++k) {
+    const int g = system_ids ? system_ids[k] : 0;
+    grp[k] = (g >= 1 && g <= h->n_isb) ? (double)g : 0.0;
+  }
+  if (stage_obs(h, n_sat, sat_ecef, nullptr, pseudoranges, grp.data(), nullptr, weights_sat,
+                nullptr, 0.0))
+    return -1;
+  gg_rt::MixArgs mx;
+  gg_rt::make_mix(sigma_pr_los, sigma_pr_nlos, nlos_bias, blocked_nlos_prob, clear_nlos_prob, mx);
+  return launch_weight_3d(h, bvh, n_sat, mx);
 }
 
 int gg_pf_device_weight_carrier_afv(gg_pf* h, const double* sat_ecef, const double* carrier_phase,

@@ -1,4 +1,5 @@
 // Particle-filter kernels (see gg_pf_kernels.h for the launch plan).
+#include "gg_bvh.h"
 #include "gg_pf_kernels.h"
 
 #include <climits>
@@ -1172,6 +1173,87 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_obs(gg_arena a, const do
   }
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) lw[q] = (i0 + q < F.n) ? lw[q] : 0.0f;
+  *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Ray-traced NLOS weight on the filter's particles: the device-filter
+// counterpart of the reference's pf_weight_3d_bvh (weight_3d_bvh.cu:127-220).
+// Each particle is taken to ECEF in binary64 (anchor + R^T offset) and one
+// ray per satellite walks the tree (gg_bvh.h); the LOS / NLOS mixture is
+// evaluated in binary64 and stored as the binary32 log-weight (overwritten,
+// as the reference does).  obs: 11 binary64 rows (stage_obs): ECEF position
+// [0..2], pseudorange [6], system group [7], weight [9].  The clock bias and,
+// for group g >= 1, the inter-system bias come from the particle's state.
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_3d_pf(gg_arena a,
+                                                              const double* __restrict__ obs,
+                                                              int n_obs,
+                                                              const gg_bvh_node* __restrict__ nodes,
+                                                              const double* __restrict__ tris,
+                                                              int n_nodes, gg_rt::MixArgs mx) {
+  constexpr int D = 4 + NISB;
+  extern __shared__ __align__(16) char smem[];
+  double* so = reinterpret_cast<double*>(smem);  // [n_obs][6]: x y z pr w group
+  const int tile = blockIdx.x;
+  const int f = a.tile_filter[tile];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int lt = tile - F.tile_off;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  const int64_t base = F.p_off + i0;
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, dy->cur, base, p);
+  for (int k = threadIdx.x; k < n_obs; k += blockDim.x) {
+    const double* o = obs + (int64_t)k * 11;
+    so[6 * k + 0] = o[0];
+    so[6 * k + 1] = o[1];
+    so[6 * k + 2] = o[2];
+    so[6 * k + 3] = o[6];
+    so[6 * k + 4] = o[9];
+    so[6 * k + 5] = o[7];
+  }
+  __syncthreads();
+  float lw[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    lw[q] = 0.0f;
+    if (i0 + q >= F.n) continue;
+    const double off[3] = {(double)p[q][0], (double)p[q][1], (double)p[q][2]};
+    double x[3];
+    gg_rot_ecef(dy->R, off, x);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) x[j] += dy->anchor[j];
+    const double cb = dy->cb_anchor + (double)p[q][3];
+    double acc = 0.0;
+    for (int k = 0; k < n_obs; ++k) {
+      const double* sk = so + 6 * k;
+      gg_rt::Ray r;
+      r.o[0] = x[0];
+      r.o[1] = x[1];
+      r.o[2] = x[2];
+      const double dx = sk[0] - x[0], dy0 = sk[1] - x[1], dz = sk[2] - x[2];
+      const double dist = sqrt(dx * dx + dy0 * dy0 + dz * dz);
+      const double inv = 1.0 / dist;
+      r.d[0] = dx * inv;
+      r.d[1] = dy0 * inv;
+      r.d[2] = dz * inv;
+      r.tmax = dist;
+      gg_rt::make_inv(r);
+      const bool nl = n_nodes > 0 && gg_rt::gg_bvh_blocked(r, 1e-6, nodes, tris);
+      const int g = (int)sk[5];
+      double bias = 0.0;
+      if (NISB > 0 && g >= 1 && g <= NISB) bias = dy->isb_anchor[g - 1] + (double)p[q][3 + g];
+      const double res = sk[3] - (dist + cb + bias);
+      const double los = -0.5 * sk[4] * res * res * mx.inv_los2;
+      const double rn = res > 0.0 ? res - mx.nlos_bias : res;
+      const double nlos = -0.5 * sk[4] * rn * rn * mx.inv_nlos2;
+      const int m = nl ? 1 : 0;
+      acc += gg_rt::mixture(los, nlos, mx.mode[m], mx.lp_keep[m], mx.lp_nlos[m]);
+    }
+    lw[q] = (float)acc;
+  }
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
 }
 

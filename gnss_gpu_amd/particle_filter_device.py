@@ -198,6 +198,25 @@ class ParticleFilterDevice:
         if resample:
             _ = self.resample_if_needed()
 
+    def update_3d_bvh(self, bvh, sat_ecef, pseudoranges, weights=None, sigma_los=3.0,
+                      sigma_nlos=30.0, nlos_bias=20.0, blocked_nlos_prob=1.0,
+                      clear_nlos_prob=0.0, resample=True, *, system_ids=None):
+        """Ray-traced NLOS likelihood on the device particles (the update of the
+        reference's ParticleFilter3DBVH, particle_filter_3d_bvh.py:79-120, on this
+        filter's resident state); overwrites the log-weights.  ``bvh`` is a
+        ``BVHAccelerator`` over the building mesh (ECEF)."""
+        self._require_init()
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
+        n_sat = len(pr)
+        w = (np.ones(n_sat, dtype=np.float64) if weights is None
+             else np.asarray(weights, dtype=np.float64).ravel())
+        _dev.pf_device_weight_3d_bvh(self._state, bvh, sat.ravel(), pr, w, n_sat, sigma_los,
+                                     sigma_nlos, nlos_bias, blocked_nlos_prob, clear_nlos_prob,
+                                     system_ids=system_ids)
+        if resample:
+            _ = self.resample_if_needed()
+
     def update_gmm(self, *args, **kwargs):
         return _dev.pf_device_weight_gmm(self._state, *args, **kwargs)
 

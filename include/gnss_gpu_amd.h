@@ -281,6 +281,17 @@ int gg_pf_weight_3d_bvh_device(const gg_bvh* bvh, const double* px, const double
                                double blocked_nlos_prob, double clear_nlos_prob,
                                void* hip_stream);
 
+/* Ray-traced NLOS weight on a single device filter's particles (the device-filter
+ * counterpart of pf_weight_3d_bvh, src/particle_filter/weight_3d_bvh.cu:127-220):
+ * overwrites the log-weights with the LOS / NLOS mixture of gg_pf_weight_3d_bvh for
+ * each particle's ECEF position and clock bias (plus the inter-system bias of
+ * system_ids[k] in 1..n_isb; system_ids may be NULL).  n_sat <= 1024. */
+int gg_pf_device_weight_3d_bvh(gg_pf* h, const gg_bvh* bvh, const double* sat_ecef,
+                               const double* pseudoranges, const double* weights_sat,
+                               const int32_t* system_ids, int32_t n_sat, double sigma_pr_los,
+                               double sigma_pr_nlos, double nlos_bias, double blocked_nlos_prob,
+                               double clear_nlos_prob);
+
 /* --- backward trajectory sampling over a particle history ----------------- */
 /* reference: ffbsi_sample_indices / genealogy_smooth_indices,
  * python/gnss_gpu/particle_ffbsi.py:32-104, 121-151 (NumPy).  n_traj trajectories,

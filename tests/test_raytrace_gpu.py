@@ -270,3 +270,70 @@ def test_device_weight_on_resident_particles(gpu):
         np.testing.assert_allclose(got[idx], ref, rtol=1e-9, atol=1e-7)  # range rounding
     finally:
         _lib.call("gg_bvh_destroy", h)
+
+
+def _pf_states(pf):
+    p = pf.get_particles()
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), p[:, 3].copy()
+
+
+def test_device_filter_ray_traced_update_matches_oracle(gpu):
+    """ParticleFilterDevice.update_3d_bvh on resident particles equals the
+    oracle's ray-traced weight at the particles' ECEF states (log-weights are
+    binary32 on the device)."""
+    from gnss_gpu_amd import ParticleFilterDevice
+    from gnss_gpu_amd.bvh import BVHAccelerator
+    from gnss_gpu_amd.sim import enu_rotation
+
+    anchor = np.array([-3957199.0, 3310205.0, 3737911.0])
+    R = enu_rotation(anchor)
+    enu_tris = _random_boxes(60, 12).reshape(-1, 3) * np.array([0.2, 0.2, 1.0])
+    tris = (anchor + enu_tris @ R).reshape(-1, 3, 3)
+    b = BVHAccelerator(tris)
+    rng = np.random.RandomState(8)
+    ns = 10
+    u = _sky(rng, ns, 1.0, 1.0, el_lo=np.radians(10))
+    sat = anchor + 2.2e7 * (u @ R)
+    pr = np.linalg.norm(sat - anchor, axis=1) + 50.0 + rng.normal(0, 5, ns)
+    ws = rng.uniform(0.5, 1.5, ns)
+    pf = ParticleFilterDevice(n_particles=100_000, sigma_pos=1.0, sigma_cb=1.0, seed=3)
+    pf.initialize(anchor + np.array([3.0, -4.0, 1.0]) @ R, clock_bias=50.0, spread_pos=40.0,
+                  spread_cb=5.0)
+    pf.update_3d_bvh(b, sat, pr, ws, sigma_los=4.0, sigma_nlos=25.0, nlos_bias=15.0,
+                     blocked_nlos_prob=0.9, clear_nlos_prob=0.05, resample=False)
+    lw = pf.get_log_weights()
+    px, py, pz, pcb = _pf_states(pf)
+    ref = oray.weight_3d(px, py, pz, pcb, sat, pr, ws, tris, 4.0, 25.0, 15.0, 0.9, 0.05)
+    np.testing.assert_allclose(lw, ref.astype(np.float32), rtol=2e-6, atol=2e-5)
+    blocked = ~oray.los_check(np.stack([px[:2000], py[:2000], pz[:2000]], 1),
+                              np.broadcast_to(sat, (2000, ns, 3)), tris, batch=True)
+    assert 0.02 < blocked.mean() < 0.98
+    pf.update_3d_bvh(b, sat, pr, ws)
+    est = pf.estimate()
+    assert np.all(np.isfinite(est))
+
+
+def test_device_filter_ray_traced_update_with_isb(gpu):
+    """A second system's satellites carry the particle's inter-system bias."""
+    from gnss_gpu_amd import ParticleFilterDevice
+    from gnss_gpu_amd.bvh import BVHAccelerator
+
+    b = BVHAccelerator.from_building_model(_box())
+    sat = np.array([[0.0, 0.0, 20000.0], [200.0, 0.0, 25.0], [0.0, 200.0, 5000.0],
+                    [-200.0, 0.0, 5000.0]])
+    sys_ids = np.array([0, 0, 1, 1], dtype=np.int32)
+    pr = np.linalg.norm(sat, axis=1) + 100.0 + np.array([0.0, 0.0, 7.0, 7.0])
+    pf = ParticleFilterDevice(n_particles=4096, sigma_pos=1.0, sigma_cb=1.0, seed=5, n_isb=1)
+    pf.initialize([0.0, 0.0, 0.0], clock_bias=100.0, spread_pos=5.0, spread_cb=2.0)
+    pf.update_3d_bvh(b, sat, pr, None, resample=False, system_ids=sys_ids)
+    from gnss_gpu_amd import _gnss_gpu_pf_device as dev
+
+    lw = pf.get_log_weights()
+    full = dev.pf_device_get_particles_full(pf._state)
+    px, py, pz, pcb, isb = (full[:, k].copy() for k in range(5))
+    ref = np.zeros_like(px)
+    for i in range(0, 4096, 16):
+        adj = pr - np.array([0.0, 0.0, isb[i], isb[i]])
+        ref[i] = oray.weight_3d(px[i:i + 1], py[i:i + 1], pz[i:i + 1], pcb[i:i + 1], sat, adj,
+                                np.ones(4), b.triangles, 3.0, 30.0, 20.0)[0]
+    np.testing.assert_allclose(lw[::16], ref[::16].astype(np.float32), rtol=2e-6, atol=2e-5)
