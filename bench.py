@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--learned-particles", type=int, default=1_000_000)
     ap.add_argument("--learned-epochs", type=int, default=120)
     ap.add_argument("--learned-steps", type=int, default=100)
-    ap.add_argument("--learned-train-epochs", type=int, default=60000)
+    ap.add_argument("--learned-train-epochs", type=int, default=125000,
+                    help="training epochs of 16 satellites (125,000: 2.0M satellite-epochs)")
     ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--no-scale", action="store_true", help="skip the 4M x 32 scale leg")
     ap.add_argument("--scale-particles", type=int, default=4_000_000)
