@@ -578,6 +578,30 @@ def multi_gnss_leg(args):
             "cpu_baseline": cpu}
 
 
+def measured_copy_gbs(mib=1024, reps=20):
+    """Achievable HBM bandwidth: a device-to-device copy of ``mib`` MiB (read +
+    write bytes over the copy time, torch's copy kernel), median of ``reps``."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    del src, dst
+    return 2.0 * mib * (1 << 20) / (ms * 1e-3) / 1e9
+
+
 def _box_city(n_boxes, span, seed):
     """Random axis-aligned buildings (12 triangles each) over a square of
     side 2 * span metres around the origin."""
@@ -792,6 +816,9 @@ def main():
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": bytes_pp[dom] * N, "avg_launch_ms": round(stage_ms[dom], 5)}
+    copy_gbs = measured_copy_gbs()
+    roofline["copy_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
     resample_gbs = (resample_bytes * N / (ms_res[gather_key] * 1e-3) / 1e9) if n_res else None
 
     # ---- CPU baseline (bounded sample of the same workload, host cores)
