@@ -32,6 +32,7 @@ struct gg_pf {
   int32_t* E = nullptr;
   int32_t* anc = nullptr;
   double* part = nullptr;
+  double* p4 = nullptr;
   double* g = nullptr;
   double* Pb = nullptr;
   int32_t* emax = nullptr;
@@ -91,6 +92,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.lw = h->lw;
   a.anc = h->anc;
   a.part = h->part;
+  a.p4 = h->p4;
   a.g = h->g;
   a.Pb = h->Pb;
   a.emax = h->emax;
@@ -487,6 +489,7 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->lw, (size_t)h->P);
   if (!rc) rc = dev_alloc(&h->anc, (size_t)2 * h->P);
   if (!rc) rc = dev_alloc(&h->part, (size_t)tiles * GG_NPART);
+  if (!rc) rc = dev_alloc(&h->p4, (size_t)tiles * 4);
   if (!rc) rc = dev_alloc(&h->g, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->Pb, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
@@ -536,7 +539,7 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
-  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->g, h->Pb,
+  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->p4, h->g, h->Pb,
                   h->emax,  h->csum,  h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,

@@ -34,6 +34,7 @@ struct gg_arena {
   float* mw;              // [P] Megopolis weights exp32(lw - M) (same memory as E)
   int32_t* anc;           // [2][P]
   double* part;           // [n_tiles_total][GG_NPART]
+  double* p4;             // [n_tiles_total][4]: m_b, S_b, S2_b, 0 (the combine's inputs, packed)
   double* g;              // [n_tiles_total]
   double* Pb;             // [n_tiles_total]
   int32_t* emax;          // [n_tiles_total] tile maxima of E
@@ -820,7 +821,7 @@ template <int D>
 __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
                                                  const bool (&valid)[GG_ITEMS],
                                                  const float (&p)[GG_ITEMS][D], double* lds,
-                                                 double* part_out) {
+                                                 double* part_out, double* p4_out) {
 #pragma clang fp contract(off)
   float m = -__builtin_huge_valf();
 #pragma unroll
@@ -864,6 +865,7 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
     part_out[0] = (double)mb;
     part_out[1] = tot[0];
     part_out[2] = tot[1];
+    *reinterpret_cast<double4*>(p4_out) = make_double4((double)mb, tot[0], tot[1], 0.0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const float* l = lf + d * 8;
@@ -985,6 +987,7 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
   const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
+  const double* p4 = a.p4 + (int64_t)F.tile_off * 4;  // m, S, S2 packed: coalesced reads
   double vv[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) vv[d] = 0.0;
@@ -1007,10 +1010,11 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int b = t + k * GG_BLOCK;
-      const double* pb = part + (int64_t)min(b, nt - 1) * GG_NPART;
-      pm[k] = pb[0];
-      pS[k] = pb[1];
-      pS2[k] = pb[2];
+      const double2* q = reinterpret_cast<const double2*>(p4 + (int64_t)min(b, nt - 1) * 4);
+      const double2 ms = q[0];
+      pm[k] = ms.x;
+      pS[k] = ms.y;
+      pS2[k] = q[1].x;
     }
     double m = -__builtin_huge_val();
 #pragma unroll
@@ -1024,11 +1028,11 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
     }
   } else {
     double m = -__builtin_huge_val();
-    for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, part[(int64_t)b * GG_NPART]);
+    for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, p4[(int64_t)b * 4]);
     M = gg_block_max256(m, lds);
     for (int b = t; b < nt; b += GG_BLOCK) {
-      const double* pb = part + (int64_t)b * GG_NPART;
-      pass2(b, pb[0], pb[1], pb[2], pb, M);
+      const double* q = p4 + (int64_t)b * 4;
+      pass2(b, q[0], q[1], q[2], part + (int64_t)b * GG_NPART, M);
     }
   }
   __syncthreads();

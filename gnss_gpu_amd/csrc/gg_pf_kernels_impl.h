@@ -177,7 +177,8 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? lw[q] : 0.0f;
   *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
-  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
+  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART,
+                      a.p4 + (int64_t)tile * 4);
 }
 
 // One round of workgroups covers 1M particles only at 4 waves per SIMD, so the
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_reduce(gg_arena a) {
     lw[2] = v.z;
     lw[3] = v.w;
   }
-  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART);
+  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART,
+                      a.p4 + (int64_t)tile * 4);
 }
 
 // ---------------------------------------------------------------------------
