@@ -174,6 +174,8 @@ _SIGS = {
     "gg_pf_device_weight_3d_bvh": [_P, _P, _DP, _DP, _DP, _IP, _I, _D, _D, _D, _D, _D],
     "gg_ffbsi_sample": [_DP, _DP, _DP, _DP, _DP, _D, _IP, _I, _I, _I, _DP, _IP, _P, _IP],
     "gg_transition_logpdf": [_DP, _DP, _I, _DP, _D, _D, _D, _DP],
+    "gg_ffbsi_sample_device": [_P, _P, _P, _P, _P, _D, _P, _I, _I, _I, _P, _P, _P, _P, _P],
+    "gg_pf_device_record": [_P, _P, _P],
     "gg_bvh_create": [_DP, _I, _DP, _I, _P],
     "gg_bvh_destroy": [_P],
     "gg_raytrace_los_check_bvh": [_DP, _DP, _DP, _I, _DP, _I, _IP, _I],

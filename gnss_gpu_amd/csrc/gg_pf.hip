@@ -1143,6 +1143,21 @@ static int export_rows(gg_pf* h, int f, int layout, double* out) {
   return 0;
 }
 
+int gg_pf_device_record(gg_pf* h, double* states_dev, double* log_weights_dev) {
+  if (int rc = check_single(h)) return rc;
+  if (!states_dev || !log_weights_dev) return gg_fail(-2, "null device pointer");
+  const int n = h->filt_h[0].n;
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_export<NI>), dim3(grid), dim3(GG_BLOCK),
+                                                0, h->stream, a, 0, 0, states_dev));
+  GG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gg_k_export_lw, dim3(grid), dim3(GG_BLOCK), 0, h->stream, a, 0,
+                     log_weights_dev);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
 int gg_pf_device_get_particles(gg_pf* h, double* out) {
   if (int rc = check_single(h)) return rc;
   return export_rows(h, 0, 0, out);

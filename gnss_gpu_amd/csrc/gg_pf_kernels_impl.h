@@ -933,6 +933,15 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_spread_combine(gg_arena a) {
   }
 }
 
+// Log-weights as binary64 (0 while they are logically zero), for a history slot.
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_export_lw(gg_arena a, int f, double* out) {
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  out[i] = dy->lw_zero ? 0.0 : (double)a.lw[F.p_off + i];
+}
+
 // ---------------------------------------------------------------------------
 // State export / import (binary64 absolute values, AoS rows).
 // layout 0: [x y z cb]; 1: [x y z cb vx vy vz S00..S22] (16); 2: [x y z cb isb..]

@@ -237,6 +237,23 @@ int gg_ffbsi_sample(const double* log_weights, const double* states, const doubl
   return rc;
 }
 
+int gg_ffbsi_sample_device(const double* log_weights, const double* states, const double* vel,
+                           const double* dt, const double* sigma_pos, double sigma_cb,
+                           const int32_t* ancestors, int32_t T, int32_t N, int32_t n_traj,
+                           const double* u, const int32_t* forced, int64_t* indices,
+                           int32_t* status, void* hip_stream) {
+  if (T <= 0 || N <= 0 || n_traj < 0) return gg_fail(-2, "T, N must be positive");
+  if (n_traj == 0) return 0;
+  if (!log_weights || !u || !forced || !indices || !status) return gg_fail(-2, "null array");
+  if (!ancestors && (!states || !vel || !dt || !sigma_pos))
+    return gg_fail(-2, "FFBSi needs states, vel, dt and sigma_pos");
+  hipLaunchKernelGGL(gg_k_ffbsi, dim3(n_traj), dim3(GG_FB_BLOCK), 0, (hipStream_t)hip_stream,
+                     log_weights, states, vel, dt, sigma_pos, 1.0 / (sigma_cb * sigma_cb),
+                     ancestors, T, N, u, forced, indices, status);
+  GG_HIP(hipGetLastError());
+  return 0;
+}
+
 int gg_transition_logpdf(const double* x_next, const double* x_t, int32_t N, const double* vel,
                          double dt, double sigma_pos, double sigma_cb, double* out) {
   if (N < 0) return gg_fail(-2, "negative size");

@@ -72,8 +72,10 @@ def _launch(a, sigma_cb, u, forced):
     return idx, status
 
 
-def _sample(a, sigma_cb, rngs, last_only):
-    T, N = a["T"], a["N"]
+def _sample(launch, T, N, rngs, last_only):
+    """Draw the uniforms from each generator in the reference's order, run the
+    backward walk, and redraw around degenerate steps (see the module note).
+    launch(u [M, T], forced [M, T]) -> (indices [M, T], status [M])."""
     M = len(rngs)
     saved = [r.bit_generator.state for r in rngs]
     degenerate = [set() for _ in range(M)]
@@ -96,8 +98,7 @@ def _sample(a, sigma_cb, rngs, last_only):
     result = np.zeros((M, T), dtype=np.int64)
     todo = list(range(M))
     while todo:
-        idx, status = _launch(a, sigma_cb, np.ascontiguousarray(u[todo]),
-                              np.ascontiguousarray(forced[todo]))
+        idx, status = launch(np.ascontiguousarray(u[todo]), np.ascontiguousarray(forced[todo]))
         again = []
         for j, k in enumerate(todo):
             st = int(status[j])
@@ -113,16 +114,20 @@ def _sample(a, sigma_cb, rngs, last_only):
     return result
 
 
+def _host(a, sigma_cb, rngs, last_only):
+    return _sample(lambda u, f: _launch(a, sigma_cb, u, f), a["T"], a["N"], rngs, last_only)
+
+
 def ffbsi_sample_indices(log_weights, X, vel, dt, sigma_pos, sigma_cb, rng):
     """One FFBSi backward trajectory as ancestor indices I[0..T-1] (int64)."""
     a = _checked(log_weights, X, vel, dt, sigma_pos)
-    return _sample(a, sigma_cb, [rng], last_only=False)[0]
+    return _host(a, sigma_cb, [rng], last_only=False)[0]
 
 
 def ffbsi_sample_indices_batch(log_weights, X, vel, dt, sigma_pos, sigma_cb, rngs):
     """len(rngs) trajectories in one launch: int64 [M, T]."""
     a = _checked(log_weights, X, vel, dt, sigma_pos)
-    return _sample(a, sigma_cb, list(rngs), last_only=False)
+    return _host(a, sigma_cb, list(rngs), last_only=False)
 
 
 def ffbsi_smooth_sample(log_weights, X, vel, dt, sigma_pos, sigma_cb, rng):
@@ -135,7 +140,7 @@ def genealogy_smooth_indices(log_weights, ancestors, rng):
     """Backward trace along the resampling genealogy (ancestors[t, j]: source of
     slot j after the resample at the end of epoch t; row T-1 is ignored)."""
     a = _checked(log_weights, ancestors=ancestors)
-    return _sample(a, 1.0, [rng], last_only=True)[0]  # sigma_cb: unused by the trace
+    return _host(a, 1.0, [rng], last_only=True)[0]  # sigma_cb: unused by the trace
 
 
 def genealogy_smooth_sample(log_weights, X, ancestors, rng):
@@ -145,6 +150,81 @@ def genealogy_smooth_sample(log_weights, X, ancestors, rng):
                       dtype=np.float64)
 
 
-__all__ = ["transition_logpdf", "ffbsi_sample_indices", "ffbsi_sample_indices_batch",
-           "ffbsi_smooth_sample", "genealogy_smooth_indices", "genealogy_smooth_sample"]
+class ParticleHistory:
+    """Device-resident particle history of a ``ParticleFilterDevice`` for FFBSi
+    (no reference counterpart: the reference's smoother experiments copy
+    get_particles() / get_log_weights() to host lists every epoch,
+    experiments/exp_ffbsi_eval.py:189-190, and pass the stacked arrays).
+
+    ``capacity`` epochs of N particles are held in HBM as binary64 ECEF rows
+    [T, N, 4] and log-weights [T, N] (40 B per particle and epoch: 1M particles
+    x 1000 epochs is 40 GB).  ``record`` stores the filter's current particles
+    and log-weights (after the update, before the resample, as the reference
+    experiment does) with the predict parameters of that epoch;
+    ``ffbsi_sample_indices`` samples trajectories on the stored history and
+    returns what ``ffbsi_sample_indices_batch`` returns for the equivalent host
+    arrays and generators.
+    """
+
+    def __init__(self, pf, capacity):
+        import torch
+
+        self.pf = pf
+        self.N = int(pf.n_particles)
+        self.capacity = int(capacity)
+        dev = torch.device("cuda")
+        self.X = torch.empty((self.capacity, self.N, 4), dtype=torch.float64, device=dev)
+        self.lw = torch.empty((self.capacity, self.N), dtype=torch.float64, device=dev)
+        self.vel = np.zeros((self.capacity, 3))
+        self.dt = np.zeros(self.capacity)
+        self.sigma_pos = np.zeros(self.capacity)
+        self.T = 0
+
+    def record(self, velocity=None, dt=1.0, sigma_pos=None):
+        """Store the current epoch; velocity / dt / sigma_pos are the predict
+        parameters that led into it (row t drives the t-1 -> t transition)."""
+        if self.T >= self.capacity:
+            raise RuntimeError("ParticleHistory is full")
+        t = self.T
+        call("gg_pf_device_record", self.pf._state.handle, C.c_void_p(self.X[t].data_ptr()),
+             C.c_void_p(self.lw[t].data_ptr()))
+        self.vel[t] = (np.zeros(3) if velocity is None
+                       else np.asarray(velocity, dtype=np.float64).ravel()[:3])
+        self.dt[t] = float(dt)
+        self.sigma_pos[t] = float(self.pf.sigma_pos if sigma_pos is None else sigma_pos)
+        self.T += 1
+
+    def ffbsi_sample_indices(self, sigma_cb, rngs):
+        """len(rngs) FFBSi trajectories over the recorded epochs: int64 [M, T]."""
+        import torch
+
+        T, N = self.T, self.N
+        if T == 0:
+            raise RuntimeError("ParticleHistory is empty")
+        self.pf.sync()
+        dev = self.X.device
+        vel = torch.from_numpy(np.ascontiguousarray(self.vel[:T])).to(dev)
+        dt = torch.from_numpy(np.ascontiguousarray(self.dt[:T])).to(dev)
+        sp = torch.from_numpy(np.ascontiguousarray(self.sigma_pos[:T])).to(dev)
+
+        def launch(u, forced):
+            M = u.shape[0]
+            tu = torch.from_numpy(u).to(dev)
+            tf = torch.from_numpy(forced).to(dev)
+            ti = torch.zeros((M, T), dtype=torch.int64, device=dev)
+            ts = torch.zeros(M, dtype=torch.int32, device=dev)
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            call("gg_ffbsi_sample_device", C.c_void_p(self.lw.data_ptr()),
+                 C.c_void_p(self.X.data_ptr()), C.c_void_p(vel.data_ptr()),
+                 C.c_void_p(dt.data_ptr()), C.c_void_p(sp.data_ptr()), float(sigma_cb), None,
+                 T, N, M, C.c_void_p(tu.data_ptr()), C.c_void_p(tf.data_ptr()),
+                 C.c_void_p(ti.data_ptr()), C.c_void_p(ts.data_ptr()), C.c_void_p(stream))
+            return ti.cpu().numpy(), ts.cpu().numpy()
+
+        return _sample(launch, T, N, list(rngs), last_only=False)
+
+
+__all__ = ["ParticleHistory", "transition_logpdf", "ffbsi_sample_indices",
+           "ffbsi_sample_indices_batch", "ffbsi_smooth_sample", "genealogy_smooth_indices",
+           "genealogy_smooth_sample"]
 _ = _lib

@@ -309,11 +309,26 @@ int gg_ffbsi_sample(const double* log_weights, const double* states, const doubl
                     const double* dt, const double* sigma_pos, double sigma_cb,
                     const int32_t* ancestors, int32_t T, int32_t N, int32_t n_traj,
                     const double* u, const int32_t* forced, int64_t* indices, int32_t* status);
+/* Device-pointer variant of gg_ffbsi_sample (every array device-resident), enqueued on
+ * hip_stream without synchronising; no reference counterpart (the reference smoother
+ * takes host arrays).  Pairs with gg_pf_device_record, which fills a history in HBM. */
+int gg_ffbsi_sample_device(const double* log_weights, const double* states, const double* vel,
+                           const double* dt, const double* sigma_pos, double sigma_cb,
+                           const int32_t* ancestors, int32_t T, int32_t N, int32_t n_traj,
+                           const double* u, const int32_t* forced, int64_t* indices,
+                           int32_t* status, void* hip_stream);
 /* reference: transition_logpdf, particle_ffbsi.py:14-29.  out [N] = log p(x_next | x_t[i])
  * of one predict step (Gaussian position noise sigma_pos per axis after v*dt, clock
  * noise sigma_cb), up to the normalising constant. */
 int gg_transition_logpdf(const double* x_next, const double* x_t, int32_t N, const double* vel,
                          double dt, double sigma_pos, double sigma_cb, double* out);
+
+/* Writes the single filter's particles as ECEF rows [N][4] (x, y, z, cb; the values
+ * gg_pf_device_get_particles returns) and its log-weights [N] (binary64) to device
+ * memory, enqueued on the handle's stream: one epoch of a device-resident history for
+ * the smoother (the reference copies get_particles / get_log_weights to the host,
+ * experiments/exp_ffbsi_eval.py:189-190).  No reference counterpart. */
+int gg_pf_device_record(gg_pf* h, double* states_dev, double* log_weights_dev);
 
 /* --- double-differenced and carrier-phase updates (single filter) ---------- */
 /* reference: pf_device_weight_dd_pseudorange, src/particle_filter/pf_device.cu:1576-1620

@@ -102,3 +102,39 @@ def test_genealogy_and_transition(gpu):
                                rtol=1e-14, atol=1e-12)
     with pytest.raises(ValueError):
         genealogy_smooth_indices(lw, np.full((T, N), N), g)
+
+
+def test_device_history_matches_host_arrays(gpu):
+    """ParticleHistory records a running ParticleFilterDevice in HBM; its FFBSi
+    indices equal those of the host-array functions on get_particles() /
+    get_log_weights() copies taken at the same points (the reference
+    experiment's loop, experiments/exp_ffbsi_eval.py:175-195)."""
+    from gnss_gpu_amd import ParticleFilterDevice, sim
+    from gnss_gpu_amd.particle_ffbsi import ParticleHistory, ffbsi_sample_indices_batch
+
+    sc = sim.scenario_plain(n_epochs=30, seed=3)
+    N, T = 4096, sc.pr.shape[0]
+    pf = ParticleFilterDevice(n_particles=N, sigma_pos=1.0, sigma_cb=3.0, sigma_pr=5.0, seed=9)
+    pf.initialize(sc.truth[0], clock_bias=100.0, spread_pos=10.0, spread_cb=10.0)
+    hist = ParticleHistory(pf, T)
+    X, LW, V = [], [], []
+    for t in range(T):
+        v = sc.truth[t] - sc.truth[max(t - 1, 0)]
+        pf.predict(velocity=v, dt=1.0)
+        pf.update(sc.sat, sc.pr[t], resample=False)
+        hist.record(velocity=v, dt=1.0)
+        X.append(pf.get_particles().copy())
+        LW.append(pf.get_log_weights().copy())
+        V.append(v)
+        pf.resample_if_needed()
+    X, LW, V = np.stack(X), np.stack(LW), np.stack(V)
+    np.testing.assert_array_equal(hist.X.cpu().numpy(), X)
+    np.testing.assert_array_equal(hist.lw.cpu().numpy(), LW)
+    got = hist.ffbsi_sample_indices(3.0, [np.random.default_rng(k) for k in range(4)])
+    ref = ffbsi_sample_indices_batch(LW, X, V, np.ones(T), np.full(T, 1.0), 3.0,
+                                     [np.random.default_rng(k) for k in range(4)])
+    np.testing.assert_array_equal(got, ref)
+    for k in range(2):
+        np.testing.assert_array_equal(
+            got[k], of.ffbsi_indices(LW, X, V, np.ones(T), np.full(T, 1.0), 3.0,
+                                     np.random.default_rng(k)))
