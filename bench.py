@@ -713,7 +713,33 @@ def ffbsi_leg(args):
         cpu = {"kind": "port", "ms_per_call": round(one * M, 1), "cores": 1,
                "sample": "NumPy restatement of particle_ffbsi.py (oracle/ffbsi.py), one "
                          "trajectory timed, times 8"}
+    # the same sampling on a history recorded in HBM by a running device filter
+    from gnss_gpu_amd import ParticleFilterDevice, sim
+    from gnss_gpu_amd.particle_ffbsi import ParticleHistory
+
+    sc = sim.scenario_plain(n_epochs=T, seed=5)
+    pf = ParticleFilterDevice(n_particles=N, sigma_pos=1.0, sigma_cb=3.0, sigma_pr=5.0, seed=4)
+    pf.initialize(sc.truth[0], clock_bias=100.0, spread_pos=10.0, spread_cb=10.0)
+    hist = ParticleHistory(pf, T)
+    t0 = time.perf_counter()
+    for t in range(T):
+        v = sc.truth[t] - sc.truth[max(t - 1, 0)]
+        pf.predict(velocity=v, dt=1.0)
+        pf.update(sc.sat, sc.pr[t], resample=False)
+        hist.record(velocity=v, dt=1.0)
+        pf.resample_if_needed()
+    pf.sync()
+    run_ms = (time.perf_counter() - t0) * 1e3 / T
+    hist.ffbsi_sample_indices(3.0, [np.random.default_rng(0)])
+    t0 = time.perf_counter()
+    for r in range(reps):
+        hist.ffbsi_sample_indices(3.0, [np.random.default_rng(100 * r + k) for k in range(M)])
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
     return {"workload": "ffbsi_T1000_N8192_8paths", "ms_per_call": round(ms, 2),
+            "device_history": {"ms_per_call": round(dev_ms, 2),
+                               "filter_epoch_ms_with_record": round(run_ms, 3),
+                               "note": "history recorded in HBM by the running device filter; "
+                                       "no host copies"},
             "history_bytes": int(X.nbytes + lw.nbytes),
             "note": "host arrays in (PCIe upload of the history included)", "cpu_baseline": cpu}
 
