@@ -602,6 +602,95 @@ def measured_copy_gbs(mib=1024, reps=20):
     return 2.0 * mib * (1 << 20) / (ms * 1e-3) / 1e9
 
 
+def updates_leg(args):
+    """The DD pseudorange, carrier AFV, DD carrier AFV, Doppler and Doppler-KF
+    updates on one device filter of 1,000,000 particles (12 satellites, 11 DD
+    pairs, per-particle velocities): median ms per host call (staging + kernel,
+    synchronised), against the NumPy oracle on 20,000 particles scaled to 1M."""
+    from types import SimpleNamespace
+
+    from gnss_gpu_amd import ParticleFilterDevice, sim
+    from gnss_gpu_amd.sim import enu_rotation
+
+    N, S = 1_000_000, 12
+    rng = np.random.default_rng(31)
+    sc = sim.scenario_plain(n_epochs=2, n_sat=S, seed=8)
+    sat, tp = sc.sat, sc.truth[0]
+    base = tp + np.array([500.0, -600.0, 120.0])
+    rr = np.linalg.norm(sat - tp, axis=1)
+    br = np.linalg.norm(sat - base, axis=1)
+    ks = list(range(1, S))
+    wl_l1 = 0.190293673
+    dd = (rr[ks] - rr[0]) - (br[ks] - br[0]) + rng.normal(0, 0.3, S - 1)
+    wls = np.full(S - 1, wl_l1)
+    ddc = dd / wls + rng.normal(0, 0.02, S - 1)
+    wdd = rng.uniform(0.5, 1.5, S - 1)
+    res = SimpleNamespace(sat_ecef_k=sat[ks], sat_ecef_ref=np.repeat(sat[[0]], S - 1, axis=0),
+                          base_range_k=br[ks], base_range_ref=np.repeat(br[0], S - 1),
+                          dd_weights=wdd, n_dd=S - 1, dd_pseudorange_m=dd,
+                          dd_carrier_cycles=ddc, wavelengths_m=wls)
+    cp = (rr + 50.0) / wl_l1 + rng.normal(0, 0.02, S) + 1234.0
+    R = enu_rotation(tp)
+    v0 = np.array([1.0, 0.5, -0.2]) @ R
+    sv = np.cross(sat, R[2])
+    sv = 3000.0 * sv / np.linalg.norm(sv, axis=1)[:, None]
+    los = (sat - tp) / rr[:, None]
+    dop = -np.sum((sv - v0) * los, axis=1) / wl_l1 + rng.normal(0, 0.3, S)
+    pf = ParticleFilterDevice(n_particles=N, seed=1, resampling="systematic")
+    pf.initialize(tp, clock_bias=50.0, spread_pos=4.0, spread_cb=2.0, velocity=v0,
+                  spread_vel=0.7, velocity_init_sigma=3.0)
+    calls = {
+        "dd_pseudorange": lambda: pf.update_dd_pseudorange(res, sigma_pr=0.8, resample=False),
+        "carrier_afv": lambda: pf.update_carrier_afv(sat, cp, sigma_cycles=0.05, resample=False),
+        "dd_carrier_afv": lambda: pf.update_dd_carrier_afv(res, sigma_cycles=0.05,
+                                                           resample=False),
+        "doppler": lambda: pf.update_doppler(sat, sv, dop, resample=False),
+        "doppler_kf": lambda: pf.update_doppler_kf(sat, sv, dop, resample=False),
+    }
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        pf.sync()
+        per = []
+        for _ in range(30):  # median of synchronised calls (a single host stall skews a mean)
+            t0 = time.perf_counter()
+            fn()
+            pf.sync()
+            per.append((time.perf_counter() - t0) * 1e3)
+        out[name] = round(float(np.median(per)), 4)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pf as opf
+
+        k = 20_000
+        st = pf.get_particle_states()[:k]
+        pos, cb, vel, cov = st[:, :3], st[:, 3], st[:, 4:7], st[:, 7:16]
+        fns = {
+            "dd_pseudorange": lambda: opf.dd_pseudorange_loglik(
+                pos, res.sat_ecef_k, res.sat_ecef_ref, dd, res.base_range_k,
+                res.base_range_ref, wdd, 0.8),
+            "carrier_afv": lambda: opf.carrier_afv_loglik(pos, cb, sat, cp, np.ones(S), wl_l1,
+                                                          0.05),
+            "dd_carrier_afv": lambda: opf.dd_carrier_afv_loglik(
+                pos, res.sat_ecef_k, res.sat_ecef_ref, ddc, res.base_range_k,
+                res.base_range_ref, wdd, wls, 0.05),
+            "doppler": lambda: opf.doppler_update(pos, vel, sat, sv, dop, np.ones(S), wl_l1, 0.5,
+                                                  0.25, 10.0),
+            "doppler_kf": lambda: opf.doppler_kf_update(pos, vel, cov, sat, sv, dop, np.ones(S),
+                                                        wl_l1, 0.5),
+        }
+        per = {}
+        for name, fn in fns.items():
+            t0 = time.perf_counter()
+            fn()
+            per[name] = round((time.perf_counter() - t0) * 1e3 * (N / k), 1)
+        cpu = {"kind": "port", "ms_per_update_1M": per, "cores": 1,
+               "sample": "NumPy binary64 oracle (oracle/pf.py) on 20,000 particles, scaled "
+                         "to 1,000,000"}
+    return {"workload": "updates_1M_12sats", "particles": N, "satellites": S,
+            "ms_per_update": out, "cpu_baseline": cpu}
+
+
 def _box_city(n_boxes, span, seed):
     """Random axis-aligned buildings (12 triangles each) over a square of
     side 2 * span metres around the origin."""
@@ -867,6 +956,7 @@ def main():
     mgnss = None if args.no_batched else multi_gnss_leg(args)
     rtrace = None if args.no_batched else raytrace_leg(args)
     ffbsi = None if args.no_batched else ffbsi_leg(args)
+    updates = None if args.no_batched else updates_leg(args)
     scale = None if args.no_scale else scale_leg(args)
 
     out = {
@@ -904,6 +994,7 @@ def main():
         "multi_gnss_wls": mgnss,
         "raytrace": rtrace,
         "ffbsi": ffbsi,
+        "updates": updates,
         "scale": scale,
         "cpu_baseline": cpu,
     }
