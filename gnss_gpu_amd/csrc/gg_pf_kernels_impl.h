@@ -1287,16 +1287,27 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_3d_pf(gg_arena a,
 // Dynamic LDS: 64 bytes per satellite.
 // ---------------------------------------------------------------------------
 __device__ bool gg_solve4(double (&A)[4][5], double (&x)[4]) {
+#pragma unroll
   for (int col = 0; col < 4; ++col) {
     int piv = col;
+    double best = fabs(A[col][col]);
+#pragma unroll
     for (int r = col + 1; r < 4; ++r)
-      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
-    if (piv != col)
-      for (int k = 0; k < 5; ++k) {
-        const double t = A[col][k];
-        A[col][k] = A[piv][k];
-        A[piv][k] = t;
+      if (fabs(A[r][col]) > best) {
+        best = fabs(A[r][col]);
+        piv = r;
       }
+    // swap through selects: a run-time row index would put A in scratch
+#pragma unroll
+    for (int r = col + 1; r < 4; ++r) {
+      const bool sw = (r == piv);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const double t0 = A[col][k], t1 = A[r][k];
+        A[col][k] = sw ? t1 : t0;
+        A[r][k] = sw ? t0 : t1;
+      }
+    }
     if (fabs(A[col][col]) < 1e-12) return false;
     for (int r = col + 1; r < 4; ++r) {
       const double fct = A[r][col] / A[col][col];
@@ -1353,41 +1364,38 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_doppler(gg_arena a, const doubl
   };
   double lw_add = 0.0;
   if (!KF) {
-    double wcd = 0.0, wsum = 0.0;
+    // one pass over the satellites: the clock-drift misfit q_k = obs_k - (sv_k - v).l_k
+    // (the residual is q_k - cd with cd the weighted mean of q), and the normal
+    // equations of the velocity WLS; sum w (q - cd)^2 = Sq2 - 2 cd Sq + cd^2 W
+    double wsum = 0.0, sq = 0.0, sq2 = 0.0;
+    double A[4][5];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) A[r][c] = 0.0;
     for (int k = 0; k < n_sat; ++k) {
       double l[3];
       if (!los(k, l)) continue;
       const double w = sd[k * 8 + 7];
-      wcd += w * (sd[k * 8 + 6] - radial(k, l) + (v[0] * l[0] + v[1] * l[1] + v[2] * l[2]));
+      const double rhs = sd[k * 8 + 6] - radial(k, l);
+      const double q = rhs + (v[0] * l[0] + v[1] * l[1] + v[2] * l[2]);
       wsum += w;
+      sq += w * q;
+      sq2 += w * q * q;
+      const double h[4] = {-l[0], -l[1], -l[2], 1.0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        A[r][4] += h[r] * w * rhs;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) A[r][c] += h[r] * w * h[c];
+      }
     }
     const bool any = wsum > 0.0;  // otherwise the particle is left as it is
-    const double cd = any ? wcd / wsum : 0.0;
+    const double cd = any ? sq / wsum : 0.0;
     const double inv_s2 = 1.0 / (sigma * sigma);
-    for (int k = 0; any && k < n_sat; ++k) {
-      double l[3];
-      if (!los(k, l)) continue;
-      const double pred = (sd[k * 8 + 3] - v[0]) * l[0] + (sd[k * 8 + 4] - v[1]) * l[1] +
-                          (sd[k * 8 + 5] - v[2]) * l[2] + cd;
-      const double r = sd[k * 8 + 6] - pred;
-      lw_add += -0.5 * sd[k * 8 + 7] * r * r * inv_s2;
-    }
+    if (any) lw_add = -0.5 * fmax(sq2 - 2.0 * cd * sq + cd * cd * wsum, 0.0) * inv_s2;
     const double g = fmin(1.0, fmax(0.0, gain));
     if (any && g > 0.0 && n_sat >= 4) {
-      double A[4][5];
-      for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 5; ++c) A[r][c] = 0.0;
-      for (int k = 0; k < n_sat; ++k) {
-        double l[3];
-        if (!los(k, l)) continue;
-        const double h[4] = {-l[0], -l[1], -l[2], 1.0};
-        const double rhs = sd[k * 8 + 6] - radial(k, l);
-        const double w = sd[k * 8 + 7];
-        for (int r = 0; r < 4; ++r) {
-          A[r][4] += h[r] * w * rhs;
-          for (int c = 0; c < 4; ++c) A[r][c] += h[r] * w * h[c];
-        }
-      }
       for (int d = 0; d < 4; ++d) A[d][d] += 1e-9;
       double sol[4] = {0.0, 0.0, 0.0, 0.0};
       if (gg_solve4(A, sol)) {
