@@ -33,17 +33,23 @@
 
 namespace {
 
-// log p(x_next | x_t) of one predict step (particle_ffbsi.py:14-29)
-__device__ __forceinline__ double trans_lp(const double xn[4], const double* __restrict__ xt,
-                                           const double v[3], double dt, double isp2,
-                                           double isc2) {
-  const double2 a = reinterpret_cast<const double2*>(xt)[0];
-  const double2 b = reinterpret_cast<const double2*>(xt)[1];
+// log p(x_next | x_t) of one predict step (particle_ffbsi.py:14-29); the
+// state x_t arrives as its two halves (x, y) and (z, clock)
+__device__ __forceinline__ double trans_lp2(const double xn[4], double2 a, double2 b,
+                                            const double v[3], double dt, double isp2,
+                                            double isc2) {
   const double d0 = xn[0] - (a.x + v[0] * dt);
   const double d1 = xn[1] - (a.y + v[1] * dt);
   const double d2 = xn[2] - (b.x + v[2] * dt);
   const double dc = xn[3] - b.y;
   return -0.5 * ((d0 * d0 + d1 * d1) + d2 * d2) * isp2 - 0.5 * (dc * dc) * isc2;
+}
+
+__device__ __forceinline__ double trans_lp(const double xn[4], const double* __restrict__ xt,
+                                           const double v[3], double dt, double isp2,
+                                           double isc2) {
+  const double2* p = reinterpret_cast<const double2*>(xt);
+  return trans_lp2(xn, p[0], p[1], v, dt, isp2, isc2);
 }
 
 struct StepIn {
@@ -154,6 +160,323 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_ffbsi(
   if (tid == 0) status[m] = -1;
 }
 
+// ---------------------------------------------------------------------------
+// FFBSi in particle rows, for N <= GG_FR_BLOCK * GG_FR_MAX_ROWS.  One
+// 1024-thread workgroup per trajectory; particle i = j * 1024 + k is element j
+// of thread k, so every load of a step is coalesced and all of them are in
+// flight at once, and the step's log-probabilities and weights stay in
+// registers.  Per step:
+//   1. lp_i = lw[t][i] + log p(x_{t+1} | x_t[i]); block max M (degenerate
+//      steps reported as in gg_k_ffbsi);
+//   2. w_i = exp(lp_i - M); per wavefront and row the sum W[j][w] in the
+//      balanced pairwise order (all rows folded together, fold_rows), which
+//      equals, bit for bit, the last lane of the Hillis-Steele scan of step 4;
+//   3. wavefront 0: row totals R_j = (((W[j][0] + W[j][1]) + W[j][2]) + ...)
+//      and row offsets P_{j+1} = P_j + R_j; target = u * P_R; row J is the
+//      first with P_{J+1} > target and, inside it, wavefront w* the first
+//      whose running end P_J + E_w exceeds the target (E_w the running sum of
+//      W[J][.]);
+//   4. wavefront w* scans its lanes (Hillis-Steele) and picks the first
+//      positive-weight lane with P_J + O_w* + scan > target.
+// The running end of w* equals P_{J+1} bit for bit, so the crossing lies in
+// that wavefront; if the target is not below the total (u * S rounded up to S)
+// the last positive weight is taken, as the sequential walk of gg_k_ffbsi
+// does.  Only the order of the sums differs from gg_k_ffbsi.
+// ---------------------------------------------------------------------------
+#define GG_FR_BLOCK 1024
+#define GG_FR_WAVES (GG_FR_BLOCK / 64)
+#define GG_FR_MAX_ROWS 8
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+  GG_BUTTERFLY32(f64, v, gg_max_f64);
+  return fmax(v, __shfl_xor(v, 32, 64));
+}
+
+// The value of lane ^ (1 << S) (DPP where the pattern exists, else the
+// swizzle or permute network).
+template <int S>
+__device__ __forceinline__ double xor_lane(double v) {
+  if constexpr (S == 0) return gg_dpp_f64<0xB1>(v);        // quad_perm [1,0,3,2]
+  else if constexpr (S == 1) return gg_dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  else if constexpr (S == 3) return gg_dpp_f64<0x128>(v);  // row_ror 8
+  else if constexpr (S == 5) return __shfl_xor(v, 32, 64);
+  else {  // ds_swizzle bit mode, xor mask 4 or 16
+    constexpr int OFF = 0x1F | ((1 << S) << 10);
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, OFF);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), OFF);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// Wavefront sums of the N values a[] of every lane, all rows at once: at level
+// S each lane keeps half of its values (by bit S of the lane) and adds the
+// partner's copy of that half, so level S pairs the aligned lane blocks of size
+// 2^S -- the balanced pairwise tree of the Hillis-Steele scan's last lane, for
+// every row.  Afterwards lane L holds, in a[0], the total of row
+// sum_s bit_s(L) * (N >> (s + 1)) (a bit reversal of L for L < N).
+template <int S, int N>
+__device__ __forceinline__ void fold_rows(double* a, int lane) {
+  if constexpr (N > 1) {
+    constexpr int H = N / 2;
+    const bool b = (lane >> S) & 1;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const double send = b ? a[i] : a[i + H];
+      const double keep = b ? a[i + H] : a[i];
+      a[i] = keep + xor_lane<S>(send);
+    }
+    fold_rows<S + 1, H>(a, lane);
+  } else if constexpr (S < 6) {
+    a[0] = a[0] + xor_lane<S>(a[0]);
+    fold_rows<S + 1, 1>(a, lane);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ int folded_row(int lane) {
+  int j = 0;
+#pragma unroll
+  for (int s = 0; (1 << s) < N; ++s) j += ((lane >> s) & 1) * (N >> (s + 1));
+  return j;
+}
+
+// inclusive scan over the wavefront: x[l] = x[l - d] + x[l] for d = 1 .. 32
+__device__ __forceinline__ double wave_scan_f64(double x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x = y + x;
+  }
+  return x;
+}
+
+// Read-only per-step inputs through the constant address space, so uniform
+// reads of them are scalar loads.
+typedef const __attribute__((address_space(4))) double* const_f64_ptr;
+
+__device__ __forceinline__ const_f64_ptr const_view(const double* p) { return (const_f64_ptr)p; }
+
+struct RowsLds {
+  double M[GG_FR_WAVES];
+  int bad[GG_FR_WAVES];
+  double PJ, Os, target;
+  int J, ws, last, sel;
+};
+
+// One backward step at epoch t given the pick nxt at t + 1 (ignored at the
+// last epoch) and the caller's uniform ut: the pick at t, or -1 when the step
+// is degenerate (the same in every thread).
+template <int R>
+__device__ __forceinline__ int rows_step(const double* __restrict__ lw,
+                                         const double* __restrict__ X, const_f64_ptr vel,
+                                         const_f64_ptr dts, const_f64_ptr sps, double isc2,
+                                         int T, int N, int t, int nxt, double ut,
+                                         double (*sW)[GG_FR_WAVES], RowsLds& sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const bool trans = t < T - 1;
+  const int64_t row = (int64_t)t * N;
+  // all loads of the step first; row j of the epoch starts at a uniform address
+  double lwv[R];
+  double2 xa[R], xb[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int64_t r0 = row + (int64_t)j * GG_FR_BLOCK;
+    lwv[j] = 0.0;
+    xa[j] = make_double2(0.0, 0.0);
+    xb[j] = xa[j];
+    if (j * GG_FR_BLOCK + tid < N) {
+      lwv[j] = (lw + r0)[tid];
+      if (trans) {
+        const double2* p = reinterpret_cast<const double2*>(X + r0 * 4);
+        xa[j] = p[2 * tid];
+        xb[j] = p[2 * tid + 1];
+      }
+    }
+  }
+  double xn[4] = {0.0, 0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0}, dtt = 0.0, isp2 = 0.0;
+  if (trans) {
+    const double* xr = X + ((int64_t)(t + 1) * N + nxt) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xn[k] = xr[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = vel[(t + 1) * 3 + k];
+    dtt = dts[t + 1];
+    isp2 = 1.0 / (sps[t + 1] * sps[t + 1]);
+  }
+  // 1. log-probabilities and the block max
+  double lp[R];
+  double mloc = -HUGE_VAL;
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    double x = -HUGE_VAL;
+    if (j * GG_FR_BLOCK + tid < N) {
+      x = lwv[j];
+      if (trans) x += trans_lp2(xn, xa[j], xb[j], v, dtt, isp2, isc2);
+      if (isnan(x) || x == HUGE_VAL)
+        bad = 1;
+      else
+        mloc = fmax(mloc, x);
+    }
+    lp[j] = x;
+  }
+  const double wm = wave_max_f64(mloc);
+  const int wbad = __any(bad);
+  if (lane == 0) {
+    sh.M[wv] = wm;
+    sh.bad[wv] = wbad;
+  }
+  __syncthreads();
+  double M = sh.M[0];
+  int anyb = sh.bad[0];
+#pragma unroll
+  for (int k = 1; k < GG_FR_WAVES; ++k) {
+    M = fmax(M, sh.M[k]);
+    anyb |= sh.bad[k];
+  }
+  if (anyb || !(M > -HUGE_VAL)) return -1;
+  // 2. weights and their per-wavefront row sums W[j][w]
+  double w[R], f[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    w[j] = lp[j] > -HUGE_VAL ? exp(lp[j] - M) : 0.0;
+    f[j] = w[j];
+  }
+  fold_rows<0, R>(f, lane);
+  if (lane < R) sW[folded_row<R>(lane)][wv] = f[0];
+  __syncthreads();
+  // 3. wavefront 0: row totals (lane j holds R_j), row offsets, the row J and
+  //    the wavefront ws of the crossing
+  if (wv == 0) {
+    double rj = 0.0;
+    if (lane < R) {
+      rj = sW[lane][0];
+      for (int k = 1; k < GG_FR_WAVES; ++k) rj = rj + sW[lane][k];
+    }
+    double P[R + 1];
+    P[0] = 0.0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) P[j + 1] = P[j] + __shfl(rj, j, 64);
+    const double target = ut * P[R];
+    int J = -1;
+#pragma unroll
+    for (int j = R - 1; j >= 0; --j)
+      if (P[j + 1] > target) J = j;
+    const bool last = J < 0;  // target >= total: the last positive weight
+    if (last) {
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (__shfl(rj, j, 64) > 0.0) J = j;
+    }
+    J = max(J, 0);
+    double PJ = 0.0;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j == J) PJ = P[j];
+    const double Ww = (lane < GG_FR_WAVES) ? sW[J][lane] : 0.0;
+    int ws = -1, wpos = -1;
+    double Os = 0.0, Opos = 0.0, O = 0.0;
+#pragma unroll
+    for (int k = 0; k < GG_FR_WAVES; ++k) {
+      const double Wk = __shfl(Ww, k, 64);
+      const double E = (k == 0) ? Wk : O + Wk;
+      if (ws < 0 && !last && PJ + E > target) {
+        ws = k;
+        Os = O;
+      }
+      if (Wk > 0.0) {
+        wpos = k;
+        Opos = O;
+      }
+      O = E;
+    }
+    if (ws < 0) {
+      ws = max(wpos, 0);
+      Os = Opos;
+    }
+    if (lane == 0) {
+      sh.J = J;
+      sh.ws = ws;
+      sh.last = last;
+      sh.PJ = PJ;
+      sh.Os = Os;
+      sh.target = target;
+    }
+  }
+  __syncthreads();
+  // 4. the crossing inside wavefront ws
+  const int ws = sh.ws;
+  if (wv == ws) {
+    const int J = sh.J;
+    const bool last = sh.last;
+    const double PJ = sh.PJ, target = sh.target;
+    double x = 0.0;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j == J) x = w[j];
+    double inc = wave_scan_f64(x, lane);
+    if (ws > 0) inc = sh.Os + inc;
+    const unsigned long long cand = __ballot(!last && x > 0.0 && PJ + inc > target);
+    const unsigned long long pos = __ballot(x > 0.0);
+    int pick = 0;
+    if (cand)
+      pick = __ffsll((long long)cand) - 1;
+    else if (pos)
+      pick = 63 - __clzll((long long)pos);
+    if (lane == 0) sh.sel = min(J * GG_FR_BLOCK + ws * 64 + pick, N - 1);
+  }
+  __syncthreads();
+  return sh.sel;
+}
+
+template <int R>
+__global__ __launch_bounds__(GG_FR_BLOCK) void gg_k_ffbsi_rows(
+    const double* __restrict__ lw, const double* __restrict__ X, const double* __restrict__ vel,
+    const double* __restrict__ dts, const double* __restrict__ sps, double isc2, int T, int N,
+    const double* __restrict__ u, const int32_t* __restrict__ forced, int64_t* __restrict__ idx,
+    int32_t* __restrict__ status) {
+  __shared__ double sW[R][GG_FR_WAVES];
+  __shared__ RowsLds sh;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t m = blockIdx.x;
+  int64_t* out = idx + m * T;
+  // The forced draws, uniforms and picks of 64 steps at a time, lane l of a
+  // wavefront holding step t0 - l: a step reads its inputs with readlane
+  // instead of a memory round trip, and wavefront 0 stores 64 picks at once.
+  int fch = -1, och = 0, nxt = 0;
+  double uch = 0.0;
+  for (int t = T - 1; t >= 0; --t) {
+    const int l = (T - 1 - t) & 63;
+    if (l == 0) {
+      const int tt = t - lane;
+      fch = tt >= 0 ? forced[m * T + tt] : -1;
+      uch = tt >= 0 ? u[m * T + tt] : 0.0;
+    }
+    const int f = __builtin_amdgcn_readlane(fch, l);
+    if (f >= 0) {  // the caller's integer draw for a degenerate step
+      nxt = f;
+    } else {
+      const uint64_t ub = __builtin_bit_cast(uint64_t, uch);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ub >> 32), l);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ub, l);
+      const double ut = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+      nxt = rows_step<R>(lw, X, const_view(vel), const_view(dts), const_view(sps), isc2, T, N,
+                         t, nxt, ut, sW, sh);
+      if (nxt < 0) {
+        if (threadIdx.x == 0) status[m] = (t == T - 1) ? -2 : t;
+        return;
+      }
+    }
+    if (lane == l) och = nxt;
+    if ((l == 63 || t == 0) && wv == 0 && lane <= l) out[t + l - lane] = och;
+  }
+  if (threadIdx.x == 0) status[m] = -1;
+}
+
 __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_trans_lp(const double* __restrict__ xn,
                                                              const double* __restrict__ xt, int n,
                                                              const double* __restrict__ v,
@@ -164,6 +487,34 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_trans_lp(const double* __res
   const double a[4] = {xn[0], xn[1], xn[2], xn[3]};
   const double vv[3] = {v[0], v[1], v[2]};
   out[i] = trans_lp(a, xt + (int64_t)i * 4, vv, dt, isp2, isc2);
+}
+
+// FFBSi up to 8192 particles in rows; larger populations and the genealogy
+// trace in the per-thread-range kernel
+hipError_t launch_backward(const double* lw, const double* X, const double* vel, const double* dt,
+                           const double* sp, double sigma_cb, const int32_t* anc, int T, int N,
+                           int n_traj, const double* u, const int32_t* forced, int64_t* idx,
+                           int32_t* status, hipStream_t st) {
+  const double isc2 = 1.0 / (sigma_cb * sigma_cb);
+  const int rows = (N + GG_FR_BLOCK - 1) / GG_FR_BLOCK;
+  if (anc || rows > GG_FR_MAX_ROWS) {
+    hipLaunchKernelGGL(gg_k_ffbsi, dim3(n_traj), dim3(GG_FB_BLOCK), 0, st, lw, X, vel, dt, sp,
+                       isc2, anc, T, N, u, forced, idx, status);
+  } else {
+#define GG_FR_LAUNCH(RR)                                                                      \
+  hipLaunchKernelGGL(gg_k_ffbsi_rows<RR>, dim3(n_traj), dim3(GG_FR_BLOCK), 0, st, lw, X, vel, \
+                     dt, sp, isc2, T, N, u, forced, idx, status)
+    if (rows <= 1)
+      GG_FR_LAUNCH(1);
+    else if (rows <= 2)
+      GG_FR_LAUNCH(2);
+    else if (rows <= 4)
+      GG_FR_LAUNCH(4);
+    else
+      GG_FR_LAUNCH(8);
+#undef GG_FR_LAUNCH
+  }
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -224,12 +575,10 @@ int gg_ffbsi_sample(const double* log_weights, const double* states, const doubl
   }
   chk(hipMemcpy(d_u, u, sizeof(double) * n_traj * T, hipMemcpyHostToDevice), "H2D u");
   chk(hipMemcpy(d_f, forced, sizeof(int32_t) * n_traj * T, hipMemcpyHostToDevice), "H2D forced");
-  if (!rc) {
-    hipLaunchKernelGGL(gg_k_ffbsi, dim3(n_traj), dim3(GG_FB_BLOCK), 0, nullptr, d_lw, d_X, d_vel,
-                       d_dt, d_sp, 1.0 / (sigma_cb * sigma_cb), d_anc, T, N, d_u, d_f, d_idx,
-                       d_st);
-    chk(hipGetLastError(), "gg_k_ffbsi");
-  }
+  if (!rc)
+    chk(launch_backward(d_lw, d_X, d_vel, d_dt, d_sp, sigma_cb, d_anc, T, N, n_traj, d_u, d_f,
+                        d_idx, d_st, nullptr),
+        "gg_k_ffbsi");
   chk(hipMemcpy(indices, d_idx, sizeof(int64_t) * n_traj * T, hipMemcpyDeviceToHost), "D2H idx");
   chk(hipMemcpy(status, d_st, sizeof(int32_t) * n_traj, hipMemcpyDeviceToHost), "D2H status");
   const hipError_t fe = hipFree(buf);
@@ -247,10 +596,8 @@ int gg_ffbsi_sample_device(const double* log_weights, const double* states, cons
   if (!log_weights || !u || !forced || !indices || !status) return gg_fail(-2, "null array");
   if (!ancestors && (!states || !vel || !dt || !sigma_pos))
     return gg_fail(-2, "FFBSi needs states, vel, dt and sigma_pos");
-  hipLaunchKernelGGL(gg_k_ffbsi, dim3(n_traj), dim3(GG_FB_BLOCK), 0, (hipStream_t)hip_stream,
-                     log_weights, states, vel, dt, sigma_pos, 1.0 / (sigma_cb * sigma_cb),
-                     ancestors, T, N, u, forced, indices, status);
-  GG_HIP(hipGetLastError());
+  GG_HIP(launch_backward(log_weights, states, vel, dt, sigma_pos, sigma_cb, ancestors, T, N,
+                         n_traj, u, forced, indices, status, (hipStream_t)hip_stream));
   return 0;
 }
 

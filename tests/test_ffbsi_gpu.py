@@ -57,6 +57,37 @@ def test_ffbsi_large_population(gpu):
             got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 5.0, np.random.default_rng(k)))
 
 
+@pytest.mark.parametrize("N", [1, 64, 1023, 1024, 1025, 4097, 8192, 8193])
+def test_ffbsi_population_edges(gpu, N):
+    """Populations at the edges of the particle-row kernel (1024 particles a
+    row, up to 8 rows) and just past it (the per-thread-range kernel)."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
+
+    lw, X, vel, dt, sp = _history(24, N, 10 + N % 7)
+    got = ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 6.0,
+                                     [np.random.default_rng(k) for k in range(3)])
+    for k in range(3):
+        np.testing.assert_array_equal(
+            got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 6.0, np.random.default_rng(k)))
+
+
+def test_ffbsi_long_history_with_redraws(gpu):
+    """T = 300 (the row kernel reads draws and stores picks 64 steps at a
+    time; 300 is not a multiple of 64) with degenerate steps in two different
+    64-step blocks, which relaunch with integer draws there."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
+
+    lw, X, vel, dt, sp = _history(300, 1500, 4)
+    lw[150, 7] = np.nan
+    lw[40] = -np.inf
+    gens = [np.random.default_rng(30 + k) for k in range(4)]
+    refs = [np.random.default_rng(30 + k) for k in range(4)]
+    got = ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 8.0, gens)
+    for k in range(4):
+        np.testing.assert_array_equal(got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 8.0, refs[k]))
+        assert _same_state(gens[k], refs[k])
+
+
 def test_ffbsi_degenerate_steps(gpu):
     """A NaN and an all -inf row mid-history redraw integers(0, N) at those
     steps (two extra launches); a degenerate last step gives zeros and leaves
