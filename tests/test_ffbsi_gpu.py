@@ -71,6 +71,21 @@ def test_ffbsi_population_edges(gpu, N):
             got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 6.0, np.random.default_rng(k)))
 
 
+@pytest.mark.parametrize("T", [1, 2, 64, 65])
+def test_ffbsi_short_and_block_edge_histories(gpu, T):
+    """One and two epochs, and histories at the 64-step block edge of the row
+    kernel's staged draws and picks."""
+    from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
+
+    lw, X, vel, dt, sp = _history(T, 300, 20 + T)
+    gens = [np.random.default_rng(50 + k) for k in range(3)]
+    refs = [np.random.default_rng(50 + k) for k in range(3)]
+    got = ffbsi_sample_indices_batch(lw, X, vel, dt, sp, 7.0, gens)
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 7.0, refs[k]))
+        assert _same_state(gens[k], refs[k])
+
+
 def test_ffbsi_long_history_with_redraws(gpu):
     """T = 300 (the row kernel reads draws and stores picks 64 steps at a
     time; 300 is not a multiple of 64) with degenerate steps in two different
