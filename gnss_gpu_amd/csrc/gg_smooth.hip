@@ -83,12 +83,13 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_ffbsi(
     const int f = forced[m * T + t];
     if (f >= 0) {  // the caller's integer draw for a degenerate step
       if (tid == 0) out[t] = f;
-      nxt = f;
+      nxt = min(f, N - 1);  // state loads stay inside the history whatever f is
       continue;
     }
     if (anc && t < T - 1) {  // genealogy: follow the resampling ancestor
-      nxt = anc[(int64_t)t * N + nxt];
-      if (tid == 0) out[t] = nxt;
+      const int a = anc[(int64_t)t * N + nxt];
+      if (tid == 0) out[t] = a;
+      nxt = min(max(a, 0), N - 1);  // the next lookup stays inside the table
       continue;
     }
     StepIn s;
@@ -457,8 +458,10 @@ __global__ __launch_bounds__(GG_FR_BLOCK) void gg_k_ffbsi_rows(
       uch = tt >= 0 ? u[m * T + tt] : 0.0;
     }
     const int f = __builtin_amdgcn_readlane(fch, l);
+    int pick;
     if (f >= 0) {  // the caller's integer draw for a degenerate step
-      nxt = f;
+      pick = f;
+      nxt = min(f, N - 1);  // state loads stay inside the history whatever f is
     } else {
       const uint64_t ub = __builtin_bit_cast(uint64_t, uch);
       const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ub >> 32), l);
@@ -470,8 +473,9 @@ __global__ __launch_bounds__(GG_FR_BLOCK) void gg_k_ffbsi_rows(
         if (threadIdx.x == 0) status[m] = (t == T - 1) ? -2 : t;
         return;
       }
+      pick = nxt;
     }
-    if (lane == l) och = nxt;
+    if (lane == l) och = pick;
     if ((l == 63 || t == 0) && wv == 0 && lane <= l) out[t + l - lane] = och;
   }
   if (threadIdx.x == 0) status[m] = -1;
@@ -530,6 +534,8 @@ int gg_ffbsi_sample(const double* log_weights, const double* states, const doubl
   if (!log_weights || !u || !forced || !indices || !status) return gg_fail(-2, "null array");
   if (!ancestors && (!states || !vel || !dt || !sigma_pos))
     return gg_fail(-2, "FFBSi needs states, vel, dt and sigma_pos");
+  for (size_t k = 0; k < (size_t)n_traj * T; ++k)
+    if (forced[k] < -1 || forced[k] >= N) return gg_fail(-2, "forced index outside [-1, N)");
   const size_t tn = (size_t)T * N;
   const bool ffbsi = ancestors == nullptr;
   size_t bytes = sizeof(double) * tn + 256;  // log-weights

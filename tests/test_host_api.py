@@ -223,3 +223,23 @@ def test_bvh_and_pf3d_argument_checks():
         pf_weight_3d_bvh(np.zeros(4), np.zeros(4), np.zeros(4), np.zeros(4), np.zeros(3),
                          np.zeros(1), np.ones(1), b._nodes_flat, b._sorted_tris,
                          np.zeros(4, dtype=np.float32), 4, 1, 3.0, 30.0, 20.0)
+
+
+def test_ffbsi_rejects_forced_draws_outside_the_population():
+    """gg_ffbsi_sample checks the forced draws before it touches the device."""
+    import ctypes as C
+
+    from gnss_gpu_amd._lib import call, dptr, iptr
+
+    T, N = 3, 5
+    lw = np.zeros((T, N))
+    X = np.zeros((T, N, 4))
+    vel, dt, sp = np.zeros((T, 3)), np.ones(T), np.ones(T)
+    u = np.full((1, T), 0.5)
+    idx = np.zeros((1, T), dtype=np.int64)
+    st = np.zeros(1, dtype=np.int32)
+    for bad in (N, -2):
+        forced = np.array([[-1, bad, -1]], dtype=np.int32)
+        with pytest.raises(RuntimeError, match="forced index"):
+            call("gg_ffbsi_sample", dptr(lw), dptr(X), dptr(vel), dptr(dt), dptr(sp), 3.0, None,
+                 T, N, 1, dptr(u), iptr(forced), C.c_void_p(idx.ctypes.data), iptr(st))
