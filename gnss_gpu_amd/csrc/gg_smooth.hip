@@ -164,9 +164,9 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_ffbsi(
 // ---------------------------------------------------------------------------
 // FFBSi in particle rows, for N <= GG_FR_BLOCK * GG_FR_MAX_ROWS.  One
 // 1024-thread workgroup per trajectory; particle i = j * 1024 + k is element j
-// of thread k, so every load of a step is coalesced and all of them are in
-// flight at once, and the step's log-probabilities and weights stay in
-// registers.  Per step:
+// of thread k, so every load of a step is coalesced and the loads of up to 8
+// rows are in flight at once, and the step's log-probabilities (and, up to 8
+// rows, its weights) stay in registers.  Per step:
 //   1. lp_i = lw[t][i] + log p(x_{t+1} | x_t[i]); block max M (degenerate
 //      steps reported as in gg_k_ffbsi);
 //   2. w_i = exp(lp_i - M); per wavefront and row the sum W[j][w] in the
@@ -186,7 +186,8 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_ffbsi(
 // ---------------------------------------------------------------------------
 #define GG_FR_BLOCK 1024
 #define GG_FR_WAVES (GG_FR_BLOCK / 64)
-#define GG_FR_MAX_ROWS 8
+#define GG_FR_MAX_ROWS 16
+#define GG_FR_CHUNK 8  // up to 8 rows load at once; 16 rows load 4 at a time
 
 __device__ __forceinline__ double wave_max_f64(double v) {
   GG_BUTTERFLY32(f64, v, gg_max_f64);
@@ -258,6 +259,10 @@ typedef const __attribute__((address_space(4))) double* const_f64_ptr;
 
 __device__ __forceinline__ const_f64_ptr const_view(const double* p) { return (const_f64_ptr)p; }
 
+__device__ __forceinline__ double row_weight(double lp, double M) {
+  return lp > -HUGE_VAL ? exp(lp - M) : 0.0;
+}
+
 struct RowsLds {
   double M[GG_FR_WAVES];
   int bad[GG_FR_WAVES];
@@ -279,50 +284,57 @@ __device__ __forceinline__ int rows_step(const double* __restrict__ lw,
   const int wv = tid >> 6;
   const bool trans = t < T - 1;
   const int64_t row = (int64_t)t * N;
-  // all loads of the step first; row j of the epoch starts at a uniform address
-  double lwv[R];
-  double2 xa[R], xb[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int64_t r0 = row + (int64_t)j * GG_FR_BLOCK;
-    lwv[j] = 0.0;
-    xa[j] = make_double2(0.0, 0.0);
-    xb[j] = xa[j];
-    if (j * GG_FR_BLOCK + tid < N) {
-      lwv[j] = (lw + r0)[tid];
-      if (trans) {
-        const double2* p = reinterpret_cast<const double2*>(X + r0 * 4);
-        xa[j] = p[2 * tid];
-        xb[j] = p[2 * tid + 1];
-      }
-    }
-  }
   double xn[4] = {0.0, 0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0}, dtt = 0.0, isp2 = 0.0;
-  if (trans) {
-    const double* xr = X + ((int64_t)(t + 1) * N + nxt) * 4;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) xn[k] = xr[k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) v[k] = vel[(t + 1) * 3 + k];
-    dtt = dts[t + 1];
-    isp2 = 1.0 / (sps[t + 1] * sps[t + 1]);
-  }
-  // 1. log-probabilities and the block max
+  // 1. log-probabilities and the block max.  Rows are loaded GG_FR_CHUNK at a
+  //    time, every load of a chunk in flight at once; row j of the epoch starts
+  //    at a uniform address.
+  constexpr int CH = R <= GG_FR_CHUNK ? R : GG_FR_CHUNK / 2;
   double lp[R];
   double mloc = -HUGE_VAL;
   int bad = 0;
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    double x = -HUGE_VAL;
-    if (j * GG_FR_BLOCK + tid < N) {
-      x = lwv[j];
-      if (trans) x += trans_lp2(xn, xa[j], xb[j], v, dtt, isp2, isc2);
-      if (isnan(x) || x == HUGE_VAL)
-        bad = 1;
-      else
-        mloc = fmax(mloc, x);
+  for (int c0 = 0; c0 < R; c0 += CH) {
+    double lwv[CH];
+    double2 xa[CH], xb[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int j = c0 + c;
+      const int64_t r0 = row + (int64_t)j * GG_FR_BLOCK;
+      lwv[c] = 0.0;
+      xa[c] = make_double2(0.0, 0.0);
+      xb[c] = xa[c];
+      if (j * GG_FR_BLOCK + tid < N) {
+        lwv[c] = (lw + r0)[tid];
+        if (trans) {
+          const double2* p = reinterpret_cast<const double2*>(X + r0 * 4);
+          xa[c] = p[2 * tid];
+          xb[c] = p[2 * tid + 1];
+        }
+      }
     }
-    lp[j] = x;
+    if (c0 == 0 && trans) {  // the picked state of epoch t + 1, with the first rows
+      const double* xr = X + ((int64_t)(t + 1) * N + nxt) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) xn[k] = xr[k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[k] = vel[(t + 1) * 3 + k];
+      dtt = dts[t + 1];
+      isp2 = 1.0 / (sps[t + 1] * sps[t + 1]);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int j = c0 + c;
+      double x = -HUGE_VAL;
+      if (j * GG_FR_BLOCK + tid < N) {
+        x = lwv[c];
+        if (trans) x += trans_lp2(xn, xa[c], xb[c], v, dtt, isp2, isc2);
+        if (isnan(x) || x == HUGE_VAL)
+          bad = 1;
+        else
+          mloc = fmax(mloc, x);
+      }
+      lp[j] = x;
+    }
   }
   const double wm = wave_max_f64(mloc);
   const int wbad = __any(bad);
@@ -340,11 +352,14 @@ __device__ __forceinline__ int rows_step(const double* __restrict__ lw,
   }
   if (anyb || !(M > -HUGE_VAL)) return -1;
   // 2. weights and their per-wavefront row sums W[j][w]
-  double w[R], f[R];
+  // up to 8 rows keep their weights for step 4; 16 rows recompute row J's
+  // (the same expression, so the same bits) to leave registers free
+  constexpr bool KEEP = R <= GG_FR_CHUNK;
+  double f[R], w[KEEP ? R : 1];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    w[j] = lp[j] > -HUGE_VAL ? exp(lp[j] - M) : 0.0;
-    f[j] = w[j];
+    f[j] = row_weight(lp[j], M);
+    if constexpr (KEEP) w[j] = f[j];
   }
   fold_rows<0, R>(f, lane);
   if (lane < R) sW[folded_row<R>(lane)][wv] = f[0];
@@ -415,9 +430,17 @@ __device__ __forceinline__ int rows_step(const double* __restrict__ lw,
     const bool last = sh.last;
     const double PJ = sh.PJ, target = sh.target;
     double x = 0.0;
+    if constexpr (KEEP) {
 #pragma unroll
-    for (int j = 0; j < R; ++j)
-      if (j == J) x = w[j];
+      for (int j = 0; j < R; ++j)
+        if (j == J) x = w[j];
+    } else {
+      double lpJ = -HUGE_VAL;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j == J) lpJ = lp[j];
+      x = row_weight(lpJ, M);
+    }
     double inc = wave_scan_f64(x, lane);
     if (ws > 0) inc = sh.Os + inc;
     const unsigned long long cand = __ballot(!last && x > 0.0 && PJ + inc > target);
@@ -493,7 +516,7 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_trans_lp(const double* __res
   out[i] = trans_lp(a, xt + (int64_t)i * 4, vv, dt, isp2, isc2);
 }
 
-// FFBSi up to 8192 particles in rows; larger populations and the genealogy
+// FFBSi up to 16384 particles in rows; larger populations and the genealogy
 // trace in the per-thread-range kernel
 hipError_t launch_backward(const double* lw, const double* X, const double* vel, const double* dt,
                            const double* sp, double sigma_cb, const int32_t* anc, int T, int N,
@@ -514,8 +537,10 @@ hipError_t launch_backward(const double* lw, const double* X, const double* vel,
       GG_FR_LAUNCH(2);
     else if (rows <= 4)
       GG_FR_LAUNCH(4);
-    else
+    else if (rows <= 8)
       GG_FR_LAUNCH(8);
+    else
+      GG_FR_LAUNCH(16);
 #undef GG_FR_LAUNCH
   }
   return hipGetLastError();

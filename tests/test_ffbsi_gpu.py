@@ -57,10 +57,10 @@ def test_ffbsi_large_population(gpu):
             got[k], of.ffbsi_indices(lw, X, vel, dt, sp, 5.0, np.random.default_rng(k)))
 
 
-@pytest.mark.parametrize("N", [1, 64, 1023, 1024, 1025, 4097, 8192, 8193])
+@pytest.mark.parametrize("N", [1, 64, 1023, 1024, 1025, 4097, 8192, 8193, 16384, 16385])
 def test_ffbsi_population_edges(gpu, N):
     """Populations at the edges of the particle-row kernel (1024 particles a
-    row, up to 8 rows) and just past it (the per-thread-range kernel)."""
+    row, up to 16 rows) and just past it (the per-thread-range kernel)."""
     from gnss_gpu_amd.particle_ffbsi import ffbsi_sample_indices_batch
 
     lw, X, vel, dt, sp = _history(24, N, 10 + N % 7)
