@@ -894,7 +894,15 @@ __device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
 #pragma clang fp contract(off)
   const double dn = (double)n;
   if (c != c) return 0;
-  const double est = floor(c * dn - u0);
+  const double x = c * dn - u0;
+  const double est = floor(x);
+  // x is within 2^-52 (n + |x|) of c n - u0, and a target's rounded quotient
+  // fl(fl(u0 + j) / n) within 2^-52 of (u0 + j) / n, so when x is farther
+  // than 1e-14 (n + |x|) from every integer the count is floor(x) + 1 exactly
+  // (clamped to [0, n]); otherwise the exact comparisons below decide.
+  const double fr = x - est;
+  const double eps = 1e-14 * (dn + fabs(x));
+  if (fr > eps && fr < 1.0 - eps) return est < 0.0 ? 0 : (est >= dn ? n : (int)est + 1);
   int j = (est < -1.0) ? -1 : (est > (double)(n - 1) ? n - 1 : (int)est);
   while (j + 1 < n && (u0 + (double)(j + 1)) / dn <= c) ++j;
   while (j >= 0 && (u0 + (double)j) / dn > c) --j;

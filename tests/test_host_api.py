@@ -243,3 +243,31 @@ def test_ffbsi_rejects_forced_draws_outside_the_population():
         with pytest.raises(RuntimeError, match="forced index"):
             call("gg_ffbsi_sample", dptr(lw), dptr(X), dptr(vel), dptr(dt), dptr(sp), 3.0, None,
                  T, N, 1, dptr(u), iptr(forced), C.c_void_p(idx.ctypes.data), iptr(st))
+
+
+def test_systematic_count_fast_path_bound():
+    """The resampling kernel counts the systematic targets (u0 + j) / n <= c
+    as floor(c n - u0) + 1 whenever c n - u0 lies farther than
+    1e-14 (n + |c n - u0|) from an integer, and compares exactly otherwise.
+    Check that rule against the exact count on targets, their neighbours and
+    random CDF values, for small and large n."""
+    from oracle import detmath
+
+    rng = np.random.default_rng(3)
+    for n in (1, 7, 1000, 1_000_000, 5_000_000):
+        u0 = float(rng.random()) / n
+        j = np.unique(np.concatenate([rng.integers(0, n, 4000), [0, n - 1]]))
+        t = (u0 + j.astype(np.float64)) / n
+        c = np.concatenate([t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf),
+                            rng.random(4000), [0.0, 1.0, 1.0 + 1e-12, -1e-300]])
+        x = c * n - u0
+        est = np.floor(x)
+        fr = x - est
+        eps = 1e-14 * (n + np.abs(x))
+        fast = (fr > eps) & (fr < 1.0 - eps)
+        k_fast = np.clip(est + 1, 0, n).astype(np.int64)
+        ref = np.searchsorted(detmath.systematic_targets(n, u0), c, side="right")
+        assert fast[3 * t.size: 3 * t.size + 4000].sum() > 3900  # random CDF values: fast
+        np.testing.assert_array_equal(k_fast[fast], ref[fast])
+        # every exact target and its neighbours fall to the exact comparisons
+        assert not np.any(fast[: 3 * t.size] & (np.abs(fr[: 3 * t.size] - 0.5) > 0.49))
