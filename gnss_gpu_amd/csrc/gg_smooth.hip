@@ -3,9 +3,12 @@
 //
 // One workgroup per trajectory walks t = T-1 .. 0 inside the kernel; the
 // history (log-weights [T][N], states [T][N][4], binary64) stays in HBM and
-// each step streams one epoch of it.  Thread k owns the contiguous particle
-// range [k*C, (k+1)*C), C = ceil(N / 256), so the inverse-CDF search is a
-// block scan of the 256 range sums followed by a walk through one range.
+// each step streams one epoch of it.  Up to 16384 particles, gg_k_ffbsi_rows
+// holds the epoch in registers as rows of 1024 particles (one per thread) and
+// locates the crossing by row, wavefront and lane.  Larger populations and the
+// genealogy trace use gg_k_ffbsi: thread k owns the contiguous particle range
+// [k*C, (k+1)*C), C = ceil(N / 256), so the inverse-CDF search is a block
+// scan of the 256 range sums followed by a walk through one range.
 //
 // A step's backward weights are w_i = exp(lp_i - max lp) with
 // lp_i = lw[t][i] + log p(x_{t+1} | x_t[i]) (particle_ffbsi.py:81-102; the
