@@ -192,6 +192,25 @@ def test_systematic_ancestors_bit_exact(gpu, n):
     assert dev.pf_device_ess(st) == pytest.approx(n)
 
 
+@pytest.mark.parametrize("n", [1000, 4099, 1_000_000])
+@pytest.mark.parametrize("seed", [2_000_000, 2_000_001])
+def test_systematic_counts_on_the_targets(gpu, n, seed):
+    """Equal weights put every CDF value on a systematic target (u0 = 0 for a
+    seed that is a multiple of 10^6) or within rounding of one, so every
+    output count goes through the exact target comparisons instead of the
+    floor(c n - u0) + 1 fast path; ancestors must still equal the oracle's."""
+    dev = _dev()
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, 0.0, 0.0, 0.0, 0.0, 10.0, 10.0, 5)
+    lw = np.zeros(n)
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_systematic(st, seed)
+    anc = np.empty(n, np.int32)
+    dev.pf_device_get_resample_ancestors(st, anc)
+    ref = detmath.systematic_ancestors(lw.astype(np.float32), n, detmath.reference_u0(seed, n))
+    np.testing.assert_array_equal(anc, ref)
+
+
 def test_systematic_fresh_u0_and_degenerate(gpu):
     dev = _dev()
     n = 3000
