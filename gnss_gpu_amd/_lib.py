@@ -145,6 +145,8 @@ _SIGS = {
     "gg_pf_device_get_particle_states": [_P, _DP],
     "gg_pf_device_set_particle_states": [_P, _DP],
     "gg_pf_device_get_log_weights": [_P, _DP],
+    "gg_pf_device_get_positions": [_P, _DP, _DP, _DP, _DP],
+    "gg_pf_device_set_positions": [_P, _DP, _DP, _DP, _DP],
     "gg_pf_device_set_log_weights": [_P, _DP],
     "gg_pf_device_get_resample_ancestors": [_P, _IP],
     "gg_pf_device_shift_clock_bias": [_P, _D],
@@ -164,6 +166,9 @@ _SIGS = {
     "gg_pfb_set_epoch": [_P, _I],
     "gg_pfb_get_particles": [_P, _I, _DP],
     "gg_pfb_get_log_weights": [_P, _I, _DP],
+    "gg_pfb_step_part": [_P, C.POINTER(StepArgs), _I, _I, _I],
+    "gg_pfb_get_log_weights_raw": [_P, _I, _FP],
+    "gg_pfb_get_ancestors": [_P, _I, _IP],
     "gg_wls_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_batch_device": [_P, _P, _P, _P, _I, _I, _I, _D, _P, _P, _I, _P],
     "gg_wls_position": [_DP, _DP, _DP, _I, _I, _D, _DP, _IP],

@@ -130,6 +130,12 @@ class BatchedParticleFilter:
     def step(self, args, resampler="systematic", n_iter=32):
         call("gg_pfb_step", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter))
 
+    def step_part(self, args, part, resampler="systematic", n_iter=32):
+        """Half of step(): part 0 the likelihood launch (predict + weight + tile
+        partials), part 1 the decision, resampling and finalize."""
+        call("gg_pfb_step_part", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
+             int(part))
+
     def capture(self, args, resampler="systematic", n_iter=32, steps=1):
         call("gg_pfb_capture", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(steps))
@@ -196,6 +202,20 @@ class BatchedParticleFilter:
     def get_particles(self, f):
         out = np.empty((int(self.n[f]), self.D), dtype=np.float64)
         call("gg_pfb_get_particles", self._h, int(f), dptr(out))
+        return out
+
+    def get_log_weights_raw(self, f):
+        """Filter f's binary32 log-weights as the last likelihood launch wrote
+        them (also after a resampling, which only marks them zero)."""
+        out = np.empty(int(self.n[f]), dtype=np.float32)
+        call("gg_pfb_get_log_weights_raw", self._h, int(f),
+             out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def get_ancestors(self, f):
+        """Ancestor indices of filter f's last resampling: out[j] = source of slot j."""
+        out = np.empty(int(self.n[f]), dtype=np.int32)
+        call("gg_pfb_get_ancestors", self._h, int(f), iptr(out))
         return out
 
     def get_log_weights(self, f):

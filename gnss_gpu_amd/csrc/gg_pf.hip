@@ -383,11 +383,18 @@ int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter)
   return enqueue_resample_after_decide(h, s, resampler, n_iter);
 }
 
-// One full device-driven step (no host synchronisation).
-int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
-  if (launch_weight(h, s, true, false)) return -1;
+// One device-driven step in two parts (no host synchronisation): part 0 is the
+// likelihood launch (predict + weight + tile partials), part 1 the decision,
+// resampling and finalize.
+int enqueue_step_part(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, int part) {
+  if (part == 0) return launch_weight(h, s, true, false);
   if (enqueue_resample(h, s, resampler, n_iter)) return -1;
   return launch_finalize(h, 0);
+}
+
+int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
+  if (enqueue_step_part(h, s, resampler, n_iter, 0)) return -1;
+  return enqueue_step_part(h, s, resampler, n_iter, 1);
 }
 
 gg_step_args default_args() {
@@ -1237,6 +1244,76 @@ int gg_pf_device_set_particle_states(gg_pf* h, const double* in) {
   return 0;
 }
 
+int gg_pf_device_get_positions(gg_pf* h, double* px, double* py, double* pz, double* pcb) {
+  if (int rc = check_single(h)) return rc;
+  if (!px || !py || !pz || !pcb) return gg_fail(-2, "null array");
+  const int n = h->filt_h[0].n;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  if (ensure_io(h, 4 * bytes)) return -1;
+  double* d = h->io_tmp;
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_export_soa<NI>), dim3(grid), dim3(GG_BLOCK),
+                                                0, h->stream, a, 0, d, d + n, d + 2 * (size_t)n,
+                                                d + 3 * (size_t)n));
+  GG_HIP(hipGetLastError());
+  double* outs[4] = {px, py, pz, pcb};
+  for (int k = 0; k < 4; ++k)
+    GG_HIP(hipMemcpyAsync(outs[k], d + (size_t)k * n, bytes, hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pf_device_set_positions(gg_pf* h, const double* px, const double* py, const double* pz,
+                               const double* pcb) {
+  if (int rc = check_single(h)) return rc;
+  if (!px || !py || !pz || !pcb) return gg_fail(-2, "null array");
+  const int n = h->filt_h[0].n;
+  gg_dyn d;
+  if (read_dyn(h, 0, &d)) return -1;
+  // new anchor: the mean of an evenly strided sample of at most 4096 particles
+  // (binary64); the offsets from it keep binary32 resolution for any cloud the
+  // filter tracks, without an O(N) host pass
+  const int stride = std::max(1, n / 4096);
+  double m[4] = {0, 0, 0, 0};
+  int cnt = 0;
+  for (int i = 0; i < n; i += stride, ++cnt) {
+    m[0] += px[i];
+    m[1] += py[i];
+    m[2] += pz[i];
+    m[3] += pcb[i];
+  }
+  for (int k = 0; k < 4; ++k) m[k] /= (double)cnt;
+  d.anchor[0] = m[0];
+  d.anchor[1] = m[1];
+  d.anchor[2] = m[2];
+  d.cb_anchor = m[3];
+  for (int k = 0; k < GG_MAX_D; ++k) d.mean[k] = 0.0;
+  d.vel_mode = 0;
+  d.lw_zero = 1;
+  d.ess = (double)n;
+  d.flag = 0;
+  d.est_valid = 0;
+  if (write_dyn(h, 0, d)) return -1;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  if (ensure_io(h, 4 * bytes)) return -1;
+  double* dv = h->io_tmp;
+  const double* ins[4] = {px, py, pz, pcb};
+  for (int k = 0; k < 4; ++k)
+    GG_HIP(hipMemcpyAsync(dv + (size_t)k * n, ins[k], bytes, hipMemcpyHostToDevice, h->stream));
+  const gg_arena a = arena_of(h);
+  const int grid = (n + GG_BLOCK - 1) / GG_BLOCK;
+  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_import_soa<NI>), dim3(grid), dim3(GG_BLOCK),
+                                                0, h->stream, a, 0, dv, dv + n, dv + 2 * (size_t)n,
+                                                dv + 3 * (size_t)n));
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(h->stream));
+  h->decide_valid = false;
+  h->ess_valid = false;
+  h->est_valid = false;
+  return 0;
+}
+
 int gg_pf_device_get_log_weights(gg_pf* h, double* out) {
   if (int rc = check_single(h)) return rc;
   return gg_pfb_get_log_weights(h, 0, out);
@@ -1384,6 +1461,17 @@ int gg_pfb_step(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n
   if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
   if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
   if (enqueue_step(h, *args, resampler, n_iter)) return -1;
+  h->decide_valid = h->ess_valid = h->est_valid = false;
+  return 0;
+}
+
+int gg_pfb_step_part(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                     int32_t part) {
+  if (!h || !args) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  if (part != 0 && part != 1) return gg_fail(-2, "part must be 0 or 1");
+  if (enqueue_step_part(h, *args, resampler, n_iter, part)) return -1;
   h->decide_valid = h->ess_valid = h->est_valid = false;
   return 0;
 }
@@ -1591,6 +1679,24 @@ int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out) {
   GG_HIP(hipStreamSynchronize(h->stream));
   const float* pf = (const float*)h->pin;
   for (int i = 0; i < n; ++i) out[i] = (double)pf[i];
+  return 0;
+}
+
+int gg_pfb_get_log_weights_raw(gg_pf* h, int32_t f, float* out) {
+  if (!h || !out) return gg_fail(-2, "null argument");
+  if (f < 0 || f >= h->n_filters) return gg_fail(-2, "filter index out of range");
+  GG_HIP(hipMemcpyAsync(out, h->lw + h->filt_h[f].p_off, sizeof(float) * h->filt_h[f].n,
+                        hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out) {
+  if (!h || !out) return gg_fail(-2, "null argument");
+  if (f < 0 || f >= h->n_filters) return gg_fail(-2, "filter index out of range");
+  GG_HIP(hipMemcpyAsync(out, h->anc + h->filt_h[f].p_off, sizeof(int32_t) * h->filt_h[f].n,
+                        hipMemcpyDeviceToHost, h->stream));
+  GG_HIP(hipStreamSynchronize(h->stream));
   return 0;
 }
 

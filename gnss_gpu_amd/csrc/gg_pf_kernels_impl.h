@@ -1037,6 +1037,46 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_import(gg_arena a, int f, const
   }
 }
 
+// Positions and clock biases as four binary64 arrays (structure of arrays),
+// the layout of the legacy host-array functions.  Export: x, y, z, cb
+// absolute; import: offsets from the record's anchor (ISB and velocity
+// components untouched, the host resets the velocity mode).
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_export_soa(gg_arena a, int f, double* px,
+                                                            double* py, double* pz, double* pcb) {
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  const double pl[3] = {(double)gg_comp(a, cur, 0)[s], (double)gg_comp(a, cur, 1)[s],
+                        (double)gg_comp(a, cur, 2)[s]};
+  double pe[3];
+  gg_rot_ecef(dy->R, pl, pe);
+  px[i] = dy->anchor[0] + pe[0];
+  py[i] = dy->anchor[1] + pe[1];
+  pz[i] = dy->anchor[2] + pe[2];
+  pcb[i] = dy->cb_anchor + (double)gg_comp(a, cur, 3)[s];
+}
+
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_import_soa(gg_arena a, int f, const double* px,
+                                                            const double* py, const double* pz,
+                                                            const double* pcb) {
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int i = blockIdx.x * GG_BLOCK + threadIdx.x;
+  if (i >= F.n) return;
+  const int cur = dy->cur;
+  const int64_t s = F.p_off + i;
+  const double de[3] = {px[i] - dy->anchor[0], py[i] - dy->anchor[1], pz[i] - dy->anchor[2]};
+  double dl[3];
+  gg_rot_local(dy->R, de, dl);
+  for (int k = 0; k < 3; ++k) gg_comp(a, cur, k)[s] = (float)dl[k];
+  gg_comp(a, cur, 3)[s] = (float)(pcb[i] - dy->cb_anchor);
+}
+
 // O(1) frame edits: shift position (ECEF) and clock bias of every particle.
 __global__ void gg_k_frame_shift(gg_dyn* dyn, int f, double dx, double dy_, double dz,
                                  double dcb) {

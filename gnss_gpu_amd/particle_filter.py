@@ -1,51 +1,58 @@
-"""Host-array particle filter (drop-in for python/gnss_gpu/particle_filter.py:10-222).
+"""Host-array particle filter: drop-in for ``gnss_gpu.particle_filter.ParticleFilter``
+(python/gnss_gpu/particle_filter.py:10-222): same constructor, methods,
+defaults, return types and errors.
 
-Particle arrays live in host NumPy float64 buffers between calls, exactly like
-the reference's legacy ParticleFilter; each call runs on the GPU through the
-``_gnss_gpu_pf`` functions.
+Between calls the particles live in host memory, as the reference's legacy
+filter keeps them: one float64 block of four rows (x, y, z, clock bias) and the
+log-weights.  Every method hands those buffers to the in-place functions of
+``_gnss_gpu_pf``, which run the step on the GPU and write the results back.
 """
+
+from __future__ import annotations
 
 import numpy as np
 
+from . import _gnss_gpu_pf as _pf
+
+# The reference wrapper resamples with 15 Megopolis iterations
+# (python/gnss_gpu/particle_filter.py:168).
+MEGOPOLIS_ITERATIONS = 15
+
+
+def _megopolis(pf, rows):
+    _pf.pf_resample_megopolis(*rows, pf._log_weights, pf.n_particles, MEGOPOLIS_ITERATIONS,
+                              pf.seed)
+
+
+def _systematic(pf, rows):
+    _pf.pf_resample_systematic(*rows, pf._log_weights, pf.n_particles, pf.seed)
+
+
+# resampling method -> routine; any other name resamples systematically, as the
+# reference's if/else does (particle_filter.py:165-172)
+_RESAMPLERS = {"megopolis": _megopolis}
+
 
 class ParticleFilter:
-    """Particle filter for GNSS positioning in ECEF coordinates.
+    """GNSS particle filter over host arrays, ECEF coordinates.
 
     Parameters
     ----------
     n_particles : int
     sigma_pos, sigma_cb : float
-        Predict noise [m].
+        Predict noise standard deviations [m] (position per axis, clock bias).
     sigma_pr : float
-        Pseudorange sigma [m].
+        Pseudorange standard deviation [m] of the Gaussian weight.
     resampling : str
-        "megopolis" (15 iterations) or "systematic".
+        "megopolis" (15 iterations of the reference rule) or "systematic".
     ess_threshold : float
+        Resample when ESS < ess_threshold * n_particles.
     seed : int
     """
 
     def __init__(self, n_particles=1_000_000, sigma_pos=1.0, sigma_cb=300.0,
                  sigma_pr=5.0, resampling="megopolis", ess_threshold=0.5,
                  seed=42):
-        from ._gnss_gpu_pf import (
-            pf_initialize as _pf_initialize,
-            pf_predict as _pf_predict,
-            pf_weight as _pf_weight,
-            pf_compute_ess as _pf_compute_ess,
-            pf_resample_systematic as _pf_resample_systematic,
-            pf_resample_megopolis as _pf_resample_megopolis,
-            pf_estimate as _pf_estimate,
-            pf_get_particles as _pf_get_particles,
-        )
-        self._pf_initialize = _pf_initialize
-        self._pf_predict = _pf_predict
-        self._pf_weight = _pf_weight
-        self._pf_compute_ess = _pf_compute_ess
-        self._pf_resample_systematic = _pf_resample_systematic
-        self._pf_resample_megopolis = _pf_resample_megopolis
-        self._pf_estimate = _pf_estimate
-        self._pf_get_particles = _pf_get_particles
-
         self.n_particles = n_particles
         self.sigma_pos = sigma_pos
         self.sigma_cb = sigma_cb
@@ -53,85 +60,79 @@ class ParticleFilter:
         self.resampling = resampling
         self.ess_threshold = ess_threshold
         self.seed = seed
-
-        self._px = None
-        self._py = None
-        self._pz = None
-        self._pcb = None
-        self._log_weights = None
+        self._block = None          # float64 [4, N]: rows x, y, z, clock bias
+        self._log_weights = None    # float64 [N]
         self._initialized = False
         self._step = 0
 
-    def _require_init(self):
+    def _rows(self):
+        """The particle rows (x, y, z, cb) in the argument order of _gnss_gpu_pf;
+        raises until initialize() has run."""
         if not self._initialized:
             raise RuntimeError("ParticleFilter not initialized. Call initialize() first.")
+        return tuple(self._block)
+
+    # Subclasses written against the reference (its ParticleFilter3D, for one)
+    # pass these per-coordinate arrays to the in-place functions; they are views
+    # of the block, so such writes land in the filter's state.
+    def _row(k):
+        return property(lambda self: None if self._block is None else self._block[k])
+
+    _px, _py, _pz, _pcb = _row(0), _row(1), _row(2), _row(3)
+    del _row
 
     def initialize(self, position_ecef, clock_bias=0.0, spread_pos=100.0, spread_cb=1000.0):
-        pos = np.asarray(position_ecef, dtype=np.float64).ravel()
+        """Scatter N particles around position_ecef (3,) [m] and clock_bias [m]."""
+        x0 = np.asarray(position_ecef, dtype=np.float64).reshape(-1)[:3]
         n = self.n_particles
-        self._px = np.empty(n, dtype=np.float64)
-        self._py = np.empty(n, dtype=np.float64)
-        self._pz = np.empty(n, dtype=np.float64)
-        self._pcb = np.empty(n, dtype=np.float64)
+        self._block = np.empty((4, n), dtype=np.float64)
         self._log_weights = np.zeros(n, dtype=np.float64)
-        self._pf_initialize(self._px, self._py, self._pz, self._pcb,
-                            float(pos[0]), float(pos[1]), float(pos[2]), float(clock_bias),
-                            float(spread_pos), float(spread_cb), n, self.seed)
+        _pf.pf_initialize(*self._block, float(x0[0]), float(x0[1]), float(x0[2]),
+                          float(clock_bias), float(spread_pos), float(spread_cb), n, self.seed)
         self._initialized = True
         self._step = 0
 
     def predict(self, velocity=None, dt=1.0):
-        self._require_init()
-        if velocity is None:
-            velocity = [0.0, 0.0, 0.0]
-        vel = np.asarray(velocity, dtype=np.float64).ravel()
-        vx = np.array([vel[0]], dtype=np.float64)
-        vy = np.array([vel[1]], dtype=np.float64)
-        vz = np.array([vel[2]], dtype=np.float64)
+        """Move every particle by velocity * dt (ECEF, default 0) plus noise."""
+        rows = self._rows()
+        v = np.zeros(3) if velocity is None else np.asarray(velocity, dtype=np.float64).reshape(-1)
         self._step += 1
-        self._pf_predict(self._px, self._py, self._pz, self._pcb, vx, vy, vz,
-                         float(dt), float(self.sigma_pos), float(self.sigma_cb),
-                         self.n_particles, self.seed, self._step)
+        _pf.pf_predict(*rows, *(np.array([c], dtype=np.float64) for c in v[:3]), float(dt),
+                       float(self.sigma_pos), float(self.sigma_cb), self.n_particles, self.seed,
+                       self._step)
 
     def update(self, sat_ecef, pseudoranges, weights=None):
-        self._require_init()
+        """Weight by the pseudoranges (overwrites the log-weights), then resample
+        when the effective sample size falls below the threshold."""
+        rows = self._rows()
         sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
-        pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
-        n_sat = len(pr)
-        if weights is None:
-            weights = np.ones(n_sat, dtype=np.float64)
-        else:
-            weights = np.asarray(weights, dtype=np.float64).ravel()
-        self._pf_weight(self._px, self._py, self._pz, self._pcb,
-                        sat.ravel(), pr, weights, self._log_weights,
-                        self.n_particles, n_sat, float(self.sigma_pr))
-        ess = self.get_ess()
-        if ess < self.ess_threshold * self.n_particles:
+        pr = np.asarray(pseudoranges, dtype=np.float64).reshape(-1)
+        w = (np.ones(pr.size, dtype=np.float64) if weights is None
+             else np.asarray(weights, dtype=np.float64).reshape(-1))
+        _pf.pf_weight(*rows, sat.reshape(-1), pr, w, self._log_weights, self.n_particles,
+                      pr.size, float(self.sigma_pr))
+        if self.get_ess() < self.ess_threshold * self.n_particles:
             self._resample()
 
     def _resample(self):
-        if self.resampling == "megopolis":
-            self._pf_resample_megopolis(self._px, self._py, self._pz, self._pcb,
-                                        self._log_weights, self.n_particles, 15, self.seed)
-        else:
-            self._pf_resample_systematic(self._px, self._py, self._pz, self._pcb,
-                                         self._log_weights, self.n_particles, self.seed)
-        self._log_weights[:] = 0.0
+        _RESAMPLERS.get(self.resampling, _systematic)(self, self._rows())
+        self._log_weights.fill(0.0)
 
     def estimate(self):
-        self._require_init()
-        result = np.empty(4, dtype=np.float64)
-        self._pf_estimate(self._px, self._py, self._pz, self._pcb,
-                          self._log_weights, result, self.n_particles)
-        return result
+        """Weighted mean [x, y, z, clock_bias] as ndarray (4,)."""
+        rows = self._rows()
+        out = np.empty(4, dtype=np.float64)
+        _pf.pf_estimate(*rows, self._log_weights, out, self.n_particles)
+        return out
 
     def get_particles(self):
-        self._require_init()
-        output = np.empty(self.n_particles * 4, dtype=np.float64)
-        self._pf_get_particles(self._px, self._py, self._pz, self._pcb, output,
-                               self.n_particles)
-        return output.reshape(self.n_particles, 4)
+        """All particles as an (N, 4) array of [x, y, z, clock_bias]."""
+        rows = self._rows()
+        out = np.empty(4 * self.n_particles, dtype=np.float64)
+        _pf.pf_get_particles(*rows, out, self.n_particles)
+        return out.reshape(self.n_particles, 4)
 
     def get_ess(self):
-        self._require_init()
-        return self._pf_compute_ess(self._log_weights, self.n_particles)
+        """Effective sample size of the current log-weights."""
+        self._rows()
+        return _pf.pf_compute_ess(self._log_weights, self.n_particles)

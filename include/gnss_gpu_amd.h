@@ -95,6 +95,14 @@ int gg_pf_device_get_particles_full(gg_pf* h, double* out);
  * pf_device.cu:1995-2031; rows [x y z cb vx vy vz S(3x3)] */
 int gg_pf_device_get_particle_states(gg_pf* h, double* out);
 int gg_pf_device_set_particle_states(gg_pf* h, const double* in);
+/* Positions and clock biases as four host arrays px, py, pz, pcb [N] (the
+ * layout of the legacy functions of python/gnss_gpu/_particle_filter_bindings.cpp:
+ * 10-145, which the reference keeps on the host between calls).  set replaces
+ * them (weights reset, ISB offsets kept, velocities back to the uniform one);
+ * no reference counterpart in pf_device.cu. */
+int gg_pf_device_get_positions(gg_pf* h, double* px, double* py, double* pz, double* pcb);
+int gg_pf_device_set_positions(gg_pf* h, const double* px, const double* py, const double* pz,
+                               const double* pcb);
 /* reference: pf_device_get_log_weights / set_log_weights, pf_device.cu:2033-2051 */
 int gg_pf_device_get_log_weights(gg_pf* h, double* out);
 int gg_pf_device_set_log_weights(gg_pf* h, const double* in);
@@ -144,6 +152,17 @@ int gg_pfb_set_epoch(gg_pf* h, int32_t epoch);
 /* Export filter f's particles as binary64 rows [x y z cb isb...]. */
 int gg_pfb_get_particles(gg_pf* h, int32_t f, double* out);
 int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out);
+/* Inspection of the device-driven step (no reference counterpart):
+ * gg_pfb_step_part(.., 0) enqueues the step's likelihood launch (predict +
+ * weight + tile partials) and (.., 1) the rest (decision, resampling,
+ * finalize); part 0 then part 1 is exactly gg_pfb_step.
+ * gg_pfb_get_log_weights_raw copies filter f's binary32 log-weight buffer as
+ * the last likelihood launch wrote it (no zeroed-weights flag applied);
+ * gg_pfb_get_ancestors the ancestor indices of its last resampling. */
+int gg_pfb_step_part(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                     int32_t part);
+int gg_pfb_get_log_weights_raw(gg_pf* h, int32_t f, float* out);
+int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out);
 
 /* --- batched Gauss-Newton WLS ---------------------------------------------- */
 /* reference: wls_batch, src/positioning/wls.cu:172-278 (Sagnac range model).

@@ -267,10 +267,12 @@ def megopolis_ancestors(lw, n, n_iter, seed, call, lw_zero=False):
     wk = w.copy()
     nseg = int(seg.max()) + 1 if n else 0
     segs = np.arange(nseg)
+    ru = None
     for b in range(n_iter):
         r = philox.draw(seed, segs, call, philox.PURPOSE_MEGO, b)
         O = ((r[0] * np.uint64(n)) >> np.uint64(32)).astype(np.int64)
-        ru = philox.draw(seed, i, call, philox.PURPOSE_MEGO_U, b >> 2)
+        if b & 3 == 0:  # one block per particle serves four iterations
+            ru = philox.draw(seed, i, call, philox.PURPOSE_MEGO_U, b >> 2)
         u = philox.u01f(ru[b & 3])
         j = i + O[seg]
         j = np.where(j >= n, j - n, j)

@@ -244,14 +244,15 @@ class OraclePF:
         self.n_isb = int(n_isb)
         self.state = None          # [N, 4 + n_isb] absolute
         self.vel = None            # [N, 3] ECEF velocity
+        self.vcov = None           # [N, 3, 3] ECEF velocity covariance (RBPF)
         self.lw = np.zeros(self.n)
         self.R = np.eye(3)
         self.resample_count = 0
         self.ancestors = np.arange(self.n, dtype=np.int32)
 
-    # pf_device.cu:67-104
+    # pf_device.cu:67-104 (velocity and its covariance: :84-101)
     def initialize(self, pos, cb=0.0, spread_pos=100.0, spread_cb=1000.0, isb=None,
-                   spread_isb=0.0, velocity=None):
+                   spread_isb=0.0, velocity=None, spread_vel=0.0, velocity_init_sigma=0.0):
         pos = np.asarray(pos, dtype=np.float64).reshape(3)
         self.R = enu_rotation(pos)
         i = np.arange(self.n)
@@ -267,21 +268,60 @@ class OraclePF:
         self.state = st
         v = np.zeros(3) if velocity is None else np.asarray(velocity, dtype=np.float64)
         self.vel = np.tile(v, (self.n, 1))
+        if spread_vel > 0.0:
+            zv = philox.normal4(self.seed, i, 0, philox.PURPOSE_INIT, 2)
+            self.vel = self.vel + (spread_vel * zv[:, :3]) @ self.R
+        self.vcov = np.tile(np.eye(3) * float(velocity_init_sigma) ** 2, (self.n, 1, 1))
         self.lw = np.zeros(self.n)
         self.resample_count = 0
 
-    # pf_device.cu:106-187 (velocity guide with alpha = 1, no velocity noise)
-    def predict(self, velocity, dt, sigma_pos, sigma_cb, step, sigma_isb=0.0, alpha=1.0):
-        i = np.arange(self.n)
-        z = philox.normal4(self.seed, i, step, philox.PURPOSE_PREDICT, 0)
+    @staticmethod
+    def _chol3(a):
+        """Lower Cholesky factor of the symmetric 3x3 matrices a [N, 3, 3] with
+        the reference's guards (pf_device.cu:153-165): sqrt of max(., 0), the
+        column below a pivot of at most 1e-12 left at zero."""
+        l00 = np.sqrt(np.maximum(a[:, 0, 0], 0.0))
+        ok0 = l00 > 1e-12
+        l10 = np.where(ok0, a[:, 0, 1] / np.where(ok0, l00, 1.0), 0.0)
+        l20 = np.where(ok0, a[:, 0, 2] / np.where(ok0, l00, 1.0), 0.0)
+        l11 = np.sqrt(np.maximum(a[:, 1, 1] - l10 * l10, 0.0))
+        ok1 = l11 > 1e-12
+        l21 = np.where(ok1, (a[:, 1, 2] - l20 * l10) / np.where(ok1, l11, 1.0), 0.0)
+        l22 = np.sqrt(np.maximum(a[:, 2, 2] - l20 * l20 - l21 * l21, 0.0))
+        L = np.zeros_like(a)
+        L[:, 0, 0], L[:, 1, 0], L[:, 1, 1] = l00, l10, l11
+        L[:, 2, 0], L[:, 2, 1], L[:, 2, 2] = l20, l21, l22
+        return L
+
+    # pf_device.cu:106-187: velocity guide blend v = (1 - a) v + a v_guide
+    # (:121-128), velocity noise N(0, sigma_vel) unless the velocity Kalman
+    # filter is on (:129-133), position noise N(0, sigma_pos) per axis or, with
+    # the RBPF, the Cholesky draw from sigma_pos^2 I + dt^2 Sigma_v followed by
+    # Sigma_v += q dt I (:141-176), then p += v dt + n and the clock walk.
+    # The noise is drawn in the library's local ENU frame (R fixed at
+    # initialisation) and rotated to ECEF; the Cholesky factor is that of the
+    # local-frame covariance R Sigma_v R^T, as the library factors it.
+    def predict(self, velocity, dt, sigma_pos, sigma_cb, step, sigma_isb=0.0, alpha=1.0,
+                sigma_vel=0.0, velocity_kf=False, velocity_process_noise=0.0):
+        z, zv, zi = philox.predict_normals(self.seed, self.n, step, self.n_isb)
         vg = np.asarray(velocity, dtype=np.float64).reshape(3)
         a = min(1.0, max(0.0, alpha))
         self.vel = (1.0 - a) * self.vel + a * vg[None, :]
-        self.state[:, :3] += self.vel * dt + (sigma_pos * z[:, :3]) @ self.R
+        if not velocity_kf and sigma_vel > 0.0:
+            self.vel = self.vel + (sigma_vel * zv) @ self.R
+        if velocity_kf:
+            Sl = np.einsum("ij,njk,lk->nil", self.R, self.vcov, self.R)
+            A = sigma_pos * sigma_pos * np.eye(3)[None] + dt * dt * Sl
+            L = self._chol3(A)
+            noise = np.einsum("nij,nj->ni", L, z[:, :3]) @ self.R
+            qdt = max(velocity_process_noise, 0.0) * max(dt, 0.0)
+            self.vcov = self.vcov + qdt * np.eye(3)[None]
+        else:
+            noise = (sigma_pos * z[:, :3]) @ self.R
+        self.state[:, :3] += self.vel * dt + noise
         self.state[:, 3] += sigma_cb * z[:, 3]
         if self.n_isb:
-            zi = philox.normal4(self.seed, i, step, philox.PURPOSE_PREDICT, 2)
-            self.state[:, 4:] += sigma_isb * zi[:, : self.n_isb]
+            self.state[:, 4:] += sigma_isb * zi
 
     def weight(self, sat, pr, w=None, sys=None, sigma_pr=5.0, nu=0.0, gate_thr=0.0,
                huber=False, huber_k=1.5):
@@ -302,6 +342,7 @@ class OraclePF:
         self.ancestors = anc
         self.state = self.state[anc]
         self.vel = self.vel[anc]
+        self.vcov = self.vcov[anc]
         self.lw = np.zeros(self.n)
         self.resample_count += 1
 

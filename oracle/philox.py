@@ -71,3 +71,16 @@ def normal4(seed, idx, step, purpose, call):
         out.append(rad * np.cos(2.0 * np.pi * b))
         out.append(rad * np.sin(2.0 * np.pi * b))
     return np.stack(out, axis=-1)
+
+
+def predict_normals(seed, n, step, n_isb):
+    """The normals one predict call draws per particle i (gg_predict4 /
+    gg_predict_one): z [n, 4] position noise x, y, z (local ENU frame) and the
+    clock bias; zv [n, 3] velocity noise (local frame); zi [n, n_isb] the
+    inter-system-bias random walk."""
+    i = np.arange(int(n))
+    z = normal4(seed, i, step, PURPOSE_PREDICT, 0)
+    zv = normal4(seed, i, step, PURPOSE_PREDICT, 1)[:, :3]
+    zi = normal4(seed, i, step, PURPOSE_PREDICT, 2)[:, :n_isb]
+    return z, zv, zi
+

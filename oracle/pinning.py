@@ -20,6 +20,9 @@ PINS = {
     "philox.u01f": ("pinned", "range (0,1) and exact dyadic values (test_uniform_ranges_and_exactness)"),
     "philox.u01d": ("pinned", "range (0,1] and 53-bit exactness (test_uniform_ranges_and_exactness)"),
     "philox.normal4": ("pinned", "moments of 4e5 draws (test_normals_are_standard)"),
+    "philox.predict_normals": ("pinned", "stream layout of the predict draws; moments through "
+                               "test_oracle_predict_velocity_noise_moments and the RBPF noise "
+                               "covariance test; GPU particle clouds equal the oracle's"),
     # detmath.py
     "detmath.exp64": ("pinned", "<= 2 ulp against libm over [-745, 0] (test_exp64_against_libm)"),
     "detmath.exp32": ("pinned", "<= 4 ulp (binary32) against binary64 exp "
@@ -56,14 +59,19 @@ PINS = {
                         "(test_scenario_geometry)"),
     "pf.huber_cost": ("pinned", "closed form -28.875 of test_cuda_streams.py:689-692"),
     "pf.log_likelihood": ("pinned", "closed forms -200.0 (Gaussian) and -28.875 (Huber); "
-                          "zero residual -> 0; gate all-pass / all-reject; Student-t brute force"),
+                          "zero residual -> 0; gate all-pass / all-reject; Student-t brute force; "
+                          "inter-system biases: 0 at the truth, -n_k delta^2 / (2 sigma^2) for "
+                          "ISB k-1 off by delta (test_isb_closed_forms)"),
     "pf.ess": ("pinned", "uniform -> N, one dominant -> < 2 (test_particle_filter.py:174-186)"),
     "pf.weighted_mean": ("pinned", "uniform weights -> arithmetic mean (test_particle_filter.py:293-315)"),
     "pf.position_spread": ("pinned", "uniform weights -> RMS distance, closed form on a 2-point cloud"),
     "pf.OraclePF.initialize": ("pinned", "moments; the draw order x,y,z,cb follows pf_device.cu:67-104. "
                                "The reference's cuRAND stream itself is parity unpinned"),
     "pf.OraclePF.predict": ("pinned", "sigma=0 closed form p += v*dt (test_cuda_streams.py:239-255); "
-                            "noise values parity unpinned (cuRAND)"),
+                            "guide blend and alpha clamp closed forms; velocity-noise moments; "
+                            "RBPF Sigma_v 4.5 / 1.5 and position std (test_cuda_streams.py:257-283, "
+                            "486-505); RBPF noise covariance sigma^2 I + dt^2 Sigma_v for a "
+                            "non-diagonal Sigma_v; noise values parity unpinned (cuRAND)"),
     "pf.OraclePF.weight": ("pinned", "as log_likelihood"),
     "pf.OraclePF.resample_systematic": ("pinned", "as systematic_ancestors"),
     "pf._gated_sum": ("pinned", "inside the DD closed forms (gate: all kept / too few kept)"),

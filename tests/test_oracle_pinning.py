@@ -154,6 +154,47 @@ def test_systematic_invariants_large():
     assert np.mean(ref == a) > 0.999
 
 
+def test_systematic_flip_count_at_1M_against_binary64_reference():
+    """How far the library's resampler (binary32 log-weights, fixed-order
+    binary64 CDF) is from the reference's semantics on binary64 weights
+    (sequential cumulative sum, first index with cdf >= (u0 + j) / N) at the
+    full-size workload: 1,000,000 particles scattered 10 m around the truth of
+    the 12-satellite multi-GNSS scenario, Cauchy log-weights.  Two counts are
+    reported: outputs whose ancestor differs because of the binary32 rounding of
+    the log-weights plus the summation order, and because of the summation
+    order alone (the same binary32 log-weights in both).  Every flipped output
+    has its target within 5e-9 of a binary64 CDF step of the reference (the
+    binary32 rounding of the log-weights moves the CDF by a few 1e-10)."""
+    from gnss_gpu_amd import sim
+
+    n = 1_000_000
+    sc = sim.scenario_multi_gnss(n_epochs=2, seed=0)
+    rng = np.random.default_rng(11)
+    pos = sc.truth[1][None, :] + rng.normal(0.0, 10.0, (n, 3))
+    cb = sc.cb + rng.normal(0.0, 10.0, n)
+    isb = sc.isb[None, :] + rng.normal(0.0, 1.0, (n, 3))
+    lw64 = opf.log_likelihood(pos, cb, isb, sc.sat, sc.pr[1], np.ones(sc.n_sat), sc.sys, 5.0,
+                              nu=1.0)
+    lw32 = lw64.astype(np.float32)
+    u0 = detmath.reference_u0(12345, n)
+    got = detmath.systematic_ancestors(lw32, n, u0)
+    w64 = np.exp(lw64 - lw64.max())
+    ref64 = detmath.sequential_systematic(w64 / w64.sum(), u0)
+    w32 = np.exp(lw32.astype(np.float64) - float(lw32.max()))
+    ref32 = detmath.sequential_systematic(w32 / w32.sum(), u0)
+    flips64 = int(np.sum(got != ref64))
+    flips32 = int(np.sum(got != ref32))
+    print(f"\n[systematic flips at N=1M] binary32 log-weights vs binary64 reference: {flips64}; "
+          f"summation order alone: {flips32}", flush=True)
+    cdf = np.cumsum(w64 / w64.sum())
+    j = np.nonzero(got != ref64)[0]
+    t = detmath.systematic_targets(n, u0)[j]
+    r = ref64[j].astype(np.int64)
+    near = np.minimum(np.abs(cdf[r] - t), np.abs(cdf[np.maximum(r - 1, 0)] - t))
+    assert np.all(near < 5e-9), float(near.max())
+    assert flips64 <= n // 1000 and flips32 <= flips64
+
+
 def test_systematic_uniform_weights_identity():
     n = 5000
     a = detmath.systematic_ancestors(np.zeros(n, np.float32), n, detmath.reference_u0(42, n))
@@ -279,6 +320,40 @@ def test_gate_semantics():
     l2 = opf.log_likelihood(pos, cb, isb, sat, pr + 100.0, np.ones(5), None, 5.0,
                             gate_thr=g["gate_threshold_m"])
     assert l2[0] < g["gate_reject_all_max_log_weight"]
+
+
+def _isb_case():
+    """Noise-free pseudoranges of 2 + 3 + 4 + 5 satellites of four systems
+    (ids 0..3) with clock bias 100 m and inter-system biases 12 / -7.5 / 20 m."""
+    from gnss_gpu_amd import sim
+
+    rng = np.random.default_rng(17)
+    counts = (2, 3, 4, 5)
+    sys_ids = np.repeat(np.arange(4), counts).astype(np.int32)
+    truth = sim.ANCHOR_CFG1.copy()
+    sat = sim.place_satellites(truth, rng.uniform(15, 85, 14), rng.uniform(0, 360, 14))
+    cb, isb = 100.0, np.array([12.0, -7.5, 20.0])
+    pr = np.linalg.norm(sat - truth, axis=1) + cb + np.concatenate([[0.0], isb])[sys_ids]
+    return truth, sat, pr, sys_ids, cb, isb, counts
+
+
+def test_isb_closed_forms():
+    """System k > 0 of a satellite uses ISB state k - 1, system 0 none: at the
+    truth the Gaussian log-weight is 0; ISB j off by delta gives
+    -n_{j+1} delta^2 / (2 sigma^2) with n_k the satellite count of system k
+    (distinct counts, so a shifted index changes the value)."""
+    truth, sat, pr, sys_ids, cb, isb, counts = _isb_case()
+    w = np.ones(len(pr))
+    lw0 = opf.log_likelihood(truth[None], np.array([cb]), isb[None], sat, pr, w, sys_ids, 5.0)
+    assert abs(lw0[0]) < 1e-12
+    for j, delta in enumerate((1.5, -2.0, 3.0)):
+        off = isb.copy()
+        off[j] += delta
+        lw = opf.log_likelihood(truth[None], np.array([cb]), off[None], sat, pr, w, sys_ids, 5.0)
+        assert lw[0] == pytest.approx(-counts[j + 1] * delta * delta / 50.0, rel=1e-9, abs=1e-12)
+    # the clock bias moves every satellite
+    lw = opf.log_likelihood(truth[None], np.array([cb + 2.0]), isb[None], sat, pr, w, sys_ids, 5.0)
+    assert lw[0] == pytest.approx(-14 * 4.0 / 50.0, rel=1e-9)
 
 
 def test_student_t_cauchy_brute_force():
@@ -732,6 +807,97 @@ def test_oracle_predict_deterministic_closed_form():
     np.testing.assert_allclose(pf.state[:, :3] - before[:, :3],
                                np.broadcast_to(np.array([3.0, -4.0, 0.5]), (100, 3)), atol=1e-9)
     np.testing.assert_array_equal(pf.state[:, 3], before[:, 3])
+
+
+def test_oracle_predict_guide_blend_closed_form():
+    """sigma = 0, alpha = 0.25: v <- 0.75 v + 0.25 v_guide exactly, then p += v dt
+    (pf_device.cu:121-128)."""
+    pf = opf.OraclePF(64, seed=3)
+    v0 = np.array([2.0, -1.0, 0.5])
+    pf.initialize(np.array([1.0e6, 2.0e6, 3.0e6]), cb=5.0, spread_pos=4.0, spread_cb=2.0,
+                  velocity=v0)
+    before = pf.state.copy()
+    vg = np.array([-6.0, 3.0, 1.5])
+    pf.predict(vg, 2.0, 0.0, 0.0, step=1, alpha=0.25)
+    v = 0.75 * v0 + 0.25 * vg
+    np.testing.assert_allclose(pf.vel, np.broadcast_to(v, (64, 3)), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(pf.state[:, :3] - before[:, :3], np.broadcast_to(2.0 * v, (64, 3)),
+                               atol=1e-9)
+    # alpha is clamped to [0, 1]: 1.7 acts as 1, -3 as 0
+    pf.predict(vg, 1.0, 0.0, 0.0, step=2, alpha=1.7)
+    np.testing.assert_allclose(pf.vel, np.broadcast_to(vg, (64, 3)), atol=1e-15)
+    pf.predict(v0, 1.0, 0.0, 0.0, step=3, alpha=-3.0)
+    np.testing.assert_allclose(pf.vel, np.broadcast_to(vg, (64, 3)), atol=1e-15)
+
+
+def test_oracle_predict_velocity_noise_moments():
+    """sigma_vel > 0 without the velocity filter: per-particle velocity noise
+    N(0, sigma_vel^2 I) around the blend (pf_device.cu:129-133); positions move
+    by the noisy velocity."""
+    n = 200_000
+    pf = opf.OraclePF(n, seed=9)
+    pf.initialize(np.array([-3.96e6, 3.35e6, 3.70e6]), spread_pos=0.0, spread_cb=0.0)
+    before = pf.state.copy()
+    vg = np.array([1.0, -2.0, 0.5])
+    pf.predict(vg, 0.5, 0.0, 0.0, step=1, sigma_vel=0.8)
+    dv = pf.vel - vg
+    np.testing.assert_allclose(dv.mean(axis=0), 0.0, atol=0.01)
+    np.testing.assert_allclose(np.cov(dv.T), 0.64 * np.eye(3), atol=0.01)
+    np.testing.assert_allclose(pf.state[:, :3] - before[:, :3], 0.5 * pf.vel, atol=1e-8)
+    # with the velocity filter on, the velocity is not sampled (pf_device.cu:129)
+    pf2 = opf.OraclePF(1000, seed=9)
+    pf2.initialize(np.zeros(3), spread_pos=0.0, spread_cb=0.0, velocity=vg,
+                   velocity_init_sigma=1.0)
+    pf2.predict(vg, 1.0, 0.0, 0.0, step=1, sigma_vel=0.8, velocity_kf=True)
+    np.testing.assert_array_equal(pf2.vel, np.broadcast_to(vg, (1000, 3)))
+
+
+def test_oracle_rbpf_predict_reference_closed_forms():
+    """The reference's RBPF predict pins (test_cuda_streams.py:257-283, 486-505):
+    Sigma_v = 4 I, dt = 1, sigma_pos = 0, q = 0.5 -> Sigma_v = 4.5 I, mu_v
+    unchanged, position std 2 (within (1.6, 2.4)); Sigma_v = I, dt = 2,
+    q = 0.25 -> 1.5 I."""
+    n = 20_000
+    pf = opf.OraclePF(n, seed=42)
+    pf.initialize(np.zeros(3), cb=0.0, spread_pos=0.0, spread_cb=0.0,
+                  velocity=np.array([1.0, 0.0, 0.0]), velocity_init_sigma=2.0)
+    pf.predict(np.array([1.0, 0.0, 0.0]), 1.0, 0.0, 0.0, step=1, sigma_vel=100.0,
+               velocity_kf=True, velocity_process_noise=0.5)
+    np.testing.assert_allclose(pf.vel, np.tile([1.0, 0.0, 0.0], (n, 1)))
+    for k, c in enumerate((1.0, 0.0, 0.0)):
+        assert 1.6 < float(np.std(pf.state[:, k] - c)) < 2.4
+    np.testing.assert_allclose(pf.vcov, np.broadcast_to(4.5 * np.eye(3), (n, 3, 3)))
+    pf = opf.OraclePF(4096, seed=42)
+    pf.initialize(np.array([-3957199.0, 3310205.0, 3737911.0]), spread_pos=0.0, spread_cb=0.0,
+                  velocity=np.array([0.5, -0.25, 0.0]), velocity_init_sigma=1.0)
+    pf.predict(np.array([0.5, -0.25, 0.0]), 2.0, 0.0, 0.0, step=1, velocity_kf=True,
+               velocity_process_noise=0.25)
+    np.testing.assert_allclose(pf.vel, np.tile([0.5, -0.25, 0.0], (4096, 1)))
+    np.testing.assert_allclose(pf.vcov[:, [0, 1, 2], [0, 1, 2]], 1.5)
+
+
+def test_oracle_cholesky_against_numpy_and_noise_covariance():
+    """_chol3 equals numpy's Cholesky factor on SPD matrices and keeps the
+    reference's guards on singular ones; the RBPF position noise has covariance
+    sigma_pos^2 I + dt^2 Sigma_v in ECEF for a non-diagonal Sigma_v (a
+    transposed factor or a frame error fails this)."""
+    rng = np.random.default_rng(0)
+    M = rng.normal(size=(500, 3, 3))
+    A = M @ np.transpose(M, (0, 2, 1)) + 0.1 * np.eye(3)
+    np.testing.assert_allclose(opf.OraclePF._chol3(A), np.linalg.cholesky(A), rtol=0, atol=1e-12)
+    S = np.zeros((1, 3, 3))
+    S[0, 1, 1] = S[0, 2, 2] = 4.0   # first pivot zero: its column stays zero
+    L = opf.OraclePF._chol3(S)
+    np.testing.assert_allclose(L[0], np.diag([0.0, 2.0, 2.0]))
+    n = 400_000
+    pf = opf.OraclePF(n, seed=5)
+    pf.initialize(np.array([-3.96e6, 3.35e6, 3.70e6]), spread_pos=0.0, spread_cb=0.0)
+    C = np.array([[2.0, 0.8, -0.3], [0.8, 1.5, 0.4], [-0.3, 0.4, 1.0]])
+    pf.vcov = np.broadcast_to(C, (n, 3, 3)).copy()
+    before = pf.state.copy()
+    pf.predict(np.zeros(3), 1.5, 0.7, 0.0, step=1, velocity_kf=True)
+    d = pf.state[:, :3] - before[:, :3]
+    np.testing.assert_allclose(np.cov(d.T), 0.49 * np.eye(3) + 2.25 * C, rtol=0, atol=0.03)
 
 
 def test_every_oracle_function_has_a_pinning_entry():
