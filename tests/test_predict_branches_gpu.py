@@ -6,7 +6,8 @@ and on per-particle covariances.  The reference's own closed forms
 (test_cuda_streams.py:239-283, 486-505) run through the same entry points.
 
 Bars: positions and clock biases within 1 mm, velocities within 1e-5 m/s,
-covariances within 1e-5 relative (binary32 storage), of the oracle driven by
+covariances within 1e-5 relative or 2e-6 absolute (binary32 storage of
+entries up to a few m^2/s^2), of the oracle driven by
 the same normals.
 """
 
@@ -37,7 +38,7 @@ def _check(pf, orc, cov=False):
     np.testing.assert_allclose(s[:, 3], orc.state[:, 3], rtol=0, atol=1e-3)
     np.testing.assert_allclose(s[:, 4:7], orc.vel, rtol=0, atol=1e-5)
     if cov:
-        np.testing.assert_allclose(s[:, 7:16], orc.vcov.reshape(-1, 9), rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(s[:, 7:16], orc.vcov.reshape(-1, 9), rtol=1e-5, atol=2e-6)
     return s
 
 
