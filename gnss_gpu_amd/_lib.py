@@ -171,6 +171,7 @@ _SIGS = {
     "gg_pfb_get_ancestors": [_P, _I, _IP],
     "gg_wls_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_batch_device": [_P, _P, _P, _P, _I, _I, _I, _D, _P, _P, _I, _P],
+    "gg_wls_batch_noalloc": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_position": [_DP, _DP, _DP, _I, _I, _D, _DP, _IP],
     "gg_wls_multi_gnss_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_multi_gnss": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],

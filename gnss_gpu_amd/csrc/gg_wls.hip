@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "gg_common_host.h"
@@ -511,6 +512,97 @@ int gg_wls_batch(const double* sat_ecef, const double* pseudoranges, const doubl
                  double tol, double* results, int32_t* iters) {
   return wls_host(sat_ecef, pseudoranges, weights, n_sat, n_epoch, n_sat_max, max_iter, tol,
                   results, iters, 0);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free host-array batch (gg_wls_batch_noalloc): the inputs and
+// outputs are staged in `scratch`, a file-scope device array that every GPU
+// the library runs on holds until the process ends.  Host code never passes
+// the array's name to a kernel or a copy: the device address comes from
+// hipGetSymbolAddress, taken once per device (with that device current) on
+// the entry point's first call there.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[GG_WLS_NOALLOC_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;           // guards g_scratch_ptr and serialises the calls
+char* g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device (null: not taken)
+
+int scratch_of_current_device(char** out) {
+  int dev = 0;
+  GG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return gg_fail(-2, "device index beyond 63");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return gg_fail(-1, "hipGetSymbolAddress returned a null staging address");
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+__global__ void gg_k_fill_f64(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int gg_wls_batch_noalloc(const double* sat_ecef, const double* pseudoranges,
+                         const double* weights, const int32_t* n_sat, int32_t n_epoch,
+                         int32_t n_sat_max, int32_t max_iter, double tol, double* results,
+                         int32_t* iters) {
+  if (n_epoch < 0 || n_sat_max < 0) return gg_fail(-2, "negative size");
+  if (n_epoch == 0) return 0;
+  if (!sat_ecef || !pseudoranges || !results) return gg_fail(-2, "null array");
+  const size_t ns = (size_t)n_epoch * (size_t)n_sat_max;
+  const size_t o_pr = round256(sizeof(double) * ns * 3);
+  const size_t o_w = o_pr + round256(sizeof(double) * ns);
+  const size_t o_res = o_w + round256(sizeof(double) * ns);
+  const size_t o_it = o_res + round256(sizeof(double) * (size_t)n_epoch * 4);
+  const size_t o_ns = o_it + round256(sizeof(int32_t) * (size_t)n_epoch);
+  const size_t need = o_ns + round256(sizeof(int32_t) * (size_t)n_epoch);
+  if (need > (size_t)GG_WLS_NOALLOC_BYTES)
+    return gg_fail(-2, "gg_wls_batch_noalloc: n_epoch * n_sat_max exceeds the 16 MiB staging "
+                       "area (at most 40 * n_epoch * n_sat_max + 40 * n_epoch bytes plus "
+                       "alignment)");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  char* base = nullptr;
+  if (scratch_of_current_device(&base)) return -1;
+  double* d_sat = reinterpret_cast<double*>(base);
+  double* d_pr = reinterpret_cast<double*>(base + o_pr);
+  double* d_w = reinterpret_cast<double*>(base + o_w);
+  double* d_res = reinterpret_cast<double*>(base + o_res);
+  int32_t* d_it = reinterpret_cast<int32_t*>(base + o_it);
+  int32_t* d_ns = reinterpret_cast<int32_t*>(base + o_ns);
+  GG_HIP(hipMemcpy(d_sat, sat_ecef, sizeof(double) * ns * 3, hipMemcpyHostToDevice));
+  GG_HIP(hipMemcpy(d_pr, pseudoranges, sizeof(double) * ns, hipMemcpyHostToDevice));
+  if (weights) {
+    GG_HIP(hipMemcpy(d_w, weights, sizeof(double) * ns, hipMemcpyHostToDevice));
+  } else if (ns > 0) {
+    // the previous call's contents stay in the area: the weights are written in full
+    hipLaunchKernelGGL(gg_k_fill_f64, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, nullptr,
+                       d_w, (int64_t)ns, 1.0);
+    GG_HIP(hipGetLastError());
+  }
+  if (n_sat)
+    GG_HIP(hipMemcpy(d_ns, n_sat, sizeof(int32_t) * (size_t)n_epoch, hipMemcpyHostToDevice));
+  if (launch_wls(d_sat, d_pr, d_w, n_sat ? d_ns : nullptr, n_epoch, n_sat_max, max_iter, tol,
+                 d_res, d_it, 0, nullptr))
+    return -1;
+  GG_HIP(hipMemcpy(results, d_res, sizeof(double) * (size_t)n_epoch * 4, hipMemcpyDeviceToHost));
+  if (iters)
+    GG_HIP(hipMemcpy(iters, d_it, sizeof(int32_t) * (size_t)n_epoch, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const double* weights,

@@ -177,6 +177,21 @@ int gg_wls_batch_device(const double* sat_ecef, const double* pseudoranges,
                         const double* weights, const int32_t* n_sat, int32_t n_epoch,
                         int32_t n_sat_max, int32_t max_iter, double tol, double* results,
                         int32_t* iters, int32_t mode, void* hip_stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gg_wls_batch with the same arguments and results, staged
+ * in a static device area of GG_WLS_NOALLOC_BYTES per GPU (held until the
+ * process ends) instead of a per-call hipMalloc.  Calls no hipMalloc/hipFree.
+ * Sizes needing more than the area (about 40 B per epoch-satellite plus
+ * 40 B per epoch) are rejected with an error.  Every call on a GPU shares
+ * that GPU's area and leaves its contents behind: calls that overlap on one
+ * GPU race (this library serialises its own callers with a process-wide
+ * lock).  Synchronous on the null stream, like gg_wls_batch. */
+#define GG_WLS_NOALLOC_BYTES (16 * 1024 * 1024)
+int gg_wls_batch_noalloc(const double* sat_ecef, const double* pseudoranges,
+                         const double* weights, const int32_t* n_sat, int32_t n_epoch,
+                         int32_t n_sat_max, int32_t max_iter, double tol, double* results,
+                         int32_t* iters);
 /* reference: wls_position, src/positioning/wls.cu:10-123 (no Sagnac) */
 int gg_wls_position(const double* sat_ecef, const double* pseudoranges, const double* weights,
                     int32_t n_sat, int32_t max_iter, double tol, double* result,
