@@ -136,6 +136,15 @@ class BatchedParticleFilter:
         call("gg_pfb_step_part", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(part))
 
+    def set_tuning(self, weight_tpw):
+        """Launch shapes of the step (results do not depend on them)."""
+        call("gg_pfb_set_tuning", self._h, int(weight_tpw))
+
+    def get_tuning(self):
+        w = C.c_int32()
+        call("gg_pfb_get_tuning", self._h, C.byref(w))
+        return {"weight_tpw": int(w.value)}
+
     def capture(self, args, resampler="systematic", n_iter=32, steps=1):
         call("gg_pfb_capture", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(steps))

@@ -59,16 +59,19 @@ __global__ void gg_k_debug_math(int op, const double* in, double* out, int n) {
     out[i] = gg_exp64(in[i]);
   } else if (op == 1) {
     out[i] = (double)gg_exp32((float)in[i]);
-  } else {
+  } else if (op == 2) {
     const gg_u32x4 r = gg_draw(0x123456789abcdef0ull, (uint32_t)in[i], 7u, 3u, 1u);
     for (int k = 0; k < 4; ++k) out[4 * i + k] = (double)r.v[k];
+  } else {
+    gg_xo x = gg_xo_seed(0x123456789abcdef0ull, (uint32_t)in[i], 7u, 3u, 1u);
+    for (int k = 0; k < 8; ++k) out[8 * i + k] = (double)gg_xo_next(x);
   }
 }
 
 extern "C" int gg_debug_math(int32_t op, const double* in, double* out, int32_t n) {
   if (n <= 0) return 0;
-  if (op < 0 || op > 2) return gg_fail(-2, "unknown op");
-  const size_t nout = (op == 2) ? 4 * (size_t)n : (size_t)n;
+  if (op < 0 || op > 3) return gg_fail(-2, "unknown op");
+  const size_t nout = (op == 2) ? 4 * (size_t)n : (op == 3 ? 8 * (size_t)n : (size_t)n);
   double* d = nullptr;
   GG_HIP(hipMalloc((void**)&d, sizeof(double) * (n + nout)));
   int rc = 0;

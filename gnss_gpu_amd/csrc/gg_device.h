@@ -89,6 +89,53 @@ __device__ __forceinline__ void gg_normal4(uint64_t seed, uint32_t idx, uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// xoshiro128+ (Blackman and Vigna, "Scrambled linear pseudorandom number
+// generators", ACM TOMS 2021), seeded per stream by one Philox4x32-10 block.
+// Long runs of uniforms for one thread (the predict noise of its particles,
+// a Megopolis chain's acceptance draws) cost one Philox block plus about
+// seven full-rate 32-bit operations per output, where a Philox block per four
+// outputs costs twenty 64-bit multiplies.  The top 23 bits feed gg_u01f; the
+// generator's weak lowest bits are never used.  The stream stays a pure
+// function of (seed, index, step, purpose, call): the seed block is counter-
+// based, the state advances only inside one thread.
+// ---------------------------------------------------------------------------
+struct gg_xo {
+  uint32_t s0, s1, s2, s3;
+};
+
+__host__ __device__ __forceinline__ gg_xo gg_xo_seed(uint64_t seed, uint32_t idx, uint32_t step,
+                                                     uint32_t purpose, uint32_t call) {
+  const gg_u32x4 r = gg_draw(seed, idx, step, purpose, call);
+  gg_xo x;
+  x.s0 = r.v[0];
+  x.s1 = r.v[1];
+  x.s2 = r.v[2];
+  x.s3 = r.v[3];
+  // the all-zero state is the generator's one fixed point
+  x.s0 |= ((x.s0 | x.s1 | x.s2 | x.s3) == 0u) ? 1u : 0u;
+  return x;
+}
+
+__host__ __device__ __forceinline__ uint32_t gg_xo_next(gg_xo& x) {
+  const uint32_t out = x.s0 + x.s3;
+  const uint32_t t = x.s1 << 9;
+  x.s2 ^= x.s0;
+  x.s3 ^= x.s1;
+  x.s1 ^= x.s2;
+  x.s0 ^= x.s3;
+  x.s2 ^= t;
+  x.s3 = (x.s3 << 11) | (x.s3 >> 21);
+  return out;
+}
+
+// Two standard normals (one Box-Muller pair) from the next two outputs.
+__device__ __forceinline__ void gg_xo_normal2(gg_xo& x, float& z0, float& z1) {
+  const float u1 = gg_u01f(gg_xo_next(x));
+  const float u2 = gg_u01f(gg_xo_next(x));
+  gg_box_muller(u1, u2, z0, z1);
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic binary64 exp.  Cody-Waite reduction x = k ln2 + r with ln2
 // split into a 32-bit head and a tail, then the degree-13 Taylor polynomial of
 // exp(r) on |r| <= ln2/2 evaluated by Horner, then exact scaling by 2^k.

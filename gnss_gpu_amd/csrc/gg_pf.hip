@@ -70,6 +70,9 @@ struct gg_pf {
   bool initialized = false;
   std::vector<double> init_isb;
   uint64_t seed0 = 0;
+  // launch shapes of the step's kernels (gg_pfb_set_tuning / gg_pfb_autotune):
+  // tiles per workgroup of the likelihood kernel
+  int32_t weight_tpw = 1;
 };
 
 namespace {
@@ -116,16 +119,17 @@ gg_arena arena_of(const gg_pf* h) {
   a.has_vel = h->has_vel;
   a.sat_rows = h->sat_rows;
   a.n_filters = h->n_filters;
-  a.pad = 0;
+  a.n_tiles = h->n_tiles;
   return a;
 }
 
-// LDS of the staged satellites: a 32-byte record and a group index per
-// satellite (+ 64 first-layer biases per satellite for the learned likelihood)
+// LDS of the staged satellites: a record of GG_REC float4 and a group index
+// per satellite (+ 64 first-layer biases per satellite for the learned
+// likelihood)
 size_t sat_smem(const gg_pf* h, bool learned = false) {
   int cap = 0;
   for (const auto& f : h->filt_h) cap = std::max(cap, f.sat_cap);
-  return (size_t)std::max(cap, 1) * (9 + (learned ? GG_MLP_H : 0)) * sizeof(float);
+  return (size_t)std::max(cap, 1) * (4 * GG_REC + 1 + (learned ? GG_MLP_H : 0)) * sizeof(float);
 }
 
 int ensure_pinned(gg_pf* h, size_t bytes) {
@@ -269,6 +273,11 @@ int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide =
       GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, false>), dim3(h->n_tiles),
                                                     dim3(GG_BLOCK), smem, h->stream, a, s));
     }
+  } else if (pred && h->weight_tpw > 1) {
+    const int tpw = h->weight_tpw;
+    const int grid = (h->n_tiles + tpw - 1) / tpw;
+    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_mt<NI, true>), dim3(grid),
+                                                  dim3(GG_BLOCK), smem, h->stream, a, s, tpw));
   } else if (pred) {
     GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, true>), dim3(h->n_tiles),
                                                   dim3(GG_BLOCK), smem, h->stream, a, s));
@@ -1697,6 +1706,19 @@ int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out) {
   GG_HIP(hipMemcpyAsync(out, h->anc + h->filt_h[f].p_off, sizeof(int32_t) * h->filt_h[f].n,
                         hipMemcpyDeviceToHost, h->stream));
   GG_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw) {
+  if (!h) return gg_fail(-2, "null handle");
+  if (weight_tpw < 1 || weight_tpw > 64) return gg_fail(-2, "weight_tpw must be in [1, 64]");
+  h->weight_tpw = weight_tpw;
+  return 0;
+}
+
+int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw) {
+  if (!h || !weight_tpw) return gg_fail(-2, "null argument");
+  *weight_tpw = h->weight_tpw;
   return 0;
 }
 

@@ -61,7 +61,7 @@ struct gg_arena {
   int32_t has_vel;
   int32_t sat_rows;
   int32_t n_filters;
-  int32_t pad;
+  int32_t n_tiles;  // tiles of all filters
 };
 
 __device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) {
@@ -70,16 +70,19 @@ __device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) 
 
 // Satellite geometry staged in LDS as binary32 values relative to the frame
 // anchor, grouped by inter-system-bias component: the reference system first,
-// then ISB 0, 1, 2, each group in the epoch row's order.  Record k is two
-// float4:
-//   rec[2k]     = (-2 ux, -2 uy, -2 uz, 1/rho)   u: unit vector anchor -> sat,
+// then ISB 0, 1, 2, each group in the epoch row's order.  Record k is GG_REC
+// (three) float4:
+//   rec[3k]     = (-2 ux, -2 uy, -2 uz, 1/rho)   u: unit vector anchor -> sat,
 //                                                rho = |sat - anchor|
-//   rec[2k + 1] = (c, wl, row index bits, isb bits)
+//   rec[3k + 1] = (c, wl, row index bits, isb bits)
 //     c  = pr - rho - cb_anchor - isb_anchor of the satellite's system
 //     wl = the likelihood's weight factor: -w / (2 sigma^2) (Gaussian),
 //          w / (nu sigma^2) (Student-t), w (Huber, learned)
 //     row index: position of the satellite in its epoch row; isb: ISB index,
 //     -1 for the reference system.
+//   rec[3k + 2] = (s c, s/2, -s/(8 rho), s/(16 rho^2)) with s = sqrt(|wl|):
+//     the residual scaled by s, e' = s e, for the Gaussian and Student-t
+//     terms (-e'^2 and log(1 + e'^2)), with s h(t) expanded in q (t = q/rho).
 // gend[g]: end of group g (g = 0 the reference system, g = 1 + ISB index).
 struct gg_sat_view {
   const float4* rec;
@@ -192,14 +195,72 @@ __device__ __forceinline__ gg_pred_plan gg_make_plan(const gg_dyn* dy, const gg_
   return pl;
 }
 
-// Predict one particle in place (binary32 offsets).
+// Predict noise.  The thread that owns particles 4g .. 4g+3 of a filter draws
+// them from one stream per predict call: xoshiro128+ seeded by the Philox
+// block (filter seed, g, step, GG_PURPOSE_PREDICT, call).  Call 0 gives the
+// 4 D normals z[q D + c] of component c (x, y, z in the local ENU frame, clock
+// bias, then the inter-system biases) of particle 4g + q, as consecutive
+// Box-Muller pairs; call 1 the 12 velocity-noise normals zv[3 q + k].  The
+// stream hands out one normal at a time (a pair may straddle two particles),
+// so no thread holds more than one pending normal.
+struct gg_normal_stream {
+  gg_xo x;
+  float carry;
+  bool has;
+  __device__ __forceinline__ gg_normal_stream(uint64_t seed, uint32_t g, uint32_t step,
+                                              uint32_t call)
+      : x(gg_xo_seed(seed, g, step, GG_PURPOSE_PREDICT, call)), carry(0.0f), has(false) {}
+  __device__ __forceinline__ float next() {
+    if (has) {
+      has = false;
+      return carry;
+    }
+    float z0;
+    gg_xo_normal2(x, z0, carry);
+    has = true;
+    return z0;
+  }
+};
+
+// Noise of the fast predict path (uniform velocity carried by the anchor, no
+// velocity filter): zn[q][c] = normal of component c of particle 4g + q times
+// its standard deviation.  It depends on nothing the step computes, so the
+// weight kernel draws it before it waits for the step frame.
+template <int NISB>
+__device__ __forceinline__ void gg_noise4_fast(uint64_t seed, uint32_t g, uint32_t step,
+                                               const gg_step_args& s,
+                                               float (&zn)[GG_ITEMS][4 + NISB]) {
+  gg_normal_stream zs(seed, g, step, 0u);
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    zn[q][0] = zs.next() * s.sigma_pos;
+    zn[q][1] = zs.next() * s.sigma_pos;
+    zn[q][2] = zs.next() * s.sigma_pos;
+    zn[q][3] = zs.next() * s.sigma_cb;
+#pragma unroll
+    for (int k = 0; k < NISB; ++k) zn[q][4 + k] = zs.next() * s.sigma_isb;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void gg_apply_noise4_fast(const float (&zn)[GG_ITEMS][D],
+                                                     float (&p)[GG_ITEMS][D]) {
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[q][c] += 0.0f + zn[q][c];
+#pragma unroll
+    for (int c = 3; c < D; ++c) p[q][c] += zn[q][c];
+  }
+}
+
+// Predict one particle in place (binary32 offsets) with its normals z
+// [4 + NISB] (position, clock bias, ISBs) and zv [3] (velocity noise).
 template <int NISB>
 __device__ __forceinline__ void gg_predict_one(const gg_arena& a, int cur, int64_t slot,
-                                               uint64_t seed, uint32_t lidx, uint32_t step,
                                                const gg_step_args& s, const gg_pred_plan& pl,
-                                               float* p /*[4+NISB]*/) {
-  float z0[4];
-  gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 0u, z0);
+                                               const float (&z0)[4 + NISB], const float (&zv)[3],
+                                               float (&p)[4 + NISB]) {
   const float dt = (float)s.dt;
   float dv[3] = {0.0f, 0.0f, 0.0f};
   if (pl.mode_out >= 1) {
@@ -212,11 +273,8 @@ __device__ __forceinline__ void gg_predict_one(const gg_arena& a, int cur, int64
         v[k] = (1.0f - pl.alpha) * vi + pl.alpha * pl.vg[k];
       }
     }
-    if (pl.per_particle_noise) {
-      float zv[4];
-      gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 1u, zv);
+    if (pl.per_particle_noise)
       for (int k = 0; k < 3; ++k) v[k] += zv[k] * s.sigma_vel;
-    }
     for (int k = 0; k < 3; ++k) {
       gg_comp(a, cur, 4 + NISB + k)[slot] = v[k];
       dv[k] = (v[k] - pl.vref_f[k]) * dt;
@@ -254,49 +312,39 @@ __device__ __forceinline__ void gg_predict_one(const gg_arena& a, int cur, int64
   p[1] += dv[1] + n1;
   p[2] += dv[2] + n2;
   p[3] += z0[3] * s.sigma_cb;
-  if (NISB > 0) {
-    float zi[4];
-    gg_normal4(seed, lidx, step, GG_PURPOSE_PREDICT, 2u, zi);
 #pragma unroll
-    for (int k = 0; k < NISB; ++k) p[4 + k] += zi[k] * s.sigma_isb;
-  }
+  for (int k = 0; k < NISB; ++k) p[4 + k] += z0[4 + k] * s.sigma_isb;
 }
 
-// Predict the thread's four particles.  With a uniform velocity carried by the
-// anchor (plan mode 0) and no velocity filter, the noise draws of the four
-// particles are independent straight-line code, so their Philox chains
-// interleave; every other mode goes particle by particle.  Both branches give
-// the same values for the same particle.
+// Predict the thread's four particles (i0 = 4g).  With a uniform velocity
+// carried by the anchor (plan mode 0) and no velocity filter every particle
+// only adds its noise, as straight-line code; every other mode goes particle
+// by particle.  Both branches take the same normals for the same particle.
 template <int NISB>
 __device__ __forceinline__ void gg_predict4(const gg_arena& a, int cur, int64_t base, int n,
                                             uint64_t seed, int i0, uint32_t step,
                                             const gg_step_args& s, const gg_pred_plan& pl,
                                             float (&p)[GG_ITEMS][4 + NISB]) {
+  constexpr int D = 4 + NISB;
+  const uint32_t g = (uint32_t)i0 >> 2;
   if (pl.mode_out == 0 && !s.velocity_kf) {
-    float z[GG_ITEMS][4];
-    float zi[GG_ITEMS][4];
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) {
-      gg_normal4(seed, (uint32_t)(i0 + q), step, GG_PURPOSE_PREDICT, 0u, z[q]);
-      if (NISB > 0) gg_normal4(seed, (uint32_t)(i0 + q), step, GG_PURPOSE_PREDICT, 2u, zi[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) {
-      const float n0 = z[q][0] * s.sigma_pos;
-      const float n1 = z[q][1] * s.sigma_pos;
-      const float n2 = z[q][2] * s.sigma_pos;
-      p[q][0] += 0.0f + n0;
-      p[q][1] += 0.0f + n1;
-      p[q][2] += 0.0f + n2;
-      p[q][3] += z[q][3] * s.sigma_cb;
-#pragma unroll
-      for (int k = 0; k < NISB; ++k) p[q][4 + k] += zi[q][k] * s.sigma_isb;
-    }
+    float zn[GG_ITEMS][D];
+    gg_noise4_fast<NISB>(seed, g, step, s, zn);
+    gg_apply_noise4_fast<D>(zn, p);
     return;
   }
+  gg_normal_stream zs(seed, g, step, 0u);
+  gg_normal_stream vs(seed, g, step, 1u);
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q)
-    if (i0 + q < n) gg_predict_one<NISB>(a, cur, base + q, seed, (uint32_t)(i0 + q), step, s, pl, p[q]);
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    float z[D], zv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < D; ++c) z[c] = zs.next();
+    if (pl.per_particle_noise)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) zv[k] = vs.next();
+    if (i0 + q < n) gg_predict_one<NISB>(a, cur, base + q, s, pl, z, zv, p[q]);
+  }
 }
 
 // Satellite row r relative to the frame anchor in the filter's ENU frame
@@ -325,10 +373,16 @@ __device__ __forceinline__ float gg_stage_one(const gg_arena& a, const double* s
   double c = sd[3 * a.sat_rows + r] - rho - *cb_anchor;
   if (ib >= 0) c -= isb_anchor[ib];
   const float ir = (float)inv_rho;
-  rec[2 * dest] = make_float4(-2.0f * (float)(d[0] * inv_rho), -2.0f * (float)(d[1] * inv_rho),
-                              -2.0f * (float)(d[2] * inv_rho), ir);
-  rec[2 * dest + 1] = make_float4((float)c, (float)(sd[4 * a.sat_rows + r] * wfac),
-                                  __int_as_float(k), __int_as_float(ib));
+  const double wl = sd[4 * a.sat_rows + r] * wfac;
+  rec[GG_REC * dest] = make_float4(-2.0f * (float)(d[0] * inv_rho),
+                                   -2.0f * (float)(d[1] * inv_rho),
+                                   -2.0f * (float)(d[2] * inv_rho), ir);
+  rec[GG_REC * dest + 1] = make_float4((float)c, (float)wl, __int_as_float(k), __int_as_float(ib));
+  // the scaled form: s = sqrt(|wl|), s c and the series of s h(q) in q
+  const double sw = sqrt(fabs(wl));
+  rec[GG_REC * dest + 2] = make_float4((float)(sw * c), (float)(0.5 * sw),
+                                       (float)(-0.125 * sw * inv_rho),
+                                       (float)(0.0625 * sw * inv_rho * inv_rho));
   return ir;
 }
 
@@ -352,7 +406,7 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   const int cap = F.sat_cap;
   const int t = threadIdx.x;
   float4* rec = reinterpret_cast<float4*>(smem);
-  int* grp = reinterpret_cast<int*>(rec + 2 * cap);
+  int* grp = reinterpret_cast<int*>(rec + GG_REC * cap);
   // weight factor of the likelihood (binary64, rounded once)
   const double sig2 = (double)s.sigma_pr * (double)s.sigma_pr;
   const double wfac = (s.lik == GG_LIK_GAUSS) ? -0.5 / sig2
@@ -440,6 +494,23 @@ __device__ __forceinline__ float gg_residual(const float4& g, const float4& m, c
   return fmaf(-q, h, m.x) - bias;
 }
 
+// The residual scaled by s = sqrt(|wl|) from the records rec[3k] (g) and
+// rec[3k + 2] (r): e' = s c - q s h - s bias, the 2e7 m range never formed.
+// SERIES: s h = s/2 - (s/(8 rho)) q + (s/(16 rho^2)) q^2 (gg_residual's
+// series with t = q/rho); otherwise s h = 2 (s/2) / (1 + sqrt(1 + t)).
+template <bool SERIES>
+__device__ __forceinline__ float gg_residual_scaled(const float4& g, const float4& r, const float* p,
+                                                    float pp, float bias) {
+  const float ud2 = fmaf(g.z, p[2], fmaf(g.y, p[1], g.x * p[0]));  // -2 u.x
+  const float q = fmaf(pp, g.w, ud2);
+  float sh;
+  if (SERIES)
+    sh = fmaf(fmaf(q, r.w, r.z), q, r.y);
+  else
+    sh = (r.y + r.y) * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(fmaf(q, g.w, 1.0f)));
+  return fmaf(-(r.y + r.y), bias, fmaf(-q, sh, r.x));
+}
+
 // |t| <= (pp/rho + 2 |x|)/rho <= 2^-10 for every staged satellite when
 // pp (1/rho)^2 <= 2^-25 (|x| within about 3.5 km of the anchor for GNSS
 // ranges); the margin covers the binary32 rounding of q and t.
@@ -506,7 +577,7 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
     gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
-        const float e = gg_residual<false>(sv.rec[2 * k], sv.rec[2 * k + 1], p, pp, bias);
+        const float e = gg_residual<false>(sv.rec[GG_REC * k], sv.rec[GG_REC * k + 1], p, pp, bias);
         kept += (fabsf(e) <= s.gate_thr) ? 1 : 0;
       }
     });
@@ -522,8 +593,8 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
     gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
-        const float4 m = sv.rec[2 * k + 1];
-        const float e = gg_residual<false>(sv.rec[2 * k], m, p, pp, bias);
+        const float4 m = sv.rec[GG_REC * k + 1];
+        const float e = gg_residual<false>(sv.rec[GG_REC * k], m, p, pp, bias);
         if (gate && fabsf(e) > s.gate_thr) continue;
         prod *= fmaf(m.y * e, e, 1.0f);
         if (++cnt == 3) {
@@ -541,8 +612,8 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
     gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
-        const float4 m = sv.rec[2 * k + 1];
-        const float e = gg_residual<false>(sv.rec[2 * k], m, p, pp, bias);
+        const float4 m = sv.rec[GG_REC * k + 1];
+        const float e = gg_residual<false>(sv.rec[GG_REC * k], m, p, pp, bias);
         if (gate && fabsf(e) > s.gate_thr) continue;
         if (s.lik == GG_LIK_HUBER) {
           const float z = fabsf(e) * isig;
@@ -579,12 +650,12 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
       for (int k = k0; k < k1; ++k) {
-        const float4 g = sv.rec[2 * k];
-        const float4 m = sv.rec[2 * k + 1];
+        const float4 g = sv.rec[GG_REC * k];
+        const float4 r = sv.rec[GG_REC * k + 2];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
-          prod[q] *= fmaf(m.y * e, e, 1.0f);
+          const float e = gg_residual_scaled<SERIES>(g, r, p[q], pp[q], bias[q]);
+          prod[q] *= fmaf(e, e, 1.0f);
         }
         if (++cnt == 3) {
 #pragma unroll
@@ -608,8 +679,8 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
       for (int k = k0; k < k1; ++k) {
-        const float4 g = sv.rec[2 * k];
-        const float4 m = sv.rec[2 * k + 1];
+        const float4 g = sv.rec[GG_REC * k];
+        const float4 m = sv.rec[GG_REC * k + 1];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
           const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
@@ -628,12 +699,12 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
       for (int k = k0; k < k1; ++k) {
-        const float4 g = sv.rec[2 * k];
-        const float4 m = sv.rec[2 * k + 1];
+        const float4 g = sv.rec[GG_REC * k];
+        const float4 r = sv.rec[GG_REC * k + 2];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
-          lw[q] = fmaf(m.y * e, e, lw[q]);
+          const float e = gg_residual_scaled<SERIES>(g, r, p[q], pp[q], bias[q]);
+          lw[q] = fmaf(-e, e, lw[q]);
         }
       }
     });
@@ -670,8 +741,8 @@ __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_fil
   for (int i = threadIdx.x; i < ns * GG_MLP_H; i += blockDim.x) {
     const int k = i / GG_MLP_H;
     const int u = i % GG_MLP_H;
-    const float4 g = sv.rec[2 * k];
-    const int64_t r = F.sat_off + __float_as_int(sv.rec[2 * k + 1].z);
+    const float4 g = sv.rec[GG_REC * k];
+    const int64_t r = F.sat_off + __float_as_int(sv.rec[GG_REC * k + 1].z);
     float f[GG_MLP_NF - 1];
     gg_mlp_features(-0.5f * g.x, -0.5f * g.y, -0.5f * g.z, sf[r], sf[a.sat_rows + r], ns, essf,
                     f);
@@ -698,8 +769,8 @@ __device__ __forceinline__ void gg_loglik_table_form(const gg_sat_view& sv, int6
 #pragma unroll
     for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
     for (int k = k0; k < k1; ++k) {
-      const float4 g = sv.rec[2 * k];
-      const float4 m = sv.rec[2 * k + 1];
+      const float4 g = sv.rec[GG_REC * k];
+      const float4 m = sv.rec[GG_REC * k + 1];
       const float2* tk = tab + (row0 + __float_as_int(m.z)) * GG_MLP_CODES;
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q) {
@@ -747,8 +818,8 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv,
 #pragma unroll
     for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
     for (int k = k0; k < k1; ++k) {
-      const float4 g = sv.rec[2 * k];
-      const float4 m = sv.rec[2 * k + 1];
+      const float4 g = sv.rec[GG_REC * k];
+      const float4 m = sv.rec[GG_REC * k + 1];
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q)
         se[t * GG_ITEMS + q] = series ? gg_residual<true>(g, m, p[q], pp[q], bias[q])
@@ -817,6 +888,32 @@ __device__ __forceinline__ void gg_block_totals(const double (&v)[D + 2], double
   gg_block_totals_n<D + 2>(v, lds, out);
 }
 
+// Scale of a tile: its weights are e_i 2^k c relative to exp(lw_i) up to a
+// common factor, with k = floor(m_b / ln 2) and c = exp64(m_b - k ln 2) in
+// [1, 2) (Cody-Waite split of ln 2, as gg_exp64).  Tiles then combine through
+// exact powers of two (2^(k_b - K)), so the combine needs no exponential.  An
+// empty tile (m_b = -inf) gets k = -2^30, c = 0; a non-finite maximum k = 0,
+// c = 1 (its NaN or infinite sums propagate as before).
+__device__ __forceinline__ void gg_tile_scale(double mb, double& k, double& c) {
+#pragma clang fp contract(off)
+  if (!(mb > -__builtin_huge_val())) {
+    k = -1073741824.0;
+    c = 0.0;
+  } else if (!(mb < __builtin_huge_val())) {
+    k = 0.0;
+    c = 1.0;
+  } else {
+    k = floor(mb * 0x1.71547652b82fep+0);
+    c = gg_exp64((mb - k * 0x1.62e42fee00000p-1) - k * 0x1.a39ef35793c76p-33);
+  }
+}
+
+// 2^(k_b - K) x v for a tile at most 1000 binary orders below the largest
+// (0 below that): the result stays a normal number, so the scaling is exact.
+__device__ __forceinline__ double gg_tile_shift(double v, double dk) {
+  return (dk >= -1000.0) ? __builtin_ldexp(v, (int)dk) : 0.0;
+}
+
 template <int D>
 __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
                                                  const bool (&valid)[GG_ITEMS],
@@ -839,7 +936,7 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
 #pragma unroll
   for (int q = 1; q < GG_ITEMS; ++q) {
     v[0] = v[0] + (double)e[q];
-    v[1] = v[1] + (double)e[q] * (double)e[q];
+    v[1] = __builtin_fma((double)e[q], (double)e[q], v[1]);  // the ESS only: no twin
   }
   // weighted state sums (the estimate): binary32 inside the wavefront,
   // binary64 across the 8 half-wavefront partials
@@ -848,7 +945,7 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
   for (int d = 0; d < D; ++d) {
     ms[d] = e[0] * p[0][d];
 #pragma unroll
-    for (int q = 1; q < GG_ITEMS; ++q) ms[d] = ms[d] + e[q] * p[q][d];
+    for (int q = 1; q < GG_ITEMS; ++q) ms[d] = __builtin_fmaf(e[q], p[q][d], ms[d]);
   }
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -862,15 +959,22 @@ __device__ __forceinline__ void gg_tile_partials(const float (&lw)[GG_ITEMS],
   double tot[2];
   gg_block_totals_n<2>(v, lds, tot);  // synchronises: lf is complete here
   if (t == 0) {
+    double kb, cb;
+    gg_tile_scale((double)mb, kb, cb);
+    const double Sc = tot[0] * cb;
+    const double S2c = tot[1] * (cb * cb);
     part_out[0] = (double)mb;
-    part_out[1] = tot[0];
-    part_out[2] = tot[1];
-    *reinterpret_cast<double4*>(p4_out) = make_double4((double)mb, tot[0], tot[1], 0.0);
+    part_out[1] = Sc;
+    part_out[2] = S2c;
+    part_out[GG_NPART - 2] = kb;
+    part_out[GG_NPART - 1] = cb;
+    *reinterpret_cast<double4*>(p4_out) = make_double4((double)mb, Sc, S2c, kb);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const float* l = lf + d * 8;
-      part_out[3 + d] = (((double)l[0] + (double)l[1]) + ((double)l[2] + (double)l[3])) +
-                        (((double)l[4] + (double)l[5]) + ((double)l[6] + (double)l[7]));
+      part_out[3 + d] = ((((double)l[0] + (double)l[1]) + ((double)l[2] + (double)l[3])) +
+                         (((double)l[4] + (double)l[5]) + ((double)l[6] + (double)l[7]))) *
+                        cb;
     }
   }
 }
@@ -978,9 +1082,11 @@ struct gg_comb {
 // Combine a filter's tile partials (one workgroup per filter, or redundantly
 // in every workgroup that needs the result; the operations and their order
 // are the same wherever it runs):
-//   pass 1 (strided): M = max m_b;
-//   pass 2 (strided): g_b = exp64(m_b - M), T_b = S_b g_b staged in LDS (sT),
-//           S2 and (FULL) weighted-mean sums accumulated in the strided order;
+//   pass 1 (strided): M = max m_b and K = max k_b (exact, any order);
+//   pass 2 (strided): T_b = 2^(k_b - K) c_b S_b staged in LDS (sT), the ESS
+//           sum and (FULL) the weighted-mean sums scaled the same way, all
+//           exact power-of-two shifts (gg_tile_scale), accumulated in the
+//           strided order; g_b = 2^(k_b - K) c_b;
 //   pass 3: W and the CDF tile offsets P_b in the fixed chunked order (thread
 //           t owns tiles [t*c, t*c+c), sums them left to right from 0.0, the
 //           per-thread values go through gg_block_scan256, then
@@ -995,52 +1101,65 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
   const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
-  const double* p4 = a.p4 + (int64_t)F.tile_off * 4;  // m, S, S2 packed: coalesced reads
+  const double* p4 = a.p4 + (int64_t)F.tile_off * 4;  // m, cS, c^2 S2, k packed: coalesced reads
   double vv[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) vv[d] = 0.0;
-  // pass 2 for tile b given M (m_b, S_b, S2_b and the sums read from pb)
-  auto pass2 = [&](int b, double mb, double Sb, double S2b, const double* pb, double M) {
-    const double gb = (mb > -__builtin_huge_val()) ? gg_exp64(mb - M) : 0.0;
-    if (ALL) a.g[F.tile_off + b] = gb;
-    if (!ALL && b == own) sx[1] = gb;
-    sT[b] = Sb * gb;
-    vv[0] = vv[0] + S2b * (gb * gb);
+  // pass 2 for tile b given K (scaled S, S2 and the tile's exponent from p4)
+  auto pass2 = [&](int b, double Sc, double S2c, double kb, double K) {
+    const double dk = kb - K;
+    if (ALL || b == own) {
+      const double gb = gg_tile_shift(part[(int64_t)b * GG_NPART + GG_NPART - 1], dk);
+      if (ALL) a.g[F.tile_off + b] = gb;
+      if (!ALL) sx[1] = gb;
+    }
+    sT[b] = gg_tile_shift(Sc, dk);
+    vv[0] = vv[0] + gg_tile_shift(S2c, 2.0 * dk);
     if (FULL) {
+      const double* pb = part + (int64_t)b * GG_NPART;
 #pragma unroll
-      for (int d = 0; d < D; ++d) vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + pb[3 + d] * gb;
+      for (int d = 0; d < D; ++d)
+        vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + gg_tile_shift(pb[3 + d], dk);
     }
   };
-  constexpr int R = 4;  // tiles per thread held in registers across the max
-  double M;
+  constexpr int R = 4;  // tiles per thread held in registers across the maxima
+  double M, K;
   if (nt <= R * GG_BLOCK) {
-    double pm[R], pS[R], pS2[R];
+    double pm[R], pS[R], pS2[R], pk[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int b = t + k * GG_BLOCK;
-      const double2* q = reinterpret_cast<const double2*>(p4 + (int64_t)min(b, nt - 1) * 4);
-      const double2 ms = q[0];
-      pm[k] = ms.x;
-      pS[k] = ms.y;
-      pS2[k] = q[1].x;
+      const double4 q = *reinterpret_cast<const double4*>(p4 + (int64_t)min(b, nt - 1) * 4);
+      pm[k] = q.x;
+      pS[k] = q.y;
+      pS2[k] = q.z;
+      pk[k] = q.w;
     }
-    double m = -__builtin_huge_val();
+    double m = -__builtin_huge_val(), km = -__builtin_huge_val();
 #pragma unroll
     for (int k = 0; k < R; ++k)
-      if (t + k * GG_BLOCK < nt) m = fmax(m, pm[k]);
+      if (t + k * GG_BLOCK < nt) {
+        m = fmax(m, pm[k]);
+        km = fmax(km, pk[k]);
+      }
     M = gg_block_max256(m, lds);
+    K = gg_block_max256(km, lds);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int b = t + k * GG_BLOCK;
-      if (b < nt) pass2(b, pm[k], pS[k], pS2[k], part + (int64_t)b * GG_NPART, M);
+      if (b < nt) pass2(b, pS[k], pS2[k], pk[k], K);
     }
   } else {
-    double m = -__builtin_huge_val();
-    for (int b = t; b < nt; b += GG_BLOCK) m = fmax(m, p4[(int64_t)b * 4]);
+    double m = -__builtin_huge_val(), km = -__builtin_huge_val();
+    for (int b = t; b < nt; b += GG_BLOCK) {
+      m = fmax(m, p4[(int64_t)b * 4]);
+      km = fmax(km, p4[(int64_t)b * 4 + 3]);
+    }
     M = gg_block_max256(m, lds);
+    K = gg_block_max256(km, lds);
     for (int b = t; b < nt; b += GG_BLOCK) {
       const double* q = p4 + (int64_t)b * 4;
-      pass2(b, q[0], q[1], q[2], part + (int64_t)b * GG_NPART, M);
+      pass2(b, q[1], q[2], q[3], K);
     }
   }
   __syncthreads();

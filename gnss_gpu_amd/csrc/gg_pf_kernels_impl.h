@@ -111,96 +111,151 @@ __global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
 // ---------------------------------------------------------------------------
 // predict (PRED) + likelihood + tile partials.  LEARN: 0 the analytic
 // likelihood of s.lik, 1 the learned network per pair, 2 the learned network
-// through its per-satellite code tables.  The step frame is computed once per
-// workgroup into LDS, so its binary64 values do not occupy registers for the
-// whole kernel.
-template <int NISB, bool PRED, int LEARN>
+// through its per-satellite code tables.  The workgroup handles tiles
+// [tb, te) in order; the step frame (computed by one thread into LDS, so its
+// binary64 values do not occupy registers) and the satellite records are
+// staged once per filter met.  PREFETCH issues the loads of tile k + 1 before
+// tile k is computed, so the memory traffic of a multi-tile workgroup
+// overlaps its arithmetic instead of waiting for it.
+template <int NISB, bool PRED, int LEARN, bool PREFETCH>
 __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_args& s,
-                                               char* smem) {
+                                               char* smem, int tb, int te) {
   constexpr int D = 4 + NISB;
   __shared__ double lds[8 + GG_BLOCK];
   __shared__ gg_frame sfr;
-  const int tile = blockIdx.x;
-  const int f = a.tile_filter[tile];
-  const gg_filter F = a.filt[f];
-  const gg_dyn* dy = a.dyn + f;
-  const int lt = tile - F.tile_off;
-  const int cur = dy->cur;
-  const int epoch = dy->epoch;
-  const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
-  const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
-  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
-  const int64_t base = F.p_off + i0;
-  float p[GG_ITEMS][D];
-  gg_load4<D>(a, cur, base, p);
-  if (threadIdx.x == 0) {
-    if (PRED) {
-      const double* vg = gg_guide_ptr(a, s, f, epoch);
-      double vge[3] = {vg[0], vg[1], vg[2]};
-      gg_step_frame<D>(dy, s, vge, sfr);
-    } else {
-      for (int k = 0; k < 3; ++k) sfr.anchor[k] = dy->anchor[k];
-      sfr.cb_anchor = dy->cb_anchor;
-      for (int k = 0; k < GG_MAX_ISB; ++k) sfr.isb_anchor[k] = dy->isb_anchor[k];
-    }
+  // the slot of particle 4t of tile `tile` (filter, buffer, index)
+  auto slot_of = [&](int tile, int& f, int64_t& base, int& i0, int& cur) {
+    f = a.tile_filter[tile];
+    const gg_filter F = a.filt[f];
+    i0 = (tile - F.tile_off) * GG_TILE + threadIdx.x * GG_ITEMS;
+    base = F.p_off + i0;
+    cur = a.dyn[f].cur;
+  };
+  float pn[GG_ITEMS][D];
+  if (PREFETCH) {
+    int f, i0, cur;
+    int64_t base;
+    slot_of(tb, f, base, i0, cur);
+    gg_load4<D>(a, cur, base, pn);
   }
-  __syncthreads();
+  int f_staged = -1;
+  int ns = 0;
   gg_sat_view sv;
-  const int ns = gg_stage_sats(a, f, F, dy, sfr.anchor, &sfr.cb_anchor, sfr.isb_anchor, slot, s,
-                               sv, smem);
-  bool valid[GG_ITEMS];
+  for (int tile = tb; tile < te; ++tile) {
+    int f, i0, cur;
+    int64_t base;
+    slot_of(tile, f, base, i0, cur);
+    const gg_filter F = a.filt[f];
+    const gg_dyn* dy = a.dyn + f;
+    const int epoch = dy->epoch;
+    const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
+    const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
+    float p[GG_ITEMS][D];
+    if (PREFETCH) {
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
-  if (PRED) {
-    gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, sfr.pl, p);
-    gg_apply_shift<D>(sfr, p);
-    gg_store4<D>(a, cur, base, p);
-  }
-  float lw[GG_ITEMS];
-  if constexpr (LEARN == 2) {
-    gg_loglik_table<NISB>(sv, F.sat_off, a.mlp_table, p, lw);
-  } else if constexpr (LEARN == 1) {
-    __shared__ gg_mlp_w smlp;
-    __shared__ float se[GG_TILE];
-    __shared__ float slw[GG_TILE];
-    {
-      const uint4* src = reinterpret_cast<const uint4*>(a.mlp);
-      uint4* dst = reinterpret_cast<uint4*>(&smlp);
-      for (int i = threadIdx.x; i < (int)(sizeof(gg_mlp_w) / 16); i += GG_BLOCK) dst[i] = src[i];
+      for (int q = 0; q < GG_ITEMS; ++q)
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[q][d] = pn[q][d];
+      if (tile + 1 < te) {
+        int f1, i1, cur1;
+        int64_t base1;
+        slot_of(tile + 1, f1, base1, i1, cur1);
+        gg_load4<D>(a, cur1, base1, pn);
+      }
+    } else {
+      gg_load4<D>(a, cur, base, p);
     }
-    __syncthreads();
-    gg_stage_learned(a, F, dy, slot, ns, smlp, sv);
-    gg_loglik_learned<NISB>(sv, p, smlp, se, slw, lw);
-  } else {
-    gg_loglik4<NISB>(sv, ns, p, s, lw);
-  }
+    // the fast predict path (pl.mode_out == 0, no velocity filter) needs no
+    // frame: its noise is drawn while the state loads are in flight and added
+    // before the staging below
+    const bool fast = PRED && dy->vel_mode == 0 && !s.velocity_kf && !(s.sigma_vel > 0.0f);
+    if (fast) {
+      float zn[GG_ITEMS][D];
+      gg_noise4_fast<NISB>(F.seed, (uint32_t)i0 >> 2, step, s, zn);
+      gg_apply_noise4_fast<D>(zn, p);
+    }
+    if (f != f_staged) {  // uniform over the workgroup
+      if (f_staged >= 0) __syncthreads();  // the previous filter's records are no longer read
+      if (threadIdx.x == 0) {
+        if (PRED) {
+          const double* vg = gg_guide_ptr(a, s, f, epoch);
+          double vge[3] = {vg[0], vg[1], vg[2]};
+          gg_step_frame<D>(dy, s, vge, sfr);
+        } else {
+          for (int k = 0; k < 3; ++k) sfr.anchor[k] = dy->anchor[k];
+          sfr.cb_anchor = dy->cb_anchor;
+          for (int k = 0; k < GG_MAX_ISB; ++k) sfr.isb_anchor[k] = dy->isb_anchor[k];
+        }
+      }
+      __syncthreads();
+      ns = gg_stage_sats(a, f, F, dy, sfr.anchor, &sfr.cb_anchor, sfr.isb_anchor, slot, s, sv,
+                         smem);
+      f_staged = f;
+    }
+    bool valid[GG_ITEMS];
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? lw[q] : 0.0f;
-  *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
-  gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART,
-                      a.p4 + (int64_t)tile * 4);
+    for (int q = 0; q < GG_ITEMS; ++q) valid[q] = (i0 + q) < F.n;
+    if (PRED) {
+      if (!fast) gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, sfr.pl, p);
+      gg_apply_shift<D>(sfr, p);
+      gg_store4<D>(a, cur, base, p);
+    }
+    float lw[GG_ITEMS];
+    if constexpr (LEARN == 2) {
+      gg_loglik_table<NISB>(sv, F.sat_off, a.mlp_table, p, lw);
+    } else if constexpr (LEARN == 1) {
+      __shared__ gg_mlp_w smlp;
+      __shared__ float se[GG_TILE];
+      __shared__ float slw[GG_TILE];
+      {
+        const uint4* src = reinterpret_cast<const uint4*>(a.mlp);
+        uint4* dst = reinterpret_cast<uint4*>(&smlp);
+        for (int i = threadIdx.x; i < (int)(sizeof(gg_mlp_w) / 16); i += GG_BLOCK) dst[i] = src[i];
+      }
+      __syncthreads();
+      gg_stage_learned(a, F, dy, slot, ns, smlp, sv);
+      gg_loglik_learned<NISB>(sv, p, smlp, se, slw, lw);
+    } else {
+      gg_loglik4<NISB>(sv, ns, p, s, lw);
+    }
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) lw[q] = valid[q] ? lw[q] : 0.0f;
+    *reinterpret_cast<float4*>(a.lw + base) = make_float4(lw[0], lw[1], lw[2], lw[3]);
+    gg_tile_partials<D>(lw, valid, p, lds, a.part + (int64_t)tile * GG_NPART,
+                        a.p4 + (int64_t)tile * 4);
+  }
 }
 
 // One round of workgroups covers 1M particles only at 4 waves per SIMD, so the
-// register budget of the plain likelihood is held to 128 VGPRs.
+// register budget of the plain likelihood is held to 128 VGPRs (one tile per
+// workgroup).  The multi-tile form (gg_k_weight_mt) runs tpw tiles per
+// workgroup with the next tile's state prefetched.
 template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void gg_k_weight(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, 0>(a, s, smem);
+  gg_weight_body<NISB, PRED, 0, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
+}
+
+template <int NISB, bool PRED>
+__global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+void gg_k_weight_mt(gg_arena a, gg_step_args s, int tpw) {
+  extern __shared__ __align__(16) char smem[];
+  const int tb = blockIdx.x * tpw;
+  gg_weight_body<NISB, PRED, 0, true>(a, s, smem, tb, min(tb + tpw, a.n_tiles));
 }
 
 template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_learned(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, 1>(a, s, smem);
+  gg_weight_body<NISB, PRED, 1, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
 }
 
 template <int NISB, bool PRED>
 __global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void gg_k_weight_table(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, 2>(a, s, smem);
+  gg_weight_body<NISB, PRED, 2, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
 }
 
 // Per-satellite code tables of the learned likelihood: for satellite row
@@ -804,6 +859,13 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
     k[q] = i;
     wk[q] = (i < n) ? w[i] : 0.0f;
   }
+  // acceptance uniforms: one xoshiro128+ stream per chain, seeded by the
+  // Philox block (seed, particle, call, GG_PURPOSE_MEGO_U, 0); iteration b
+  // takes the stream's output b
+  gg_xo xs[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q)
+    xs[q] = gg_xo_seed(F.seed, (uint32_t)(ib + 64 * q), call, GG_PURPOSE_MEGO_U, 0u);
   for (int b0 = 0; b0 < B; b0 += 64) {
     uint32_t myO = 0;
     if (b0 + lane < B) {
@@ -811,30 +873,19 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
       myO = (uint32_t)(((uint64_t)r.v[0] * (uint64_t)(uint32_t)n) >> 32);
     }
     const int bend = min(B - b0, 64);
-    // one Philox block per particle serves four iterations (b0 is a multiple
-    // of 64, so the groups of four start at multiples of four)
-    for (int g0 = 0; g0 < bend; g0 += 4) {
-      gg_u32x4 r[GG_ITEMS];
+    for (int c = 0; c < bend; ++c) {
+      const int O = __builtin_amdgcn_readlane((int)myO, c);
 #pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q)
-        r[q] = gg_draw(F.seed, (uint32_t)(ib + 64 * q), call, GG_PURPOSE_MEGO_U,
-                       (uint32_t)((b0 + g0) >> 2));
-      const int gend = min(bend - g0, 4);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c >= gend) break;
-        const int O = __builtin_amdgcn_readlane((int)myO, g0 + c);
-#pragma unroll
-        for (int q = 0; q < GG_ITEMS; ++q) {
-          const int i = ib + 64 * q;
-          if (i >= n) continue;
-          int j = i + O;
-          if (j >= n) j -= n;
-          const float wj = w[j];
-          const bool acc = gg_u01f(r[q].v[c]) * wk[q] < wj;
-          k[q] = acc ? j : k[q];
-          wk[q] = acc ? wj : wk[q];
-        }
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const int i = ib + 64 * q;
+        const float u = gg_u01f(gg_xo_next(xs[q]));
+        if (i >= n) continue;
+        int j = i + O;
+        if (j >= n) j -= n;
+        const float wj = w[j];
+        const bool acc = u * wk[q] < wj;
+        k[q] = acc ? j : k[q];
+        wk[q] = acc ? wj : wk[q];
       }
     }
   }

@@ -163,6 +163,13 @@ int gg_pfb_step_part(gg_pf* h, const gg_step_args* args, int32_t resampler, int3
                      int32_t part);
 int gg_pfb_get_log_weights_raw(gg_pf* h, int32_t f, float* out);
 int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out);
+/* Launch shapes of the device-driven step (no reference counterpart):
+ * weight_tpw tiles of 1024 particles per workgroup of the predict + likelihood
+ * kernel (1: one tile per workgroup, one round over the batch; > 1: fewer
+ * workgroups streaming tiles with the next tile prefetched).  Results do not
+ * depend on the shapes; only the speed does. */
+int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw);
+int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw);
 
 /* --- batched Gauss-Newton WLS ---------------------------------------------- */
 /* reference: wls_batch, src/positioning/wls.cu:172-278 (Sagnac range model).

@@ -29,7 +29,9 @@
 #define GG_MAX_ISB 3
 #define GG_MAX_D (4 + GG_MAX_ISB)
 #define GG_NVEL 9
-#define GG_NPART 10  // doubles per tile partial: m, S, S2, sum[GG_MAX_D]
+// doubles per tile partial: m, c S, c^2 S2, c sum[GG_MAX_D], k, c (see gg_tile_scale)
+#define GG_NPART 12
+#define GG_REC 3  // float4 per staged satellite record (gg_sat_view)
 
 #define GG_PURPOSE_INIT 1u
 #define GG_PURPOSE_PREDICT 2u

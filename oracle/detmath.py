@@ -154,15 +154,42 @@ def kcount(c, n, u0):
     return np.where(np.isnan(c), 0, k)
 
 
+def tile_scale(m):
+    """Twin of gg_tile_scale: k_b = floor(m_b / ln 2) and c_b = exp64((m_b - k_b
+    ln2_hi) - k_b ln2_lo) in [1, 2), so that exp(lw_i) is proportional to
+    e_i c_b 2^k_b for e_i = exp32(lw_i - m_b); empty tiles (m_b = -inf) get
+    k = -2^30, c = 0; a non-finite maximum k = 0, c = 1."""
+    m = np.asarray(m, dtype=np.float64)
+    k = np.zeros_like(m)
+    c = np.ones_like(m)
+    empty = ~(m > -np.inf)
+    fin = ~empty & (m < np.inf)
+    k[empty] = -1073741824.0
+    c[empty] = 0.0
+    kf = np.floor(m[fin] * _INV_LN2)
+    k[fin] = kf
+    c[fin] = exp64((m[fin] - kf * _LN2_HI) - kf * _LN2_LO)
+    return k, c
+
+
+def tile_shift(v, dk):
+    """Twin of gg_tile_shift: 2^dk v, exactly, for dk >= -1000, else 0."""
+    v = np.asarray(v, dtype=np.float64)
+    dk = np.asarray(dk, dtype=np.float64)
+    ok = dk >= -1000.0
+    return np.where(ok, np.ldexp(v, np.where(ok, dk, 0.0).astype(np.int32)), 0.0)
+
+
 def decide(m, S):
-    """Twin of gg_combine: M, g_b = exp64(m_b - M), the tile offsets P_b and
-    the total W in the fixed chunked order."""
+    """Twin of gg_combine: M = max m_b, K = max k_b, the exact power-of-two
+    tile scales g_b = 2^(k_b - K) c_b, T_b = 2^(k_b - K) (c_b S_b), the tile
+    offsets P_b and the total W in the fixed chunked order."""
     nt = m.shape[0]
     M = float(np.max(m))
-    g = np.zeros(nt)
-    ok = m > -np.inf
-    g[ok] = exp64(m[ok] - M)
-    T = S * g
+    k, c = tile_scale(m)
+    K = float(np.max(k))
+    g = tile_shift(c, k - K)
+    T = tile_shift(S * c, k - K)
     c = (nt + BLOCK - 1) // BLOCK
     TT = np.zeros((BLOCK, c))
     idx = np.arange(nt)
@@ -259,7 +286,8 @@ def megopolis_ancestors(lw, n, n_iter, seed, call, lw_zero=False):
     """Megopolis of Chesser et al. (2021) as implemented by gg_k_mego: Metropolis
     chains on the original weights, one offset per 256-particle segment; a
     chain at k moves to proposal j when u * w_k < w_j (binary32 product),
-    w = megopolis_weights."""
+    w = megopolis_weights; iteration b's uniform is output b of the chain's
+    xoshiro128+ stream."""
     w = megopolis_weights(lw, n, lw_zero)
     i = np.arange(n, dtype=np.int64)
     seg = i // 256
@@ -267,13 +295,11 @@ def megopolis_ancestors(lw, n, n_iter, seed, call, lw_zero=False):
     wk = w.copy()
     nseg = int(seg.max()) + 1 if n else 0
     segs = np.arange(nseg)
-    ru = None
+    xs = philox.xo_seed(seed, i, call, philox.PURPOSE_MEGO_U, 0)  # one stream per chain
     for b in range(n_iter):
         r = philox.draw(seed, segs, call, philox.PURPOSE_MEGO, b)
         O = ((r[0] * np.uint64(n)) >> np.uint64(32)).astype(np.int64)
-        if b & 3 == 0:  # one block per particle serves four iterations
-            ru = philox.draw(seed, i, call, philox.PURPOSE_MEGO_U, b >> 2)
-        u = philox.u01f(ru[b & 3])
+        u = philox.u01f(philox.xo_next(xs))
         j = i + O[seg]
         j = np.where(j >= n, j - n, j)
         wj = w[j]

@@ -73,14 +73,56 @@ def normal4(seed, idx, step, purpose, call):
     return np.stack(out, axis=-1)
 
 
+def xo_seed(seed, idx, step, purpose, call):
+    """xoshiro128+ state (Blackman and Vigna, ACM TOMS 2021) seeded by the
+    Philox block (seed, idx, step, purpose, call); the all-zero state, the
+    generator's fixed point, becomes (1, 0, 0, 0) (twin of gg_xo_seed).
+    Returns a list of four uint64 arrays holding 32-bit words."""
+    s0, s1, s2, s3 = draw(seed, idx, step, purpose, call)
+    zero = (s0 | s1 | s2 | s3) == 0
+    return [s0 | zero.astype(np.uint64), s1, s2, s3]
+
+
+def xo_next(st):
+    """Next xoshiro128+ output of every stream in st (advanced in place):
+    out = s0 + s3, then the xorshift-rotate state update (twin of gg_xo_next)."""
+    s0, s1, s2, s3 = st
+    out = (s0 + s3) & M32
+    t = (s1 << np.uint64(9)) & M32
+    s2 = s2 ^ s0
+    s3 = s3 ^ s1
+    s1 = s1 ^ s2
+    s0 = s0 ^ s3
+    s2 = s2 ^ t
+    s3 = ((s3 << np.uint64(11)) | (s3 >> np.uint64(21))) & M32
+    st[:] = [s0, s1, s2, s3]
+    return out
+
+
+def xo_normals(st, count):
+    """count (even) standard normals per stream: consecutive Box-Muller pairs of
+    the streams' next outputs (binary64 arithmetic on the exact binary32
+    uniforms).  Returns [streams, count]."""
+    out = np.empty((np.asarray(st[0]).size, count))
+    for k in range(0, count, 2):
+        a = u01f(xo_next(st)).astype(np.float64)
+        b = u01f(xo_next(st)).astype(np.float64)
+        rad = np.sqrt(-2.0 * np.log(a))
+        out[:, k] = rad * np.cos(2.0 * np.pi * b)
+        out[:, k + 1] = rad * np.sin(2.0 * np.pi * b)
+    return out
+
+
 def predict_normals(seed, n, step, n_isb):
-    """The normals one predict call draws per particle i (gg_predict4 /
-    gg_predict_one): z [n, 4] position noise x, y, z (local ENU frame) and the
-    clock bias; zv [n, 3] velocity noise (local frame); zi [n, n_isb] the
-    inter-system-bias random walk."""
-    i = np.arange(int(n))
-    z = normal4(seed, i, step, PURPOSE_PREDICT, 0)
-    zv = normal4(seed, i, step, PURPOSE_PREDICT, 1)[:, :3]
-    zi = normal4(seed, i, step, PURPOSE_PREDICT, 2)[:, :n_isb]
-    return z, zv, zi
+    """The normals one predict call draws (gg_pred_normals): particles 4g..4g+3
+    share the stream (seed, g, step, PURPOSE_PREDICT, call).  Call 0 gives
+    particle 4g + q the normals 4 q D .. of its components (x, y, z in the
+    local ENU frame, clock bias, ISBs; D = 4 + n_isb), call 1 its three
+    velocity-noise normals.  Returns z [n, 4], zv [n, 3], zi [n, n_isb]."""
+    n = int(n)
+    D = 4 + int(n_isb)
+    g = np.arange((n + 3) // 4)
+    zp = xo_normals(xo_seed(seed, g, step, PURPOSE_PREDICT, 0), 4 * D).reshape(-1, D)[:n]
+    zv = xo_normals(xo_seed(seed, g, step, PURPOSE_PREDICT, 1), 12).reshape(-1, 3)[:n]
+    return zp[:, :4], zv, zp[:, 4:]
 

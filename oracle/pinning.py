@@ -20,10 +20,23 @@ PINS = {
     "philox.u01f": ("pinned", "range (0,1) and exact dyadic values (test_uniform_ranges_and_exactness)"),
     "philox.u01d": ("pinned", "range (0,1] and 53-bit exactness (test_uniform_ranges_and_exactness)"),
     "philox.normal4": ("pinned", "moments of 4e5 draws (test_normals_are_standard)"),
-    "philox.predict_normals": ("pinned", "stream layout of the predict draws; moments through "
-                               "test_oracle_predict_velocity_noise_moments and the RBPF noise "
-                               "covariance test; GPU particle clouds equal the oracle's"),
+    "philox.predict_normals": ("pinned", "moments and independence across components, group "
+                               "slots and groups (test_predict_normals_moments_and_independence); "
+                               "velocity-noise moments, RBPF noise covariance; GPU particle "
+                               "clouds equal the oracle's"),
+    "philox.xo_seed": ("pinned", "Philox KATs for the seed block; zero-state rule "
+                       "(test_xoshiro_outputs_and_seed_rules); GPU bit-exact (gg_debug_math op 3)"),
+    "philox.xo_next": ("pinned", "state transition T with T^(2^64) equal to the published jump "
+                       "polynomial of xoshiro128+ (test_xoshiro_transition_matches_published_"
+                       "jump_polynomial); hand-computed first outputs; uniformity and lag-1 "
+                       "correlation; GPU bit-exact"),
+    "philox.xo_normals": ("pinned", "Box-Muller of xo_next uniforms, moments through "
+                          "test_predict_normals_moments_and_independence"),
     # detmath.py
+    "detmath.tile_scale": ("pinned", "c 2^k = exp(m) against libm, c in [1, 2), edge cases "
+                           "(test_tile_scale_and_shift_exact)"),
+    "detmath.tile_shift": ("pinned", "exact against rational arithmetic; 0 below 2^-1000 "
+                           "(test_tile_scale_and_shift_exact)"),
     "detmath.exp64": ("pinned", "<= 2 ulp against libm over [-745, 0] (test_exp64_against_libm)"),
     "detmath.exp32": ("pinned", "<= 4 ulp (binary32) against binary64 exp "
                       "(test_exp32_against_float64_exp)"),

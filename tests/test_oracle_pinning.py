@@ -37,6 +37,80 @@ def test_philox_known_answers(ctr, key, expect):
     assert tuple(int(v) for v in out) == expect
 
 
+def _xo_transition_matrix():
+    """The xoshiro128+ state update as a 128 x 128 matrix over GF(2), column k
+    the update of the state with only bit k set (bit 32 w + b = bit b of s_w)."""
+    T = np.zeros((128, 128), dtype=np.int64)
+    for k in range(128):
+        st = [np.zeros(1, dtype=np.uint64) for _ in range(4)]
+        st[k // 32][0] = np.uint64(1) << np.uint64(k % 32)
+        philox.xo_next(st)
+        for w in range(4):
+            v = int(st[w][0])
+            for b in range(32):
+                T[32 * w + b, k] = (v >> b) & 1
+    return T
+
+
+def test_xoshiro_transition_matches_published_jump_polynomial():
+    """The published jump of xoshiro128+ (Blackman and Vigna's reference code,
+    JUMP = {0x8764000b, 0xf542d2d3, 0x6fa035c3, 0x77f2db5b}) advances the state
+    by 2^64 steps: sum_b c_b T^b must equal T^(2^64).  A wrong shift, rotation
+    or xor order in xo_next changes T and fails."""
+    T = _xo_transition_matrix()
+    P = T.copy()
+    for _ in range(64):
+        P = (P @ P) % 2
+    J = np.zeros_like(T)
+    Tb = np.eye(128, dtype=np.int64)
+    for i, word in enumerate((0x8764000B, 0xF542D2D3, 0x6FA035C3, 0x77F2DB5B)):
+        for b in range(32):
+            if (word >> b) & 1:
+                J = (J + Tb) % 2
+            Tb = (Tb @ T) % 2
+    np.testing.assert_array_equal(J, P)
+
+
+def test_xoshiro_outputs_and_seed_rules():
+    """Output = s0 + s3 before the update (state (1, 2, 3, 4) gives 5, then
+    7 + 12288 after one update by hand: s2 = 3^1, s3 = 4^2, s1 = 2^2, s0 = 1^6,
+    s2 ^= 2 << 9, s3 = rotl(6, 11)); the all-zero seed becomes (1, 0, 0, 0);
+    uniforms from the top bits are uniform (chi-square over 64 bins of 1e6
+    draws) and consecutive outputs uncorrelated."""
+    st = [np.array([v], dtype=np.uint64) for v in (1, 2, 3, 4)]
+    assert int(philox.xo_next(st)[0]) == 5
+    assert [int(v[0]) for v in st] == [7, 0, 1026, 12288]
+    assert int(philox.xo_next(st)[0]) == 7 + 12288
+    z = [np.zeros(1, dtype=np.uint64) for _ in range(4)]
+    zero = (z[0] | z[1] | z[2] | z[3]) == 0
+    assert int((z[0] | zero.astype(np.uint64))[0]) == 1
+    xs = philox.xo_seed(7, np.arange(1000), 3, philox.PURPOSE_MEGO_U, 0)
+    u = np.stack([philox.u01f(philox.xo_next(xs)) for _ in range(1000)], axis=1).astype(np.float64)
+    counts = np.histogram(u.ravel(), bins=64, range=(0.0, 1.0))[0]
+    chi2 = float(np.sum((counts - u.size / 64) ** 2 / (u.size / 64)))
+    assert chi2 < 63 + 5 * np.sqrt(2 * 63)
+    c = np.corrcoef(u[:, :-1].ravel(), u[:, 1:].ravel())[0, 1]
+    assert abs(c) < 5.0 / np.sqrt(u.size)
+
+
+def test_predict_normals_moments_and_independence():
+    """Group streams: every component and particle slot is N(0, 1), and the
+    normals of different components, of the four particles of one group and of
+    neighbouring groups are uncorrelated."""
+    n = 400_000
+    z, zv, zi = philox.predict_normals(99, n, 5, 3)
+    allz = np.concatenate([z, zi, zv], axis=1)
+    np.testing.assert_allclose(allz.mean(axis=0), 0.0, atol=0.01)
+    np.testing.assert_allclose(allz.std(axis=0), 1.0, atol=0.01)
+    C = np.corrcoef(allz.T)
+    assert np.max(np.abs(C - np.eye(C.shape[0]))) < 0.01
+    for q in range(4):  # slot q of each group vs slot q + 1, and group g vs g + 1
+        a = z[q::4, 0]
+        b = z[(q + 1) % 4::4, 0][: a.size]
+        assert abs(np.corrcoef(a, b)[0, 1]) < 0.01
+    assert abs(np.corrcoef(z[:-4, 1], z[4:, 1])[0, 1]) < 0.01
+
+
 def test_uniform_ranges_and_exactness():
     x = np.array([0, 1, 255, 256, 0xFFFFFFFF], dtype=np.uint64)
     u = philox.u01f(x)
@@ -102,6 +176,33 @@ def test_block_scan_close_to_exact_rational():
 
 
 # ---------------------------------------------------- systematic resample ----
+def test_tile_scale_and_shift_exact():
+    """tile_scale: c 2^k = exp(m) to binary64 accuracy with c in [1, 2) (libm's
+    exp as the reference); tile_shift: exactly v 2^dk (rational arithmetic) for
+    dk >= -1000 and 0 below; decide: g_b / exp(m_b - M) is one constant over
+    the tiles, i.e. the tiles keep their exact relative weights."""
+    rng = np.random.default_rng(4)
+    m = np.concatenate([rng.uniform(-700, 700, 2000), [0.0, -0.5, 1e-300, -1e-9]])
+    k, c = detmath.tile_scale(m)
+    assert np.all((c >= 1.0) & (c < 2.0 + 1e-15))
+    np.testing.assert_array_equal(k, np.floor(k))
+    got = c * np.exp2(np.minimum(k, 1000.0)) * np.exp2(np.maximum(k - 1000.0, 0.0))
+    ref = np.array([math.exp(x) for x in m])
+    np.testing.assert_allclose(got, ref, rtol=3e-16, atol=0)
+    k2, c2 = detmath.tile_scale(np.array([-np.inf, np.inf]))
+    assert list(k2) == [-1073741824.0, 0.0] and list(c2) == [0.0, 1.0]
+    v = rng.uniform(1.0, 2048.0, 500)
+    dk = rng.integers(-1000, 1, 500).astype(np.float64)
+    out = detmath.tile_shift(v, dk)
+    for a, b, o in zip(v[:100], dk[:100], out[:100]):
+        assert Fraction(o) == Fraction(a) * Fraction(2) ** int(b)
+    assert np.all(detmath.tile_shift(v, dk - 1001.0) == 0.0)
+    mb = rng.uniform(-80.0, 0.0, 300).astype(np.float32).astype(np.float64)
+    M, g, P, W = detmath.decide(mb, np.ones(300))
+    ratio = g / np.exp(mb - M)
+    np.testing.assert_allclose(ratio, ratio[0], rtol=4e-15)
+
+
 def test_reference_u0_int32_product():
     """pf_device.cu:1852 multiplies 1000000 * N in int: N=10000 wraps."""
     assert detmath.reference_u0(42, 100) == 42 / 1e8

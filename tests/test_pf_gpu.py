@@ -57,6 +57,11 @@ def test_device_exp_and_philox_bit_exact(gpu):
     _lib.call("gg_debug_math", 2, _lib.dptr(idx), _lib.dptr(out), 1000)
     ref = philox.draw(0x123456789ABCDEF0, np.arange(1000), 7, 3, 1)
     np.testing.assert_array_equal(out.reshape(1000, 4).astype(np.uint64), np.stack(ref, axis=1))
+    out = np.empty(8000)
+    _lib.call("gg_debug_math", 3, _lib.dptr(idx), _lib.dptr(out), 1000)
+    st = philox.xo_seed(0x123456789ABCDEF0, np.arange(1000), 7, 3, 1)
+    ref = np.stack([philox.xo_next(st) for _ in range(8)], axis=1)
+    np.testing.assert_array_equal(out.reshape(1000, 8).astype(np.uint64), ref)
     _ = C
 
 
