@@ -132,7 +132,7 @@ class BatchedParticleFilter:
 
     def step_part(self, args, part, resampler="systematic", n_iter=32):
         """Half of step(): part 0 the likelihood launch (predict + weight + tile
-        partials), part 1 the decision, resampling and finalize."""
+        partials), part 1 the decision and resampling (with its finalize)."""
         call("gg_pfb_step_part", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(part))
 
@@ -167,19 +167,19 @@ class BatchedParticleFilter:
     def profile_steps(self, args, n_steps, resampler="systematic", n_iter=32):
         """Eager steps with hipEvents around each launch group.  Returns
         ({stage: ms} for steps that resampled, {stage: ms} for the others,
-        (n_resampled, n_other)); stages are predict_weight, decide_cdf, gather,
-        finalize (systematic) or predict_weight, decide, megopolis_gather,
-        finalize."""
-        ms = np.zeros(8, dtype=np.float32)
+        (n_resampled, n_other)); stages are predict_weight, decide_cdf, gather
+        (systematic) or predict_weight, decide, megopolis_gather; the gather
+        includes the resampling's finalize."""
+        ms = np.zeros(6, dtype=np.float32)
         cnt = np.zeros(2, dtype=np.int32)
         call("gg_pfb_profile_steps", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(n_steps), ms.ctypes.data_as(C.POINTER(C.c_float)),
              cnt.ctypes.data_as(C.POINTER(C.c_int32)))
         if resampler == "systematic":
-            names = ("predict_weight", "decide_cdf", "gather", "finalize")
+            names = ("predict_weight", "decide_cdf", "gather")
         else:
-            names = ("predict_weight", "decide", "megopolis_gather", "finalize")
-        return (dict(zip(names, map(float, ms[:4]))), dict(zip(names, map(float, ms[4:]))),
+            names = ("predict_weight", "decide", "megopolis_gather")
+        return (dict(zip(names, map(float, ms[:3]))), dict(zip(names, map(float, ms[3:]))),
                 (int(cnt[0]), int(cnt[1])))
 
     def time_stage(self, args, stage, reps, n_iter=32):

@@ -37,6 +37,7 @@ struct gg_pf {
   double* Pb = nullptr;
   int32_t* emax = nullptr;
   double* csum = nullptr;
+  uint32_t* fin = nullptr;
   double* scratch_f = nullptr;
   gg_filter* filt = nullptr;
   gg_dyn* dyn = nullptr;
@@ -102,6 +103,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.E = h->E;
   a.mw = reinterpret_cast<float*>(h->E);  // Megopolis steps do not use the ends
   a.csum = h->csum;
+  a.fin = h->fin;
   a.scratch_f = h->scratch_f;
   a.filt = h->filt;
   a.dyn = h->dyn;
@@ -327,23 +329,16 @@ int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
   return 0;
 }
 
-int launch_finalize(gg_pf* h, int force) {
-  const gg_arena a = arena_of(h);
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_finalize<NI>), dim3(h->n_filters),
-                                                dim3(GG_BLOCK), 0, h->stream, a, force));
-  GG_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf,
-                  bool with_finalize = true) {
+// Resampling gather; its last work item per filter finalises the resampling
+// (gg_gather_finish), so no launch follows it.
+int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf) {
   const gg_arena a = arena_of(h);
   const size_t smem = (anc_mode == 0) ? gather_smem(h) : 0;
   GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_gather<NI>), dim3(h->n_chunks),
                                                 dim3(GG_BLOCK), smem, h->stream, a, s, force,
                                                 anc_mode, src_buf));
   GG_HIP(hipGetLastError());
-  return with_finalize ? launch_finalize(h, force) : 0;
+  return 0;
 }
 
 int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
@@ -372,33 +367,32 @@ int launch_mego(gg_pf* h, int n_iter, int force) {
 int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
   if (resampler == 1) {
     if (launch_mego(h, n_iter, 0)) return -1;
-    return launch_gather(h, s, 0, 1, 1, false);
+    return launch_gather(h, s, 0, 1, 1);
   }
   int fb = 0;
-  if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0, false);
+  if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0);
   if (launch_mego_ref(h, n_iter, 0, 0, 0, &fb)) return -1;
-  return launch_gather(h, s, 0, 1, fb, false);
+  return launch_gather(h, s, 0, 1, fb);
 }
 
-// Resampling part of a device-driven step, after the weight launch, up to (not
-// including) finalize.  Systematic: the CDF kernel takes the decision in every
+// Resampling part of a device-driven step, after the weight launch (the
+// gather's last work item per filter finalises).  Systematic: the CDF kernel takes the decision in every
 // workgroup (and commits the predict's frame), so no decision kernel runs.
 int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
   if (resampler == 0) {
     if (launch_cdf(h, s, 0, 1)) return -1;
-    return launch_gather(h, s, 0, 0, 0, false);
+    return launch_gather(h, s, 0, 0, 0);
   }
   if (launch_decide(h, s, 3)) return -1;
   return enqueue_resample_after_decide(h, s, resampler, n_iter);
 }
 
 // One device-driven step in two parts (no host synchronisation): part 0 is the
-// likelihood launch (predict + weight + tile partials), part 1 the decision,
-// resampling and finalize.
+// likelihood launch (predict + weight + tile partials), part 1 the decision
+// and resampling.
 int enqueue_step_part(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, int part) {
   if (part == 0) return launch_weight(h, s, true, false);
-  if (enqueue_resample(h, s, resampler, n_iter)) return -1;
-  return launch_finalize(h, 0);
+  return enqueue_resample(h, s, resampler, n_iter);
 }
 
 int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
@@ -511,6 +505,7 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
   if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
+  if (!rc) rc = dev_alloc(&h->fin, (size_t)n_filters * GG_ARRIVE_WORDS);
   if (!rc) rc = dev_alloc(&h->scratch_f, (size_t)n_filters * 16);
   if (!rc) rc = dev_alloc(&h->filt, (size_t)n_filters);
   if (!rc) rc = dev_alloc(&h->dyn, (size_t)n_filters);
@@ -556,7 +551,7 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
   void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->p4, h->g, h->Pb,
-                  h->emax,  h->csum,  h->scratch_f, h->filt,
+                  h->emax,  h->csum, h->fin, h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,
                   h->io_tmp, h->obsd};
@@ -1565,33 +1560,31 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
   if (!h || !args || !stage_ms || !counts) return gg_fail(-2, "null argument");
   if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
   if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int rc = 0;
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) rc = gg_fail(-1, "hipEventCreate failed");
-  double acc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
   int cnt[2] = {0, 0};
   std::vector<gg_dyn> dv(h->n_filters);
   const gg_step_args s = *args;
   for (int k = 0; rc == 0 && k < n_steps; ++k) {
-    // stages: weight | decision (+ CDF for systematic) | gather (+ Megopolis) | finalize
+    // stages: weight | decision (+ CDF for systematic) | gather (+ Megopolis, finalize)
     bool bad = hipEventRecord(ev[0], h->stream) != hipSuccess || launch_weight(h, s, true, false) ||
                hipEventRecord(ev[1], h->stream) != hipSuccess;
     if (!bad) {
       if (resampler == 0)
         bad = launch_cdf(h, s, 0, 1) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
-              launch_gather(h, s, 0, 0, 0, false);
+              launch_gather(h, s, 0, 0, 0);
       else
         bad = launch_decide(h, s, 3) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
               enqueue_resample_after_decide(h, s, resampler, n_iter);
     }
-    if (bad ||
-        hipEventRecord(ev[3], h->stream) != hipSuccess || launch_finalize(h, 0) ||
-        hipEventRecord(ev[4], h->stream) != hipSuccess) {
+    if (bad || hipEventRecord(ev[3], h->stream) != hipSuccess) {
       rc = gg_fail(-1, "profiled step launch failed");
       break;
     }
-    if (hipEventSynchronize(ev[4]) != hipSuccess ||
+    if (hipEventSynchronize(ev[3]) != hipSuccess ||
         hipMemcpy(dv.data(), h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost) !=
             hipSuccess) {
       rc = gg_fail(-1, "profiled step failed");
@@ -1601,7 +1594,7 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
     for (const auto& d : dv) any |= d.did_resample;
     const int r = any ? 0 : 1;
     cnt[r] += 1;
-    for (int st = 0; st < 4; ++st) {
+    for (int st = 0; st < 3; ++st) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, ev[st], ev[st + 1]) != hipSuccess) {
         rc = gg_fail(-1, "hipEventElapsedTime failed");
@@ -1613,8 +1606,8 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   for (int r = 0; r < 2; ++r)
-    for (int st = 0; st < 4; ++st)
-      stage_ms[r * 4 + st] = cnt[r] ? (float)(acc[r][st] / cnt[r]) : 0.0f;
+    for (int st = 0; st < 3; ++st)
+      stage_ms[r * 3 + st] = cnt[r] ? (float)(acc[r][st] / cnt[r]) : 0.0f;
   counts[0] = cnt[0];
   counts[1] = cnt[1];
   h->decide_valid = h->ess_valid = h->est_valid = false;
@@ -1632,9 +1625,9 @@ int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t
       case 0: rc = launch_weight(h, *args, true, false); break;
       case 1: rc = launch_weight(h, *args, false, false); break;
       case 2: rc = launch_cdf(h, *args, 1, 1); break;
-      case 3: rc = launch_gather(h, *args, 1, 0, 0, false); break;
+      case 3: rc = launch_gather(h, *args, 1, 0, 0); break;
       case 4: rc = launch_mego(h, n_iter, 1); break;
-      case 5: rc = launch_gather(h, *args, 1, 1, 1, false); break;
+      case 5: rc = launch_gather(h, *args, 1, 1, 1); break;
       case 6: rc = launch_decide(h, *args, 0); break;
       default: return gg_fail(-2, "unknown stage");
     }

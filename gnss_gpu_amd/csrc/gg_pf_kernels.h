@@ -5,7 +5,7 @@
 // particles and every SoA load/store is a 16-byte vector access that the
 // wavefront coalesces into one contiguous 1 KiB transaction.
 //
-// The systematic step for a batch of filters is four launches, each over all
+// The systematic step for a batch of filters is three launches, each over all
 // filters:
 //   gg_k_weight    predict (optional) + pseudorange likelihood + per-tile
 //                  partials (max, sums of exp32(lw - m_b), ESS and mean sums).
@@ -15,17 +15,46 @@
 //                  and estimate; when resampling, per-tile binary64 CDF and
 //                  output ends.
 //   gg_k_gather    merge-path partition of (ends, outputs), ancestor indices,
-//                  state gather.
-//   gg_k_finalize  mean of the resampled particles, buffer swap.
-// Kernel boundaries are the only cross-workgroup synchronisation: the L2s of
-// the 8 XCDs are not coherent, and an agent-scope fence in every workgroup
-// (L2 writeback + invalidate) costs more than a launch inside a graph.
+//                  state gather; the last work item of each filter to finish
+//                  adds the chunk sums and finalises (estimate, buffer swap).
+// Besides that counter (device-coherent atomics, no L2 writeback), kernel
+// boundaries are the only cross-workgroup synchronisation: the L2s of the 8
+// XCDs are not coherent, an agent-scope release in every workgroup writes back
+// its XCD's whole L2, and waiting on other workgroups would need every one of
+// them resident.
 // Megopolis: gg_k_weight, gg_k_decide (one workgroup per filter), gg_k_mego
-// (or B launches of gg_k_mego_ref), the ancestor-driven gather, finalize.
+// (or B launches of gg_k_mego_ref), the ancestor-driven gather.
 #pragma once
 
 #include "gg_device.h"
 #include "gg_mlp.h"
+
+// Last-arrival detection without waiting: each of n_items work items calls
+// gg_arrive_last once (one thread); exactly one call, the last to arrive,
+// returns true.  Two levels of device-scope atomic counters (32 groups by
+// item mod 32, then the groups) keep the arrivals on one address to about
+// n_items / 32: a single counter serialises every arrival (about 11 ns each
+// here).  Each counter sits in its own 128-byte line and is back at zero when
+// the last arrival returns.  Callers make their data visible first: stores
+// through the device-coherent level (device-scope atomic stores) completed
+// before the call.
+#define GG_ARRIVE_SUBS 32
+#define GG_ARRIVE_STRIDE 32  // uint32 per counter (128 bytes)
+#define GG_ARRIVE_WORDS ((GG_ARRIVE_SUBS + 1) * GG_ARRIVE_STRIDE)
+
+__device__ __forceinline__ bool gg_arrive_last(uint32_t* c, int item, int n_items) {
+  const int j = item % GG_ARRIVE_SUBS;
+  const uint32_t group = (uint32_t)((n_items - 1 - j) / GG_ARRIVE_SUBS + 1);
+  uint32_t* sub = c + GG_ARRIVE_STRIDE * (1 + j);
+  if (__hip_atomic_fetch_add(sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u != group)
+    return false;
+  __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t groups = (uint32_t)(n_items < GG_ARRIVE_SUBS ? n_items : GG_ARRIVE_SUBS);
+  if (__hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u != groups)
+    return false;
+  __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
 
 struct gg_arena {
   float* state;           // [2][ncomp][P]
@@ -38,7 +67,8 @@ struct gg_arena {
   double* g;              // [n_tiles_total]
   double* Pb;             // [n_tiles_total]
   int32_t* emax;          // [n_tiles_total] tile maxima of E
-  double* csum;           // [n_chunks_total][GG_MAX_D]
+  double* csum;           // [n_chunks_total][GG_MAX_D]: state sums of each gather chunk
+  uint32_t* fin;          // [n_filters][GG_ARRIVE_WORDS]: gather arrival counters
   double* scratch_f;      // [n_filters][16] misc per-filter results
   const gg_filter* filt;  // [n_filters]
   gg_dyn* dyn;            // [n_filters]

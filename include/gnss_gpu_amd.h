@@ -136,10 +136,10 @@ int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total);
 /* Device time of each of n_replays graph launches (ms_each[n_replays]). */
 int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each);
 /* n_steps eager steps with hipEvents around each launch group on the handle's
- * stream: stage_ms[0..3] = mean ms of (predict+weight, decision [+ CDF for
- * systematic], gather [+ Megopolis], finalize) over steps in which some filter
- * resampled, stage_ms[4..7] over the other steps; counts[0..1] = the two step
- * counts. */
+ * stream: stage_ms[0..2] = mean ms of (predict+weight, decision [+ CDF for
+ * systematic], gather [+ Megopolis] with its finalize) over steps in which some
+ * filter resampled, stage_ms[3..5] over the other steps; counts[0..1] = the two
+ * step counts. */
 int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
                          int32_t n_steps, float* stage_ms, int32_t* counts);
 /* Device time of `reps` launches of one kernel: 0 weight+predict, 1 weight,
@@ -154,7 +154,7 @@ int gg_pfb_get_particles(gg_pf* h, int32_t f, double* out);
 int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out);
 /* Inspection of the device-driven step (no reference counterpart):
  * gg_pfb_step_part(.., 0) enqueues the step's likelihood launch (predict +
- * weight + tile partials) and (.., 1) the rest (decision, resampling,
+ * weight + tile partials) and (.., 1) the rest (decision, resampling and its
  * finalize); part 0 then part 1 is exactly gg_pfb_step.
  * gg_pfb_get_log_weights_raw copies filter f's binary32 log-weight buffer as
  * the last likelihood launch wrote it (no zeroed-weights flag applied);

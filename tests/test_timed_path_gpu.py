@@ -5,7 +5,7 @@ The timed step is BatchedParticleFilter's device-driven step on one filter of
 the multi-GNSS workload (GPS/Galileo/BeiDou/GLONASS, 3 satellites each, state
 [e n u cb isb x3], Cauchy scale 5 m, 30% NLOS): the fused predict + likelihood
 + tile-partial launch, then the decision and the resampler (systematic, or
-Megopolis B = 32), then finalize.  Tests drive it through step_part, which
+Megopolis B = 32) whose gather finalises.  Tests drive it through step_part, which
 splits gg_pfb_step after its likelihood launch so that the binary32
 log-weights the resampler consumes can be read.
 
