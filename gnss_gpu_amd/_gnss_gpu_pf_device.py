@@ -333,6 +333,23 @@ def pf_device_sync(state):
         call("gg_pf_device_sync", state.handle)
 
 
+def stream_handle(stream):
+    """hipStream_t of ``stream``: a torch.cuda.Stream, a raw handle (int), or
+    None (the filter's own stream)."""
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(int(stream.cuda_stream))
+    return C.c_void_p(int(stream))
+
+
+def pf_device_set_stream(state, stream=None):
+    """Enqueue this filter's work on ``stream`` from now on (after finishing
+    what is queued on its current stream); None returns to a stream of its own.
+    Calls that hand results back to the host synchronise that stream."""
+    call("gg_pf_set_stream", state.handle, stream_handle(stream))
+
+
 def _not_in_scope(name):
     def fn(*args, **kwargs):
         raise NotImplementedError(

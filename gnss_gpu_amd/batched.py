@@ -194,6 +194,13 @@ class BatchedParticleFilter:
     def sync(self):
         call("gg_pf_device_sync", self._h)
 
+    def set_stream(self, stream=None):
+        """Enqueue steps, captures and replays on ``stream`` (torch.cuda.Stream
+        or raw hipStream_t) from now on; None: the handle's own stream."""
+        from ._gnss_gpu_pf_device import stream_handle
+
+        call("gg_pf_set_stream", self._h, stream_handle(stream))
+
     # -- results ---------------------------------------------------------------
     def read_dyn(self):
         arr = (_lib.Dyn * self.n_filters)()

@@ -1484,7 +1484,6 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
                    int32_t steps) {
   if (!h || !args) return gg_fail(-2, "null argument");
   if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
-  if (!h->own_stream) return gg_fail(-2, "graph capture needs the handle's own stream");
   if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
   GG_HIP(hipStreamSynchronize(h->stream));
   if (h->graph_exec) {
@@ -1497,11 +1496,21 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
   }
   // allocations cannot happen inside the capture
   if (args->lik == GG_LIK_LEARNED && args->learned_mode == 1 && ensure_table(h)) return -1;
-  GG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  // captured on a private stream, so that a caller's stream (gg_pf_set_stream)
+  // never holds a capture; the graph replays on whichever stream is in use
+  hipStream_t cs = nullptr;
+  GG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipStream_t run = h->stream;
+  h->stream = cs;
   int rc = 0;
-  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter);
   hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter);
+    e = hipStreamEndCapture(cs, &g);
+  }
+  h->stream = run;
+  (void)hipStreamDestroy(cs);
   if (rc) {
     if (g) hipGraphDestroy(g);
     return -1;

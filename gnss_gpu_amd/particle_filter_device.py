@@ -486,6 +486,14 @@ class ParticleFilterDevice:
         if getattr(self, "_state", None) is not None:
             _dev.pf_device_sync(self._state)
 
+    def set_stream(self, stream=None):
+        """Run this filter's kernels and copies on ``stream`` (a
+        torch.cuda.Stream or a raw hipStream_t), in order with the caller's
+        other work there; None returns to the filter's own stream.  Work
+        already queued is finished first.  Methods that return host values
+        synchronise the stream in use."""
+        _dev.pf_device_set_stream(self._state, stream)
+
     # -- smoother (not part of the forward step on this backend) ----------------
     def enable_smoothing(self):
         raise NotImplementedError("forward-backward smoothing is not implemented on this backend")

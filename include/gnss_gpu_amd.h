@@ -127,7 +127,9 @@ int gg_pfb_load_epochs(gg_pf* h, int32_t n_slots, const double* sat_xyz, const d
                        const double* vel);
 /* resampler: 0 systematic, 1 Megopolis (Chesser), 2 Megopolis (reference rule) */
 int gg_pfb_step(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter);
-/* Capture `steps` consecutive steps in one hipGraph (replaced on each call). */
+/* Capture `steps` consecutive steps in one hipGraph (replaced on each call).
+ * The capture runs on a private stream; gg_pfb_replay launches the graph on
+ * the stream in use (the handle's own or the one given to gg_pf_set_stream). */
 int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
                    int32_t steps);
 int gg_pfb_replay(gg_pf* h, int32_t n_replays);
