@@ -169,8 +169,9 @@ _SIGS = {
     "gg_pfb_step_part": [_P, C.POINTER(StepArgs), _I, _I, _I],
     "gg_pfb_get_log_weights_raw": [_P, _I, _FP],
     "gg_pfb_get_ancestors": [_P, _I, _IP],
-    "gg_pfb_set_tuning": [_P, _I],
-    "gg_pfb_get_tuning": [_P, _IP],
+    "gg_pfb_set_tuning": [_P, _I, _I],
+    "gg_pfb_get_tuning": [_P, _IP, _IP],
+    "gg_pfb_autotune": [_P, C.POINTER(StepArgs), _I, _I, _I, _I, _I, _IP, _FP, _IP],
     "gg_wls_batch": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],
     "gg_wls_batch_device": [_P, _P, _P, _P, _I, _I, _I, _D, _P, _P, _I, _P],
     "gg_wls_batch_noalloc": [_DP, _DP, _DP, _IP, _I, _I, _I, _D, _DP, _IP],

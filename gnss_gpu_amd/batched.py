@@ -136,14 +136,34 @@ class BatchedParticleFilter:
         call("gg_pfb_step_part", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(part))
 
-    def set_tuning(self, weight_tpw):
-        """Launch shapes of the step (results do not depend on them)."""
-        call("gg_pfb_set_tuning", self._h, int(weight_tpw))
+    def set_tuning(self, weight_tpw=None, mego_depth=None):
+        """Launch shapes of the step (results do not depend on them); None
+        keeps a shape as it is.  Capture after changing them."""
+        cur = self.get_tuning()
+        tpw = cur["weight_tpw"] if weight_tpw is None else int(weight_tpw)
+        depth = cur["mego_depth"] if mego_depth is None else int(mego_depth)
+        call("gg_pfb_set_tuning", self._h, tpw, depth)
 
     def get_tuning(self):
-        w = C.c_int32()
-        call("gg_pfb_get_tuning", self._h, C.byref(w))
-        return {"weight_tpw": int(w.value)}
+        w, d = C.c_int32(), C.c_int32()
+        call("gg_pfb_get_tuning", self._h, C.byref(w), C.byref(d))
+        return {"weight_tpw": int(w.value), "mego_depth": int(d.value)}
+
+    def autotune(self, args, resampler="systematic", n_iter=32, steps=10, reps=3):
+        """Time every candidate launch shape on graphs of `steps` steps from a
+        snapshot of the filters (restored afterwards, so the trajectory is
+        unchanged) and keep the fastest.  Returns {"chosen": shapes,
+        "candidates": [(shapes, ms per step), ...]}.  Capture afterwards."""
+        cap = 16
+        cand = np.zeros(2 * cap, dtype=np.int32)
+        ms = np.zeros(cap, dtype=np.float32)
+        n = C.c_int32()
+        call("gg_pfb_autotune", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
+             int(steps), int(reps), cap, cand.ctypes.data_as(C.POINTER(C.c_int32)),
+             ms.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n))
+        rows = [({"weight_tpw": int(cand[2 * k]), "mego_depth": int(cand[2 * k + 1])},
+                 float(ms[k])) for k in range(n.value)]
+        return {"chosen": self.get_tuning(), "candidates": rows}
 
     def capture(self, args, resampler="systematic", n_iter=32, steps=1):
         call("gg_pfb_capture", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),

@@ -72,8 +72,10 @@ struct gg_pf {
   std::vector<double> init_isb;
   uint64_t seed0 = 0;
   // launch shapes of the step's kernels (gg_pfb_set_tuning / gg_pfb_autotune):
-  // tiles per workgroup of the likelihood kernel
+  // tiles per workgroup of the likelihood kernel, Megopolis iterations whose
+  // proposal loads are issued together
   int32_t weight_tpw = 1;
+  int32_t mego_depth = 1;
 };
 
 namespace {
@@ -357,8 +359,23 @@ int launch_mego(gg_pf* h, int n_iter, int force) {
   const gg_arena a = arena_of(h);
   hipLaunchKernelGGL(gg_k_mego_w, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, force);
   GG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gg_k_mego, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
-                     force);
+  switch (h->mego_depth) {
+    case 8:
+      hipLaunchKernelGGL(gg_k_mego<8>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+                         force);
+      break;
+    case 4:
+      hipLaunchKernelGGL(gg_k_mego<4>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+                         force);
+      break;
+    case 2:
+      hipLaunchKernelGGL(gg_k_mego<2>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+                         force);
+      break;
+    default:
+      hipLaunchKernelGGL(gg_k_mego<1>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+                         force);
+  }
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -1480,6 +1497,44 @@ int gg_pfb_step_part(gg_pf* h, const gg_step_args* args, int32_t resampler, int3
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// `steps` consecutive steps captured into *graph on a private stream, so that
+// a caller's stream (gg_pf_set_stream) never holds a capture; the graph
+// replays on whichever stream is in use.  The handle's stream is idle here.
+int capture_steps(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, int steps,
+                  hipGraph_t* graph) {
+  *graph = nullptr;
+  // allocations cannot happen inside the capture
+  if (s.lik == GG_LIK_LEARNED && s.learned_mode == 1 && ensure_table(h)) return -1;
+  hipStream_t cs = nullptr;
+  GG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipStream_t run = h->stream;
+  h->stream = cs;
+  int rc = 0;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, s, resampler, n_iter);
+    e = hipStreamEndCapture(cs, &g);
+  }
+  h->stream = run;
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return -1;
+  }
+  GG_HIP(e);
+  *graph = g;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
                    int32_t steps) {
   if (!h || !args) return gg_fail(-2, "null argument");
@@ -1494,28 +1549,8 @@ int gg_pfb_capture(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_
     GG_HIP(hipGraphDestroy(h->graph));
     h->graph = nullptr;
   }
-  // allocations cannot happen inside the capture
-  if (args->lik == GG_LIK_LEARNED && args->learned_mode == 1 && ensure_table(h)) return -1;
-  // captured on a private stream, so that a caller's stream (gg_pf_set_stream)
-  // never holds a capture; the graph replays on whichever stream is in use
-  hipStream_t cs = nullptr;
-  GG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  hipStream_t run = h->stream;
-  h->stream = cs;
-  int rc = 0;
   hipGraph_t g = nullptr;
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  if (e == hipSuccess) {
-    for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(h, *args, resampler, n_iter);
-    e = hipStreamEndCapture(cs, &g);
-  }
-  h->stream = run;
-  (void)hipStreamDestroy(cs);
-  if (rc) {
-    if (g) hipGraphDestroy(g);
-    return -1;
-  }
-  GG_HIP(e);
+  if (capture_steps(h, *args, resampler, n_iter, steps, &g)) return -1;
   h->graph = g;
   GG_HIP(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
   return 0;
@@ -1711,16 +1746,117 @@ int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out) {
   return 0;
 }
 
-int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw) {
+int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw, int32_t mego_depth) {
   if (!h) return gg_fail(-2, "null handle");
   if (weight_tpw < 1 || weight_tpw > 64) return gg_fail(-2, "weight_tpw must be in [1, 64]");
+  if (mego_depth != 1 && mego_depth != 2 && mego_depth != 4 && mego_depth != 8)
+    return gg_fail(-2, "mego_depth must be 1, 2, 4 or 8");
   h->weight_tpw = weight_tpw;
+  h->mego_depth = mego_depth;
   return 0;
 }
 
-int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw) {
-  if (!h || !weight_tpw) return gg_fail(-2, "null argument");
+int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw, int32_t* mego_depth) {
+  if (!h || !weight_tpw || !mego_depth) return gg_fail(-2, "null argument");
   *weight_tpw = h->weight_tpw;
+  *mego_depth = h->mego_depth;
+  return 0;
+}
+
+int gg_pfb_autotune(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                    int32_t steps, int32_t reps, int32_t max_cand, int32_t* cand, float* ms,
+                    int32_t* n_cand) {
+  if (!h || !args || !cand || !ms || !n_cand) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  if (steps < 1 || reps < 1) return gg_fail(-2, "steps and reps must be >= 1");
+  // candidate shapes: tiles per likelihood workgroup (the analytic likelihood
+  // with predict is the only launch that takes it) x Megopolis load depth
+  // (Chesser's form only)
+  std::vector<int> tpws = {h->weight_tpw};
+  if (args->lik != GG_LIK_LEARNED) tpws = {1, 2, 4};
+  std::vector<int> depths = {h->mego_depth};
+  if (resampler == 1) depths = {1, 2, 4, 8};
+  const int nc = (int)(tpws.size() * depths.size());
+  if (max_cand < nc) return gg_fail(-2, "max_cand below the candidate count");
+  GG_HIP(hipStreamSynchronize(h->stream));
+  // every trial starts from the same particles, weights and records, and the
+  // handle is left as it was found (the step's intermediate buffers aside)
+  const size_t b_state = sizeof(float) * 2 * (size_t)h->ncomp * h->P;
+  const size_t b_lw = sizeof(float) * (size_t)h->P;
+  const size_t b_anc = sizeof(int32_t) * 2 * (size_t)h->P;
+  const size_t b_dyn = sizeof(gg_dyn) * (size_t)h->n_filters;
+  char* snap = nullptr;
+  GG_HIP(hipMalloc((void**)&snap, b_state + b_lw + b_anc + b_dyn));
+  struct Buf { void* dev; size_t off, bytes; };
+  const Buf bufs[] = {{h->state, 0, b_state}, {h->lw, b_state, b_lw},
+                      {h->anc, b_state + b_lw, b_anc}, {h->dyn, b_state + b_lw + b_anc, b_dyn}};
+  auto copy_all = [&](bool save) {
+    for (const Buf& b : bufs) {
+      const hipError_t e = save ? hipMemcpyAsync(snap + b.off, b.dev, b.bytes,
+                                                 hipMemcpyDeviceToDevice, h->stream)
+                                : hipMemcpyAsync(b.dev, snap + b.off, b.bytes,
+                                                 hipMemcpyDeviceToDevice, h->stream);
+      if (e != hipSuccess) return gg_fail_hip(e, "autotune snapshot copy", __FILE__, __LINE__);
+    }
+    return 0;
+  };
+  const int tpw0 = h->weight_tpw, depth0 = h->mego_depth;
+  int rc = copy_all(true);
+  float best = 0.0f;
+  int best_k = -1;
+  int k = 0;
+  for (int tpw : tpws) {
+    for (int depth : depths) {
+      if (rc) break;
+      h->weight_tpw = tpw;
+      h->mego_depth = depth;
+      hipGraph_t g = nullptr;
+      hipGraphExec_t x = nullptr;
+      rc = capture_steps(h, *args, resampler, n_iter, steps, &g);
+      if (!rc && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess)
+        rc = gg_fail(-1, "autotune graph instantiation failed");
+      // one untimed replay, then the timed ones, each from the snapshot
+      if (!rc) rc = copy_all(false);
+      if (!rc && hipGraphLaunch(x, h->stream) != hipSuccess) rc = gg_fail(-1, "autotune replay");
+      if (!rc) rc = copy_all(false);
+      if (!rc && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = gg_fail(-1, "event");
+      for (int r = 0; r < reps && !rc; ++r)
+        if (hipGraphLaunch(x, h->stream) != hipSuccess) rc = gg_fail(-1, "autotune replay");
+      float t = 0.0f;
+      if (!rc && (hipEventRecord(h->ev1, h->stream) != hipSuccess ||
+                  hipEventSynchronize(h->ev1) != hipSuccess ||
+                  hipEventElapsedTime(&t, h->ev0, h->ev1) != hipSuccess))
+        rc = gg_fail(-1, "autotune timing");
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+      if (rc) break;
+      // reps graph launches of `steps` steps: ms per step
+      const float per = t / (float)(reps * steps);
+      cand[2 * k] = tpw;
+      cand[2 * k + 1] = depth;
+      ms[k] = per;
+      if (best_k < 0 || per < best) {
+        best = per;
+        best_k = k;
+      }
+      ++k;
+    }
+  }
+  if (!rc) rc = copy_all(false);
+  if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = gg_fail(-1, "autotune sync");
+  (void)hipFree(snap);
+  if (rc || best_k < 0) {
+    h->weight_tpw = tpw0;
+    h->mego_depth = depth0;
+    return rc ? rc : gg_fail(-1, "autotune found no candidate");
+  }
+  h->weight_tpw = cand[2 * best_k];
+  h->mego_depth = cand[2 * best_k + 1];
+  *n_cand = k;
+  // the trials overwrote the per-tile partials behind the single-filter API's
+  // cached decision
+  h->decide_valid = false;
   return 0;
 }
 

@@ -849,6 +849,11 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   *reinterpret_cast<float4*>(a.mw + F.p_off + i0) = make_float4(w[0], w[1], w[2], w[3]);
 }
 
+// DEPTH: iterations whose proposal weights are loaded before the first of
+// them is decided (the proposals depend only on the particle and the
+// iteration's offset, not on the chain's state), so DEPTH x 4 loads per lane
+// are in flight instead of 4.  Results do not depend on DEPTH.
+template <int DEPTH>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int force) {
 #pragma clang fp contract(off)
   const int tile = blockIdx.x;
@@ -873,7 +878,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
   }
   // acceptance uniforms: one xoshiro128+ stream per chain, seeded by the
   // Philox block (seed, particle, call, GG_PURPOSE_MEGO_U, 0); iteration b
-  // takes the stream's output b
+  // takes the stream's output b (also for chains past the filter's end)
   gg_xo xs[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q)
@@ -885,19 +890,33 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
       myO = (uint32_t)(((uint64_t)r.v[0] * (uint64_t)(uint32_t)n) >> 32);
     }
     const int bend = min(B - b0, 64);
-    for (int c = 0; c < bend; ++c) {
-      const int O = __builtin_amdgcn_readlane((int)myO, c);
+    for (int c0 = 0; c0 < bend; c0 += DEPTH) {
+      int jq[DEPTH][GG_ITEMS];
+      float wq[DEPTH][GG_ITEMS];
 #pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const int i = ib + 64 * q;
-        const float u = gg_u01f(gg_xo_next(xs[q]));
-        if (i >= n) continue;
-        int j = i + O;
-        if (j >= n) j -= n;
-        const float wj = w[j];
-        const bool acc = u * wk[q] < wj;
-        k[q] = acc ? j : k[q];
-        wk[q] = acc ? wj : wk[q];
+      for (int cc = 0; cc < DEPTH; ++cc) {
+        // past bend (uniform) the slot repeats iteration c0 and is not used
+        const int O = __builtin_amdgcn_readlane((int)myO, (c0 + cc < bend) ? c0 + cc : c0);
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const int i = ib + 64 * q;
+          int j = i + O;
+          if (j >= n) j -= n;
+          jq[cc][q] = j;
+          wq[cc][q] = (i < n) ? w[j] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int cc = 0; cc < DEPTH; ++cc) {
+        if (c0 + cc >= bend) break;
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          // chains past the end hold wk = wq = 0: never accept
+          const float u = gg_u01f(gg_xo_next(xs[q]));
+          const bool acc = u * wk[q] < wq[cc][q];
+          k[q] = acc ? jq[cc][q] : k[q];
+          wk[q] = acc ? wq[cc][q] : wk[q];
+        }
       }
     }
   }

@@ -168,10 +168,22 @@ int gg_pfb_get_ancestors(gg_pf* h, int32_t f, int32_t* out);
 /* Launch shapes of the device-driven step (no reference counterpart):
  * weight_tpw tiles of 1024 particles per workgroup of the predict + likelihood
  * kernel (1: one tile per workgroup, one round over the batch; > 1: fewer
- * workgroups streaming tiles with the next tile prefetched).  Results do not
- * depend on the shapes; only the speed does. */
-int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw);
-int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw);
+ * workgroups streaming tiles with the next tile prefetched); mego_depth (1, 2,
+ * 4, 8) Megopolis iterations whose proposal loads are issued together.
+ * Results do not depend on the shapes; only the speed does.  Graphs captured
+ * earlier keep the shapes they were captured with. */
+int gg_pfb_set_tuning(gg_pf* h, int32_t weight_tpw, int32_t mego_depth);
+int gg_pfb_get_tuning(gg_pf* h, int32_t* weight_tpw, int32_t* mego_depth);
+/* Autotuner (no reference counterpart): times graphs of `steps` steps, `reps`
+ * replays each, for every candidate shape (weight_tpw in {1, 2, 4} for the
+ * analytic likelihood, mego_depth in {1, 2, 4, 8} for resampler 1), every trial
+ * starting from a device snapshot of the particles, log-weights, ancestors and
+ * filter records, which are restored at the end.  Keeps the fastest shape in
+ * the handle; cand[2k], cand[2k+1] = (weight_tpw, mego_depth) of candidate k
+ * and ms[k] its device ms per step, for the n_cand <= max_cand candidates. */
+int gg_pfb_autotune(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                    int32_t steps, int32_t reps, int32_t max_cand, int32_t* cand, float* ms,
+                    int32_t* n_cand);
 
 /* --- batched Gauss-Newton WLS ---------------------------------------------- */
 /* reference: wls_batch, src/positioning/wls.cu:172-278 (Sagnac range model).
