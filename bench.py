@@ -322,8 +322,9 @@ def scale_leg(args):
             est = bf.estimates()[0][:3]
             errs.append(float(np.linalg.norm((drive.R @ (est - drive.truth[t]))[:2])))
         e = np.asarray(errs[10:])
-        # timed: graph replays over the same epochs
+        # timed: graph replays over the same epochs, shapes from the autotuner
         start(bf)
+        tune = bf.autotune(st, "megopolis", B, steps=G, reps=2)
         bf.capture(st, "megopolis", B, steps=G)
         bf.replay(1)
         bf.sync()
@@ -338,6 +339,7 @@ def scale_leg(args):
         stage = {k: round((res[k] * cnt[0] + oth[k] * cnt[1]) / tot, 5) for k in res}
         wk = stage["predict_weight"]
         rec = {"ms_per_step": round(float(ms.sum()) / (reps * G), 5),
+               "tuning": tune["chosen"],
                "p50_err_m": round(float(np.median(e)), 3),
                "rms_err_m": round(float(np.sqrt(np.mean(e * e))), 3),
                "resample_fraction": round((rc1 - rc0) / (reps * G), 3),
@@ -396,6 +398,7 @@ def megopolis_leg(args, sc, init, T, S):
         st = pf.step_args(sigma_pos=1.0, sigma_cb=1.0, sigma_isb=0.1, sigma_pr=5.0, nu=1.0,
                           lik=LIK_STUDENT, n_slots=T, ess_frac=0.5)
         G = 10
+        tune = pf.autotune(st, name, 32, steps=G, reps=2)
         pf.capture(st, name, 32, steps=G)
         pf.replay(2)
         pf.sync()
@@ -405,6 +408,7 @@ def megopolis_leg(args, sc, init, T, S):
         res, oth, cnt = pf.profile_steps(st, 20, name, 32)
         pf.close()
         out[name] = {"iterations": 32, "ms_per_step": round(float(ms.sum()) / (reps * G), 5),
+                     "tuning": tune["chosen"],
                      "stage_ms_resampling_steps": {k: round(v, 5) for k, v in res.items()},
                      "resample_fraction": round(cnt[0] / max(sum(cnt), 1), 3)}
     return out
@@ -993,6 +997,9 @@ def main():
     # trajectory stays on the device; the epoch counter advances in the graph),
     # one hipEvent pair per replay
     G = args.graph_steps if (args.graph_steps > 0 and args.steps % args.graph_steps == 0) else 1
+    # launch shapes chosen by the autotuner (timed graphs from a snapshot that
+    # is restored, so the timed trajectory is unchanged), before the capture
+    tuning = pf.autotune(st, args.resampler, args.mego_iters, steps=G, reps=3)
     pf.capture(st, args.resampler, args.mego_iters, steps=G)
     pf.replay(max(1, -(-args.warmup // G)))
     pf.sync()
@@ -1006,8 +1013,9 @@ def main():
     ms_step = float(per_replay.sum()) / args.steps
     resample_fraction = (rc1 - rc0) / args.steps
 
-    # ---- per-kernel device time inside real steps (eager steps, hipEvents
-    # around each launch group on the handle's stream)
+    # ---- per-kernel device time inside real steps: eager steps after the
+    # timed replays, every kernel bracketed by events that its own launch
+    # records at the kernel's start and end (no launch gaps)
     prof_steps = max(args.profile_steps, 1)
     ms_res, ms_skip, (n_res, n_skip) = pf.profile_steps(st, prof_steps, args.resampler,
                                                          args.mego_iters)
@@ -1043,6 +1051,29 @@ def main():
     roofline["copy_gbs"] = round(copy_gbs, 1)
     roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
     resample_gbs = (resample_bytes * N / (ms_res[gather_key] * 1e-3) / 1e9) if n_res else None
+
+    # fraction of the HBM peak per kernel (algorithmic bytes / measured launch
+    # time) and for the whole step (the bytes a step must move / ms per step)
+    def hbm_frac(nbytes, ms):
+        return round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms else None
+
+    kernels = {"predict_weight": {"bytes_per_particle": bytes_pp[dom],
+                                  "ms": round(stage_ms[dom], 5),
+                                  "hbm_frac": hbm_frac(bytes_pp[dom] * N, stage_ms[dom])}}
+    if args.resampler == "systematic":
+        cdf_ms = ms_res["decide_cdf"] if n_res else None
+        kernels["decide_cdf_resampling_steps"] = {"bytes_per_particle": 8, "ms": cdf_ms,
+                                                  "hbm_frac": hbm_frac(8 * N, cdf_ms)}
+        step_pp = bytes_pp[dom] + resample_fraction * (8 + resample_bytes)
+    else:
+        step_pp = bytes_pp[dom] + resample_fraction * resample_bytes
+    g_ms = ms_res[gather_key] if n_res else None
+    kernels[gather_key + "_resampling_steps"] = {"bytes_per_particle": resample_bytes, "ms": g_ms,
+                                                 "hbm_frac": hbm_frac(resample_bytes * N, g_ms)}
+    step_frac = {"bytes_per_particle": round(step_pp, 2),
+                 "hbm_frac": hbm_frac(step_pp * N, ms_step),
+                 "note": "algorithmic bytes of an average step (the predict + weight pass, "
+                         "plus the resampling passes in the fraction of steps that resample)"}
 
     # ---- CPU baseline (bounded sample of the same workload, host cores)
     cpu = None
@@ -1096,6 +1127,10 @@ def main():
         "resample_kernel_gbs": None if resample_gbs is None else round(resample_gbs, 1),
         "p50_err_m": None if p50 is None else round(p50, 3),
         "roofline": roofline,
+        "kernel_hbm_fractions": kernels,
+        "step_hbm_fraction": step_frac,
+        "tuning": {"chosen": tuning["chosen"],
+                   "candidates_ms_per_step": [[c, round(m, 5)] for c, m in tuning["candidates"]]},
         "megopolis": mego,
         "learned": learned,
         "batched": batched,

@@ -185,7 +185,9 @@ class BatchedParticleFilter:
         return out
 
     def profile_steps(self, args, n_steps, resampler="systematic", n_iter=32):
-        """Eager steps with hipEvents around each launch group.  Returns
+        """Eager steps, every kernel timed at its own start and end (events
+        recorded by the launch itself), a launch group from its first kernel's
+        start to its last kernel's end.  Returns
         ({stage: ms} for steps that resampled, {stage: ms} for the others,
         (n_resampled, n_other)); stages are predict_weight, decide_cdf, gather
         (systematic) or predict_weight, decide, megopolis_gather; the gather

@@ -1,6 +1,7 @@
 // Host side of the particle-filter C ABI: handle, allocation, launches,
 // hipGraph capture.  Kernels live in gg_pf_kernels_impl.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -76,6 +77,11 @@ struct gg_pf {
   // proposal loads are issued together
   int32_t weight_tpw = 1;
   int32_t mego_depth = 1;
+  // kernel timing (gg_pfb_profile_steps): while kt_ev holds events, every
+  // launch of a step stage takes the next pair, recorded at the kernel's own
+  // start and end
+  std::vector<hipEvent_t> kt_ev;
+  size_t kt_next = 0;
 };
 
 namespace {
@@ -250,6 +256,20 @@ int ensure_table(gg_pf* h) {
   return 0;
 }
 
+// Launch on the handle's stream; with kernel timing on, bracketed by the
+// next start/stop event pair (hipExtLaunchKernelGGL records them at the
+// kernel's own begin and end, without the launch gaps).
+template <typename K, typename... A>
+void gg_launch(gg_pf* h, K kernel, const dim3& grid, const dim3& block, size_t smem, A... args) {
+  if (h->kt_next + 2 <= h->kt_ev.size()) {
+    hipEvent_t e0 = h->kt_ev[h->kt_next], e1 = h->kt_ev[h->kt_next + 1];
+    h->kt_next += 2;
+    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)smem, h->stream, e0, e1, 0u, args...);
+  } else {
+    kernel<<<grid, block, smem, h->stream>>>(args...);
+  }
+}
+
 int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide = true) {
   const bool learned = s.lik == GG_LIK_LEARNED;
   if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
@@ -259,41 +279,41 @@ int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide =
   const size_t smem = sat_smem(h, learned && !table);
   if (table) {
     const dim3 tgrid((GG_MLP_CODES + 1023) / 1024, h->sat_rows);
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_mlp_table<NI>), tgrid, dim3(GG_BLOCK), 0,
-                                                  h->stream, a, s, pred ? 1 : 0, h->row_filter));
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_mlp_table<NI>), tgrid, dim3(GG_BLOCK), 0,
+                                                   a, s, pred ? 1 : 0, h->row_filter));
     GG_HIP(hipGetLastError());
     if (pred) {
-      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_table<NI, true>), dim3(h->n_tiles),
-                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_table<NI, true>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, a, s));
     } else {
-      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_table<NI, false>), dim3(h->n_tiles),
-                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_table<NI, false>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, a, s));
     }
   } else if (learned) {
     if (pred) {
-      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, true>), dim3(h->n_tiles),
-                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_learned<NI, true>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, a, s));
     } else {
-      GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_learned<NI, false>), dim3(h->n_tiles),
-                                                    dim3(GG_BLOCK), smem, h->stream, a, s));
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_learned<NI, false>), dim3(h->n_tiles),
+                                                    dim3(GG_BLOCK), smem, a, s));
     }
   } else if (pred && h->weight_tpw > 1) {
     const int tpw = h->weight_tpw;
     const int grid = (h->n_tiles + tpw - 1) / tpw;
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight_mt<NI, true>), dim3(grid),
-                                                  dim3(GG_BLOCK), smem, h->stream, a, s, tpw));
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_mt<NI, true>), dim3(grid),
+                                                  dim3(GG_BLOCK), smem, a, s, tpw));
   } else if (pred) {
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, true>), dim3(h->n_tiles),
-                                                  dim3(GG_BLOCK), smem, h->stream, a, s));
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight<NI, true>), dim3(h->n_tiles),
+                                                  dim3(GG_BLOCK), smem, a, s));
   } else {
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_weight<NI, false>), dim3(h->n_tiles),
-                                                  dim3(GG_BLOCK), smem, h->stream, a, s));
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight<NI, false>), dim3(h->n_tiles),
+                                                  dim3(GG_BLOCK), smem, a, s));
   }
   GG_HIP(hipGetLastError());
   if (with_decide) {
     const int mode = (pred ? 1 : 0) | 2;
-    GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
-                                                  dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_decide<NI>), dim3(h->n_filters),
+                                                  dim3(GG_BLOCK), decide_smem(h), a, s,
                                                   mode));
     GG_HIP(hipGetLastError());
   }
@@ -302,11 +322,11 @@ int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide =
 
 int launch_reduce(gg_pf* h, const gg_step_args& s) {
   const gg_arena a = arena_of(h);
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_reduce<NI>), dim3(h->n_tiles),
-                                                dim3(GG_BLOCK), 0, h->stream, a));
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_reduce<NI>), dim3(h->n_tiles),
+                                                dim3(GG_BLOCK), 0, a));
   GG_HIP(hipGetLastError());
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
-                                                dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_decide<NI>), dim3(h->n_filters),
+                                                dim3(GG_BLOCK), decide_smem(h), a, s,
                                                 0));
   GG_HIP(hipGetLastError());
   return 0;
@@ -314,8 +334,8 @@ int launch_reduce(gg_pf* h, const gg_step_args& s) {
 
 int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
   const gg_arena a = arena_of(h);
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_decide<NI>), dim3(h->n_filters),
-                                                dim3(GG_BLOCK), decide_smem(h), h->stream, a, s,
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_decide<NI>), dim3(h->n_filters),
+                                                dim3(GG_BLOCK), decide_smem(h), a, s,
                                                 mode));
   GG_HIP(hipGetLastError());
   return 0;
@@ -325,8 +345,8 @@ int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
 int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
   const gg_arena a = arena_of(h);
   const size_t smem = with_decide ? decide_smem(h) : 0;
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_cdf<NI>), dim3(h->n_tiles), dim3(GG_BLOCK),
-                                                smem, h->stream, a, s, force, with_decide));
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_cdf<NI>), dim3(h->n_tiles), dim3(GG_BLOCK),
+                                                smem, a, s, force, with_decide));
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -336,8 +356,8 @@ int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
 int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf) {
   const gg_arena a = arena_of(h);
   const size_t smem = (anc_mode == 0) ? gather_smem(h) : 0;
-  GG_DISPATCH_ISB(h->n_isb, hipLaunchKernelGGL((gg_k_gather<NI>), dim3(h->n_chunks),
-                                                dim3(GG_BLOCK), smem, h->stream, a, s, force,
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_gather<NI>), dim3(h->n_chunks),
+                                                dim3(GG_BLOCK), smem, a, s, force,
                                                 anc_mode, src_buf));
   GG_HIP(hipGetLastError());
   return 0;
@@ -347,7 +367,7 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
                     int* final_buf) {
   const gg_arena a = arena_of(h);
   for (int it = 0; it < n_iter; ++it) {
-    hipLaunchKernelGGL(gg_k_mego_ref, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, it,
+    gg_launch(h, gg_k_mego_ref, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, it,
                        force, seed, use_ovr);
     GG_HIP(hipGetLastError());
   }
@@ -357,23 +377,23 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
 
 int launch_mego(gg_pf* h, int n_iter, int force) {
   const gg_arena a = arena_of(h);
-  hipLaunchKernelGGL(gg_k_mego_w, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, force);
+  gg_launch(h, gg_k_mego_w, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, force);
   GG_HIP(hipGetLastError());
   switch (h->mego_depth) {
     case 8:
-      hipLaunchKernelGGL(gg_k_mego<8>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+      gg_launch(h, gg_k_mego<8>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
                          force);
       break;
     case 4:
-      hipLaunchKernelGGL(gg_k_mego<4>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+      gg_launch(h, gg_k_mego<4>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
                          force);
       break;
     case 2:
-      hipLaunchKernelGGL(gg_k_mego<2>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+      gg_launch(h, gg_k_mego<2>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
                          force);
       break;
     default:
-      hipLaunchKernelGGL(gg_k_mego<1>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, h->stream, a, n_iter,
+      gg_launch(h, gg_k_mego<1>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
                          force);
   }
   GG_HIP(hipGetLastError());
@@ -1599,36 +1619,62 @@ int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each) {
   return rc;
 }
 
-int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
-                         int32_t n_steps, float* stage_ms, int32_t* counts) {
-  if (!h || !args || !stage_ms || !counts) return gg_fail(-2, "null argument");
-  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
-  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+}  // extern "C"
+
+namespace {
+
+// One step in its three launch groups: weight | decision (+ CDF for
+// systematic) | gather (+ Megopolis, finalize); ends[g] = the kernel-timing
+// event index after group g.
+int enqueue_staged_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
+                        size_t (&ends)[3]) {
+  int rc = launch_weight(h, s, true, false);
+  ends[0] = h->kt_next;
+  if (!rc) {
+    if (resampler == 0) {
+      rc = launch_cdf(h, s, 0, 1);
+      ends[1] = h->kt_next;
+      if (!rc) rc = launch_gather(h, s, 0, 0, 0);
+    } else {
+      rc = launch_decide(h, s, 3);
+      ends[1] = h->kt_next;
+      if (!rc) rc = enqueue_resample_after_decide(h, s, resampler, n_iter);
+    }
+  }
+  ends[2] = h->kt_next;
+  return rc ? gg_fail(-1, "profiled step launch failed") : 0;
+}
+
+// n_steps eager steps with every kernel timed at its own start and end
+// (hipExtLaunchKernelGGL events), each step waited for.  A group's time runs
+// from its first kernel's start to its last kernel's end.  Per-group means
+// over the steps in which some filter resampled (r = 0) and over the others.
+int profile_staged(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, int n_steps,
+                   float* stage_ms, int32_t* counts) {
+  // launches per step: weight (+ table), decision or CDF, up to n_iter + 2
+  // resampling kernels and the gather
+  const size_t n_ev = 2 * (size_t)(6 + std::max(n_iter, 0));
+  std::vector<hipEvent_t> ev(n_ev, nullptr);
   int rc = 0;
   for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) rc = gg_fail(-1, "hipEventCreate failed");
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = gg_fail(-1, "hipEventCreate failed");
   double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
   int cnt[2] = {0, 0};
   std::vector<gg_dyn> dv(h->n_filters);
-  const gg_step_args s = *args;
   for (int k = 0; rc == 0 && k < n_steps; ++k) {
-    // stages: weight | decision (+ CDF for systematic) | gather (+ Megopolis, finalize)
-    bool bad = hipEventRecord(ev[0], h->stream) != hipSuccess || launch_weight(h, s, true, false) ||
-               hipEventRecord(ev[1], h->stream) != hipSuccess;
-    if (!bad) {
-      if (resampler == 0)
-        bad = launch_cdf(h, s, 0, 1) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
-              launch_gather(h, s, 0, 0, 0);
-      else
-        bad = launch_decide(h, s, 3) || hipEventRecord(ev[2], h->stream) != hipSuccess ||
-              enqueue_resample_after_decide(h, s, resampler, n_iter);
-    }
-    if (bad || hipEventRecord(ev[3], h->stream) != hipSuccess) {
-      rc = gg_fail(-1, "profiled step launch failed");
+    size_t ends[3] = {0, 0, 0};
+    h->kt_ev = ev;
+    h->kt_next = 0;
+    rc = enqueue_staged_step(h, s, resampler, n_iter, ends);
+    h->kt_ev.clear();
+    h->kt_next = 0;
+    if (rc) break;
+    if (ends[2] == 0 || ends[0] == 0 || ends[1] == ends[0] || ends[2] == ends[1] ||
+        ends[2] > n_ev) {
+      rc = gg_fail(-1, "profiled step: a launch group ran without timing events");
       break;
     }
-    if (hipEventSynchronize(ev[3]) != hipSuccess ||
+    if (hipEventSynchronize(ev[ends[2] - 1]) != hipSuccess ||
         hipMemcpy(dv.data(), h->dyn, sizeof(gg_dyn) * h->n_filters, hipMemcpyDeviceToHost) !=
             hipSuccess) {
       rc = gg_fail(-1, "profiled step failed");
@@ -1638,13 +1684,13 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
     for (const auto& d : dv) any |= d.did_resample;
     const int r = any ? 0 : 1;
     cnt[r] += 1;
-    for (int st = 0; st < 3; ++st) {
+    size_t begin = 0;
+    for (int st = 0; st < 3 && !rc; ++st) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, ev[st], ev[st + 1]) != hipSuccess) {
+      if (hipEventElapsedTime(&ms, ev[begin], ev[ends[st] - 1]) != hipSuccess)
         rc = gg_fail(-1, "hipEventElapsedTime failed");
-        break;
-      }
       acc[r][st] += ms;
+      begin = ends[st];
     }
   }
   for (auto& e : ev)
@@ -1656,6 +1702,18 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
   counts[1] = cnt[1];
   h->decide_valid = h->ess_valid = h->est_valid = false;
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
+                         int32_t n_steps, float* stage_ms, int32_t* counts) {
+  if (!h || !args || !stage_ms || !counts) return gg_fail(-2, "null argument");
+  if (!h->initialized) return gg_fail(-3, "particle filter not initialized");
+  if (args->n_slots <= 0 || args->n_slots > h->n_slots) return gg_fail(-2, "bad n_slots");
+  return profile_staged(h, *args, resampler, n_iter, n_steps, stage_ms, counts);
 }
 
 int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,

@@ -137,8 +137,10 @@ int gg_pfb_replay(gg_pf* h, int32_t n_replays);
 int gg_pfb_time_replays(gg_pf* h, int32_t n_replays, float* ms_total);
 /* Device time of each of n_replays graph launches (ms_each[n_replays]). */
 int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each);
-/* n_steps eager steps with hipEvents around each launch group on the handle's
- * stream: stage_ms[0..2] = mean ms of (predict+weight, decision [+ CDF for
+/* n_steps eager steps, each waited for, every kernel timed at its own start
+ * and end (hipExtLaunchKernelGGL events on the handle's stream); a launch
+ * group's time runs from its first kernel's start to its last kernel's end.
+ * stage_ms[0..2] = mean ms of (predict+weight, decision [+ CDF for
  * systematic], gather [+ Megopolis] with its finalize) over steps in which some
  * filter resampled, stage_ms[3..5] over the other steps; counts[0..1] = the two
  * step counts. */

@@ -95,3 +95,23 @@ def test_autotune_keeps_the_trajectory(gpu, resampler):
     assert pf.read_dyn()[0].epoch == epoch0
     _same(_run(pf, st, resampler), ref)
     pf.close()
+
+
+@pytest.mark.parametrize("resampler", ["systematic", "megopolis"])
+def test_profiled_steps_are_the_steps(gpu, resampler):
+    """Stage profiling (every kernel bracketed by its own start/stop events)
+    runs exactly the step's kernels: the trajectory equals plain steps."""
+    twin, st = _make()
+    ref = _run(twin, st, resampler, steps=5)
+    twin.close()
+    pf, st = _make()
+    res, oth, cnt = pf.profile_steps(st, 5, resampler, 32)
+    assert sum(cnt) == 5 and cnt[0] >= 1
+    names = ({"predict_weight", "decide_cdf", "gather"} if resampler == "systematic"
+             else {"predict_weight", "decide", "megopolis_gather"})
+    assert set(res) == names
+    assert all(v > 0.0 for v in res.values())
+    np.testing.assert_array_equal(pf.estimates(), ref[0][-1])
+    for f in range(len(SIZES)):
+        np.testing.assert_array_equal(pf.get_particles(f), ref[2][f])
+    pf.close()
