@@ -29,6 +29,48 @@ _LAZY = {
 
 __all__ = list(_LAZY)
 
+# Reference module names whose contents live in another module here:
+# gnss_gpu.particle_filter_3d_bvh (ParticleFilter3DBVH) and gnss_gpu._gnss_gpu_pf3d
+# (pf_weight_3d, python/gnss_gpu/_pf3d_bindings.cpp:10-52).  Importing
+# gnss_gpu_amd.<alias> returns the target module itself.
+_MODULE_ALIASES = {
+    "particle_filter_3d_bvh": "particle_filter_3d",
+    "_gnss_gpu_pf3d": "_gnss_gpu_pf3d_bvh",
+}
+
+
+class _AliasFinder:
+    """Meta-path finder that resolves the aliased module names."""
+
+    @staticmethod
+    def find_spec(fullname, path=None, target=None):
+        prefix, _, leaf = fullname.rpartition(".")
+        if prefix != __name__ or leaf not in _MODULE_ALIASES:
+            return None
+        import importlib.util
+
+        return importlib.util.spec_from_loader(fullname, _AliasFinder)
+
+    @staticmethod
+    def create_module(spec):
+        import importlib
+
+        return importlib.import_module(f"{__name__}.{_MODULE_ALIASES[spec.name.rpartition('.')[2]]}")
+
+    @staticmethod
+    def exec_module(module):
+        return None
+
+
+def _install_aliases():
+    import sys
+
+    if not any(f is _AliasFinder for f in sys.meta_path):
+        sys.meta_path.append(_AliasFinder)
+
+
+_install_aliases()
+
 
 def __getattr__(name):
     if name in _LAZY:

@@ -271,3 +271,18 @@ def test_systematic_count_fast_path_bound():
         np.testing.assert_array_equal(k_fast[fast], ref[fast])
         # every exact target and its neighbours fall to the exact comparisons
         assert not np.any(fast[: 3 * t.size] & (np.abs(fr[: 3 * t.size] - 0.5) > 0.49))
+
+
+def test_reference_module_aliases():
+    """gnss_gpu.particle_filter_3d_bvh and gnss_gpu._gnss_gpu_pf3d resolve to the
+    modules that hold their names here."""
+    import importlib
+
+    m = importlib.import_module("gnss_gpu_amd.particle_filter_3d_bvh")
+    from gnss_gpu_amd.particle_filter_3d import ParticleFilter3DBVH
+
+    assert m.ParticleFilter3DBVH is ParticleFilter3DBVH
+    from gnss_gpu_amd._gnss_gpu_pf3d import pf_weight_3d
+    from gnss_gpu_amd._gnss_gpu_pf3d_bvh import pf_weight_3d as direct
+
+    assert pf_weight_3d is direct
