@@ -31,6 +31,7 @@ struct gg_pf {
   float* state = nullptr;
   float* lw = nullptr;
   int32_t* E = nullptr;
+  float* aos = nullptr;
   int32_t* anc = nullptr;
   double* part = nullptr;
   double* p4 = nullptr;
@@ -110,6 +111,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.emax = h->emax;
   a.E = h->E;
   a.mw = reinterpret_cast<float*>(h->E);  // Megopolis steps do not use the ends
+  a.aos = h->aos;
   a.csum = h->csum;
   a.fin = h->fin;
   a.scratch_f = h->scratch_f;
@@ -377,24 +379,27 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
 
 int launch_mego(gg_pf* h, int n_iter, int force) {
   const gg_arena a = arena_of(h);
-  gg_launch(h, gg_k_mego_w, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, force);
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego_w<NI>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a,
+                                       force));
   GG_HIP(hipGetLastError());
+  // the chain kernel finalises the resampling itself (one gather item per
+  // tile: the filter's gather item count equals its tile count)
   switch (h->mego_depth) {
     case 8:
-      gg_launch(h, gg_k_mego<8>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
-                         force);
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego<NI, 8>, dim3(h->n_tiles), dim3(GG_BLOCK), 0,
+                                           a, n_iter, force));
       break;
     case 4:
-      gg_launch(h, gg_k_mego<4>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
-                         force);
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego<NI, 4>, dim3(h->n_tiles), dim3(GG_BLOCK), 0,
+                                           a, n_iter, force));
       break;
     case 2:
-      gg_launch(h, gg_k_mego<2>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
-                         force);
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego<NI, 2>, dim3(h->n_tiles), dim3(GG_BLOCK), 0,
+                                           a, n_iter, force));
       break;
     default:
-      gg_launch(h, gg_k_mego<1>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a, n_iter,
-                         force);
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego<NI, 1>, dim3(h->n_tiles), dim3(GG_BLOCK), 0,
+                                           a, n_iter, force));
   }
   GG_HIP(hipGetLastError());
   return 0;
@@ -403,8 +408,7 @@ int launch_mego(gg_pf* h, int n_iter, int force) {
 // Megopolis part of a step, after the decision kernel.
 int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
   if (resampler == 1) {
-    if (launch_mego(h, n_iter, 0)) return -1;
-    return launch_gather(h, s, 0, 1, 1);
+    return launch_mego(h, n_iter, 0);
   }
   int fb = 0;
   if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0);
@@ -541,6 +545,7 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->Pb, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
+  if (!rc) rc = dev_alloc(&h->aos, (size_t)h->P * 8);
   if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
   if (!rc) rc = dev_alloc(&h->fin, (size_t)n_filters * GG_ARRIVE_WORDS);
   if (!rc) rc = dev_alloc(&h->scratch_f, (size_t)n_filters * 16);
@@ -587,7 +592,7 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
-  void* bufs[] = {h->state, h->lw,    h->E,         h->anc,  h->part, h->p4, h->g, h->Pb,
+  void* bufs[] = {h->state, h->lw,    h->E,  h->aos,  h->anc,  h->part, h->p4, h->g, h->Pb,
                   h->emax,  h->csum, h->fin, h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,
@@ -1153,7 +1158,6 @@ int gg_pf_device_resample_megopolis_chesser(gg_pf* h, int32_t n_iterations) {
   if (ensure_decide(h)) return -1;
   gg_step_args s = default_args();
   if (launch_mego(h, n_iterations, 1)) return -1;
-  if (launch_gather(h, s, 1, 1, 1)) return -1;
   h->decide_valid = false;
   h->ess_valid = true;
   h->est_valid = true;
@@ -1729,7 +1733,7 @@ int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t
       case 2: rc = launch_cdf(h, *args, 1, 1); break;
       case 3: rc = launch_gather(h, *args, 1, 0, 0); break;
       case 4: rc = launch_mego(h, n_iter, 1); break;
-      case 5: rc = launch_gather(h, *args, 1, 1, 1); break;
+      case 5: rc = launch_gather(h, *args, 1, 3, 0); break;
       case 6: rc = launch_decide(h, *args, 0); break;
       default: return gg_fail(-2, "unknown stage");
     }

@@ -61,6 +61,7 @@ struct gg_arena {
   float* lw;              // [P]
   int32_t* E;             // [P] tile-local prefix max of the output ends
   float* mw;              // [P] Megopolis weights exp32(lw - M) (same memory as E)
+  float* aos;             // [P][8] Megopolis: the state of every particle, packed per particle
   int32_t* anc;           // [2][P]
   double* part;           // [n_tiles_total][GG_NPART]
   double* p4;             // [n_tiles_total][4]: m_b, S_b, S2_b, 0 (the combine's inputs, packed)

@@ -553,7 +553,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s,
 //      search from the previous ancestor,
 //   5. copies the state of the ancestors (all loads of a thread in flight
 //      before its stores).
-// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity.
+// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity,
+// 3 ancestors from anc[src_buf] with the states read from the packed copy that
+// gg_k_mego_w wrote (aos).
 // Dynamic LDS: 4 * max(n_tiles) bytes for PM.
 // ---------------------------------------------------------------------------
 
@@ -743,13 +745,28 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
       const int k = k0 + u * GG_BLOCK + t;
       ok[u] = k < nout;
       const int j = jbase + (ok[u] ? k : 0);
-      src[u] = (anc_mode == 0) ? sA[ok[u] ? k : 0] : (anc_mode == 1 ? asrc[j] : j);
+      src[u] = (anc_mode == 0) ? sA[ok[u] ? k : 0] : (anc_mode == 2 ? j : asrc[j]);
     }
     float v[U][D];
+    if (anc_mode == 3) {
+      const float4* ao = reinterpret_cast<const float4*>(a.aos) + 2 * F.p_off;
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), y = x;
+        if (ok[u]) {
+          x = ao[2 * src[u]];
+          y = ao[2 * src[u] + 1];
+        }
+        const float r[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
 #pragma unroll
-      for (int d = 0; d < D; ++d) v[u][d] = ok[u] ? gg_comp(a, cur, d)[F.p_off + src[u]] : 0.0f;
+        for (int d = 0; d < D; ++d) v[u][d] = r[d];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[u][d] = ok[u] ? gg_comp(a, cur, d)[F.p_off + src[u]] : 0.0f;
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (!ok[u]) continue;
@@ -830,7 +847,13 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, in
 // segment 4 * tile + w; lane l holds its particles l + 64 q, so every load and
 // store of a wavefront is 64 consecutive values.  Writes anc[1].
 // ---------------------------------------------------------------------------
+// Megopolis weights w = exp32(lw - M), and the particles' D state components
+// packed 8 floats per particle (aos): a chain's final ancestor is any particle,
+// so the gather that follows reads one 32-byte record per output instead of D
+// scattered words.
+template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
+  constexpr int D = 4 + NISB;
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
@@ -847,15 +870,34 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   for (int q = 0; q < GG_ITEMS; ++q)
     w[q] = (i0 + q >= F.n) ? 0.0f : (lwz ? 1.0f : gg_exp32(l4[q] - M));
   *reinterpret_cast<float4*>(a.mw + F.p_off + i0) = make_float4(w[0], w[1], w[2], w[3]);
+  float p[GG_ITEMS][D];
+  gg_load4<D>(a, dy->cur, F.p_off + i0, p);
+  float4* ao = reinterpret_cast<float4*>(a.aos) + 2 * (F.p_off + i0);
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    float r[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) r[d] = (d < D) ? p[q][d] : 0.0f;
+    ao[2 * q] = make_float4(r[0], r[1], r[2], r[3]);
+    ao[2 * q + 1] = make_float4(r[4], r[5], r[6], r[7]);
+  }
 }
 
 // DEPTH: iterations whose proposal weights are loaded before the first of
 // them is decided (the proposals depend only on the particle and the
 // iteration's offset, not on the chain's state), so DEPTH x 4 loads per lane
 // are in flight instead of 4.  Results do not depend on DEPTH.
-template <int DEPTH>
+// The chains end in the resampling itself: each chain's particle receives its
+// ancestor's state from the packed copy (aos, gg_k_mego_w), the ancestors go to
+// anc[0], and the workgroup's state sums enter the gather's finalize
+// (gg_gather_finish, one item per tile), so no gather launch follows.
+static_assert(GG_QCHUNK == 2 * GG_TILE, "one gather item per tile: equal item counts");
+template <int NISB, int DEPTH>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int force) {
 #pragma clang fp contract(off)
+  constexpr int D = 4 + NISB;
+  __shared__ int ilds[8];
+  __shared__ double lds[8 + GG_BLOCK];
   const int tile = blockIdx.x;
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
@@ -920,10 +962,49 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
       }
     }
   }
-  int32_t* out = a.anc + a.P + F.p_off;
+  // state of each chain's ancestor into the chain's particle (next buffer)
+  const int cur = dy->cur;
+  const int nxt = 1 - cur;
+  const float4* ao = reinterpret_cast<const float4*>(a.aos) + 2 * F.p_off;
+  double acc[D + 2];
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q)
-    if (ib + 64 * q < n) out[ib + 64 * q] = k[q];
+  for (int d = 0; d < D + 2; ++d) acc[d] = 0.0;
+  float4 x[GG_ITEMS], y[GG_ITEMS];
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    x[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    y[q] = x[q];
+    if (ib + 64 * q < n) {
+      x[q] = ao[2 * k[q]];
+      y[q] = ao[2 * k[q] + 1];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    const int i = ib + 64 * q;
+    if (i >= n) continue;
+    const float r[8] = {x[q].x, x[q].y, x[q].z, x[q].w, y[q].x, y[q].y, y[q].z, y[q].w};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      gg_comp(a, nxt, d)[F.p_off + i] = r[d];
+      acc[2 + d] += (double)r[d];
+    }
+    a.anc[F.p_off + i] = k[q];
+  }
+  const int vm = dy->vel_mode;
+  if (vm >= 1) {
+    const int nv = (vm == 2) ? 9 : 3;
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const int i = ib + 64 * q;
+      if (i >= n) continue;
+      for (int k2 = 0; k2 < nv; ++k2)
+        gg_comp(a, nxt, D + k2)[F.p_off + i] = gg_comp(a, cur, D + k2)[F.p_off + k[q]];
+    }
+  }
+  double tot[D + 2];
+  gg_block_totals<D>(acc, lds, tot);
+  gg_gather_finish<D>(a, f, F, F.chunk_off + lt, tot, lds, ilds);
 }
 
 // ---------------------------------------------------------------------------

@@ -148,7 +148,9 @@ int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, 
                          int32_t n_steps, float* stage_ms, int32_t* counts);
 /* Device time of `reps` launches of one kernel: 0 weight+predict, 1 weight,
  * 2 CDF with the per-workgroup decision, 3 gather (systematic), 4 Megopolis
- * (Chesser), 5 gather (ancestors), 6 decision (one workgroup per filter). */
+ * (Chesser: weights and packed states, chains, state copy and finalize),
+ * 5 gather (ancestors, from the packed states), 6 decision (one workgroup per
+ * filter). */
 int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t n_iter,
                       int32_t reps, float* ms_total);
 int gg_pfb_read_dyn(gg_pf* h, gg_dyn* out);
