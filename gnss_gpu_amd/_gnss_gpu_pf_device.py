@@ -350,13 +350,15 @@ def pf_device_set_stream(state, stream=None):
     call("gg_pf_set_stream", state.handle, stream_handle(stream))
 
 
-def _not_in_scope(name):
-    def fn(*args, **kwargs):
-        raise NotImplementedError(
-            f"{name}: the GMM update is not implemented on this backend")
-
-    fn.__name__ = name
-    return fn
-
-
-pf_device_weight_gmm = _not_in_scope("pf_device_weight_gmm")
+def pf_device_weight_gmm(state, sat_ecef, pseudoranges, weights_sat, n_sat, sigma_pr,
+                         w_los=0.7, mu_nlos=15.0, sigma_nlos=30.0):
+    """LOS / NLOS Gaussian-mixture weight update (_pf_device_bindings.cpp:147-165,
+    pf_device.cu:373-437, 1622-1680): overwrites the log-weights."""
+    n_sat = int(n_sat)
+    sat = f64(sat_ecef).ravel()
+    pr = f64(pseudoranges).ravel()
+    w = f64(weights_sat).ravel()
+    if sat.size < 3 * n_sat or pr.size < n_sat or w.size < n_sat:
+        raise ValueError("satellite arrays are shorter than n_sat")
+    call("gg_pf_device_weight_gmm", state.handle, dptr(sat), dptr(pr), dptr(w), n_sat,
+         float(sigma_pr), float(w_los), float(mu_nlos), float(sigma_nlos))

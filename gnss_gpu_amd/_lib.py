@@ -24,6 +24,7 @@ LIK_GAUSS = 0
 LIK_STUDENT = 1
 LIK_HUBER = 2
 LIK_LEARNED = 3
+LIK_GMM = 4
 
 RESAMPLER_SYSTEMATIC = 0
 RESAMPLER_MEGOPOLIS = 1
@@ -57,6 +58,10 @@ class StepArgs(C.Structure):
         ("recenter", C.c_int32),
         ("learned_mode", C.c_int32),
         ("u0_value", C.c_double),
+        ("gmm_w_los", C.c_float),
+        ("gmm_mu_nlos", C.c_float),
+        ("gmm_sigma_nlos", C.c_float),
+        ("gmm_pad", C.c_int32),
     ]
 
 
@@ -127,6 +132,7 @@ _SIGS = {
     "gg_pf_device_initialize": [_P, _D, _D, _D, _D, _D, _D, _U64, _D, _D, _D, _D, _D, _DP, _D],
     "gg_pf_device_predict": [_P, _D, _D, _D, _D, _D, _D, _U64, _I, _D, _D, _I, _D, _D],
     "gg_pf_device_weight": [_P, _DP, _DP, _DP, _IP, _I, _D, _D, _D, _I, _D],
+    "gg_pf_device_weight_gmm": [_P, _DP, _DP, _DP, _I, _D, _D, _D, _D],
     "gg_pf_device_weight_dd_pseudorange": [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I, _D, _D, _I, _D],
     "gg_pf_device_weight_carrier_afv": [_P, _DP, _DP, _DP, _I, _D, _D],
     "gg_pf_device_weight_dd_carrier_afv": [_P, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _I, _D, _D, _I,
@@ -312,6 +318,9 @@ def default_step_args(**kw) -> StepArgs:
     s.u0_mode = 0
     s.recenter = 1
     s.u0_value = 0.0
+    s.gmm_w_los = 0.7
+    s.gmm_mu_nlos = 15.0
+    s.gmm_sigma_nlos = 30.0
     for k, v in kw.items():
         if not hasattr(s, k):
             raise AttributeError(f"StepArgs has no field {k!r}")

@@ -453,6 +453,9 @@ gg_step_args default_args() {
   s.n_slots = 1;
   s.ess_frac = 0.5;
   s.recenter = 1;
+  s.gmm_w_los = 0.7f;
+  s.gmm_mu_nlos = 15.0f;
+  s.gmm_sigma_nlos = 30.0f;
   return s;
 }
 
@@ -1033,6 +1036,26 @@ int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudora
   s.lik = student ? GG_LIK_STUDENT
                   : ((per_particle_huber && per_particle_huber_k > 0.0) ? GG_LIK_HUBER
                                                                         : GG_LIK_GAUSS);
+  s.auto_flag = 0;
+  if (launch_weight(h, s, false)) return -1;
+  h->decide_valid = true;
+  h->ess_valid = true;
+  h->est_valid = true;
+  return 0;
+}
+
+int gg_pf_device_weight_gmm(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                            const double* weights_sat, int32_t n_sat, double sigma_pr,
+                            double w_los, double mu_nlos, double sigma_nlos) {
+  if (int rc = check_single(h)) return rc;
+  if (stage_single(h, sat_ecef, pseudoranges, weights_sat, nullptr, n_sat, nullptr, nullptr))
+    return -1;
+  gg_step_args s = default_args();
+  s.sigma_pr = (float)sigma_pr;
+  s.lik = GG_LIK_GMM;
+  s.gmm_w_los = (float)w_los;
+  s.gmm_mu_nlos = (float)mu_nlos;
+  s.gmm_sigma_nlos = (float)sigma_nlos;
   s.auto_flag = 0;
   if (launch_weight(h, s, false)) return -1;
   h->decide_valid = true;

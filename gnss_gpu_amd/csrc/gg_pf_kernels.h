@@ -440,7 +440,7 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
   int* grp = reinterpret_cast<int*>(rec + GG_REC * cap);
   // weight factor of the likelihood (binary64, rounded once)
   const double sig2 = (double)s.sigma_pr * (double)s.sigma_pr;
-  const double wfac = (s.lik == GG_LIK_GAUSS) ? -0.5 / sig2
+  const double wfac = (s.lik == GG_LIK_GAUSS || s.lik == GG_LIK_GMM) ? -0.5 / sig2
                       : (s.lik == GG_LIK_STUDENT) ? 1.0 / ((double)s.nu * sig2) : 1.0;
   if (ns <= 64) {
     if (t < 64) {
@@ -701,6 +701,39 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
     const float hn = -0.5f * (s.nu + 1.0f);
 #pragma unroll
     for (int q = 0; q < GG_ITEMS; ++q) out[q] = hn * (acc[q] + gg_lnf(prod[q]));
+  } else if (s.lik == GG_LIK_GMM) {
+    // per satellite log(w N(e; 0, sigma_los) + (1 - w) N(e; mu, sigma_nlos)) up
+    // to the common -log sqrt(2 pi), formed as the reference does (max plus
+    // log of the exponential sum); m.y = -w_s / (2 sigma_los^2), the NLOS
+    // factor is m.y sigma_los^2 / sigma_nlos^2
+    const double sl = (double)s.sigma_pr, sn = (double)s.gmm_sigma_nlos;
+    const double wl = (double)s.gmm_w_los;
+    const float c_los = (float)(::log(wl) - ::log(sl));
+    const float c_nlos = (float)(::log(1.0 - wl) - ::log(sn));
+    const float ratio = (float)((sl * sl) / (sn * sn));
+    const float mu = s.gmm_mu_nlos;
+    float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
+    gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
+      float bias[GG_ITEMS];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) bias[q] = gg_group_bias<NISB, decltype(grp)::value>(p[q]);
+      for (int k = k0; k < k1; ++k) {
+        const float4 g = sv.rec[GG_REC * k];
+        const float4 m = sv.rec[GG_REC * k + 1];
+        const float kn = m.y * ratio;
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          const float en = e - mu;
+          const float a = fmaf(m.y * e, e, c_los);
+          const float b = fmaf(kn * en, en, c_nlos);
+          const float mx = fmaxf(a, b);
+          lw[q] += mx + gg_lnf(__expf(a - mx) + __expf(b - mx));
+        }
+      }
+    });
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = lw[q];
   } else if (s.lik == GG_LIK_HUBER) {
     const float hk = s.huber_k;
     const float isig = 1.0f / s.sigma_pr;

@@ -217,8 +217,23 @@ class ParticleFilterDevice:
         if resample:
             _ = self.resample_if_needed()
 
-    def update_gmm(self, *args, **kwargs):
-        return _dev.pf_device_weight_gmm(self._state, *args, **kwargs)
+    def update_gmm(self, sat_ecef, pseudoranges, weights=None, sigma_pr=None, w_los=0.7,
+                   mu_nlos=15.0, sigma_nlos=30.0, resample=True):
+        """LOS / NLOS Gaussian-mixture likelihood (particle_filter_device.py:305-351):
+        per satellite w_los N(e; 0, sigma_pr) + (1 - w_los) N(e; mu_nlos, sigma_nlos),
+        ``weights`` scaling both exponents; overwrites the log-weights, then the
+        ESS-based resampling unless ``resample`` is False."""
+        self._require_init()
+        sat = np.asarray(sat_ecef, dtype=np.float64).reshape(-1, 3)
+        pr = np.asarray(pseudoranges, dtype=np.float64).ravel()
+        n_sat = len(pr)
+        w = (np.ones(n_sat, dtype=np.float64) if weights is None
+             else np.asarray(weights, dtype=np.float64).ravel())
+        sp = float(self.sigma_pr if sigma_pr is None else sigma_pr)
+        _dev.pf_device_weight_gmm(self._state, sat.ravel(), pr, w, n_sat, sp, float(w_los),
+                                  float(mu_nlos), float(sigma_nlos))
+        if resample:
+            _ = self.resample_if_needed()
 
     def update_dd_pseudorange(self, dd_result, sigma_pr=0.75, resample=True):
         """DD pseudorange likelihood (no clock bias); overwrites the log-weights

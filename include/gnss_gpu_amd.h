@@ -69,6 +69,14 @@ int gg_pf_device_weight(gg_pf* h, const double* sat_ecef, const double* pseudora
                         const double* weights_sat, const int32_t* sys, int32_t n_sat,
                         double sigma_pr, double nu, double per_particle_nlos_threshold_m,
                         int32_t per_particle_huber, double per_particle_huber_k);
+/* reference: pf_device_weight_gmm, src/particle_filter/pf_device.cu:1622-1680 and
+ * pfd_weight_gmm_kernel :373-437 (binding _pf_device_bindings.cpp:147-165):
+ * overwrites the log-weights with sum_s log(w_los N(e_s; 0, sigma_pr) +
+ * (1 - w_los) N(e_s; mu_nlos, sigma_nlos)) (up to the constant -log sqrt(2 pi)),
+ * e_s = pr_s - |sat_s - x| - cb, per-satellite weights scaling both exponents. */
+int gg_pf_device_weight_gmm(gg_pf* h, const double* sat_ecef, const double* pseudoranges,
+                            const double* weights_sat, int32_t n_sat, double sigma_pr,
+                            double w_los, double mu_nlos, double sigma_nlos);
 /* reference: pf_device_ess, pf_device.cu:1722-1762 */
 int gg_pf_device_ess(gg_pf* h, double* out);
 /* reference: pf_device_position_spread, pf_device.cu:1764-1807 */

@@ -45,6 +45,7 @@
 #define GG_LIK_STUDENT 1
 #define GG_LIK_HUBER 2
 #define GG_LIK_LEARNED 3  // per-pair MLP (gg_mlp), needs satellite C/N0 and LOS flags
+#define GG_LIK_GMM 4      // LOS / NLOS Gaussian mixture (gmm_* below, sigma_pr the LOS sigma)
 
 struct gg_filter {
   int64_t p_off;     // first particle slot in the arena
@@ -117,4 +118,9 @@ struct gg_step_args {
   int32_t recenter;      // gather re-centres offsets on the weighted mean
   int32_t learned_mode;  // GG_LIK_LEARNED: 0 network per pair, 1 per-satellite code table
   double u0_value;
+  // GG_LIK_GMM: weight of the LOS component, mean and sigma of the NLOS one [m]
+  float gmm_w_los;
+  float gmm_mu_nlos;
+  float gmm_sigma_nlos;
+  int32_t gmm_pad;
 };

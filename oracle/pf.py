@@ -72,6 +72,22 @@ def log_likelihood(pos, cb, isb, sat, pr, w, sys, sigma_pr, nu=0.0, gate_thr=0.0
     return np.sum(np.where(keep, term, 0.0), axis=1)
 
 
+def log_likelihood_gmm(pos, cb, sat, pr, w, sigma_los, w_los, mu_nlos, sigma_nlos):
+    """Per-particle GMM log-weight, pfd_weight_gmm_kernel pf_device.cu:373-437
+    (overwrites): sum over satellites of max(a, b) + log(exp(a - max) + exp(b - max))
+    with a = log w_los - log sigma_los - w e^2 / (2 sigma_los^2),
+    b = log(1 - w_los) - log sigma_nlos - w (e - mu_nlos)^2 / (2 sigma_nlos^2),
+    e = pr - |sat - x| - cb.  pos [N,3], cb [N], sat [S,3]."""
+    r = np.linalg.norm(pos[:, None, :] - sat[None, :, :], axis=2)
+    e = pr[None, :] - (r + cb[:, None])
+    a = np.log(w_los) - np.log(sigma_los) - 0.5 * w[None, :] * e * e / (sigma_los * sigma_los)
+    en = e - mu_nlos
+    b = np.log(1.0 - w_los) - np.log(sigma_nlos) - 0.5 * w[None, :] * en * en / (
+        sigma_nlos * sigma_nlos)
+    mx = np.maximum(a, b)
+    return np.sum(mx + np.log(np.exp(a - mx) + np.exp(b - mx)), axis=1)
+
+
 def _gated_sum(resid, w, sigma, thr, huber, huber_k):
     """Sum over observations of the Gaussian or weighted-Huber cost with the
     per-particle gate of the DD kernels (min(n, 3) kept, pf_device.cu:317-335)."""

@@ -75,6 +75,10 @@ PINS = {
                           "zero residual -> 0; gate all-pass / all-reject; Student-t brute force; "
                           "inter-system biases: 0 at the truth, -n_k delta^2 / (2 sigma^2) for "
                           "ISB k-1 off by delta (test_isb_closed_forms)"),
+    "pf.log_likelihood_gmm": ("pinned", "the two-component normal mixture density from "
+                              "scipy.stats.norm (unit satellite weights, up to the dropped "
+                              "-log sqrt(2 pi)); w_los = 1 -> Gaussian; equal components -> "
+                              "independent of w_los (test_gmm_log_likelihood_pins)"),
     "pf.ess": ("pinned", "uniform -> N, one dominant -> < 2 (test_particle_filter.py:174-186)"),
     "pf.weighted_mean": ("pinned", "uniform weights -> arithmetic mean (test_particle_filter.py:293-315)"),
     "pf.position_spread": ("pinned", "uniform weights -> RMS distance, closed form on a 2-point cloud"),

@@ -1317,3 +1317,44 @@ def test_doppler_oracle_known_answers():
     K2 = P_a @ h2 / S2
     np.testing.assert_allclose(mu1[0], mu_a + K2 * (yc2 - h2 @ mu_a), rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(P1[0], P_a - np.outer(K2, h2 @ P_a), rtol=1e-7, atol=1e-9)
+
+
+def test_gmm_log_likelihood_pins():
+    """pf.log_likelihood_gmm against what the mathematics fixes: with unit
+    satellite weights each term is log of the normal-mixture density
+    w N(e; 0, s_los) + (1 - w) N(e; mu, s_nlos) plus log sqrt(2 pi) (the
+    reference drops that constant from both components, pf_device.cu:418-425);
+    w_los -> 1 leaves the Gaussian term minus log s_los; two equal components
+    make w_los irrelevant."""
+    from scipy.stats import norm
+
+    from oracle import pf as opf
+
+    rng = np.random.default_rng(21)
+    truth = np.array([-3957199.0, 3310205.0, 3737911.0])
+    sat = truth + rng.normal(0, 1.5e7, (9, 3))
+    pos = truth + rng.normal(0, 30.0, (64, 3))
+    cb = rng.normal(100.0, 20.0, 64)
+    pr = np.linalg.norm(sat - truth, axis=1) + 100.0 + rng.normal(0, 3, 9)
+    pr[[1, 4]] += [25.0, 60.0]  # two NLOS-like biases
+    e = pr[None, :] - (np.linalg.norm(pos[:, None, :] - sat[None, :, :], axis=2) + cb[:, None])
+    ones = np.ones(9)
+    got = opf.log_likelihood_gmm(pos, cb, sat, pr, ones, 5.0, 0.7, 15.0, 30.0)
+    dens = 0.7 * norm.pdf(e, 0.0, 5.0) + 0.3 * norm.pdf(e, 15.0, 30.0)
+    ref = np.sum(np.log(dens) + 0.5 * np.log(2 * np.pi), axis=1)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-9)
+    # w_los = 1: the Gaussian log-density without its constant
+    g1 = opf.log_likelihood_gmm(pos, cb, sat, pr, ones, 5.0, 1.0, 15.0, 30.0)
+    np.testing.assert_allclose(g1, np.sum(-np.log(5.0) - 0.5 * e * e / 25.0, axis=1), rtol=1e-12)
+    # identical components: any w_los gives the single Gaussian
+    for wl in (0.1, 0.5, 0.9):
+        gi = opf.log_likelihood_gmm(pos, cb, sat, pr, ones, 4.0, wl, 0.0, 4.0)
+        np.testing.assert_allclose(gi, np.sum(-np.log(4.0) - 0.5 * e * e / 16.0, axis=1),
+                                   rtol=1e-12)
+    # at the truth the residuals are noise plus the positive NLOS biases: a
+    # positive NLOS mean explains them better than a negative one
+    t1 = opf.log_likelihood_gmm(truth[None, :], np.array([100.0]), sat, pr, ones, 5.0, 0.7,
+                                15.0, 30.0)
+    t2 = opf.log_likelihood_gmm(truth[None, :], np.array([100.0]), sat, pr, ones, 5.0, 0.7,
+                                -15.0, 30.0)
+    assert t1[0] > t2[0]

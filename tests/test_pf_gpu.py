@@ -175,6 +175,35 @@ def test_weight_ess_estimate_match_oracle(gpu, kind):
     assert pf.get_position_spread(c) == pytest.approx(opf.position_spread(st[:, :3], lw, c), rel=1e-6)
 
 
+@pytest.mark.parametrize("w_los,mu,sig_n", [(0.7, 15.0, 30.0), (0.95, 40.0, 10.0), (1.0, 15.0, 30.0)])
+def test_gmm_update_matches_oracle(gpu, w_los, mu, sig_n):
+    """update_gmm (particle_filter_device.py:305-351, pf_device.cu:373-437) on the
+    Tokyo constellation with two NLOS-biased satellites and per-satellite
+    weights: log-weights within 1e-5 relative of the binary64 oracle at the
+    device's particles, then the ESS-driven resampling of the caller's path."""
+    g = GOLD["tokyo_constellation"]
+    sat = np.array(g["sat_ecef"])
+    tp = np.array(g["true_pos"])
+    rng = np.random.default_rng(5)
+    pr = np.linalg.norm(sat - tp, axis=1) + g["true_cb"] + rng.normal(0, 3, 8)
+    pr[[2, 6]] += [30.0, 55.0]
+    w = rng.uniform(0.5, 2.0, 8)
+    pf, orc = _setup(20000)
+    pf.update_gmm(sat, pr, weights=w, w_los=w_los, mu_nlos=mu, sigma_nlos=sig_n, resample=False)
+    lw = pf.get_log_weights()
+    st = pf.get_particles()
+    ref = opf.log_likelihood_gmm(st[:, :3], st[:, 3], sat, pr, w, 5.0, w_los, mu, sig_n)
+    np.testing.assert_allclose(lw, ref, rtol=1e-5, atol=1e-5)
+    assert pf.get_ess() == pytest.approx(opf.ess(lw), rel=1e-6)
+    np.testing.assert_allclose(pf.estimate(), opf.weighted_mean(st, lw), rtol=0, atol=1e-3)
+    # the drop-in module function, then the caller's resampling
+    dev = _dev()
+    dev.pf_device_weight_gmm(pf._state, sat.ravel(), pr, w, 8, 5.0, w_los, mu, sig_n)
+    np.testing.assert_array_equal(pf.get_log_weights(), lw)
+    pf.update_gmm(sat, pr, weights=w, w_los=w_los, mu_nlos=mu, sigma_nlos=sig_n)
+    assert np.all(pf.get_log_weights() == 0.0) or pf.get_ess() >= 0.5 * 20000
+
+
 @pytest.mark.parametrize("n", [1, 5, 1000, 1024, 1025, 4099, 100_000, 1_000_000])
 def test_systematic_ancestors_bit_exact(gpu, n):
     dev = _dev()
