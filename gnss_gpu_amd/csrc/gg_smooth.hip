@@ -84,15 +84,23 @@ __global__ __launch_bounds__(GG_FB_BLOCK) void gg_k_ffbsi(
   int nxt = 0;
   for (int t = T - 1; t >= 0; --t) {
     const int f = forced[m * T + t];
+    if (f < -1 || f >= N) {  // not an index nor -1: reported, nothing out of range written
+      if (tid == 0) status[m] = -3;
+      return;
+    }
     if (f >= 0) {  // the caller's integer draw for a degenerate step
       if (tid == 0) out[t] = f;
-      nxt = min(f, N - 1);  // state loads stay inside the history whatever f is
+      nxt = f;
       continue;
     }
     if (anc && t < T - 1) {  // genealogy: follow the resampling ancestor
       const int a = anc[(int64_t)t * N + nxt];
+      if (a < 0 || a >= N) {
+        if (tid == 0) status[m] = -3;
+        return;
+      }
       if (tid == 0) out[t] = a;
-      nxt = min(max(a, 0), N - 1);  // the next lookup stays inside the table
+      nxt = a;
       continue;
     }
     StepIn s;
@@ -484,10 +492,14 @@ __global__ __launch_bounds__(GG_FR_BLOCK) void gg_k_ffbsi_rows(
       uch = tt >= 0 ? u[m * T + tt] : 0.0;
     }
     const int f = __builtin_amdgcn_readlane(fch, l);
+    if (f < -1 || f >= N) {  // not an index nor -1: reported, nothing out of range written
+      if (threadIdx.x == 0) status[m] = -3;
+      return;
+    }
     int pick;
     if (f >= 0) {  // the caller's integer draw for a degenerate step
       pick = f;
-      nxt = min(f, N - 1);  // state loads stay inside the history whatever f is
+      nxt = f;
     } else {
       const uint64_t ub = __builtin_bit_cast(uint64_t, uch);
       const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ub >> 32), l);
@@ -564,6 +576,9 @@ int gg_ffbsi_sample(const double* log_weights, const double* states, const doubl
     return gg_fail(-2, "FFBSi needs states, vel, dt and sigma_pos");
   for (size_t k = 0; k < (size_t)n_traj * T; ++k)
     if (forced[k] < -1 || forced[k] >= N) return gg_fail(-2, "forced index outside [-1, N)");
+  if (ancestors)
+    for (size_t k = 0; k < (size_t)T * N; ++k)
+      if (ancestors[k] < 0 || ancestors[k] >= N) return gg_fail(-2, "ancestor outside [0, N)");
   const size_t tn = (size_t)T * N;
   const bool ffbsi = ancestors == nullptr;
   size_t bytes = sizeof(double) * tn + 256;  // log-weights

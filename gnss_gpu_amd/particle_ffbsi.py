@@ -106,6 +106,8 @@ def _sample(launch, T, N, rngs, last_only):
                 result[k] = idx[j]
             elif st == -2:  # degenerate last step: zeros, generator untouched
                 rngs[k].bit_generator.state = saved[k]
+            elif st == -3:
+                raise ValueError("a forced draw or genealogy ancestor lies outside [0, N)")
             else:
                 degenerate[k].add(st)
                 draw(k)

@@ -367,10 +367,11 @@ int gg_pf_device_weight_3d_bvh(gg_pf* h, const gg_bvh* bvh, const double* sat_ec
  * sigma_pos [T] (row t+1 drives the t -> t+1 transition), sigma_cb; genealogy
  * (ancestors [T][N] given, values in [0, N)): only the last step samples.
  * u [n_traj][T]: the uniform used at step t; forced [n_traj][T]: an index in [0, N)
- * replaces step t's draw (-1: sample; other values are rejected with -2 here, and
- * the device variant clamps them into the history for its loads).
- * indices [n_traj][T] (int64) out.
+ * replaces step t's draw (-1: sample; other values are rejected with -2 here).
+ * indices [n_traj][T] (int64) out, always in [0, N).
  * status [n_traj]: -1 done; -2 the last step is degenerate (indices not written);
+ * -3 (device variant) a forced value outside [-1, N) or a genealogy ancestor
+ * outside [0, N) met: the walk stopped there, no out-of-range index written;
  * t >= 0 the first degenerate step met (a NaN or +inf log-weight, or all -inf),
  * where the walk stopped -- the reference draws an integer there, which the caller
  * passes through forced[] in a second call. */

@@ -184,3 +184,54 @@ def test_device_history_matches_host_arrays(gpu):
         np.testing.assert_array_equal(
             got[k], of.ffbsi_indices(LW, X, V, np.ones(T), np.full(T, 1.0), 3.0,
                                      np.random.default_rng(k)))
+
+
+@pytest.mark.parametrize("bad", [-5, "N"])
+def test_device_entry_reports_forced_values_outside_the_history(gpu, bad):
+    """gg_ffbsi_sample_device on device arrays: a forced value outside [-1, N)
+    stops that trajectory with status -3 and no index outside [0, N) is
+    written; the other trajectories are unaffected.  The host entry rejects
+    the same input, and out-of-range genealogy ancestors."""
+    import ctypes as C
+
+    import torch
+
+    from gnss_gpu_amd import _lib
+
+    T, N, M = 40, 512, 3
+    lw, X, vel, dt, sp = _history(T, N, 17)
+    rng = np.random.default_rng(2)
+    u = rng.random((M, T))
+    forced = np.full((M, T), -1, dtype=np.int32)
+    forced[1, 20] = N if bad == "N" else bad
+    dev = torch.device("cuda")
+    t = {k: torch.tensor(np.ascontiguousarray(v), device=dev)
+         for k, v in dict(lw=lw, X=X, vel=vel, dt=dt, sp=sp, u=u).items()}
+    tf = torch.tensor(forced, device=dev)
+    idx = torch.full((M, T), -7, dtype=torch.int64, device=dev)
+    st = torch.zeros(M, dtype=torch.int32, device=dev)
+    p = lambda x: C.c_void_p(x.data_ptr())  # noqa: E731
+    _lib.call("gg_ffbsi_sample_device", p(t["lw"]), p(t["X"]), p(t["vel"]), p(t["dt"]),
+              p(t["sp"]), 3.0, None, T, N, M, p(t["u"]), p(tf), p(idx), p(st), None)
+    torch.cuda.synchronize()
+    status, got = st.cpu().numpy(), idx.cpu().numpy()
+    assert status.tolist() == [-1, -3, -1]
+    written = got[1][got[1] != -7]
+    assert np.all((written >= 0) & (written < N))
+    ref = of.ffbsi_indices(lw, X, vel, dt, sp, 3.0, np.random.default_rng(0))
+    assert got[0].shape == ref.shape
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    out = np.zeros((M, T), dtype=np.int64)
+    sts = np.zeros(M, dtype=np.int32)
+    with pytest.raises(RuntimeError, match="forced index"):
+        _lib.call("gg_ffbsi_sample", lw.ctypes.data_as(DP), X.ctypes.data_as(DP),
+                  vel.ctypes.data_as(DP), dt.ctypes.data_as(DP), sp.ctypes.data_as(DP), 3.0,
+                  None, T, N, M, u.ctypes.data_as(DP), forced.ctypes.data_as(IP),
+                  C.c_void_p(out.ctypes.data), sts.ctypes.data_as(IP))
+    anc = np.zeros((T, N), dtype=np.int32)
+    anc[5, 7] = N
+    ok = np.full((M, T), -1, dtype=np.int32)
+    with pytest.raises(RuntimeError, match="ancestor outside"):
+        _lib.call("gg_ffbsi_sample", lw.ctypes.data_as(DP), None, None, None, None, 3.0,
+                  anc.ctypes.data_as(IP), T, N, M, u.ctypes.data_as(DP), ok.ctypes.data_as(IP),
+                  C.c_void_p(out.ctypes.data), sts.ctypes.data_as(IP))
