@@ -1024,9 +1024,9 @@ def main():
     stage_ms = {k: (ms_res[k] * n_res + ms_skip[k] * n_skip) / tot for k in ms_res}
 
     # algorithmic HBM bytes per particle and launch (binary32 state, D = 7)
-    gather_key = "gather" if args.resampler == "systematic" else "megopolis_gather"
+    gather_key = "resample" if args.resampler == "systematic" else "megopolis_gather"
     resample_bytes = {
-        "systematic": 4 + 2 * D * 4 + 4,          # read ends, gather state, write ancestors
+        "systematic": 4 + 2 * D * 4 + 4,          # read lw, gather state, write ancestors
         "megopolis": 4 + 4 * args.mego_iters + 2 * D * 4 + 4,
         "megopolis_ref": 4 + 8 * args.mego_iters + 2 * D * 4 + 4,
     }[args.resampler]
@@ -1060,20 +1060,14 @@ def main():
     kernels = {"predict_weight": {"bytes_per_particle": bytes_pp[dom],
                                   "ms": round(stage_ms[dom], 5),
                                   "hbm_frac": hbm_frac(bytes_pp[dom] * N, stage_ms[dom])}}
-    if args.resampler == "systematic":
-        cdf_ms = ms_res["decide_cdf"] if n_res else None
-        kernels["decide_cdf_resampling_steps"] = {"bytes_per_particle": 8, "ms": cdf_ms,
-                                                  "hbm_frac": hbm_frac(8 * N, cdf_ms)}
-        step_pp = bytes_pp[dom] + resample_fraction * (8 + resample_bytes)
-    else:
-        step_pp = bytes_pp[dom] + resample_fraction * resample_bytes
+    step_pp = bytes_pp[dom] + resample_fraction * resample_bytes
     g_ms = ms_res[gather_key] if n_res else None
     kernels[gather_key + "_resampling_steps"] = {"bytes_per_particle": resample_bytes, "ms": g_ms,
                                                  "hbm_frac": hbm_frac(resample_bytes * N, g_ms)}
     step_frac = {"bytes_per_particle": round(step_pp, 2),
                  "hbm_frac": hbm_frac(step_pp * N, ms_step),
                  "note": "algorithmic bytes of an average step (the predict + weight pass, "
-                         "plus the resampling passes in the fraction of steps that resample)"}
+                         "plus the resampling pass in the fraction of steps that resample)"}
 
     # ---- CPU baseline (bounded sample of the same workload, host cores)
     cpu = None

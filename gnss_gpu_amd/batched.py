@@ -24,8 +24,9 @@ from ._lib import call, dptr, f64, i32, iptr
 
 STAGE_WEIGHT_PREDICT = 0
 STAGE_WEIGHT = 1
-STAGE_CDF = 2
-STAGE_GATHER_SYSTEMATIC = 3
+STAGE_SYSTEMATIC = 2  # decision, CDF ends, gather and finalize in one launch
+STAGE_CDF = STAGE_SYSTEMATIC
+STAGE_GATHER_SYSTEMATIC = 3  # the same launch
 STAGE_MEGOPOLIS = 4
 STAGE_GATHER_ANCESTORS = 5
 STAGE_DECIDE = 6
@@ -189,19 +190,21 @@ class BatchedParticleFilter:
         recorded by the launch itself), a launch group from its first kernel's
         start to its last kernel's end.  Returns
         ({stage: ms} for steps that resampled, {stage: ms} for the others,
-        (n_resampled, n_other)); stages are predict_weight, decide_cdf, gather
-        (systematic) or predict_weight, decide, megopolis_gather; the gather
-        includes the resampling's finalize."""
+        (n_resampled, n_other)); stages are predict_weight and resample (the
+        systematic decision, CDF ends, gather and finalize in one launch), or
+        predict_weight, decide and megopolis_gather (weights and packed states,
+        chains with the state copy and finalize)."""
         ms = np.zeros(6, dtype=np.float32)
         cnt = np.zeros(2, dtype=np.int32)
         call("gg_pfb_profile_steps", self._h, C.byref(args), RESAMPLERS[resampler], int(n_iter),
              int(n_steps), ms.ctypes.data_as(C.POINTER(C.c_float)),
              cnt.ctypes.data_as(C.POINTER(C.c_int32)))
         if resampler == "systematic":
-            names = ("predict_weight", "decide_cdf", "gather")
+            names = ("predict_weight", "resample")
         else:
             names = ("predict_weight", "decide", "megopolis_gather")
-        return (dict(zip(names, map(float, ms[:3]))), dict(zip(names, map(float, ms[3:]))),
+        k = len(names)
+        return (dict(zip(names, map(float, ms[:k]))), dict(zip(names, map(float, ms[3:3 + k]))),
                 (int(cnt[0]), int(cnt[1])))
 
     def time_stage(self, args, stage, reps, n_iter=32):

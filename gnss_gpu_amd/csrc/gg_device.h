@@ -245,6 +245,58 @@ __device__ __forceinline__ double gg_block_scan256(double v, double* lds, double
   return x;
 }
 
+// NV independent scans of the same form as gg_block_scan256 in one pass (the
+// same operations per value, so each result equals gg_block_scan256's bit for
+// bit).  `lds` must hold at least NV * (8 + 256) doubles.
+template <int NV>
+__device__ __forceinline__ void gg_block_scan256_n(const double (&v)[NV], double* lds,
+                                                   double (&excl)[NV], double (&total)[NV]) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  double x[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x[k] = v[k];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double y = __shfl_up(x[k], d, 64);
+      if (lane >= d) x[k] = y + x[k];
+    }
+  }
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * 264 + w] = x[k];
+  __syncthreads();
+  if (t < NV) {
+    double* l = lds + t * 264;
+    const double o1 = l[0];
+    const double o2 = o1 + l[1];
+    const double o3 = o2 + l[2];
+    l[4] = o1;
+    l[5] = o2;
+    l[6] = o3;
+    l[7] = o3 + l[3];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double* l = lds + k * 264;
+    if (w > 0) x[k] = l[3 + w] + x[k];
+    l[8 + t] = x[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double* l = lds + k * 264;
+    excl[k] = (t == 0) ? 0.0 : l[8 + t - 1];
+    total[k] = l[7];
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Cross-lane exchanges on the DPP network (register to register, no LDS
 // traffic).  dpp_ctrl values of the gfx9 encoding:

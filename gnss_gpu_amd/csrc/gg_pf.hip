@@ -37,7 +37,6 @@ struct gg_pf {
   double* p4 = nullptr;
   double* g = nullptr;
   double* Pb = nullptr;
-  int32_t* emax = nullptr;
   double* csum = nullptr;
   uint32_t* fin = nullptr;
   double* scratch_f = nullptr;
@@ -108,7 +107,6 @@ gg_arena arena_of(const gg_pf* h) {
   a.p4 = h->p4;
   a.g = h->g;
   a.Pb = h->Pb;
-  a.emax = h->emax;
   a.E = h->E;
   a.mw = reinterpret_cast<float*>(h->E);  // Megopolis steps do not use the ends
   a.aos = h->aos;
@@ -234,7 +232,6 @@ int max_tiles(const gg_pf* h) {
   return nt;
 }
 
-size_t gather_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(int32_t); }
 size_t decide_smem(const gg_pf* h) { return (size_t)max_tiles(h) * sizeof(double); }
 
 // likelihood (+ predict) over all tiles, then the per-filter decision
@@ -343,12 +340,13 @@ int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
   return 0;
 }
 
-// per-tile CDF and ends; with_decide: every workgroup takes the decision itself
-int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
+// systematic resampling in one launch: decision (with_decide) or the record's,
+// CDF ends of each item's tiles, merge-path gather and finalize
+int launch_sysres(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
   const gg_arena a = arena_of(h);
   const size_t smem = with_decide ? decide_smem(h) : 0;
-  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_cdf<NI>), dim3(h->n_tiles), dim3(GG_BLOCK),
-                                                smem, a, s, force, with_decide));
+  GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_sysres<NI>), dim3(h->n_chunks), dim3(GG_BLOCK),
+                                       smem, a, s, force, with_decide));
   GG_HIP(hipGetLastError());
   return 0;
 }
@@ -357,7 +355,7 @@ int launch_cdf(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
 // (gg_gather_finish), so no launch follows it.
 int launch_gather(gg_pf* h, const gg_step_args& s, int force, int anc_mode, int src_buf) {
   const gg_arena a = arena_of(h);
-  const size_t smem = (anc_mode == 0) ? gather_smem(h) : 0;
+  const size_t smem = 0;
   GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_gather<NI>), dim3(h->n_chunks),
                                                 dim3(GG_BLOCK), smem, a, s, force,
                                                 anc_mode, src_buf));
@@ -421,8 +419,7 @@ int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler
 // workgroup (and commits the predict's frame), so no decision kernel runs.
 int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
   if (resampler == 0) {
-    if (launch_cdf(h, s, 0, 1)) return -1;
-    return launch_gather(h, s, 0, 0, 0);
+    return launch_sysres(h, s, 0, 1);
   }
   if (launch_decide(h, s, 3)) return -1;
   return enqueue_resample_after_decide(h, s, resampler, n_iter);
@@ -546,7 +543,6 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->p4, (size_t)tiles * 4);
   if (!rc) rc = dev_alloc(&h->g, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->Pb, (size_t)tiles);
-  if (!rc) rc = dev_alloc(&h->emax, (size_t)tiles);
   if (!rc) rc = dev_alloc(&h->E, (size_t)h->P);
   if (!rc) rc = dev_alloc(&h->aos, (size_t)h->P * 8);
   if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
@@ -596,7 +592,7 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
   void* bufs[] = {h->state, h->lw,    h->E,  h->aos,  h->anc,  h->part, h->p4, h->g, h->Pb,
-                  h->emax,  h->csum, h->fin, h->scratch_f, h->filt,
+                  h->csum, h->fin, h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,
                   h->io_tmp, h->obsd};
@@ -1149,8 +1145,7 @@ int gg_pf_device_resample_systematic(gg_pf* h, uint64_t seed, int32_t u0_mode) {
     s.u0_mode = 2;
     s.u0_value = gg_u01d(r.v[0], r.v[1]) - 1.1102230246251565e-16;
   }
-  if (launch_cdf(h, s, 1, 0)) return -1;
-  if (launch_gather(h, s, 1, 0, 0)) return -1;
+  if (launch_sysres(h, s, 1, 0)) return -1;
   h->decide_valid = false;
   h->ess_valid = true;
   h->est_valid = true;
@@ -1650,18 +1645,18 @@ int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each) {
 
 namespace {
 
-// One step in its three launch groups: weight | decision (+ CDF for
-// systematic) | gather (+ Megopolis, finalize); ends[g] = the kernel-timing
-// event index after group g.
+// One step in its launch groups: weight | systematic resampling (decision,
+// CDF, gather, finalize in one launch) or the Megopolis decision | Megopolis
+// chains and gather (none for systematic); ends[g] = the kernel-timing event
+// index after group g.
 int enqueue_staged_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
                         size_t (&ends)[3]) {
   int rc = launch_weight(h, s, true, false);
   ends[0] = h->kt_next;
   if (!rc) {
     if (resampler == 0) {
-      rc = launch_cdf(h, s, 0, 1);
+      rc = launch_sysres(h, s, 0, 1);
       ends[1] = h->kt_next;
-      if (!rc) rc = launch_gather(h, s, 0, 0, 0);
     } else {
       rc = launch_decide(h, s, 3);
       ends[1] = h->kt_next;
@@ -1696,8 +1691,7 @@ int profile_staged(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, i
     h->kt_ev.clear();
     h->kt_next = 0;
     if (rc) break;
-    if (ends[2] == 0 || ends[0] == 0 || ends[1] == ends[0] || ends[2] == ends[1] ||
-        ends[2] > n_ev) {
+    if (ends[2] == 0 || ends[0] == 0 || ends[1] == ends[0] || ends[2] > n_ev) {
       rc = gg_fail(-1, "profiled step: a launch group ran without timing events");
       break;
     }
@@ -1713,8 +1707,8 @@ int profile_staged(gg_pf* h, const gg_step_args& s, int resampler, int n_iter, i
     cnt[r] += 1;
     size_t begin = 0;
     for (int st = 0; st < 3 && !rc; ++st) {
-      float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, ev[begin], ev[ends[st] - 1]) != hipSuccess)
+      float ms = 0.0f;  // a group without launches (the systematic step's third) takes 0
+      if (ends[st] > begin && hipEventElapsedTime(&ms, ev[begin], ev[ends[st] - 1]) != hipSuccess)
         rc = gg_fail(-1, "hipEventElapsedTime failed");
       acc[r][st] += ms;
       begin = ends[st];
@@ -1753,8 +1747,8 @@ int gg_pfb_time_stage(gg_pf* h, const gg_step_args* args, int32_t stage, int32_t
     switch (stage) {
       case 0: rc = launch_weight(h, *args, true, false); break;
       case 1: rc = launch_weight(h, *args, false, false); break;
-      case 2: rc = launch_cdf(h, *args, 1, 1); break;
-      case 3: rc = launch_gather(h, *args, 1, 0, 0); break;
+      case 2:
+      case 3: rc = launch_sysres(h, *args, 1, 1); break;
       case 4: rc = launch_mego(h, n_iter, 1); break;
       case 5: rc = launch_gather(h, *args, 1, 3, 0); break;
       case 6: rc = launch_decide(h, *args, 0); break;

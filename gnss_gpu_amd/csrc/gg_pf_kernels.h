@@ -5,25 +5,27 @@
 // particles and every SoA load/store is a 16-byte vector access that the
 // wavefront coalesces into one contiguous 1 KiB transaction.
 //
-// The systematic step for a batch of filters is three launches, each over all
+// The systematic step for a batch of filters is two launches, each over all
 // filters:
 //   gg_k_weight    predict (optional) + pseudorange likelihood + per-tile
 //                  partials (max, sums of exp32(lw - m_b), ESS and mean sums).
-//   gg_k_cdf       every workgroup combines its filter's partials (max, total,
-//                  ESS, CDF tile offset) itself and takes the resampling
-//                  decision; the first tile of each filter writes the decision
-//                  and estimate; when resampling, per-tile binary64 CDF and
-//                  output ends.
-//   gg_k_gather    merge-path partition of (ends, outputs), ancestor indices,
-//                  state gather; the last work item of each filter to finish
-//                  adds the chunk sums and finalises (estimate, buffer swap).
-// Besides that counter (device-coherent atomics, no L2 writeback), kernel
-// boundaries are the only cross-workgroup synchronisation: the L2s of the 8
-// XCDs are not coherent, an agent-scope release in every workgroup writes back
-// its XCD's whole L2, and waiting on other workgroups would need every one of
-// them resident.
-// Megopolis: gg_k_weight, gg_k_decide (one workgroup per filter), gg_k_mego
-// (or B launches of gg_k_mego_ref), the ancestor-driven gather.
+//   gg_k_sysres    every work item combines its filter's partials (max, total,
+//                  ESS, CDF tile offsets) itself and takes the resampling
+//                  decision; the first item of each filter writes the decision
+//                  and estimate; when resampling, each item computes the CDF
+//                  ends of the tiles its inputs span, the merge-path ancestors
+//                  and the state gather; the last item of each filter to
+//                  finish adds the item sums and finalises (estimate, buffer
+//                  swap).
+// Besides the records and sums that pass through the device-coherent level
+// (device-scope atomic stores and loads, two-level arrival counters; no L2
+// writeback), kernel boundaries are the only cross-workgroup synchronisation:
+// the L2s of the 8 XCDs are not coherent, an agent-scope release in every
+// workgroup writes back its XCD's whole L2, and waiting on other workgroups
+// would need every one of them resident.
+// Megopolis: gg_k_weight, gg_k_decide (one workgroup per filter), gg_k_mego_w
+// and gg_k_mego (chains, state copy and finalize), or B launches of
+// gg_k_mego_ref and the ancestor-driven gather.
 #pragma once
 
 #include "gg_device.h"
@@ -67,7 +69,6 @@ struct gg_arena {
   double* p4;             // [n_tiles_total][4]: m_b, S_b, S2_b, 0 (the combine's inputs, packed)
   double* g;              // [n_tiles_total]
   double* Pb;             // [n_tiles_total]
-  int32_t* emax;          // [n_tiles_total] tile maxima of E
   double* csum;           // [n_chunks_total][GG_MAX_D]: state sums of each gather chunk
   uint32_t* fin;          // [n_filters][GG_ARRIVE_WORDS]: gather arrival counters
   double* scratch_f;      // [n_filters][16] misc per-filter results
@@ -1077,6 +1078,27 @@ __device__ __forceinline__ int gg_kcount(double c, double u0, int n) {
   return j + 1;
 }
 
+// Copies of a filter record through the device-coherent level (device-scope
+// atomic loads / stores, one 32-bit word per thread), for a record that one
+// workgroup writes and another reads within the same launch.  A store copy
+// completes (s_waitcnt) before the copying threads go on.
+__device__ __forceinline__ void gg_dyn_load_coherent(gg_dyn* dst, const gg_dyn* src) {
+  constexpr int NW = (int)(sizeof(gg_dyn) / 4);
+  if (threadIdx.x < NW)
+    reinterpret_cast<uint32_t*>(dst)[threadIdx.x] =
+        __hip_atomic_load(reinterpret_cast<const uint32_t*>(src) + threadIdx.x, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void gg_dyn_store_coherent(gg_dyn* dst, const gg_dyn* src) {
+  constexpr int NW = (int)(sizeof(gg_dyn) / 4);
+  if (threadIdx.x < NW)
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(dst) + threadIdx.x,
+                       reinterpret_cast<const uint32_t*>(src)[threadIdx.x], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
 // Copy a filter record between global memory and LDS, one 32-bit word per
 // thread (the record is a multiple of 4 bytes).
 __device__ __forceinline__ void gg_dyn_copy(gg_dyn* dst, const gg_dyn* src) {
@@ -1140,6 +1162,8 @@ __device__ __forceinline__ void gg_commit_frame(const gg_step_args& s, gg_dyn* d
 // Result of combining a filter's tile partials.
 struct gg_comb {
   double M, W, W2, g_own, P_own;
+  double K;     // max tile exponent k_b (g_b = 2^(k_b - K) c_b)
+  double excl;  // this thread's exclusive prefix in the chunked tile scan (pass 3)
   double sum[GG_MAX_D];
 };
 
@@ -1234,6 +1258,8 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   for (int b = b0; b < b1; ++b) T = T + sT[b];
   double excl, W;
   gg_block_scan256(T, lds, excl, W);
+  r.excl = excl;
+  r.K = K;
   if (ALL) {
     double run = 0.0;
     for (int b = b0; b < b1; ++b) {

@@ -398,142 +398,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 }
 
 // ---------------------------------------------------------------------------
-// Systematic resampling, part 1: per-tile CDF and output ends.
-// cdf_i = (P_b + g_b * L_i) / W with L the fixed-order inclusive scan of
-// e_i = exp32(lw_i - m_b) inside the tile.  The particle's output end is
-// k(cdf_i) = #{ j : (u0 + j) / n <= cdf_i } (n for the last particle, which
-// reproduces the hi = n-1 clamp of the reference's binary search,
-// pf_device.cu:1200-1215).  The ancestor of output j is
-// min{ i : cdf_i >= t_j } = #{ i : E_i <= j } with E the prefix max of k, so
-// no monotonicity of the floating-point CDF is assumed.  Writes the
-// tile-local prefix max and the tile maximum; the gather completes the prefix
-// max across tiles.
-//
-// with_decide (the device-driven step): every workgroup combines its filter's
-// tile partials itself (gg_combine, identical in every workgroup) instead of
-// waiting for a one-workgroup decision kernel, and the filter's first tile
-// commits the step frame and writes the decision and estimate to the record.
-// Dynamic LDS: 8 * max(n_tiles) bytes (with_decide).
-// ---------------------------------------------------------------------------
-template <int NISB>
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_cdf(gg_arena a, gg_step_args s, int force,
-                                                     int with_decide) {
-#pragma clang fp contract(off)
-  constexpr int D = 4 + NISB;
-  extern __shared__ __align__(16) char smem[];
-  __shared__ double lds[8 + GG_BLOCK];
-  __shared__ double sx[2];
-  __shared__ int ilds[8];
-  __shared__ int tmax[GG_BLOCK];
-  __shared__ gg_dyn sd;
-  __shared__ double svg[3];
-  __shared__ int sflag;
-  const int tile = blockIdx.x;
-  const int f = a.tile_filter[tile];
-  const gg_filter F = a.filt[f];
-  gg_dyn* dy = a.dyn + f;
-  const int lt = tile - F.tile_off;
-  const int n = F.n;
-  const int t = threadIdx.x;
-  const int i0 = lt * GG_TILE + t * GG_ITEMS;
-  const int64_t base = F.p_off + i0;
-  // this tile's log-weights; the load overlaps the combine
-  const float4 lwv = *reinterpret_cast<const float4*>(a.lw + base);
-  double mb, gb, Pb, W, u0;
-  bool lwz;
-  if (with_decide) {
-    double* sT = reinterpret_cast<double*>(smem);
-    const bool first = (lt == 0);
-    if (first) gg_dyn_copy(&sd, dy);
-    double vg = 0.0;
-    if (first && t < 3) vg = gg_guide_ptr(a, s, f, dy->epoch)[t];
-    gg_comb r;
-    if (first)
-      gg_combine<D, true, false>(a, F, lt, lds, sT, sx, r);
-    else
-      gg_combine<D, false, false>(a, F, lt, lds, sT, sx, r);
-    mb = a.part[(int64_t)tile * GG_NPART];
-    gb = r.g_own;
-    Pb = r.P_own;
-    W = r.W;
-    u0 = gg_u0(s, F.seed, n, dy->resample_count);
-    lwz = false;  // the step's weight launch wrote fresh log-weights
-    if (t == 0) {
-      const double ess = (r.W2 > 0.0) ? (r.W * r.W) / r.W2 : 0.0;
-      sflag = s.auto_flag ? ((ess < s.ess_frac * (double)n) ? 1 : 0) : dy->flag;
-    }
-    if (first && t < 3) svg[t] = vg;
-    __syncthreads();
-    if (first) {
-      if (t == 0) {
-        gg_frame fr;
-        gg_step_frame<D>(&sd, s, svg, fr);
-        gg_commit_frame<D>(s, &sd, fr);
-        gg_store_decision<D>(s, F, r, u0, &sd);
-        gg_make_estimate<D>(&sd);
-        sd.lw_zero = 0;
-        sd.did_resample = 0;
-        if (s.slot_from_device || s.step_from_device) sd.epoch = sd.epoch + 1;
-      }
-      __syncthreads();
-      gg_dyn_copy(dy, &sd);
-    }
-    if (!force && !sflag) return;
-  } else {
-    if (!force && !dy->flag) return;
-    mb = a.part[(int64_t)tile * GG_NPART];
-    gb = a.g[tile];
-    Pb = a.Pb[tile];
-    W = dy->W;
-    u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
-    lwz = dy->lw_zero != 0;
-  }
-  float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (!lwz) {
-    lw[0] = lwv.x;
-    lw[1] = lwv.y;
-    lw[2] = lwv.z;
-    lw[3] = lwv.w;
-  }
-  const float mbf = (float)mb;
-  double l[GG_ITEMS];
-  double run = 0.0;
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    const bool valid = (i0 + q) < n;
-    const double e =
-        (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[q] - mbf) : 0.0;
-    run = (q == 0) ? e : run + e;
-    l[q] = run;
-  }
-  double excl, total;
-  gg_block_scan256(l[GG_ITEMS - 1], lds, excl, total);
-  int E[GG_ITEMS];
-  int rmax = 0;
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    const int i = i0 + q;
-    int k;
-    if (i >= n - 1) {
-      k = n;
-    } else {
-      const double L = excl + l[q];
-      k = gg_kcount((Pb + gb * L) / W, u0, n);
-    }
-    rmax = max(rmax, k);
-    E[q] = rmax;
-  }
-  const int pre_incl = gg_block_prefix_max256_i(rmax, ilds);
-  tmax[t] = pre_incl;
-  __syncthreads();
-  const int pre = (t == 0) ? 0 : tmax[t - 1];
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) E[q] = max(E[q], pre);
-  *reinterpret_cast<int4*>(a.E + base) = make_int4(E[0], E[1], E[2], E[3]);
-  if (t == GG_BLOCK - 1) a.emax[tile] = E[GG_ITEMS - 1];
-}
-
-// ---------------------------------------------------------------------------
 // Resampling gather (systematic merge path, or ancestors from Megopolis).
 //
 // Systematic: work item = GG_QCHUNK consecutive steps of the merge of the
@@ -616,126 +480,46 @@ __device__ __forceinline__ void gg_gather_finish(const gg_arena& a, int f, const
                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   double sum[D + 2];
   gg_block_totals<D>(v, lds, sum);
-  if (t != 0) return;
+  // the record may have been written in this launch by another workgroup (the
+  // decision of the systematic resampler's first item): read and write it
+  // through the device-coherent level
+  __shared__ gg_dyn sd;
   gg_dyn* dy = a.dyn + f;
-  const double n = (double)F.n;
-  for (int d = 0; d < D; ++d) dy->mean[d] = sum[2 + d] / n;
-  gg_make_estimate<D>(dy);
-  dy->cur = 1 - dy->cur;
-  dy->lw_zero = 1;
-  dy->flag = 0;
-  dy->did_resample = 1;
-  dy->resample_count += 1;
-  dy->ess = n;
-  dy->M = 0.0;
-  dy->W = n;
-  dy->W2 = n;
+  gg_dyn_load_coherent(&sd, dy);
+  __syncthreads();
+  if (t == 0) {
+    const double n = (double)F.n;
+    for (int d = 0; d < D; ++d) sd.mean[d] = sum[2 + d] / n;
+    gg_make_estimate<D>(&sd);
+    sd.cur = 1 - sd.cur;
+    sd.lw_zero = 1;
+    sd.flag = 0;
+    sd.did_resample = 1;
+    sd.resample_count += 1;
+    sd.ess = n;
+    sd.M = 0.0;
+    sd.W = n;
+    sd.W2 = n;
+  }
+  __syncthreads();
+  gg_dyn_store_coherent(dy, &sd);
 }
 
-template <int NISB>
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args s, int force,
-                                                        int anc_mode, int src_buf) {
-  constexpr int D = 4 + NISB;
-  extern __shared__ __align__(16) char smem[];
-  int* spm = reinterpret_cast<int*>(smem);
-  constexpr int WT = GG_QCHUNK / GG_TILE + 1;  // tiles spanned by one item's inputs
-  constexpr int RUN = GG_QCHUNK / GG_BLOCK;    // consecutive outputs per thread
-  __shared__ int sE[WT * GG_TILE];
-  __shared__ int sA[GG_QCHUNK];
-  __shared__ int ilds[8];
-  __shared__ int sI[4];
-  __shared__ double lds[8 + GG_BLOCK];
-  const int chunk = blockIdx.x;
-  const int f = a.chunk_filter[chunk];
-  const gg_filter F = a.filt[f];
-  const gg_dyn* dy = a.dyn + f;
-  if (!force && !dy->flag) return;
-  const int cl = chunk - F.chunk_off;
-  const int n = F.n;
-  const int cur = dy->cur;
-  const int nxt = 1 - cur;
+// Copy of one work item's outputs [jbase, jbase + nout) from their ancestors
+// (sA for the systematic merge, anc[src_buf] or the identity otherwise; the
+// packed states for anc_mode 3), four outputs per thread and round with every
+// load of a thread issued before its stores, then the item's finalize.
+template <int D>
+__device__ __forceinline__ void gg_gather_copy(const gg_arena& a, int f, const gg_filter& F,
+                                               int chunk, const gg_dyn* dy, int cur, int jbase,
+                                               int nout, int anc_mode, int src_buf,
+                                               const int* sA, double* lds, int* ilds) {
   const int t = threadIdx.x;
-  int jbase, nout;
-  if (anc_mode == 0) {
-    const int nt = F.n_tiles;
-    // 1. inclusive prefix max of the tile maxima, chunked per thread
-    const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
-    const int b0 = min(t * c, nt);
-    const int b1 = min(b0 + c, nt);
-    if (t == 0) sI[0] = nt - 1;  // overwritten by the search below
-    int run = 0;
-    for (int b = b0; b < b1; ++b) {
-      run = max(run, a.emax[F.tile_off + b]);
-      spm[b] = run;
-    }
-    const int incl = gg_block_prefix_max256_i(run, ilds);
-    const int prev = __shfl_up(incl, 1, 64);
-    if ((t & 63) == 63) ilds[4 + (t >> 6)] = incl;
-    __syncthreads();
-    int pre = 0;
-    if ((t & 63) != 0)
-      pre = prev;
-    else if (t > 0)
-      pre = ilds[4 + (t >> 6) - 1];
-    for (int b = b0; b < b1; ++b) spm[b] = max(spm[b], pre);
-    __syncthreads();
-    const int64_t d0 = (int64_t)cl * GG_QCHUNK;
-    const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
-    // 2. the tile holding diagonal d0 (TE is strictly increasing, TE_{nt-1} = 2n - 1)
-    for (int b = b0; b < b1; ++b) {
-      const int64_t te = (int64_t)min((b + 1) * GG_TILE, n) - 1 + spm[b];
-      const int64_t tp = (b == 0) ? -1 : (int64_t)(b * GG_TILE - 1) + spm[b - 1];
-      if (tp < d0 && te >= d0) sI[0] = b;
-    }
-    __syncthreads();
-    const int ta = sI[0];
-    // 3. ends of tiles ta .. ta+WT-1 (global values), positions below d0 / d1
-    int c0 = 0, c1 = 0;
-    for (int k0 = 4 * t; k0 < WT * GG_TILE; k0 += 4 * GG_BLOCK) {
-      const int b = ta + k0 / GG_TILE;
-      const int i0 = ta * GG_TILE + k0;
-      int4 ev = make_int4(0x3fffffff, 0x3fffffff, 0x3fffffff, 0x3fffffff);
-      if (b < nt) ev = *reinterpret_cast<const int4*>(a.E + F.p_off + i0);
-      const int pm = (b > 0 && b < nt) ? spm[b - 1] : 0;
-      int e4[4] = {ev.x, ev.y, ev.z, ev.w};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = i0 + q;
-        const int e = (b < nt && i < n) ? max(pm, e4[q]) : 0x3fffffff;
-        sE[k0 + q] = e;
-        const int64_t pos = (int64_t)i + e;
-        c0 += (i < n && pos < d0) ? 1 : 0;
-        c1 += (i < n && pos < d1) ? 1 : 0;
-      }
-    }
-    c0 = gg_block_sum256_i(c0, ilds);
-    c1 = gg_block_sum256_i(c1, ilds);
-    const int ia = ta * GG_TILE + c0;
-    const int ib = ta * GG_TILE + c1;
-    const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
-    const int la = c0, lb = c1;  // local range in sE
-    // 4. ancestors: thread t owns outputs [t*RUN, t*RUN + RUN)
-    {
-      int idx = la;
-      const int kb = t * RUN;
-      const int ke = min(kb + RUN, j1 - j0);
-      for (int k = kb; k < ke; ++k) {
-        idx = gg_first_greater(sE, idx, lb, j0 + k);
-        sA[k] = ta * GG_TILE + idx;
-      }
-    }
-    __syncthreads();
-    jbase = j0;
-    nout = j1 - j0;
-  } else {
-    jbase = cl * GG_QCHUNK;
-    nout = min(max(n - jbase, 0), GG_QCHUNK);
-  }
+  const int nxt = 1 - cur;
   const int32_t* asrc = a.anc + (int64_t)src_buf * a.P + F.p_off;
   double acc[D + 2];
 #pragma unroll
   for (int d = 0; d < D + 2; ++d) acc[d] = 0.0;
-  // 5. copy: four outputs per thread and round, every load issued before the stores
   constexpr int U = 4;
   for (int k0 = 0; k0 < nout; k0 += U * GG_BLOCK) {
     int src[U];
@@ -792,6 +576,283 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   double tot[D + 2];
   gg_block_totals<D>(acc, lds, tot);
   gg_gather_finish<D>(a, f, F, chunk, tot, lds, ilds);
+}
+
+// Gather by ancestor indices (anc_mode 1: anc[src_buf], 2: identity, 3:
+// anc[src_buf] with the packed states); work item = GG_QCHUNK outputs.
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args s, int force,
+                                                        int anc_mode, int src_buf) {
+  constexpr int D = 4 + NISB;
+  __shared__ int ilds[8];
+  __shared__ double lds[8 + GG_BLOCK];
+  const int chunk = blockIdx.x;
+  const int f = a.chunk_filter[chunk];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  if (!force && !dy->flag) return;
+  const int cl = chunk - F.chunk_off;
+  const int jbase = cl * GG_QCHUNK;
+  const int nout = min(max(F.n - jbase, 0), GG_QCHUNK);
+  gg_gather_copy<D>(a, f, F, chunk, dy, dy->cur, jbase, nout, anc_mode, src_buf, nullptr, lds,
+                    ilds);
+}
+
+// ---------------------------------------------------------------------------
+// Systematic resampling in one launch (one workgroup per GG_QCHUNK steps of
+// the merge of the particles' positions pos_i = i + E_i with the outputs).
+//
+// with_decide (the device-driven step): every workgroup combines its filter's
+// tile partials itself (gg_combine, identical in every workgroup) and takes
+// the decision; the filter's first item commits the step frame and writes the
+// decision and estimate to the record (through the device-coherent level,
+// since the finalize below may rewrite the record in this launch).  Otherwise
+// the decision, W, u0 and the tile offsets P_b and scales g_b come from the
+// record and a.Pb / a.g (gg_k_decide).
+//
+// The CDF of particle i in tile b is cdf_i = (P_b + g_b L_i) / W, L the
+// fixed-order inclusive scan of e_i = exp32(lw_i - m_b) in the tile, and its
+// output end E_i = #{j : t_j <= cdf_i} clamped to the tile's end count
+// K_{b+1} = #{j : t_j <= P_{b+1} / W} (P_nt = W); the tile's last particle
+// takes K_{b+1} and the filter's last particle N (the hi = n-1 clamp of the
+// reference's binary search, pf_device.cu:1200-1215).  E is then
+// non-decreasing across the whole filter and every tile's last position is
+// TE_b = last_b + K_{b+1}, known from the offsets alone, so an item finds its
+// first tile without any cross-tile prefix and computes the ends of the (at
+// most three) tiles its inputs span itself.  The ancestor of output j is
+// #{i : E_i <= j} (exponential search), then the gather and the finalize.
+// Dynamic LDS: 8 * max(n_tiles) bytes (with_decide).
+// ---------------------------------------------------------------------------
+template <int NISB>
+__global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args s, int force,
+                                                        int with_decide) {
+#pragma clang fp contract(off)
+  constexpr int D = 4 + NISB;
+  constexpr int WT = GG_QCHUNK / GG_TILE + 1;  // tiles spanned by one item's inputs
+  constexpr int RUN = GG_QCHUNK / GG_BLOCK;    // consecutive outputs per thread
+  extern __shared__ __align__(16) char smem[];
+  double* sT = reinterpret_cast<double*>(smem);
+  __shared__ int sE[WT * GG_TILE];
+  __shared__ int sA[GG_QCHUNK];
+  __shared__ int ilds[8];
+  __shared__ int sI[4];
+  __shared__ int sflag;
+  __shared__ double lds[WT * 264];
+  __shared__ double sx[2];
+  __shared__ double sEx[GG_BLOCK + 1];  // exclusive prefix of each thread's tile chunk
+  __shared__ double stile[WT][4];       // P_b, g_b, m_b of tiles ta .. ta+WT-1
+  __shared__ int sK[WT];                // K_{b+1} of the same tiles
+  __shared__ gg_dyn sd;
+  __shared__ double svg[3];
+  const int chunk = blockIdx.x;
+  const int f = a.chunk_filter[chunk];
+  const gg_filter F = a.filt[f];
+  const gg_dyn* dy = a.dyn + f;
+  const int cl = chunk - F.chunk_off;
+  const int n = F.n;
+  const int nt = F.n_tiles;
+  const int t = threadIdx.x;
+  const int cur = dy->cur;
+  // thread t owns tiles [b0, b1) of the chunked scan
+  const int c = (nt + GG_BLOCK - 1) / GG_BLOCK;
+  const int b0 = min(t * c, nt);
+  const int b1 = min(b0 + c, nt);
+  double W, u0, K = 0.0;
+  bool lwz;
+  if (with_decide) {
+    const bool first = (cl == 0);
+    if (first) gg_dyn_copy(&sd, dy);
+    double vg = 0.0;
+    if (first && t < 3) vg = gg_guide_ptr(a, s, f, dy->epoch)[t];
+    gg_comb r;
+    if (first)
+      gg_combine<D, true, false>(a, F, 0, lds, sT, sx, r);
+    else
+      gg_combine<D, false, false>(a, F, 0, lds, sT, sx, r);
+    W = r.W;
+    K = r.K;
+    sEx[t] = r.excl;
+    if (t == 0) sEx[GG_BLOCK] = r.W;
+    u0 = gg_u0(s, F.seed, n, dy->resample_count);
+    lwz = false;  // the step's weight launch wrote fresh log-weights
+    if (t == 0) {
+      const double ess = (r.W2 > 0.0) ? (r.W * r.W) / r.W2 : 0.0;
+      sflag = s.auto_flag ? ((ess < s.ess_frac * (double)n) ? 1 : 0) : dy->flag;
+    }
+    if (first && t < 3) svg[t] = vg;
+    __syncthreads();
+    if (first) {
+      if (t == 0) {
+        gg_frame fr;
+        gg_step_frame<D>(&sd, s, svg, fr);
+        gg_commit_frame<D>(s, &sd, fr);
+        gg_store_decision<D>(s, F, r, u0, &sd);
+        gg_make_estimate<D>(&sd);
+        sd.lw_zero = 0;
+        sd.did_resample = 0;
+        if (s.slot_from_device || s.step_from_device) sd.epoch = sd.epoch + 1;
+      }
+      __syncthreads();
+      gg_dyn_store_coherent(a.dyn + f, &sd);
+    }
+    if (!force && !sflag) return;
+  } else {
+    if (!force && !dy->flag) return;
+    W = dy->W;
+    u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
+    lwz = dy->lw_zero != 0;
+  }
+  // P_b for b in [0, nt] (P_nt = W), from the combine's chunked scan (the
+  // owner's exclusive prefix plus the owner's running sum, as gg_combine forms
+  // them) or from a.Pb
+  auto P_of = [&](int b) -> double {
+    if (b >= nt) return W;
+    if (!with_decide) return a.Pb[F.tile_off + b];
+    const int o = b / c;
+    double run = 0.0;
+    for (int k = o * c; k < b; ++k) run = run + sT[k];
+    return sEx[o] + run;
+  };
+  const int64_t d0 = (int64_t)cl * GG_QCHUNK;
+  const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
+  // the tile holding diagonal d0 (TE strictly increasing, TE_{nt-1} = 2n - 1)
+  if (t == 0) sI[0] = nt - 1;
+  __syncthreads();
+  if (b0 < b1) {
+    const double ex = with_decide ? sEx[t] : 0.0;
+    double run = 0.0;  // this thread's running tile sum (with_decide)
+    int kb = gg_kcount((with_decide ? ex + run : a.Pb[F.tile_off + b0]) / W, u0, n);  // K_b
+    for (int b = b0; b < b1; ++b) {
+      double pn;  // P_{b+1}
+      if (b + 1 == b1) {
+        pn = P_of(b + 1);
+      } else if (with_decide) {
+        run = run + sT[b];
+        pn = ex + run;
+      } else {
+        pn = a.Pb[F.tile_off + b + 1];
+      }
+      const int kn = gg_kcount(pn / W, u0, n);  // K_{b+1}
+      // the tile's last particle ends at K_{b+1}; the filter's last at N
+      const int64_t te = (int64_t)min((b + 1) * GG_TILE, n) - 1 + ((b + 1 == nt) ? n : kn);
+      const int64_t tp = (b == 0) ? -1 : (int64_t)(b * GG_TILE - 1) + kb;
+      if (tp < d0 && te >= d0) sI[0] = b;
+      kb = kn;
+    }
+  }
+  __syncthreads();
+  const int ta = sI[0];
+  // offsets, scales and end counts of tiles ta .. ta+WT-1
+  if (t < WT) {
+    const int b = ta + t;
+    if (b < nt) {
+      const double* pb = a.part + (int64_t)(F.tile_off + b) * GG_NPART;
+      stile[t][0] = P_of(b);
+      stile[t][1] = with_decide ? gg_tile_shift(pb[GG_NPART - 1], pb[GG_NPART - 2] - K)
+                                : a.g[F.tile_off + b];
+      stile[t][2] = pb[0];
+      sK[t] = gg_kcount(P_of(b + 1) / W, u0, n);
+    }
+  }
+  __syncthreads();
+  // ends of the item's tiles: in-tile scans of e, counts clamped to K_{b+1};
+  // a tile whose first position is past the item's last diagonal is not
+  // needed (its particles are neither inputs of the item nor searched)
+  {
+    bool need[WT];
+#pragma unroll
+    for (int w = 0; w < WT; ++w) {
+      const int b = ta + w;
+      need[w] = b < nt;
+      if (w > 0 && need[w]) {
+        const int64_t te_prev = (int64_t)min(b * GG_TILE, n) - 1 + sK[w - 1];
+        need[w] = need[w - 1] && te_prev + 1 < d1;
+      }
+    }
+    float lw[WT][GG_ITEMS];
+    double l[WT][GG_ITEMS], last[WT];
+#pragma unroll
+    for (int w = 0; w < WT; ++w) {
+      const int b = ta + w;
+      const int i0 = b * GG_TILE + t * GG_ITEMS;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (need[w] && !lwz) v = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
+      lw[w][0] = v.x;
+      lw[w][1] = v.y;
+      lw[w][2] = v.z;
+      lw[w][3] = v.w;
+    }
+#pragma unroll
+    for (int w = 0; w < WT; ++w) {
+      const int b = ta + w;
+      const float mbf = (b < nt) ? (float)stile[w][2] : 0.0f;
+      const int i0 = b * GG_TILE + t * GG_ITEMS;
+      double run = 0.0;
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const bool valid = need[w] && (i0 + q) < n;
+        const double e =
+            (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[w][q] - mbf) : 0.0;
+        run = (q == 0) ? e : run + e;
+        l[w][q] = run;
+      }
+      last[w] = l[w][GG_ITEMS - 1];
+    }
+    double excl[WT], total[WT];
+    gg_block_scan256_n<WT>(last, lds, excl, total);
+#pragma unroll
+    for (int w = 0; w < WT; ++w) {
+      const int b = ta + w;
+      const int i0 = b * GG_TILE + t * GG_ITEMS;
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const int i = i0 + q;
+        int e;
+        if (!need[w] || i >= n) {
+          e = 0x3fffffff;
+        } else if (i >= n - 1) {
+          e = n;
+        } else if (t * GG_ITEMS + q == GG_TILE - 1) {
+          e = sK[w];
+        } else {
+          const double L = excl[w] + l[w][q];
+          e = min(gg_kcount((stile[w][0] + stile[w][1] * L) / W, u0, n), sK[w]);
+        }
+        sE[w * GG_TILE + t * GG_ITEMS + q] = e;
+      }
+    }
+  }
+  // positions below d0 / d1 in the item's tiles
+  int c0 = 0, c1 = 0;
+  for (int k0 = 4 * t; k0 < WT * GG_TILE; k0 += 4 * GG_BLOCK) {
+    const int i0 = ta * GG_TILE + k0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + q;
+      const int64_t pos = (int64_t)i + sE[k0 + q];
+      c0 += (i < n && pos < d0) ? 1 : 0;
+      c1 += (i < n && pos < d1) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  c0 = gg_block_sum256_i(c0, ilds);
+  c1 = gg_block_sum256_i(c1, ilds);
+  const int ia = ta * GG_TILE + c0;
+  const int ib = ta * GG_TILE + c1;
+  const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
+  const int la = c0, lb = c1;  // local range in sE
+  // ancestors: thread t owns outputs [t*RUN, t*RUN + RUN)
+  {
+    int idx = la;
+    const int kb = t * RUN;
+    const int ke = min(kb + RUN, j1 - j0);
+    for (int k = kb; k < ke; ++k) {
+      idx = gg_first_greater(sE, idx, lb, j0 + k);
+      sA[k] = ta * GG_TILE + idx;
+    }
+  }
+  __syncthreads();
+  gg_gather_copy<D>(a, f, F, chunk, dy, cur, j0, j1 - j0, 0, 0, sA, lds, ilds);
 }
 
 // ---------------------------------------------------------------------------

@@ -148,14 +148,16 @@ int gg_pfb_time_replays_each(gg_pf* h, int32_t n_replays, float* ms_each);
 /* n_steps eager steps, each waited for, every kernel timed at its own start
  * and end (hipExtLaunchKernelGGL events on the handle's stream); a launch
  * group's time runs from its first kernel's start to its last kernel's end.
- * stage_ms[0..2] = mean ms of (predict+weight, decision [+ CDF for
- * systematic], gather [+ Megopolis] with its finalize) over steps in which some
+ * stage_ms[0..2] = mean ms of (predict+weight, then for systematic the one
+ * resampling launch (decision, CDF ends, gather, finalize) and 0, for Megopolis
+ * the decision and the chains with their gather) over steps in which some
  * filter resampled, stage_ms[3..5] over the other steps; counts[0..1] = the two
  * step counts. */
 int gg_pfb_profile_steps(gg_pf* h, const gg_step_args* args, int32_t resampler, int32_t n_iter,
                          int32_t n_steps, float* stage_ms, int32_t* counts);
 /* Device time of `reps` launches of one kernel: 0 weight+predict, 1 weight,
- * 2 CDF with the per-workgroup decision, 3 gather (systematic), 4 Megopolis
+ * 2 and 3 systematic resampling (decision, CDF ends, gather and finalize in one
+ * launch, forced), 4 Megopolis
  * (Chesser: weights and packed states, chains, state copy and finalize),
  * 5 gather (ancestors, from the packed states), 6 decision (one workgroup per
  * filter). */

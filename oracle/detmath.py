@@ -227,12 +227,21 @@ def fresh_u0(seed, resample_count):
     return float(philox.u01d(r[0], r[1])) - 2.0 ** -53
 
 
+def tile_ends(P, W, n, u0):
+    """K_b = #{j : t_j <= P_b / W} for b = 0 .. nt (P_nt = W, so K_nt = N):
+    the outputs that fall before tile b."""
+    Pn = np.append(np.asarray(P, dtype=np.float64), W)
+    return kcount(Pn / W, n, u0)
+
+
 def systematic_ends(lw, n, u0):
     """Per-particle output ends E_i of the library's systematic resampler:
     cdf_i = (P_b + g_b L_i) / W with L_i the in-tile inclusive scan of e,
-    k_i = #{j : t_j <= cdf_i} (N for the last particle), E = prefix max of k.
-    The ancestors a_j = #{i : E_i <= j} equal min{i : cdf_i >= t_j} wherever
-    the rounded cdf is monotone."""
+    k_i = #{j : t_j <= cdf_i}, clamped to the tile's end count K_{b+1}
+    (tile_ends); the tile's last particle takes K_{b+1} itself and the
+    filter's last particle N.  E is non-decreasing, so the ancestors
+    a_j = #{i : E_i <= j} = min{i : E_i > j}, and every tile's outputs are
+    fixed by the tile offsets alone."""
     m, S, e = tile_partials(lw, n)
     M, g, P, W = decide(m, S)
     nt = m.shape[0]
@@ -242,9 +251,13 @@ def systematic_ends(lw, n, u0):
     _, excl, _ = block_scan256(l[..., ITEMS - 1])
     L = excl[..., None] + l
     cdf = ((P[:, None, None] + g[:, None, None] * L) / W).reshape(nt * TILE)[:n]
-    k = kcount(cdf, n, u0)
+    K = tile_ends(P, W, n, u0)
+    tile = np.arange(n) // TILE
+    k = np.minimum(kcount(cdf, n, u0), K[tile + 1])
+    last = (np.arange(n) % TILE) == TILE - 1
+    k[last] = K[tile[last] + 1]
     k[n - 1] = n
-    return np.maximum.accumulate(k), cdf
+    return k, cdf
 
 
 def systematic_ancestors(lw, n, u0):

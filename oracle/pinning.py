@@ -53,6 +53,9 @@ PINS = {
     "detmath.reference_u0": ("pinned", "int32 product of pf_device.cu:1852 on N where 1e6*N "
                              "overflows (test_reference_u0_int32_product)"),
     "detmath.fresh_u0": ("pinned", "range [0,1) and Philox stream"),
+    "detmath.tile_ends": ("pinned", "K_0 = 0 and K_nt = N, non-decreasing, equal to the "
+                          "brute-force count of targets below each tile's share of the binary64 "
+                          "cumulative weight (test_tile_ends_pins)"),
     "detmath.systematic_ends": ("pinned", "brute force cumsum+searchsorted, invariants "
                                 "(test_systematic_matches_brute_force, test_systematic_invariants_large)"),
     "detmath.systematic_ancestors": ("pinned", "brute force on tiny N, identity for uniform weights, "

@@ -107,7 +107,7 @@ def test_profiled_steps_are_the_steps(gpu, resampler):
     pf, st = _make()
     res, oth, cnt = pf.profile_steps(st, 5, resampler, 32)
     assert sum(cnt) == 5 and cnt[0] >= 1
-    names = ({"predict_weight", "decide_cdf", "gather"} if resampler == "systematic"
+    names = ({"predict_weight", "resample"} if resampler == "systematic"
              else {"predict_weight", "decide", "megopolis_gather"})
     assert set(res) == names
     assert all(v > 0.0 for v in res.values())

@@ -1358,3 +1358,27 @@ def test_gmm_log_likelihood_pins():
     t2 = opf.log_likelihood_gmm(truth[None, :], np.array([100.0]), sat, pr, ones, 5.0, 0.7,
                                 -15.0, 30.0)
     assert t1[0] > t2[0]
+
+
+def test_tile_ends_pins():
+    """detmath.tile_ends: the output count before each tile equals the count of
+    systematic targets below that tile's share of an independent binary64
+    cumulative sum of the weights (away from rounding ties); it is
+    non-decreasing, ends at N, and starts at #{t_j <= 0}."""
+    from oracle import detmath
+
+    rng = np.random.default_rng(12)
+    for n in (1, 1000, 1025, 5000, 70_000):
+        lw = (rng.normal(0, 3.0, n) - 40.0).astype(np.float32)
+        for u0 in (0.0, 0.37 / n, 0.999 / n):
+            m, S, e = detmath.tile_partials(lw, n)
+            _, _, P, W = detmath.decide(m, S)
+            K = detmath.tile_ends(P, W, n, u0)
+            assert K[-1] == n and np.all(np.diff(K) >= 0)
+            assert K[0] == (1 if u0 == 0.0 else 0)
+            w = np.exp(lw.astype(np.float64) - np.max(lw))
+            C = np.concatenate([[0.0], np.cumsum(w)])[:: detmath.TILE][: len(P)] / np.sum(w)
+            t = (u0 + np.arange(n)) / n
+            Kref = np.searchsorted(t, C, side="right")
+            near = np.array([np.min(np.abs(t - c)) < 1e-12 for c in C])
+            assert np.all((K[:-1] == Kref) | near)
