@@ -412,9 +412,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 //      TE_{A-1} < d0 <= TE_A; each thread tests its own tiles),
 //   3. loads the ends of tiles A .. A+WT-1 into LDS and counts positions below
 //      d0 and d1 -> input range [i0, i1),
-//   4. ancestor of output j in [d0 - i0, d1 - i1) = i0 + #{ i in [i0, i1) : E_i <= j },
-//      each thread handling a run of consecutive outputs with an exponential
-//      search from the previous ancestor,
+//   4. ancestor of output j in [d0 - i0, d1 - i1) = max{ i in [i0, i1] : E_{i-1} <= j }:
+//      every input marks the output where its run of copies starts, and a
+//      block prefix max over the outputs fills the runs,
 //   5. copies the state of the ancestors (all loads of a thread in flight
 //      before its stores).
 // anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity,
@@ -422,26 +422,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 // gg_k_mego_w wrote (aos).
 // Dynamic LDS: 4 * max(n_tiles) bytes for PM.
 // ---------------------------------------------------------------------------
-
-// First index in [lo, hi) with e[idx] > j, searching forward from lo.
-__device__ __forceinline__ int gg_first_greater(const int* e, int lo, int hi, int j) {
-  if (lo >= hi || e[lo] > j) return lo;
-  int step = 1;
-  int last_le = lo;  // e[last_le] <= j
-  while (last_le + step < hi && e[last_le + step] <= j) {
-    last_le += step;
-    step <<= 1;
-  }
-  int a = last_le + 1, b = min(last_le + step, hi);  // answer in [a, b]
-  while (a < b) {
-    const int mid = (a + b) >> 1;
-    if (e[mid] <= j)
-      a = mid + 1;
-    else
-      b = mid;
-  }
-  return a;
-}
 
 // End of a gather work item: the chunk's state sums go to its slot, then the
 // item counts itself in (gg_arrive_last); the item that arrives last adds every chunk's sums in
@@ -620,7 +600,8 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
 // TE_b = last_b + K_{b+1}, known from the offsets alone, so an item finds its
 // first tile without any cross-tile prefix and computes the ends of the (at
 // most three) tiles its inputs span itself.  The ancestor of output j is
-// #{i : E_i <= j} (exponential search), then the gather and the finalize.
+// #{i : E_i <= j} = max{i : E_{i-1} <= j} (a marker per input and a prefix
+// max, no search), then the gather and the finalize.
 // Dynamic LDS: 8 * max(n_tiles) bytes (with_decide).
 // ---------------------------------------------------------------------------
 template <int NISB>
@@ -641,7 +622,8 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
   __shared__ double sx[2];
   __shared__ double sEx[GG_BLOCK + 1];  // exclusive prefix of each thread's tile chunk
   __shared__ double stile[WT][4];       // P_b, g_b, m_b of tiles ta .. ta+WT-1
-  __shared__ int sK[WT];                // K_{b+1} of the same tiles
+  __shared__ int sK[WT + 1];            // K_{b+1} of the same tiles; sK[WT] = K_ta
+  __shared__ int sM[GG_BLOCK];          // per-thread prefix maxima of the ancestor markers
   __shared__ gg_dyn sd;
   __shared__ double svg[3];
   const int chunk = blockIdx.x;
@@ -753,6 +735,8 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
       stile[t][2] = pb[0];
       sK[t] = gg_kcount(P_of(b + 1) / W, u0, n);
     }
+  } else if (t == WT) {
+    sK[WT] = (ta == 0) ? 0 : gg_kcount(P_of(ta) / W, u0, n);
   }
   __syncthreads();
   // ends of the item's tiles: in-tile scans of e, counts clamped to K_{b+1};
@@ -785,16 +769,20 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
 #pragma unroll
     for (int w = 0; w < WT; ++w) {
       const int b = ta + w;
-      const float mbf = (b < nt) ? (float)stile[w][2] : 0.0f;
       const int i0 = b * GG_TILE + t * GG_ITEMS;
-      double run = 0.0;
 #pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const bool valid = need[w] && (i0 + q) < n;
-        const double e =
-            (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[w][q] - mbf) : 0.0;
-        run = (q == 0) ? e : run + e;
-        l[w][q] = run;
+      for (int q = 0; q < GG_ITEMS; ++q) l[w][q] = 0.0;
+      if (need[w]) {  // uniform over the workgroup
+        const float mbf = (float)stile[w][2];
+        double run = 0.0;
+#pragma unroll
+        for (int q = 0; q < GG_ITEMS; ++q) {
+          const bool valid = (i0 + q) < n;
+          const double e =
+              (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[w][q] - mbf) : 0.0;
+          run = (q == 0) ? e : run + e;
+          l[w][q] = run;
+        }
       }
       last[w] = l[w][GG_ITEMS - 1];
     }
@@ -841,15 +829,36 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
   const int ib = ta * GG_TILE + c1;
   const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
   const int la = c0, lb = c1;  // local range in sE
-  // ancestors: thread t owns outputs [t*RUN, t*RUN + RUN)
+  const int nout = j1 - j0;
+  // ancestors without searching: the ancestor of output j is
+  // max{ i : E_{i-1} <= j } (E_{-1} = 0, E non-decreasing), so every input
+  // i in [ia, ib] marks the output where its run of copies starts (clamped to
+  // the item's first output), and a prefix max over the item's outputs fills
+  // the runs.  The ancestors of [j0, j1) all lie in [ia, ib].
+  for (int k = t; k < GG_QCHUNK; k += GG_BLOCK) sA[k] = -1;
+  __syncthreads();
+  for (int k = la + t; k <= lb; k += GG_BLOCK) {
+    const int i = ta * GG_TILE + k;
+    if (i >= n) break;
+    const int prev = (k == 0) ? sK[WT] : sE[k - 1];  // E_{i-1}
+    const int slot = prev - j0;
+    if (slot < nout) atomicMax(&sA[slot < 0 ? 0 : slot], i);
+  }
+  __syncthreads();
   {
-    int idx = la;
-    const int kb = t * RUN;
-    const int ke = min(kb + RUN, j1 - j0);
-    for (int k = kb; k < ke; ++k) {
-      idx = gg_first_greater(sE, idx, lb, j0 + k);
-      sA[k] = ta * GG_TILE + idx;
+    int v[RUN];
+    int run = -1;
+#pragma unroll
+    for (int q = 0; q < RUN; ++q) {
+      run = max(run, sA[t * RUN + q]);
+      v[q] = run;
     }
+    const int incl = gg_block_prefix_max256_i(run, ilds);
+    sM[t] = incl;
+    __syncthreads();
+    const int excl = (t == 0) ? -1 : sM[t - 1];
+#pragma unroll
+    for (int q = 0; q < RUN; ++q) sA[t * RUN + q] = max(excl, v[q]);
   }
   __syncthreads();
   gg_gather_copy<D>(a, f, F, chunk, dy, cur, j0, j1 - j0, 0, 0, sA, lds, ilds);
