@@ -344,7 +344,10 @@ int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
 // CDF ends of each item's tiles, merge-path gather and finalize
 int launch_sysres(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
   const gg_arena a = arena_of(h);
-  const size_t smem = with_decide ? decide_smem(h) : 0;
+  // tile sums (with_decide) and the end counts K_b of every tile, later the
+  // ancestors of an item's outputs
+  const size_t smem = std::max(decide_smem(h) + sizeof(int32_t) * (size_t)(max_tiles(h) + 1),
+                               sizeof(int32_t) * (size_t)GG_QCHUNK);
   GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_sysres<NI>), dim3(h->n_chunks), dim3(GG_BLOCK),
                                        smem, a, s, force, with_decide));
   GG_HIP(hipGetLastError());
@@ -516,7 +519,9 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
     F.n = n;
     F.n_tiles = (n + GG_TILE - 1) / GG_TILE;
     F.tile_off = tiles;
-    F.n_chunks = (int32_t)((2 * (int64_t)n + GG_QCHUNK - 1) / GG_QCHUNK);
+    // systematic resampling items: one per tile plus one per GG_QCHUNK outputs bounds
+    // sum_b ceil(outputs_b / GG_QCHUNK)
+    F.n_chunks = F.n_tiles + (int32_t)(((int64_t)n + GG_QCHUNK - 1) / GG_QCHUNK);
     F.chunk_off = chunks;
     F.sat_off = rows;
     F.sat_cap = std::max(sat_cap ? sat_cap[f] : 64, 1);

@@ -398,29 +398,11 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 }
 
 // ---------------------------------------------------------------------------
-// Resampling gather (systematic merge path, or ancestors from Megopolis).
-//
-// Systematic: work item = GG_QCHUNK consecutive steps of the merge of the
-// particles' positions pos_i = i + E_i (strictly increasing) with the outputs
-// j = 0..n-1; every item handles the same number of (input + output) steps
-// whatever the weights, so its inputs lie in at most WT consecutive tiles.
-// With PM_b the inclusive prefix max of the tile maxima, the global end of
-// particle i in tile b is max(PM_{b-1}, E_i) and the last position of tile b
-// is TE_b = last_b + PM_b.  A work item:
-//   1. builds PM for its filter in LDS (thread t owns tiles [t*c, t*c+c)),
-//   2. finds the tile A holding its start diagonal d0 (the tile with
-//      TE_{A-1} < d0 <= TE_A; each thread tests its own tiles),
-//   3. loads the ends of tiles A .. A+WT-1 into LDS and counts positions below
-//      d0 and d1 -> input range [i0, i1),
-//   4. ancestor of output j in [d0 - i0, d1 - i1) = max{ i in [i0, i1] : E_{i-1} <= j }:
-//      every input marks the output where its run of copies starts, and a
-//      block prefix max over the outputs fills the runs,
-//   5. copies the state of the ancestors (all loads of a thread in flight
-//      before its stores).
-// anc_mode: 0 systematic merge path, 1 ancestors from anc[src_buf], 2 identity,
-// 3 ancestors from anc[src_buf] with the states read from the packed copy that
-// gg_k_mego_w wrote (aos).
-// Dynamic LDS: 4 * max(n_tiles) bytes for PM.
+// Resampling gather: copies the state of each output's ancestor (all loads of
+// a thread in flight before its stores), then the item's finalize.
+// anc_mode: 0 the ancestors the systematic resampler staged in LDS (sA),
+// 1 ancestors from anc[src_buf], 2 identity, 3 ancestors from anc[src_buf]
+// with the states read from the packed copy that gg_k_mego_w wrote (aos).
 // ---------------------------------------------------------------------------
 
 // End of a gather work item: the chunk's state sums go to its slot, then the
@@ -437,8 +419,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 // launch it replaces (gather 23 -> 52 us at 1M particles).
 template <int D>
 __device__ __forceinline__ void gg_gather_finish(const gg_arena& a, int f, const gg_filter& F,
-                                                 int chunk, const double (&tot)[D + 2],
-                                                 double* lds, int* ilds) {
+                                                 int chunk, int n_items,
+                                                 const double (&tot)[D + 2], double* lds,
+                                                 int* ilds) {
   const int t = threadIdx.x;
   if (t == 0) {
     for (int d = 0; d < D; ++d)
@@ -446,18 +429,31 @@ __device__ __forceinline__ void gg_gather_finish(const gg_arena& a, int f, const
                          __HIP_MEMORY_SCOPE_AGENT);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     ilds[0] = gg_arrive_last(a.fin + (int64_t)f * GG_ARRIVE_WORDS, chunk - F.chunk_off,
-                             F.n_chunks) ? 1 : 0;
+                             n_items) ? 1 : 0;
   }
   __syncthreads();
   if (!ilds[0]) return;
   double v[D + 2];
 #pragma unroll
   for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
-  for (int cc = t; cc < F.n_chunks; cc += GG_BLOCK)
+  // four chunks' sums in flight per thread before they are added (in chunk order)
+  constexpr int U = 4;
+  for (int c0 = t; c0 < n_items; c0 += U * GG_BLOCK) {
+    double x[U][D];
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-      v[2 + d] = v[2 + d] + __hip_atomic_load(a.csum + (int64_t)(F.chunk_off + cc) * GG_MAX_D + d,
-                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int u = 0; u < U; ++u) {
+      const int cc = min(c0 + u * GG_BLOCK, n_items - 1);
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        x[u][d] = __hip_atomic_load(a.csum + (int64_t)(F.chunk_off + cc) * GG_MAX_D + d,
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c0 + u * GG_BLOCK < n_items)
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + x[u][d];
+  }
   double sum[D + 2];
   gg_block_totals<D>(v, lds, sum);
   // the record may have been written in this launch by another workgroup (the
@@ -491,7 +487,8 @@ __device__ __forceinline__ void gg_gather_finish(const gg_arena& a, int f, const
 // load of a thread issued before its stores, then the item's finalize.
 template <int D>
 __device__ __forceinline__ void gg_gather_copy(const gg_arena& a, int f, const gg_filter& F,
-                                               int chunk, const gg_dyn* dy, int cur, int jbase,
+                                               int chunk, int n_items, const gg_dyn* dy, int cur,
+                                               int jbase,
                                                int nout, int anc_mode, int src_buf,
                                                const int* sA, double* lds, int* ilds) {
   const int t = threadIdx.x;
@@ -555,7 +552,7 @@ __device__ __forceinline__ void gg_gather_copy(const gg_arena& a, int f, const g
   }
   double tot[D + 2];
   gg_block_totals<D>(acc, lds, tot);
-  gg_gather_finish<D>(a, f, F, chunk, tot, lds, ilds);
+  gg_gather_finish<D>(a, f, F, chunk, n_items, tot, lds, ilds);
 }
 
 // Gather by ancestor indices (anc_mode 1: anc[src_buf], 2: identity, 3:
@@ -574,13 +571,15 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
   const int cl = chunk - F.chunk_off;
   const int jbase = cl * GG_QCHUNK;
   const int nout = min(max(F.n - jbase, 0), GG_QCHUNK);
-  gg_gather_copy<D>(a, f, F, chunk, dy, dy->cur, jbase, nout, anc_mode, src_buf, nullptr, lds,
-                    ilds);
+  gg_gather_copy<D>(a, f, F, chunk, F.n_chunks, dy, dy->cur, jbase, nout, anc_mode, src_buf,
+                    nullptr, lds, ilds);
 }
 
 // ---------------------------------------------------------------------------
-// Systematic resampling in one launch (one workgroup per GG_QCHUNK steps of
-// the merge of the particles' positions pos_i = i + E_i with the outputs).
+// Systematic resampling in one launch.  Work item = one input tile and at
+// most GG_QCHUNK of its outputs (a tile with more outputs gets several items),
+// so every item handles the same inputs whatever the weights and at most
+// GG_QCHUNK outputs.
 //
 // with_decide (the device-driven step): every workgroup combines its filter's
 // tile partials itself (gg_combine, identical in every workgroup) and takes
@@ -596,33 +595,37 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_gather(gg_arena a, gg_step_args
 // K_{b+1} = #{j : t_j <= P_{b+1} / W} (P_nt = W); the tile's last particle
 // takes K_{b+1} and the filter's last particle N (the hi = n-1 clamp of the
 // reference's binary search, pf_device.cu:1200-1215).  E is then
-// non-decreasing across the whole filter and every tile's last position is
-// TE_b = last_b + K_{b+1}, known from the offsets alone, so an item finds its
-// first tile without any cross-tile prefix and computes the ends of the (at
-// most three) tiles its inputs span itself.  The ancestor of output j is
-// #{i : E_i <= j} = max{i : E_{i-1} <= j} (a marker per input and a prefix
-// max, no search), then the gather and the finalize.
-// Dynamic LDS: 8 * max(n_tiles) bytes (with_decide).
+// non-decreasing across the whole filter and tile b's outputs are exactly
+// [K_b, K_{b+1}).  Every item computes K_b of all tiles (thread t owns tiles
+// [t*c, t*c+c) of the chunked scan), the tiles' item counts
+// ceil((K_{b+1} - K_b) / GG_QCHUNK) and their prefix sum, which gives its
+// tile b and output range [j0, j1); then the ends of tile b alone.  The
+// ancestor of output j is #{i : E_i <= j} = max{i : E_{i-1} <= j}: the input
+// whose run of copies [E_{i-1}, E_i) meets [j0, j1) marks the output where
+// that run starts (clamped to j0; no two inputs mark one output), and a block
+// prefix max over the outputs fills the runs.  Then the gather and the
+// finalize (items without outputs still count themselves in).
+// Dynamic LDS: max(8 * max(n_tiles) + 4 * (max(n_tiles) + 1), 4 * GG_QCHUNK)
+// bytes: the tile sums and end counts, then (once the item knows its tile)
+// the ancestors of its outputs in the same memory.  Six workgroups per CU
+// (at most 80 VGPRs) keep the n_tiles + n / GG_QCHUNK items of a 1M-particle
+// filter in one round.
 // ---------------------------------------------------------------------------
 template <int NISB>
-__global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args s, int force,
-                                                        int with_decide) {
+__global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
+void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
 #pragma clang fp contract(off)
   constexpr int D = 4 + NISB;
-  constexpr int WT = GG_QCHUNK / GG_TILE + 1;  // tiles spanned by one item's inputs
-  constexpr int RUN = GG_QCHUNK / GG_BLOCK;    // consecutive outputs per thread
+  constexpr int RUN = GG_QCHUNK / GG_BLOCK;  // outputs per thread in the prefix max
   extern __shared__ __align__(16) char smem[];
-  double* sT = reinterpret_cast<double*>(smem);
-  __shared__ int sE[WT * GG_TILE];
-  __shared__ int sA[GG_QCHUNK];
+  __shared__ int sE[GG_TILE];
   __shared__ int ilds[8];
-  __shared__ int sI[4];
   __shared__ int sflag;
-  __shared__ double lds[WT * 264];
+  __shared__ double lds[8 + GG_BLOCK];
   __shared__ double sx[2];
   __shared__ double sEx[GG_BLOCK + 1];  // exclusive prefix of each thread's tile chunk
-  __shared__ double stile[WT][4];       // P_b, g_b, m_b of tiles ta .. ta+WT-1
-  __shared__ int sK[WT + 1];            // K_{b+1} of the same tiles; sK[WT] = K_ta
+  __shared__ int sItem[4];              // tile b, first output j0, last output j1, K_b
+  __shared__ double sTile[3];           // P_b, g_b, m_b of the item's tile
   __shared__ int sM[GG_BLOCK];          // per-thread prefix maxima of the ancestor markers
   __shared__ gg_dyn sd;
   __shared__ double svg[3];
@@ -633,6 +636,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
   const int cl = chunk - F.chunk_off;
   const int n = F.n;
   const int nt = F.n_tiles;
+  double* sT = reinterpret_cast<double*>(smem);
+  int* sKt = reinterpret_cast<int*>(smem + sizeof(double) * (size_t)nt);  // K_b, b in [0, nt]
+  int* sA = reinterpret_cast<int*>(smem);  // ancestors of the outputs, once sT and sKt are read
   const int t = threadIdx.x;
   const int cur = dy->cur;
   // thread t owns tiles [b0, b1) of the chunked scan
@@ -684,165 +690,119 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
     u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
     lwz = dy->lw_zero != 0;
   }
-  // P_b for b in [0, nt] (P_nt = W), from the combine's chunked scan (the
-  // owner's exclusive prefix plus the owner's running sum, as gg_combine forms
-  // them) or from a.Pb
-  auto P_of = [&](int b) -> double {
-    if (b >= nt) return W;
-    if (!with_decide) return a.Pb[F.tile_off + b];
-    const int o = b / c;
-    double run = 0.0;
-    for (int k = o * c; k < b; ++k) run = run + sT[k];
-    return sEx[o] + run;
-  };
-  const int64_t d0 = (int64_t)cl * GG_QCHUNK;
-  const int64_t d1 = min(d0 + (int64_t)GG_QCHUNK, 2 * (int64_t)n);
-  // the tile holding diagonal d0 (TE strictly increasing, TE_{nt-1} = 2n - 1)
-  if (t == 0) sI[0] = nt - 1;
-  __syncthreads();
-  if (b0 < b1) {
-    const double ex = with_decide ? sEx[t] : 0.0;
-    double run = 0.0;  // this thread's running tile sum (with_decide)
-    int kb = gg_kcount((with_decide ? ex + run : a.Pb[F.tile_off + b0]) / W, u0, n);  // K_b
-    for (int b = b0; b < b1; ++b) {
-      double pn;  // P_{b+1}
-      if (b + 1 == b1) {
-        pn = P_of(b + 1);
-      } else if (with_decide) {
-        run = run + sT[b];
-        pn = ex + run;
-      } else {
-        pn = a.Pb[F.tile_off + b + 1];
-      }
-      const int kn = gg_kcount(pn / W, u0, n);  // K_{b+1}
-      // the tile's last particle ends at K_{b+1}; the filter's last at N
-      const int64_t te = (int64_t)min((b + 1) * GG_TILE, n) - 1 + ((b + 1 == nt) ? n : kn);
-      const int64_t tp = (b == 0) ? -1 : (int64_t)(b * GG_TILE - 1) + kb;
-      if (tp < d0 && te >= d0) sI[0] = b;
-      kb = kn;
-    }
-  }
-  __syncthreads();
-  const int ta = sI[0];
-  // offsets, scales and end counts of tiles ta .. ta+WT-1
-  if (t < WT) {
-    const int b = ta + t;
-    if (b < nt) {
-      const double* pb = a.part + (int64_t)(F.tile_off + b) * GG_NPART;
-      stile[t][0] = P_of(b);
-      stile[t][1] = with_decide ? gg_tile_shift(pb[GG_NPART - 1], pb[GG_NPART - 2] - K)
-                                : a.g[F.tile_off + b];
-      stile[t][2] = pb[0];
-      sK[t] = gg_kcount(P_of(b + 1) / W, u0, n);
-    }
-  } else if (t == WT) {
-    sK[WT] = (ta == 0) ? 0 : gg_kcount(P_of(ta) / W, u0, n);
-  }
-  __syncthreads();
-  // ends of the item's tiles: in-tile scans of e, counts clamped to K_{b+1};
-  // a tile whose first position is past the item's last diagonal is not
-  // needed (its particles are neither inputs of the item nor searched)
+  // K_b of the thread's tiles (P_b from the combine's chunked scan, the
+  // owner's exclusive prefix plus the owner's running sum as gg_combine forms
+  // them, or from a.Pb) and their item counts
+  int mt = 0;
   {
-    bool need[WT];
-#pragma unroll
-    for (int w = 0; w < WT; ++w) {
-      const int b = ta + w;
-      need[w] = b < nt;
-      if (w > 0 && need[w]) {
-        const int64_t te_prev = (int64_t)min(b * GG_TILE, n) - 1 + sK[w - 1];
-        need[w] = need[w - 1] && te_prev + 1 < d1;
-      }
+    double run = 0.0;
+    for (int b = b0; b < b1; ++b) {
+      const double pb = with_decide ? sEx[t] + run : a.Pb[F.tile_off + b];
+      if (with_decide) run = run + sT[b];
+      const int kb = (b == 0) ? 0 : gg_kcount(pb / W, u0, n);
+      sKt[b] = kb;
     }
-    float lw[WT][GG_ITEMS];
-    double l[WT][GG_ITEMS], last[WT];
-#pragma unroll
-    for (int w = 0; w < WT; ++w) {
-      const int b = ta + w;
-      const int i0 = b * GG_TILE + t * GG_ITEMS;
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (need[w] && !lwz) v = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
-      lw[w][0] = v.x;
-      lw[w][1] = v.y;
-      lw[w][2] = v.z;
-      lw[w][3] = v.w;
-    }
-#pragma unroll
-    for (int w = 0; w < WT; ++w) {
-      const int b = ta + w;
-      const int i0 = b * GG_TILE + t * GG_ITEMS;
-#pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) l[w][q] = 0.0;
-      if (need[w]) {  // uniform over the workgroup
-        const float mbf = (float)stile[w][2];
-        double run = 0.0;
-#pragma unroll
-        for (int q = 0; q < GG_ITEMS; ++q) {
-          const bool valid = (i0 + q) < n;
-          const double e =
-              (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[w][q] - mbf) : 0.0;
-          run = (q == 0) ? e : run + e;
-          l[w][q] = run;
-        }
-      }
-      last[w] = l[w][GG_ITEMS - 1];
-    }
-    double excl[WT], total[WT];
-    gg_block_scan256_n<WT>(last, lds, excl, total);
-#pragma unroll
-    for (int w = 0; w < WT; ++w) {
-      const int b = ta + w;
-      const int i0 = b * GG_TILE + t * GG_ITEMS;
-#pragma unroll
-      for (int q = 0; q < GG_ITEMS; ++q) {
-        const int i = i0 + q;
-        int e;
-        if (!need[w] || i >= n) {
-          e = 0x3fffffff;
-        } else if (i >= n - 1) {
-          e = n;
-        } else if (t * GG_ITEMS + q == GG_TILE - 1) {
-          e = sK[w];
-        } else {
-          const double L = excl[w] + l[w][q];
-          e = min(gg_kcount((stile[w][0] + stile[w][1] * L) / W, u0, n), sK[w]);
-        }
-        sE[w * GG_TILE + t * GG_ITEMS + q] = e;
-      }
-    }
+    if (t == 0) sKt[nt] = n;
   }
-  // positions below d0 / d1 in the item's tiles
-  int c0 = 0, c1 = 0;
-  for (int k0 = 4 * t; k0 < WT * GG_TILE; k0 += 4 * GG_BLOCK) {
-    const int i0 = ta * GG_TILE + k0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = i0 + q;
-      const int64_t pos = (int64_t)i + sE[k0 + q];
-      c0 += (i < n && pos < d0) ? 1 : 0;
-      c1 += (i < n && pos < d1) ? 1 : 0;
+  __syncthreads();
+  for (int b = b0; b < b1; ++b) mt += (sKt[b + 1] - sKt[b] + GG_QCHUNK - 1) / GG_QCHUNK;
+  double mex, mtot;
+  gg_block_scan256((double)mt, lds, mex, mtot);  // exact: integers below 2^53
+  if (t == 0) sItem[0] = -1;
+  __syncthreads();
+  {
+    int start = (int)mex;
+    for (int b = b0; b < b1; ++b) {
+      const int kb = sKt[b], kn = sKt[b + 1];
+      const int m = (kn - kb + GG_QCHUNK - 1) / GG_QCHUNK;
+      if (cl >= start && cl < start + m) {
+        const int j0 = kb + (cl - start) * GG_QCHUNK;
+        sItem[0] = b;
+        sItem[1] = j0;
+        sItem[2] = min(j0 + GG_QCHUNK, kn);
+        sItem[3] = kb;
+      }
+      start += m;
     }
   }
   __syncthreads();
-  c0 = gg_block_sum256_i(c0, ilds);
-  c1 = gg_block_sum256_i(c1, ilds);
-  const int ia = ta * GG_TILE + c0;
-  const int ib = ta * GG_TILE + c1;
-  const int j0 = (int)(d0 - ia), j1 = (int)(d1 - ib);
-  const int la = c0, lb = c1;  // local range in sE
+  const int tb = sItem[0];
+  if (tb < 0) {  // no outputs for this item: count it in with zero sums
+    gg_gather_copy<D>(a, f, F, chunk, F.n_chunks, dy, cur, 0, 0, 0, 0, sA, lds, ilds);
+    return;
+  }
+  const int j0 = sItem[1], j1 = sItem[2], kb = sItem[3];
+  const int kn = sKt[tb + 1];  // read before the __syncthreads that precedes the sA writes
   const int nout = j1 - j0;
-  // ancestors without searching: the ancestor of output j is
-  // max{ i : E_{i-1} <= j } (E_{-1} = 0, E non-decreasing), so every input
-  // i in [ia, ib] marks the output where its run of copies starts (clamped to
-  // the item's first output), and a prefix max over the item's outputs fills
-  // the runs.  The ancestors of [j0, j1) all lie in [ia, ib].
+  // the tile's offset, scale and maximum, beside its log-weights
+  const int i0 = tb * GG_TILE + t * GG_ITEMS;
+  float4 lv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!lwz && i0 < n) lv = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
+  if (t == 0) {
+    const double* pb = a.part + (int64_t)(F.tile_off + tb) * GG_NPART;
+    double P;
+    if (with_decide) {
+      const int o = tb / c;
+      double run = 0.0;
+      for (int k = o * c; k < tb; ++k) run = run + sT[k];
+      P = sEx[o] + run;
+    } else {
+      P = a.Pb[F.tile_off + tb];
+    }
+    sTile[0] = P;
+    sTile[1] = with_decide ? gg_tile_shift(pb[GG_NPART - 1], pb[GG_NPART - 2] - K)
+                           : a.g[F.tile_off + tb];
+    sTile[2] = pb[0];
+  }
+  __syncthreads();
+  // ends of the tile's particles: in-tile scan of e, counts clamped to K_{b+1}
+  int e4[GG_ITEMS];
+  {
+    const float lw[GG_ITEMS] = {lv.x, lv.y, lv.z, lv.w};
+    const float mbf = (float)sTile[2];
+    double l[GG_ITEMS];
+    double run = 0.0;
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const bool valid = (i0 + q) < n;
+      const double e =
+          (valid && mbf > -__builtin_huge_valf()) ? (double)gg_exp32(lw[q] - mbf) : 0.0;
+      run = (q == 0) ? e : run + e;
+      l[q] = run;
+    }
+    double excl, total;
+    gg_block_scan256(l[GG_ITEMS - 1], lds, excl, total);
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const int i = i0 + q;
+      int e;
+      if (i >= n) {
+        e = 0x3fffffff;
+      } else if (i >= n - 1) {
+        e = n;
+      } else if (t * GG_ITEMS + q == GG_TILE - 1) {
+        e = kn;
+      } else {
+        const double L = excl + l[q];
+        e = min(gg_kcount((sTile[0] + sTile[1] * L) / W, u0, n), kn);
+      }
+      e4[q] = e;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) sE[t * GG_ITEMS + q] = e4[q];
   for (int k = t; k < GG_QCHUNK; k += GG_BLOCK) sA[k] = -1;
   __syncthreads();
-  for (int k = la + t; k <= lb; k += GG_BLOCK) {
-    const int i = ta * GG_TILE + k;
-    if (i >= n) break;
-    const int prev = (k == 0) ? sK[WT] : sE[k - 1];  // E_{i-1}
-    const int slot = prev - j0;
-    if (slot < nout) atomicMax(&sA[slot < 0 ? 0 : slot], i);
+  // markers: the input whose run [E_{i-1}, E_i) meets [j0, j1) marks its start
+  {
+    int prev = (t == 0) ? kb : sE[t * GG_ITEMS - 1];
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) {
+      const int i = i0 + q;
+      const int e = e4[q];
+      if (i < n && e > prev && prev < j1 && e > j0) sA[prev > j0 ? prev - j0 : 0] = i;
+      prev = e;
+    }
   }
   __syncthreads();
   {
@@ -861,7 +821,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_sysres(gg_arena a, gg_step_args
     for (int q = 0; q < RUN; ++q) sA[t * RUN + q] = max(excl, v[q]);
   }
   __syncthreads();
-  gg_gather_copy<D>(a, f, F, chunk, dy, cur, j0, j1 - j0, 0, 0, sA, lds, ilds);
+  gg_gather_copy<D>(a, f, F, chunk, F.n_chunks, dy, cur, j0, nout, 0, 0, sA, lds, ilds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,7 +1034,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
   }
   double tot[D + 2];
   gg_block_totals<D>(acc, lds, tot);
-  gg_gather_finish<D>(a, f, F, F.chunk_off + lt, tot, lds, ilds);
+  gg_gather_finish<D>(a, f, F, F.chunk_off + lt, F.n_tiles, tot, lds, ilds);
 }
 
 // ---------------------------------------------------------------------------

@@ -24,7 +24,7 @@
 #define GG_BLOCK 256
 #define GG_ITEMS 4
 #define GG_TILE (GG_BLOCK * GG_ITEMS)
-#define GG_QCHUNK (2 * GG_TILE)  // merge-path steps per gather work item
+#define GG_QCHUNK (2 * GG_TILE)  // outputs per resampling work item (at most)
 #define GG_MAX_PARTICLES (7680 * GG_TILE)  // per filter (per-tile values kept in LDS)
 #define GG_MAX_ISB 3
 #define GG_MAX_D (4 + GG_MAX_ISB)
@@ -52,7 +52,7 @@ struct gg_filter {
   int32_t n;         // live particles
   int32_t n_tiles;   // ceil(n / GG_TILE)
   int32_t tile_off;  // first global tile id
-  int32_t n_chunks;  // gather work items: ceil(2n / GG_QCHUNK)
+  int32_t n_chunks;  // resampling work items: n_tiles + ceil(n / GG_QCHUNK)
   int32_t chunk_off; // first global gather chunk id
   int32_t sat_off;   // first satellite row in an epoch slot
   int32_t sat_cap;   // satellite rows reserved for this filter
