@@ -964,7 +964,9 @@ def main():
     from gnss_gpu_amd._lib import LIK_STUDENT
 
     N = args.particles
-    T = args.warmup + args.steps + args.profile_steps + 2 * max(args.graph_steps, 1) + 8
+    G0 = max(args.graph_steps, 1)
+    T = (args.warmup + args.steps + args.profile_steps + 2 * G0 + 8
+         + max(0, 20 - args.steps // G0) * G0)  # replays for the spread statistics
     sc = sim.scenario_multi_gnss(n_epochs=T, seed=0)
     S = sc.n_sat
     D = 7
@@ -1009,9 +1011,17 @@ def main():
     pf.sync()
     torch.cuda.synchronize()
     rc1 = pf.read_dyn()[0].resample_count
-    per_step = per_replay / G
     ms_step = float(per_replay.sum()) / args.steps
     resample_fraction = (rc1 - rc0) / args.steps
+    # the spread of the step time: the timed replays, plus (after the timed
+    # region, so the value above is unaffected) further replays of the same
+    # graph until at least 20 are on record
+    n_extra = max(0, 20 - len(per_replay))
+    stats_replays = per_replay
+    if n_extra:
+        stats_replays = np.concatenate([per_replay, pf.time_replays_each(n_extra)])
+        pf.sync()
+    per_step = stats_replays / G
 
     # ---- per-kernel device time inside real steps: eager steps after the
     # timed replays, every kernel bracketed by events that its own launch
@@ -1113,7 +1123,9 @@ def main():
         "particles_per_s": round(N / (ms_step * 1e-3), 1),
         "median_ms": round(float(np.median(per_step)), 5),
         "p90_ms": round(float(np.percentile(per_step, 90)), 5),
-        "per_step_stats_over": f"{len(per_replay)} replays of {G} steps",
+        "per_step_stats_over": (f"{len(stats_replays)} replays of {G} steps ({len(per_replay)} "
+                                f"timed, {len(stats_replays) - len(per_replay)} after the timed "
+                                f"region)"),
         "resample_fraction": round(resample_fraction, 3),
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "stage_ms_resampling_steps": {k: round(v, 5) for k, v in ms_res.items()},

@@ -21,6 +21,8 @@ Additions (keyword-only, reference behaviour by default):
 
 from __future__ import annotations
 
+from types import SimpleNamespace
+
 import numpy as np
 
 from . import _gnss_gpu_pf_device as _dev
@@ -509,12 +511,196 @@ class ParticleFilterDevice:
         synchronise the stream in use."""
         _dev.pf_device_set_stream(self._state, stream)
 
-    # -- smoother (not part of the forward step on this backend) ----------------
+    # -- forward-backward smoother (particle_filter_device.py:892-1251) ---------
+    # Host orchestration over this class's own device updates: the forward pass
+    # records each epoch's inputs, smooth() runs a second filter backwards over
+    # them and averages the two tracks.  Epoch records are dicts with the
+    # reference's keys, which its tests build directly.
     def enable_smoothing(self):
-        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")
+        """Start recording epochs (store_epoch) for a later smooth()."""
+        self._smooth_epochs = []
+        self._smooth_enabled = True
 
-    def store_epoch(self, *args, **kwargs):
-        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")
+    def store_epoch(self, sat_ecef, pseudoranges, weights, velocity, dt, spp_ref=None,
+                    dd_pseudorange=None, dd_pseudorange_sigma=None, dd_pseudorange_source=None,
+                    dd_carrier=None, dd_carrier_sigma=None, carrier_anchor_pseudorange=None,
+                    carrier_anchor_sigma=None, carrier_afv=None, carrier_afv_sigma=None,
+                    carrier_afv_wavelength=None, doppler_update=None, doppler_sigma_mps=None,
+                    doppler_velocity_update_gain=None, doppler_max_velocity_update_mps=None,
+                    *, system_ids=None):
+        """Record this epoch's forward estimate and the observations used (after
+        update/estimate); a no-op until enable_smoothing().  ``system_ids``
+        (keyword-only) replays multi-GNSS pseudoranges for filters with ISB states."""
+        if not getattr(self, "_smooth_enabled", False):
+            return
+        rec = {
+            "estimate": np.asarray(self.estimate(), dtype=np.float64)[:3].copy(),
+            "sat_ecef": _f64(sat_ecef),
+            "pseudoranges": _f64(pseudoranges),
+            "weights": _f64(weights),
+            "velocity": None if velocity is None else _f64(velocity),
+            "dt": float(dt),
+            "spp_ref": None if spp_ref is None else _f64(spp_ref),
+            "dd_pseudorange": _dd_record(dd_pseudorange, _DD_PR_FIELDS),
+            "dd_pseudorange_sigma": _opt_float(dd_pseudorange_sigma),
+            "dd_pseudorange_source": (None if dd_pseudorange_source is None
+                                      else str(dd_pseudorange_source)),
+            "dd_carrier": _dd_record(dd_carrier, _DD_CP_FIELDS),
+            "dd_carrier_sigma": _opt_float(dd_carrier_sigma),
+            "carrier_anchor_pseudorange": _rows_record(
+                carrier_anchor_pseudorange, ("sat_ecef", "pseudoranges", "weights"),
+                "pseudoranges"),
+            "carrier_anchor_sigma": _opt_float(carrier_anchor_sigma),
+            "carrier_afv": _rows_record(
+                carrier_afv, ("sat_ecef", "carrier_phase_cycles", "weights"),
+                "carrier_phase_cycles"),
+            "carrier_afv_sigma": _opt_float(carrier_afv_sigma),
+            "carrier_afv_wavelength": _opt_float(carrier_afv_wavelength),
+            "doppler_update": _doppler_record(doppler_update),
+            "doppler_sigma_mps": _opt_float(doppler_sigma_mps),
+            "doppler_velocity_update_gain": _opt_float(doppler_velocity_update_gain),
+            "doppler_max_velocity_update_mps": _opt_float(doppler_max_velocity_update_mps),
+        }
+        if system_ids is not None:
+            rec["system_ids"] = np.asarray(system_ids, dtype=np.int32).ravel().copy()
+        self._smooth_epochs.append(rec)
 
-    def smooth(self, *args, **kwargs):
-        raise NotImplementedError("forward-backward smoothing is not implemented on this backend")
+    def smooth(self, position_update_sigma=None, skip_widelane_dd_pseudorange=False):
+        """Backward pass over the recorded epochs; returns (smoothed, forward),
+        both (n_epochs, 3) ECEF, smoothed = (forward + backward) / 2.
+
+        The backward filter has this filter's configuration and seed + 1, starts
+        at the last forward estimate (clock bias: median pseudorange residual
+        there; spreads 10 m / 100 m) and visits the epochs last to first with the
+        velocity negated, replaying per epoch: the DD pseudorange update (or,
+        without one or for a wide-lane one when ``skip_widelane_dd_pseudorange``,
+        the clock-bias re-centring and the pseudorange update), the DD carrier
+        AFV, the carrier-anchor pseudoranges, the undifferenced carrier AFV and,
+        given ``position_update_sigma``, the SPP position update.  Doppler rows
+        are not replayed (they constrain the forward-time velocity)."""
+        epochs = getattr(self, "_smooth_epochs", None)
+        if not getattr(self, "_smooth_enabled", False) or not epochs:
+            raise RuntimeError("No stored epochs. Call enable_smoothing() before forward pass.")
+        forward = np.array([ep["estimate"] for ep in epochs], dtype=np.float64)
+        last = epochs[-1]
+        start = forward[-1]
+        sat_last = np.asarray(last["sat_ecef"], dtype=np.float64).reshape(-1, 3)
+        cb0 = float(np.median(np.asarray(last["pseudoranges"], dtype=np.float64)
+                              - np.linalg.norm(sat_last - start, axis=1)))
+        # looked up at call time, so a replacement of the module's class is used
+        bwd = ParticleFilterDevice(**_backward_config(self))
+        bwd.initialize(start, clock_bias=cb0, spread_pos=10.0, spread_cb=100.0,
+                       velocity_init_sigma=float(getattr(self, "_velocity_init_sigma", 0.0)))
+        backward = np.zeros_like(forward)
+        for i in range(len(epochs) - 1, -1, -1):
+            _replay_backward(self, bwd, epochs[i], position_update_sigma,
+                             skip_widelane_dd_pseudorange)
+            backward[i] = np.asarray(bwd.estimate(), dtype=np.float64)[:3]
+        self._smooth_enabled = False
+        self._smooth_epochs = []
+        return (forward + backward) / 2.0, forward
+
+
+# -- smoother records and the backward replay ----------------------------------
+_DD_PR_FIELDS = ("dd_pseudorange_m", "sat_ecef_k", "sat_ecef_ref", "base_range_k",
+                 "base_range_ref", "dd_weights")
+_DD_CP_FIELDS = ("dd_carrier_cycles", "sat_ecef_k", "sat_ecef_ref", "base_range_k",
+                 "base_range_ref", "dd_weights", "wavelengths_m")
+_DOPPLER_WAVELENGTH = 0.19029367279836488
+_CARRIER_WAVELENGTH = 0.190293673
+# constructor keywords copied into the backward filter (particle_filter_device.py:1117-1144)
+_CONFIG_KEYS = ("n_particles", "sigma_pos", "sigma_cb", "sigma_pr", "nu", "resampling",
+                "ess_threshold", "per_particle_nlos_gate", "per_particle_nlos_dd_pr_threshold_m",
+                "per_particle_nlos_dd_carrier_threshold_cycles",
+                "per_particle_nlos_undiff_pr_threshold_m", "per_particle_huber",
+                "per_particle_huber_dd_pr_k", "per_particle_huber_dd_carrier_k",
+                "per_particle_huber_undiff_pr_k", "sigma_vel", "velocity_guide_alpha",
+                "rbpf_velocity_kf", "velocity_process_noise")
+# this backend's keyword-only additions, carried when the filter has them
+_EXTRA_KEYS = ("n_isb", "sigma_isb", "megopolis_mode", "megopolis_iterations", "systematic_u0")
+
+
+def _f64(a):
+    return np.asarray(a, dtype=np.float64).copy()
+
+
+def _opt_float(v):
+    return None if v is None else float(v)
+
+
+def _dd_record(dd, fields):
+    """Copy of a DD result's arrays, ids and count (None passes through)."""
+    if dd is None:
+        return None
+    rec = {k: _f64(getattr(dd, k)) for k in fields}
+    rec["ref_sat_ids"] = tuple(getattr(dd, "ref_sat_ids", ()))
+    rec["n_dd"] = int(dd.n_dd)
+    return rec
+
+
+def _rows_record(obs, keys, count_key):
+    """Copy of a dict of per-satellite rows plus their count."""
+    if obs is None:
+        return None
+    rec = {k: _f64(obs[k]) for k in keys}
+    rec["n_sat"] = int(np.asarray(obs[count_key]).size)
+    return rec
+
+
+def _doppler_record(obs):
+    if obs is None:
+        return None
+    rec = {k: _f64(obs[k]) for k in ("sat_ecef", "sat_vel", "doppler_hz", "weights")}
+    rec["wavelength_m"] = float(obs.get("wavelength_m", _DOPPLER_WAVELENGTH))
+    rec["n_sat"] = int(np.asarray(obs["doppler_hz"]).size)
+    return rec
+
+
+def _backward_config(pf):
+    cfg = {k: getattr(pf, k) for k in _CONFIG_KEYS}
+    cfg["seed"] = pf.seed + 1
+    cfg.update({k: getattr(pf, k) for k in _EXTRA_KEYS if hasattr(pf, k)})
+    return cfg
+
+
+def _or(v, default):
+    return default if v is None else v
+
+
+def _replay_backward(pf, bwd, ep, position_update_sigma, skip_widelane):
+    """One epoch of the backward pass (particle_filter_device.py:1152-1243)."""
+    vel = ep["velocity"]
+    bwd.predict(velocity=None if vel is None else -np.asarray(vel, dtype=np.float64),
+                dt=ep["dt"])
+    sat = np.asarray(ep["sat_ecef"], dtype=np.float64).reshape(-1, 3)
+    dd = ep.get("dd_pseudorange")
+    if dd is not None and not (skip_widelane and ep.get("dd_pseudorange_source") == "widelane"):
+        bwd.update_dd_pseudorange(SimpleNamespace(**dd),
+                                  sigma_pr=float(_or(ep.get("dd_pseudorange_sigma"), pf.sigma_pr)))
+    else:
+        bwd.correct_clock_bias(sat, ep["pseudoranges"])
+        sys_ids = ep.get("system_ids")
+        if sys_ids is None:
+            bwd.update(sat, ep["pseudoranges"], weights=ep["weights"])
+        else:
+            bwd.update(sat, ep["pseudoranges"], weights=ep["weights"], system_ids=sys_ids)
+    dd_cp = ep.get("dd_carrier")
+    if dd_cp is not None:
+        bwd.resample_if_needed()
+        bwd.update_dd_carrier_afv(SimpleNamespace(**dd_cp),
+                                  sigma_cycles=float(_or(ep.get("dd_carrier_sigma"), 0.05)))
+    anchor = ep.get("carrier_anchor_pseudorange")
+    if anchor is not None:
+        bwd.update(anchor["sat_ecef"], anchor["pseudoranges"], weights=anchor["weights"],
+                   sigma_pr=float(_or(ep.get("carrier_anchor_sigma"), pf.sigma_pr)))
+    afv = ep.get("carrier_afv")
+    if afv is not None:
+        bwd.resample_if_needed()
+        bwd.update_carrier_afv(afv["sat_ecef"], afv["carrier_phase_cycles"],
+                               weights=afv["weights"],
+                               wavelength=float(_or(ep.get("carrier_afv_wavelength"),
+                                                    _CARRIER_WAVELENGTH)),
+                               sigma_cycles=float(_or(ep.get("carrier_afv_sigma"), 0.05)))
+    if position_update_sigma is not None and ep.get("spp_ref") is not None:
+        bwd.position_update(np.asarray(ep["spp_ref"], dtype=np.float64)[:3],
+                            sigma_pos=position_update_sigma)
