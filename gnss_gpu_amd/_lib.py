@@ -61,7 +61,10 @@ class StepArgs(C.Structure):
         ("gmm_w_los", C.c_float),
         ("gmm_mu_nlos", C.c_float),
         ("gmm_sigma_nlos", C.c_float),
-        ("gmm_pad", C.c_int32),
+        # derived by the library before every launch
+        ("gmm_c_los", C.c_float),
+        ("gmm_c_nlos", C.c_float),
+        ("gmm_ratio", C.c_float),
     ]
 
 

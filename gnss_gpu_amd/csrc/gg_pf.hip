@@ -269,7 +269,20 @@ void gg_launch(gg_pf* h, K kernel, const dim3& grid, const dim3& block, size_t s
   }
 }
 
-int launch_weight(gg_pf* h, const gg_step_args& s, bool pred, bool with_decide = true) {
+// The mixture likelihood's constants, from its binary32 parameters (binary64
+// logs, rounded once); every weight launch carries them.
+gg_step_args with_derived(const gg_step_args& in) {
+  gg_step_args s = in;
+  const double sl = (double)s.sigma_pr, sn = (double)s.gmm_sigma_nlos;
+  const double wl = (double)s.gmm_w_los;
+  s.gmm_c_los = (float)(std::log(wl) - std::log(sl));
+  s.gmm_c_nlos = (float)(std::log(1.0 - wl) - std::log(sn));
+  s.gmm_ratio = (float)((sl * sl) / (sn * sn));
+  return s;
+}
+
+int launch_weight(gg_pf* h, const gg_step_args& s_in, bool pred, bool with_decide = true) {
+  const gg_step_args s = with_derived(s_in);
   const bool learned = s.lik == GG_LIK_LEARNED;
   if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
   const bool table = learned && s.learned_mode == 1;

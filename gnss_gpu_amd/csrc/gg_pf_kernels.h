@@ -706,12 +706,11 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
     // per satellite log(w N(e; 0, sigma_los) + (1 - w) N(e; mu, sigma_nlos)) up
     // to the common -log sqrt(2 pi), formed as the reference does (max plus
     // log of the exponential sum); m.y = -w_s / (2 sigma_los^2), the NLOS
-    // factor is m.y sigma_los^2 / sigma_nlos^2
-    const double sl = (double)s.sigma_pr, sn = (double)s.gmm_sigma_nlos;
-    const double wl = (double)s.gmm_w_los;
-    const float c_los = (float)(::log(wl) - ::log(sl));
-    const float c_nlos = (float)(::log(1.0 - wl) - ::log(sn));
-    const float ratio = (float)((sl * sl) / (sn * sn));
+    // factor is m.y sigma_los^2 / sigma_nlos^2.  The log constants come from
+    // the host (binary64 logs here would run on every likelihood's path).
+    const float c_los = s.gmm_c_los;
+    const float c_nlos = s.gmm_c_nlos;
+    const float ratio = s.gmm_ratio;
     const float mu = s.gmm_mu_nlos;
     float lw[GG_ITEMS] = {0.0f, 0.0f, 0.0f, 0.0f};
     gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {

@@ -940,13 +940,17 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
   const int lane = threadIdx.x & 63;
   const uint32_t seg = (uint32_t)(lt * (GG_TILE / 256) + (threadIdx.x >> 6));
   const int ib = (int)seg * 256 + lane;  // particle of item q: ib + 64 q
+  // a chain past the filter's end holds wk = +inf: u wk = +inf is never below
+  // a proposal's weight (whatever its slot holds), so it never moves, and its
+  // proposal loads need no mask (j = (i + O) mod n stays inside the filter's
+  // padded slots)
   int k[GG_ITEMS];
   float wk[GG_ITEMS];
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) {
     const int i = ib + 64 * q;
     k[q] = i;
-    wk[q] = (i < n) ? w[i] : 0.0f;
+    wk[q] = (i < n) ? w[i] : __builtin_huge_valf();
   }
   // acceptance uniforms: one xoshiro128+ stream per chain, seeded by the
   // Philox block (seed, particle, call, GG_PURPOSE_MEGO_U, 0); iteration b
@@ -971,11 +975,13 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
         const int O = __builtin_amdgcn_readlane((int)myO, (c0 + cc < bend) ? c0 + cc : c0);
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const int i = ib + 64 * q;
-          int j = i + O;
-          if (j >= n) j -= n;
+          // (i + O) mod n for a chain of the filter (i, O < n): the smaller of
+          // i + O and i + O - n as unsigned; past the end, i + O - n lies
+          // below the filter's padded slot count
+          const uint32_t j0 = (uint32_t)(ib + 64 * q + O);
+          const int j = (int)min(j0, j0 - (uint32_t)n);
           jq[cc][q] = j;
-          wq[cc][q] = (i < n) ? w[j] : 0.0f;
+          wq[cc][q] = w[j];
         }
       }
 #pragma unroll
@@ -983,7 +989,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
         if (c0 + cc >= bend) break;
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          // chains past the end hold wk = wq = 0: never accept
           const float u = gg_u01f(gg_xo_next(xs[q]));
           const bool acc = u * wk[q] < wq[cc][q];
           k[q] = acc ? jq[cc][q] : k[q];

@@ -122,5 +122,10 @@ struct gg_step_args {
   float gmm_w_los;
   float gmm_mu_nlos;
   float gmm_sigma_nlos;
-  int32_t gmm_pad;
+  // derived by the library before every launch (callers leave them as they are):
+  // log(w_los) - log(sigma_pr), log(1 - w_los) - log(sigma_nlos) and
+  // sigma_pr^2 / sigma_nlos^2, in binary64 from the binary32 fields, rounded once
+  float gmm_c_los;
+  float gmm_c_nlos;
+  float gmm_ratio;
 };
