@@ -61,7 +61,15 @@ __host__ __device__ __forceinline__ gg_u32x4 gg_draw(uint64_t seed, uint32_t idx
 // one half, scaled by 2^-23, so every value is exact in binary32 and the
 // largest is 1 - 2^-24.
 __host__ __device__ __forceinline__ float gg_u01f(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the same value in two instructions: a funnel shift puts 0x3f800000 above
+  // the top 23 bits (1 + m 2^-23 in [1, 2)), then one exact subtraction of
+  // 1 - 2^-24 leaves (m + 0.5) 2^-23
+  const float f = __uint_as_float(__builtin_amdgcn_alignbit(0x7fu, x, 9u));
+  return f - 0.99999994039535522f;  // 1 - 2^-24
+#else
   return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f;  // 2^-23
+#endif
 }
 
 // two uint32 -> double uniform in (0, 1]; exact in binary64.

@@ -692,16 +692,19 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
         if (++cnt == 3) {
 #pragma unroll
           for (int q = 0; q < GG_ITEMS; ++q) {
-            acc[q] += gg_lnf(prod[q]);
+            acc[q] += __builtin_amdgcn_logf(prod[q]);  // log2: ln 2 applied once below
             prod[q] = 1.0f;
           }
           cnt = 0;
         }
       }
     });
-    const float hn = -0.5f * (s.nu + 1.0f);
+    const float hn = -0.5f * (s.nu + 1.0f) * 0.6931471805599453f;
+    if (cnt)  // uniform: every thread sees the same satellite count
 #pragma unroll
-    for (int q = 0; q < GG_ITEMS; ++q) out[q] = hn * (acc[q] + gg_lnf(prod[q]));
+      for (int q = 0; q < GG_ITEMS; ++q) acc[q] += __builtin_amdgcn_logf(prod[q]);
+#pragma unroll
+    for (int q = 0; q < GG_ITEMS; ++q) out[q] = hn * acc[q];
   } else if (s.lik == GG_LIK_GMM) {
     // per satellite log(w N(e; 0, sigma_los) + (1 - w) N(e; mu, sigma_nlos)) up
     // to the common -log sqrt(2 pi), formed as the reference does (max plus
