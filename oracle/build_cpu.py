@@ -1,6 +1,6 @@
 """Build the CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP): the C
-restatements of the particle-filter step (cpu_pf.c) and of the WLS solvers,
-single-clock and multi-GNSS (cpu_wls.c).
+restatements of the particle-filter step (cpu_pf.c), of the observation updates
+(cpu_updates.c) and of the WLS solvers, single-clock and multi-GNSS (cpu_wls.c).
 
 Measurement infrastructure only; the output lands in oracle/_build/.
 """
@@ -17,7 +17,8 @@ LIB = os.path.join(OUT_DIR, "liboracle_cpu_pf.so")
 
 def build() -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    srcs = [os.path.join(HERE, "cpu_pf.c"), os.path.join(HERE, "cpu_wls.c")]
+    srcs = [os.path.join(HERE, "cpu_pf.c"), os.path.join(HERE, "cpu_wls.c"),
+            os.path.join(HERE, "cpu_updates.c")]
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11",
            "-D_GNU_SOURCE", *srcs, "-o", tmp, "-lm"]
@@ -47,6 +48,17 @@ def load():
     lib.cpu_wls_batch.argtypes = [DP, DP, DP, IP, I, I, I, D, I, I, DP, IP]
     lib.cpu_wls_mg_batch.restype = None
     lib.cpu_wls_mg_batch.argtypes = [DP, DP, DP, IP, I, I, I, I, D, I, I, I, DP, IP]
+    lib.cpu_updates_threads.restype = I
+    for name, args in {
+        "cpu_dd_pr_loglik": [I, DP, I, DP, DP, DP, DP, DP, DP, D, D, I, D, DP, I],
+        "cpu_carrier_afv_loglik": [I, DP, DP, I, DP, DP, DP, D, D, DP, I],
+        "cpu_dd_carrier_afv_loglik": [I, DP, I, DP, DP, DP, DP, DP, DP, DP, D, D, I, D, DP, I],
+        "cpu_doppler_update": [I, DP, DP, I, DP, DP, DP, DP, D, D, D, D, DP, I],
+        "cpu_doppler_kf_update": [I, DP, DP, DP, I, DP, DP, DP, DP, D, D, DP, I],
+    }.items():
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = args
     return lib
 
 
