@@ -947,8 +947,16 @@ def ffbsi_leg(args):
     for r in range(reps):
         hist.ffbsi_sample_indices(3.0, [np.random.default_rng(100 * r + k) for k in range(M)])
     dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # one trajectory per workgroup: a call for 8 paths occupies 8 CUs; 256 paths
+    # in one launch fill the chip
+    Mw = 256
+    t0 = time.perf_counter()
+    hist.ffbsi_sample_indices(3.0, [np.random.default_rng(7000 + k) for k in range(Mw)])
+    wide_ms = (time.perf_counter() - t0) * 1e3
     return {"workload": "ffbsi_T1000_N8192_8paths", "ms_per_call": round(ms, 2),
             "device_history": {"ms_per_call": round(dev_ms, 2),
+                               "paths_256_ms_per_call": round(wide_ms, 2),
+                               "paths_256_ms_per_path": round(wide_ms / Mw, 3),
                                "filter_epoch_ms_with_record": round(run_ms, 3),
                                "note": "history recorded in HBM by the running device filter; "
                                        "no host copies"},
