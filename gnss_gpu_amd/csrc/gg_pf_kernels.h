@@ -255,35 +255,32 @@ struct gg_normal_stream {
 };
 
 // Noise of the fast predict path (uniform velocity carried by the anchor, no
-// velocity filter): zn[q][c] = normal of component c of particle 4g + q times
-// its standard deviation.  It depends on nothing the step computes, so the
-// weight kernel draws it before it waits for the step frame.
+// velocity filter): zn[q][c] = standard normal of component c of particle
+// 4g + q.  It depends on nothing the step computes, so the weight kernel
+// draws it before it waits for the step frame.
 template <int NISB>
 __device__ __forceinline__ void gg_noise4_fast(uint64_t seed, uint32_t g, uint32_t step,
-                                               const gg_step_args& s,
                                                float (&zn)[GG_ITEMS][4 + NISB]) {
   gg_normal_stream zs(seed, g, step, 0u);
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    zn[q][0] = zs.next() * s.sigma_pos;
-    zn[q][1] = zs.next() * s.sigma_pos;
-    zn[q][2] = zs.next() * s.sigma_pos;
-    zn[q][3] = zs.next() * s.sigma_cb;
+  for (int q = 0; q < GG_ITEMS; ++q)
 #pragma unroll
-    for (int k = 0; k < NISB; ++k) zn[q][4 + k] = zs.next() * s.sigma_isb;
-  }
+    for (int c = 0; c < 4 + NISB; ++c) zn[q][c] = zs.next();
 }
 
+// p += sigma_c z (one fused multiply-add per component): sigma_pos for x, y,
+// z, sigma_cb for the clock bias, sigma_isb for the inter-system biases.
 template <int D>
 __device__ __forceinline__ void gg_apply_noise4_fast(const float (&zn)[GG_ITEMS][D],
+                                                     const gg_step_args& s,
                                                      float (&p)[GG_ITEMS][D]) {
 #pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
+  for (int q = 0; q < GG_ITEMS; ++q)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[q][c] += 0.0f + zn[q][c];
-#pragma unroll
-    for (int c = 3; c < D; ++c) p[q][c] += zn[q][c];
-  }
+    for (int c = 0; c < D; ++c) {
+      const float sg = (c < 3) ? s.sigma_pos : (c == 3 ? s.sigma_cb : s.sigma_isb);
+      p[q][c] = __builtin_fmaf(zn[q][c], sg, p[q][c]);
+    }
 }
 
 // Predict one particle in place (binary32 offsets) with its normals z
@@ -361,8 +358,8 @@ __device__ __forceinline__ void gg_predict4(const gg_arena& a, int cur, int64_t 
   const uint32_t g = (uint32_t)i0 >> 2;
   if (pl.mode_out == 0 && !s.velocity_kf) {
     float zn[GG_ITEMS][D];
-    gg_noise4_fast<NISB>(seed, g, step, s, zn);
-    gg_apply_noise4_fast<D>(zn, p);
+    gg_noise4_fast<NISB>(seed, g, step, zn);
+    gg_apply_noise4_fast<D>(zn, s, p);
     return;
   }
   gg_normal_stream zs(seed, g, step, 0u);

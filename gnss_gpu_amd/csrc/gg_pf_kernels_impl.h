@@ -171,8 +171,8 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     const bool fast = PRED && dy->vel_mode == 0 && !s.velocity_kf && !(s.sigma_vel > 0.0f);
     if (fast) {
       float zn[GG_ITEMS][D];
-      gg_noise4_fast<NISB>(F.seed, (uint32_t)i0 >> 2, step, s, zn);
-      gg_apply_noise4_fast<D>(zn, p);
+      gg_noise4_fast<NISB>(F.seed, (uint32_t)i0 >> 2, step, zn);
+      gg_apply_noise4_fast<D>(zn, s, p);
     }
     if (f != f_staged) {  // uniform over the workgroup
       if (f_staged >= 0) __syncthreads();  // the previous filter's records are no longer read
