@@ -175,38 +175,25 @@ __host__ __device__ __forceinline__ double gg_exp64(double x) {
 }
 
 // Deterministic binary32 exp with the same construction (16-bit ln2 head,
-// degree-7 Taylor polynomial).  Used by the Megopolis acceptance test.
-// gg_exp32_core is the branch-free body, valid for |x| <= 87.
-__host__ __device__ __forceinline__ float gg_exp32_core(float x) {
-#pragma clang fp contract(off)
-  const float k = __builtin_rintf(x * 0x1.715476p+0f);
-  const float r = (x - k * 0x1.62e4p-1f) - k * 0x1.7f7d1cp-20f;
-  float p = 0x1.a01a02p-13f;
-  p = p * r + 0x1.6c16c2p-10f;
-  p = p * r + 0x1.111112p-7f;
-  p = p * r + 0x1.555556p-5f;
-  p = p * r + 0x1.555556p-3f;
-  p = p * r + 0.5f;
-  p = p * r + 1.0f;
-  p = p * r + 1.0f;
-  return __builtin_ldexpf(p, (int)k);
-}
-
+// degree-7 Taylor polynomial), every step a correctly rounded fused
+// multiply-add: r = (x - k ln2_hi) - k ln2_lo and the Horner steps, 13
+// instructions on the exact path.  The oracle's twin emulates each fused
+// multiply-add exactly (binary64 sum, round to odd, then to binary32).
 __host__ __device__ __forceinline__ float gg_exp32(float x) {
 #pragma clang fp contract(off)
   if (x != x) return x;
   if (x > 88.72283f) return __builtin_huge_valf();
   if (x < -103.97208f) return 0.0f;
   const float k = __builtin_rintf(x * 0x1.715476p+0f);
-  const float r = (x - k * 0x1.62e4p-1f) - k * 0x1.7f7d1cp-20f;
+  const float r = __builtin_fmaf(-k, 0x1.7f7d1cp-20f, __builtin_fmaf(-k, 0x1.62e4p-1f, x));
   float p = 0x1.a01a02p-13f;
-  p = p * r + 0x1.6c16c2p-10f;
-  p = p * r + 0x1.111112p-7f;
-  p = p * r + 0x1.555556p-5f;
-  p = p * r + 0x1.555556p-3f;
-  p = p * r + 0.5f;
-  p = p * r + 1.0f;
-  p = p * r + 1.0f;
+  p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+  p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+  p = __builtin_fmaf(p, r, 0.5f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
   return __builtin_ldexpf(p, (int)k);
 }
 

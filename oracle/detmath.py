@@ -60,8 +60,30 @@ def exp64(x):
     return out
 
 
+def fmaf(a, b, c):
+    """Correctly rounded binary32 fused multiply-add a*b + c (twin of
+    __builtin_fmaf / v_fma_f32): the product of two binary32 values is exact in
+    binary64, TwoSum gives the exact error of the binary64 sum, the sum is taken
+    to binary64 with round-to-odd and then rounded to binary32 (exact because
+    53 >= 24 + 2).  Finite inputs; arrays broadcast."""
+    a = np.asarray(a, dtype=np.float32).astype(np.float64)
+    b = np.asarray(b, dtype=np.float32).astype(np.float64)
+    c = np.asarray(c, dtype=np.float32).astype(np.float64)
+    p = a * b
+    s = p + c
+    bp = s - p
+    ap = s - bp
+    err = (p - ap) + (c - bp)
+    # truncation toward zero of the exact p + c: one step toward zero when the
+    # rounded sum lies beyond it
+    away = ((s > 0) & (err < 0)) | ((s < 0) & (err > 0))
+    t = np.where(away, np.nextafter(s, 0.0), s)
+    bits = t.view(np.int64) | (err != 0).astype(np.int64)
+    return bits.view(np.float64).astype(np.float32)
+
+
 def exp32(x):
-    """Twin of gg_exp32 (binary32)."""
+    """Twin of gg_exp32 (binary32, fused multiply-adds)."""
     x = np.asarray(x, dtype=np.float32)
     out = np.zeros_like(x)
     nan = np.isnan(x)
@@ -70,10 +92,10 @@ def exp32(x):
     mid = ~(nan | hi | lo)
     xm = x[mid]
     k = np.rint(xm * _INV_LN2_F)
-    r = (xm - k * _LN2_HI_F) - k * _LN2_LO_F
+    r = fmaf(-k, _LN2_LO_F, fmaf(-k, _LN2_HI_F, xm))
     p = np.full_like(r, _EXP_CF[0])
     for c in _EXP_CF[1:]:
-        p = p * r + c
+        p = fmaf(p, r, c)
     out[mid] = np.ldexp(p, k.astype(np.int32)).astype(np.float32)
     out[nan] = x[nan]
     out[hi] = np.inf

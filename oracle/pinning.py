@@ -40,6 +40,8 @@ PINS = {
     "detmath.exp64": ("pinned", "<= 2 ulp against libm over [-745, 0] (test_exp64_against_libm)"),
     "detmath.exp32": ("pinned", "<= 4 ulp (binary32) against binary64 exp "
                       "(test_exp32_against_float64_exp)"),
+    "detmath.fmaf": ("pinned", "bit-exact against the correctly rounded binary32 of the exact "
+                     "rational a*b + c, ties included (test_fmaf_against_rational_arithmetic)"),
     "detmath.block_scan256": ("pinned", "exact on integers, rational bound on reals "
                               "(test_block_scan_exact_on_integers, test_block_scan_close_to_exact_rational)"),
     "detmath._hs_wave": ("pinned", "inside block_scan256 (exact on integers)"),

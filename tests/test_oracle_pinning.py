@@ -144,6 +144,44 @@ def test_exp64_against_libm():
     assert detmath.exp64(np.array([0.0]))[0] == 1.0
 
 
+def _round_f32_exact(q):
+    """Correctly rounded binary32 of a rational (ties to even), by comparing
+    the neighbours of a candidate exactly."""
+    from fractions import Fraction
+
+    f = np.float32(float(q))
+    best = None
+    for cand in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        key = (abs(Fraction(float(cand)) - q), int(np.array(cand).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, cand)
+    return best[1]
+
+
+def test_fmaf_against_rational_arithmetic():
+    """detmath.fmaf (the oracle's fused multiply-add) equals the correctly
+    rounded binary32 of a*b + c computed exactly, on random triples over many
+    magnitudes and on constructed ties and near-ties."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(11)
+    n = 6000
+    a = (rng.normal(size=n) * rng.choice([1e-3, 1.0, 1e3], n)).astype(np.float32)
+    b = rng.normal(size=n).astype(np.float32)
+    c = (rng.normal(size=n) * rng.choice([1e-8, 1.0, 1e8], n)).astype(np.float32)
+    # ties: (1 + 2^-12)^2 = 1 + 2^-11 + 2^-24 sits halfway between two binary32
+    # values; c = +-2^-60 pushes it just off the tie either way
+    t = np.float32(1.0 + 2.0 ** -12)
+    a = np.concatenate([a, [t, t, t, -t, np.float32(1.0 + 3 * 2.0 ** -12)]]).astype(np.float32)
+    b = np.concatenate([b, [t, t, t, t, t]]).astype(np.float32)
+    c = np.concatenate([c, [0.0, 2.0 ** -60, -(2.0 ** -60), 0.0, 0.0]]).astype(np.float32)
+    got = detmath.fmaf(a, b, c)
+    for i in range(a.size):
+        want = _round_f32_exact(Fraction(float(a[i])) * Fraction(float(b[i])) +
+                                Fraction(float(c[i])))
+        assert np.array(got[i]).view(np.uint32) == np.array(want).view(np.uint32), i
+
+
 def test_exp32_against_float64_exp():
     x = np.linspace(-87.0, 88.0, 50001, dtype=np.float32)
     got = detmath.exp32(x).astype(np.float64)
