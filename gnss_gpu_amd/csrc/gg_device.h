@@ -127,12 +127,24 @@ __host__ __device__ __forceinline__ gg_xo gg_xo_seed(uint64_t seed, uint32_t idx
 __host__ __device__ __forceinline__ uint32_t gg_xo_next(gg_xo& x) {
   const uint32_t out = x.s0 + x.s3;
   const uint32_t t = x.s1 << 9;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the same update with three-input exclusive ors (v_bitop3_b32, truth
+  // table 0x96): s1 ^ s2 ^ s0 and s2 ^ s0 ^ t without the shared s2 ^ s0
+  const uint32_t s1 = __builtin_amdgcn_bitop3_b32(x.s1, x.s2, x.s0, 0x96);
+  const uint32_t s2 = __builtin_amdgcn_bitop3_b32(x.s2, x.s0, t, 0x96);
+  const uint32_t s3 = x.s3 ^ x.s1;
+  x.s0 = x.s0 ^ s3;
+  x.s1 = s1;
+  x.s2 = s2;
+  x.s3 = (s3 << 11) | (s3 >> 21);
+#else
   x.s2 ^= x.s0;
   x.s3 ^= x.s1;
   x.s1 ^= x.s2;
   x.s0 ^= x.s3;
   x.s2 ^= t;
   x.s3 = (x.s3 << 11) | (x.s3 >> 21);
+#endif
   return out;
 }
 
@@ -347,6 +359,14 @@ __device__ __forceinline__ double gg_swz16_f64(double x) {
     x = OP(x, gg_dpp_##T<0x140>(x));      \
     x = OP(x, gg_swz16_##T(x));           \
   } while (0)
+
+// Lane 0's value of a double, in every lane of the wavefront.
+__device__ __forceinline__ double gg_readlane0_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 0);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 0);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
 
 __device__ __forceinline__ double gg_add_f64(double a, double b) {
 #pragma clang fp contract(off)

@@ -39,6 +39,7 @@ struct gg_pf {
   double* Pb = nullptr;
   double* csum = nullptr;
   uint32_t* fin = nullptr;
+  int64_t* acc = nullptr;
   double* scratch_f = nullptr;
   gg_filter* filt = nullptr;
   gg_dyn* dyn = nullptr;
@@ -112,6 +113,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.aos = h->aos;
   a.csum = h->csum;
   a.fin = h->fin;
+  a.acc = h->acc;
   a.scratch_f = h->scratch_f;
   a.filt = h->filt;
   a.dyn = h->dyn;
@@ -130,6 +132,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.sat_rows = h->sat_rows;
   a.n_filters = h->n_filters;
   a.n_tiles = h->n_tiles;
+  a.n_chunks = h->n_chunks;
   return a;
 }
 
@@ -354,13 +357,13 @@ int launch_decide(gg_pf* h, const gg_step_args& s, int mode) {
 }
 
 // systematic resampling in one launch: decision (with_decide) or the record's,
-// CDF ends of each item's tiles, merge-path gather and finalize
+// work item search over the merged tile ends and grid points, CDF ends of
+// the item's tile, gather and finalize
 int launch_sysres(gg_pf* h, const gg_step_args& s, int force, int with_decide) {
   const gg_arena a = arena_of(h);
-  // tile sums (with_decide) and the end counts K_b of every tile, later the
-  // ancestors of an item's outputs
-  const size_t smem = std::max(decide_smem(h) + sizeof(int32_t) * (size_t)(max_tiles(h) + 1),
-                               sizeof(int32_t) * (size_t)GG_QCHUNK);
+  // tile sums, then CDF offsets, of every tile; later the ancestors of an
+  // item's outputs
+  const size_t smem = std::max(decide_smem(h), sizeof(int32_t) * (size_t)GG_QCHUNK);
   GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_sysres<NI>), dim3(h->n_chunks), dim3(GG_BLOCK),
                                        smem, a, s, force, with_decide));
   GG_HIP(hipGetLastError());
@@ -565,6 +568,7 @@ int gg_pf_create(int32_t n_filters, const int32_t* n_particles, const int32_t* s
   if (!rc) rc = dev_alloc(&h->aos, (size_t)h->P * 8);
   if (!rc) rc = dev_alloc(&h->csum, (size_t)chunks * GG_MAX_D);
   if (!rc) rc = dev_alloc(&h->fin, (size_t)n_filters * GG_ARRIVE_WORDS);
+  if (!rc) rc = dev_alloc(&h->acc, (size_t)n_filters * GG_ARRIVE_SUBS * GG_ACC_W);
   if (!rc) rc = dev_alloc(&h->scratch_f, (size_t)n_filters * 16);
   if (!rc) rc = dev_alloc(&h->filt, (size_t)n_filters);
   if (!rc) rc = dev_alloc(&h->dyn, (size_t)n_filters);
@@ -610,7 +614,7 @@ int gg_pf_destroy(gg_pf* h) {
   if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->graph) hipGraphDestroy(h->graph);
   void* bufs[] = {h->state, h->lw,    h->E,  h->aos,  h->anc,  h->part, h->p4, h->g, h->Pb,
-                  h->csum, h->fin, h->scratch_f, h->filt,
+                  h->csum, h->fin, h->acc, h->scratch_f, h->filt,
                   h->dyn,   h->satd,  h->sati,      h->tile_filter, h->chunk_filter, h->vel_guide,
                   h->satf,  h->mlp_table, h->row_filter,
                   h->io_tmp, h->obsd};

@@ -12,9 +12,11 @@
 //   gg_k_sysres    every work item combines its filter's partials (max, total,
 //                  ESS, CDF tile offsets) itself and takes the resampling
 //                  decision; the first item of each filter writes the decision
-//                  and estimate; when resampling, each item computes the CDF
-//                  ends of the tiles its inputs span, the merge-path ancestors
-//                  and the state gather; the last item of each filter to
+//                  and estimate; when resampling, each item finds its
+//                  interval of outputs (one tile's, at most GG_QCHUNK) by a
+//                  search over the merged tile ends and grid points, computes
+//                  that tile's CDF ends, the ancestors and the state gather;
+//                  the last item of each filter to
 //                  finish adds the item sums and finalises (estimate, buffer
 //                  swap).
 // Besides the records and sums that pass through the device-coherent level
@@ -43,6 +45,12 @@
 #define GG_ARRIVE_SUBS 32
 #define GG_ARRIVE_STRIDE 32  // uint32 per counter (128 bytes)
 #define GG_ARRIVE_WORDS ((GG_ARRIVE_SUBS + 1) * GG_ARRIVE_STRIDE)
+// Per arrival sub-group: the state sums in fixed point, units of 2^-24
+// (words 0 .. GG_MAX_D-1) and, in the last word, a flag for sums that did
+// not fit.
+#define GG_ACC_W 8
+#define GG_ACC_SCALE 0x1p24
+static_assert(GG_MAX_D + 1 <= GG_ACC_W, "fixed-point sum record too small");
 
 __device__ __forceinline__ bool gg_arrive_last(uint32_t* c, int item, int n_items) {
   const int j = item % GG_ARRIVE_SUBS;
@@ -69,8 +77,9 @@ struct gg_arena {
   double* p4;             // [n_tiles_total][4]: m_b, S_b, S2_b, 0 (the combine's inputs, packed)
   double* g;              // [n_tiles_total]
   double* Pb;             // [n_tiles_total]
-  double* csum;           // [n_chunks_total][GG_MAX_D]: state sums of each gather chunk
+  double* csum;           // [GG_MAX_D][n_chunks]: state sums of each gather chunk
   uint32_t* fin;          // [n_filters][GG_ARRIVE_WORDS]: gather arrival counters
+  int64_t* acc;           // [n_filters][GG_ARRIVE_SUBS][GG_ACC_W]: fixed-point gather sums
   double* scratch_f;      // [n_filters][16] misc per-filter results
   const gg_filter* filt;  // [n_filters]
   gg_dyn* dyn;            // [n_filters]
@@ -93,7 +102,8 @@ struct gg_arena {
   int32_t has_vel;
   int32_t sat_rows;
   int32_t n_filters;
-  int32_t n_tiles;  // tiles of all filters
+  int32_t n_tiles;   // tiles of all filters
+  int32_t n_chunks;  // resampling work items of all filters
 };
 
 __device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) {
@@ -506,17 +516,25 @@ __device__ __forceinline__ int gg_stage_sats(const gg_arena& a, int f, const gg_
 // (pp = |x|^2; bias = clock bias + the ISB of the satellite's system).
 // |sat - x| - rho = q h with q = pp/rho - 2 u.x, t = q/rho and
 // h = 1 / (1 + sqrt(1 + t)); the 2e7 m range is never formed in binary32.
-// SERIES evaluates h = 1/2 - t/8 + t^2/16, whose truncation error
-// (5/128) t^3 is below 1e-10 relative for |t| <= 2^-10; gg_series_ok tells
-// when every (particle, satellite) pair of a thread is in that range.
-template <bool SERIES>
+// FORM selects how h is evaluated (gg_range_form picks it per thread):
+//   GG_FORM_EXACT  the closed form;
+//   GG_FORM_QUAD   h = 1/2 - t/8 + t^2/16, truncation error (5/128) |t|^3,
+//                  below 1e-10 relative for |t| <= 2^-10;
+//   GG_FORM_LIN    h = 1/2 - t/8, truncation error t^2/16 (relative t^2/8),
+//                  below 1e-10 relative for |t| <= 2^-15.3.
+#define GG_FORM_EXACT 0
+#define GG_FORM_QUAD 1
+#define GG_FORM_LIN 2
+template <int FORM>
 __device__ __forceinline__ float gg_residual(const float4& g, const float4& m, const float* p,
                                              float pp, float bias) {
   const float ud2 = fmaf(g.z, p[2], fmaf(g.y, p[1], g.x * p[0]));  // -2 u.x
   const float q = fmaf(pp, g.w, ud2);
   const float t = q * g.w;
   float h;
-  if (SERIES)
+  if (FORM == GG_FORM_LIN)
+    h = fmaf(t, -0.125f, 0.5f);
+  else if (FORM == GG_FORM_QUAD)
     h = fmaf(fmaf(t, 0.0625f, -0.125f), t, 0.5f);
   else
     h = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f + t));
@@ -525,40 +543,48 @@ __device__ __forceinline__ float gg_residual(const float4& g, const float4& m, c
 
 // The residual scaled by s = sqrt(|wl|) from the records rec[3k] (g) and
 // rec[3k + 2] (r): e' = s c - q s h - s bias, the 2e7 m range never formed.
-// SERIES: s h = s/2 - (s/(8 rho)) q + (s/(16 rho^2)) q^2 (gg_residual's
+// Series forms: s h = s/2 - (s/(8 rho)) q [+ (s/(16 rho^2)) q^2] (gg_residual's
 // series with t = q/rho); otherwise s h = 2 (s/2) / (1 + sqrt(1 + t)).
-template <bool SERIES>
+template <int FORM>
 __device__ __forceinline__ float gg_residual_scaled(const float4& g, const float4& r, const float* p,
                                                     float pp, float bias) {
   const float ud2 = fmaf(g.z, p[2], fmaf(g.y, p[1], g.x * p[0]));  // -2 u.x
   const float q = fmaf(pp, g.w, ud2);
   float sh;
-  if (SERIES)
+  if (FORM == GG_FORM_LIN)
+    sh = fmaf(q, r.z, r.y);
+  else if (FORM == GG_FORM_QUAD)
     sh = fmaf(fmaf(q, r.w, r.z), q, r.y);
   else
     sh = (r.y + r.y) * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(fmaf(q, g.w, 1.0f)));
   return fmaf(-(r.y + r.y), bias, fmaf(-q, sh, r.x));
 }
 
-// |t| <= (pp/rho + 2 |x|)/rho <= 2^-10 for every staged satellite when
-// pp (1/rho)^2 <= 2^-25 (|x| within about 3.5 km of the anchor for GNSS
-// ranges); the margin covers the binary32 rounding of q and t.
-__device__ __forceinline__ bool gg_series_ok(float ppmax, float irmax) {
-  return ppmax * irmax * irmax <= 2.98023223876953125e-08f;  // 2^-25
+// Range form of a thread whose particles have |x|^2 <= ppmax, against staged
+// satellites with 1/rho <= irmax.  |t| <= (pp/rho + 2 |x|)/rho, so
+// pp (1/rho)^2 <= 2^-25 gives |t| <= 2^-11.4 (the quadratic series) and
+// pp (1/rho)^2 <= 2^-33 gives |t| <= 2^-15.4 (the linear one): |x| within
+// about 3.5 km and 220 m of the anchor at GNSS ranges.  The margins cover the
+// binary32 rounding of q and t.
+__device__ __forceinline__ int gg_range_form(float ppmax, float irmax) {
+  const float v = ppmax * irmax * irmax;
+  return (v <= 1.16415321826934814e-10f) ? GG_FORM_LIN             // 2^-33
+         : (v <= 2.98023223876953125e-08f) ? GG_FORM_QUAD          // 2^-25
+                                           : GG_FORM_EXACT;
 }
 
-// |x|^2 of the thread's particles and whether the series range form holds
-// for all of them (gg_series_ok).
+// |x|^2 of the thread's particles and the range form that holds for all of
+// them (gg_range_form).
 template <int D>
-__device__ __forceinline__ bool gg_pp4(const gg_sat_view& sv, const float (&p)[GG_ITEMS][D],
-                                       float (&pp)[GG_ITEMS]) {
+__device__ __forceinline__ int gg_pp4(const gg_sat_view& sv, const float (&p)[GG_ITEMS][D],
+                                      float (&pp)[GG_ITEMS]) {
   float ppmax = 0.0f;
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) {
     pp[q] = fmaf(p[q][2], p[q][2], fmaf(p[q][1], p[q][1], p[q][0] * p[q][0]));
     ppmax = fmaxf(ppmax, pp[q]);
   }
-  return gg_series_ok(ppmax, sv.irmax);
+  return gg_range_form(ppmax, sv.irmax);
 }
 
 // Record k's bias term for group G (-1: reference system) of particle p.
@@ -606,7 +632,8 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
     gg_for_groups<NISB>(sv, [&](auto grp, int k0, int k1) {
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
-        const float e = gg_residual<false>(sv.rec[GG_REC * k], sv.rec[GG_REC * k + 1], p, pp, bias);
+        const float e =
+            gg_residual<GG_FORM_EXACT>(sv.rec[GG_REC * k], sv.rec[GG_REC * k + 1], p, pp, bias);
         kept += (fabsf(e) <= s.gate_thr) ? 1 : 0;
       }
     });
@@ -623,7 +650,7 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
         const float4 m = sv.rec[GG_REC * k + 1];
-        const float e = gg_residual<false>(sv.rec[GG_REC * k], m, p, pp, bias);
+        const float e = gg_residual<GG_FORM_EXACT>(sv.rec[GG_REC * k], m, p, pp, bias);
         if (gate && fabsf(e) > s.gate_thr) continue;
         prod *= fmaf(m.y * e, e, 1.0f);
         if (++cnt == 3) {
@@ -642,7 +669,7 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
       const float bias = gg_group_bias<NISB, decltype(grp)::value>(p);
       for (int k = k0; k < k1; ++k) {
         const float4 m = sv.rec[GG_REC * k + 1];
-        const float e = gg_residual<false>(sv.rec[GG_REC * k], m, p, pp, bias);
+        const float e = gg_residual<GG_FORM_EXACT>(sv.rec[GG_REC * k], m, p, pp, bias);
         if (gate && fabsf(e) > s.gate_thr) continue;
         if (s.lik == GG_LIK_HUBER) {
           const float z = fabsf(e) * isig;
@@ -661,7 +688,7 @@ __device__ __forceinline__ float gg_loglik(const gg_sat_view& sv, int ns, const 
 // loop, so that the four residual chains interleave; a group's bias is formed
 // once per particle.  With the NLOS gate on, the kept set differs per
 // particle and gg_loglik is used.
-template <int NISB, bool SERIES>
+template <int NISB, int FORM>
 __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
                                                 const float (&p)[GG_ITEMS][4 + NISB],
                                                 const float (&pp)[GG_ITEMS],
@@ -683,7 +710,7 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
         const float4 r = sv.rec[GG_REC * k + 2];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual_scaled<SERIES>(g, r, p[q], pp[q], bias[q]);
+          const float e = gg_residual_scaled<FORM>(g, r, p[q], pp[q], bias[q]);
           prod[q] *= fmaf(e, e, 1.0f);
         }
         if (++cnt == 3) {
@@ -723,7 +750,7 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
         const float kn = m.y * ratio;
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          const float e = gg_residual<FORM>(g, m, p[q], pp[q], bias[q]);
           const float en = e - mu;
           const float a = fmaf(m.y * e, e, c_los);
           const float b = fmaf(kn * en, en, c_nlos);
@@ -747,7 +774,7 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
         const float4 m = sv.rec[GG_REC * k + 1];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+          const float e = gg_residual<FORM>(g, m, p[q], pp[q], bias[q]);
           const float z = fabsf(e) * isig;
           const float rho = (z < hk) ? 0.5f * z * z : hk * z - 0.5f * hk * hk;
           lw[q] -= m.y * rho;
@@ -767,7 +794,7 @@ __device__ __forceinline__ void gg_loglik4_form(const gg_sat_view& sv,
         const float4 r = sv.rec[GG_REC * k + 2];
 #pragma unroll
         for (int q = 0; q < GG_ITEMS; ++q) {
-          const float e = gg_residual_scaled<SERIES>(g, r, p[q], pp[q], bias[q]);
+          const float e = gg_residual_scaled<FORM>(g, r, p[q], pp[q], bias[q]);
           lw[q] = fmaf(-e, e, lw[q]);
         }
       }
@@ -787,10 +814,13 @@ __device__ __forceinline__ void gg_loglik4(const gg_sat_view& sv, int ns,
     return;
   }
   float pp[GG_ITEMS];
-  if (gg_pp4(sv, p, pp))
-    gg_loglik4_form<NISB, true>(sv, p, pp, s, out);
+  const int form = gg_pp4(sv, p, pp);
+  if (form == GG_FORM_LIN)
+    gg_loglik4_form<NISB, GG_FORM_LIN>(sv, p, pp, s, out);
+  else if (form == GG_FORM_QUAD)
+    gg_loglik4_form<NISB, GG_FORM_QUAD>(sv, p, pp, s, out);
   else
-    gg_loglik4_form<NISB, false>(sv, p, pp, s, out);
+    gg_loglik4_form<NISB, GG_FORM_EXACT>(sv, p, pp, s, out);
 }
 
 // Learned likelihood, per-satellite part: features (elevation and azimuth
@@ -820,7 +850,7 @@ __device__ __forceinline__ void gg_stage_learned(const gg_arena& a, const gg_fil
 // scaled value, so a lookup returns exactly what the per-pair network returns.
 // The residuals and the per-particle sum follow gg_loglik_learned (record
 // order, same range form), so both give identical log-weights.
-template <int NISB, bool SERIES>
+template <int NISB, int FORM>
 __device__ __forceinline__ void gg_loglik_table_form(const gg_sat_view& sv, int64_t row0,
                                                      const float2* __restrict__ tab,
                                                      const float (&p)[GG_ITEMS][4 + NISB],
@@ -838,7 +868,7 @@ __device__ __forceinline__ void gg_loglik_table_form(const gg_sat_view& sv, int6
       const float2* tk = tab + (row0 + __float_as_int(m.z)) * GG_MLP_CODES;
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q) {
-        const float e = gg_residual<SERIES>(g, m, p[q], pp[q], bias[q]);
+        const float e = gg_residual<FORM>(g, m, p[q], pp[q], bias[q]);
         const float2 z = tk[gg_mlp_code_index(gg_mlp_xe(e))];
         out[q] += gg_mlp_ll(e, z.x, z.y);
       }
@@ -852,10 +882,13 @@ __device__ __forceinline__ void gg_loglik_table(const gg_sat_view& sv, int64_t r
                                                 const float (&p)[GG_ITEMS][4 + NISB],
                                                 float (&out)[GG_ITEMS]) {
   float pp[GG_ITEMS];
-  if (gg_pp4(sv, p, pp))
-    gg_loglik_table_form<NISB, true>(sv, row0, tab, p, pp, out);
+  const int form = gg_pp4(sv, p, pp);
+  if (form == GG_FORM_LIN)
+    gg_loglik_table_form<NISB, GG_FORM_LIN>(sv, row0, tab, p, pp, out);
+  else if (form == GG_FORM_QUAD)
+    gg_loglik_table_form<NISB, GG_FORM_QUAD>(sv, row0, tab, p, pp, out);
   else
-    gg_loglik_table_form<NISB, false>(sv, row0, tab, p, pp, out);
+    gg_loglik_table_form<NISB, GG_FORM_EXACT>(sv, row0, tab, p, pp, out);
 }
 
 // Learned likelihood of the workgroup's tile: for every satellite the 1024
@@ -872,7 +905,7 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv,
   const int l = t & 63;
   const int base = (t >> 6) * 256;
   float pp[GG_ITEMS];
-  const bool series = gg_pp4(sv, p, pp);
+  const int form = gg_pp4(sv, p, pp);
 #pragma unroll
   for (int q = 0; q < GG_ITEMS; ++q) slw[t * GG_ITEMS + q] = 0.0f;
   gg_mlp_regs wr;
@@ -886,8 +919,10 @@ __device__ __forceinline__ void gg_loglik_learned(const gg_sat_view& sv,
       const float4 m = sv.rec[GG_REC * k + 1];
 #pragma unroll
       for (int q = 0; q < GG_ITEMS; ++q)
-        se[t * GG_ITEMS + q] = series ? gg_residual<true>(g, m, p[q], pp[q], bias[q])
-                                      : gg_residual<false>(g, m, p[q], pp[q], bias[q]);
+        se[t * GG_ITEMS + q] =
+            (form == GG_FORM_LIN)    ? gg_residual<GG_FORM_LIN>(g, m, p[q], pp[q], bias[q])
+            : (form == GG_FORM_QUAD) ? gg_residual<GG_FORM_QUAD>(g, m, p[q], pp[q], bias[q])
+                                     : gg_residual<GG_FORM_EXACT>(g, m, p[q], pp[q], bias[q]);
       __syncthreads();
       gg_bf16x8 a1[4];
       gg_mlp_a1(w.w_e, sv.cs + k * GG_MLP_H, a1);

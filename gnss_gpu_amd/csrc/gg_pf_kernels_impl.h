@@ -405,15 +405,18 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
 // with the states read from the packed copy that gg_k_mego_w wrote (aos).
 // ---------------------------------------------------------------------------
 
-// End of a gather work item: the chunk's state sums go to its slot, then the
-// item counts itself in (gg_arrive_last); the item that arrives last adds every chunk's sums in
-// chunk order (the same binary64 operations whichever item is last) and
-// finalises the resampling: mean of the resampled particles as the estimate,
-// buffer swap, weights reset, counters.  Nobody waits on anybody.
+// End of a gather work item: the chunk's state sums go to its slot and, as
+// fixed-point integers (units of 2^-24), into its arrival sub-group's totals
+// (device-scope integer atomics), then the item counts itself in (gg_arrive_last); the item
+// that arrives last reads the 32 sub-group totals in one round of loads (the
+// same integers whichever item is last) and finalises the resampling: mean of
+// the resampled particles as the estimate, buffer swap, weights reset,
+// counters.  Only when a sum did not fit the fixed point does it add every
+// chunk's binary64 sums in chunk order instead.  Nobody waits on anybody.
 // Visibility without an L2 writeback: the sums are device-scope atomic stores
-// (written through to the device-coherent level), the wave waits for their
-// completion before its counter increment (an atomic, device-coherent), and
-// the last item reads the sums with device-scope atomic loads (from the
+// and additions (performed at the device-coherent level), the wave waits for
+// their completion before its counter increment (an atomic, device-coherent),
+// and the last item reads them with device-scope atomic loads (from the
 // coherent level).  A release/acquire pair would also write back and
 // invalidate the whole L2 of every item's XCD, which costs more than the
 // launch it replaces (gather 23 -> 52 us at 1M particles).
@@ -423,46 +426,97 @@ __device__ __forceinline__ void gg_gather_finish(const gg_arena& a, int f, const
                                                  const double (&tot)[D + 2], double* lds,
                                                  int* ilds) {
   const int t = threadIdx.x;
-  if (t == 0) {
-    for (int d = 0; d < D; ++d)
-      __hip_atomic_store(a.csum + (int64_t)chunk * GG_MAX_D + d, tot[2 + d], __ATOMIC_RELAXED,
+  int64_t* acc = a.acc + (int64_t)f * GG_ARRIVE_SUBS * GG_ACC_W;
+  if (t < 64) {
+    // the item's sums (formed in thread 0), one component per lane: binary64 in
+    // the chunk's slot, and rounded to fixed point (units of 2^-24) added into
+    // the arrival sub-group's words; integer additions give the same totals in
+    // any order.  A sum too large for the 63-bit totals (or not finite) raises
+    // the sub-group's flag instead.
+    double x = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double v = gg_readlane0_f64(tot[2 + d]);
+      if (t == d) x = v;
+    }
+    int64_t* ag = acc + (int64_t)((chunk - F.chunk_off) % GG_ARRIVE_SUBS) * GG_ACC_W;
+    const double lim = 0x1p62 / GG_ACC_SCALE / (double)n_items;
+    const bool fits = !(t < D) || fabs(x) < lim;
+    if (t < D) {
+      __hip_atomic_store(a.csum + (int64_t)t * a.n_chunks + chunk, x, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
+      const int64_t q = fits ? (int64_t)__builtin_rint(x * GG_ACC_SCALE) : 0;
+      if (q != 0) __hip_atomic_fetch_add(ag + t, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (__builtin_amdgcn_ballot_w64(!fits) != 0 && t == 0)
+      __hip_atomic_fetch_or(ag + GG_ACC_W - 1, (int64_t)1, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    ilds[0] = gg_arrive_last(a.fin + (int64_t)f * GG_ARRIVE_WORDS, chunk - F.chunk_off,
-                             n_items) ? 1 : 0;
+    if (t == 0)
+      ilds[0] = gg_arrive_last(a.fin + (int64_t)f * GG_ARRIVE_WORDS, chunk - F.chunk_off,
+                               n_items) ? 1 : 0;
   }
   __syncthreads();
   if (!ilds[0]) return;
-  double v[D + 2];
+  // The last item.  The record may have been written in this launch by another
+  // workgroup (the decision of the systematic resampler's first item): it is
+  // read and written through the device-coherent level.  Its loads and the
+  // sub-groups' fixed-point words are one round of loads.
+  gg_dyn* dy = a.dyn + f;
+  constexpr int NW = (int)(sizeof(gg_dyn) / 4);
+  static_assert(NW <= GG_BLOCK, "gg_dyn larger than one word per thread");
+  static_assert(GG_ARRIVE_SUBS * GG_ACC_W == GG_BLOCK, "one fixed-point word per thread");
+  uint32_t dw = 0u;
+  if (t < NW)
+    dw = __hip_atomic_load(reinterpret_cast<const uint32_t*>(dy) + t, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t w = __hip_atomic_load(acc + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __shared__ gg_dyn sd;
+  __shared__ unsigned long long sacc[GG_ACC_W];
+  if (t < NW) reinterpret_cast<uint32_t*>(&sd)[t] = dw;
+  if (t < GG_ACC_W) sacc[t] = 0ull;
+  __syncthreads();
+  // word t % GG_ACC_W of sub-group t / GG_ACC_W (two's complement sums, exact
+  // and order-free)
+  atomicAdd(&sacc[t % GG_ACC_W], (unsigned long long)w);
+  // cleared for the next launch
+  __hip_atomic_store(acc + t, (int64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const bool fixed = (sacc[GG_ACC_W - 1] == 0ull);  // uniform
+  double sum[D + 2];
 #pragma unroll
-  for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
-  // four chunks' sums in flight per thread before they are added (in chunk order)
-  constexpr int U = 4;
-  for (int c0 = t; c0 < n_items; c0 += U * GG_BLOCK) {
-    double x[U][D];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int cc = min(c0 + u * GG_BLOCK, n_items - 1);
+  for (int d = 0; d < D + 2; ++d) sum[d] = 0.0;
+  if (fixed) {
+    if (t == 0)
 #pragma unroll
       for (int d = 0; d < D; ++d)
-        x[u][d] = __hip_atomic_load(a.csum + (int64_t)(F.chunk_off + cc) * GG_MAX_D + d,
-                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sum[2 + d] = (double)(int64_t)sacc[d] * (1.0 / GG_ACC_SCALE);
+  } else {
+    // the chunks' binary64 sums, added in chunk order per thread, then over the block
+    double v[D + 2];
+#pragma unroll
+    for (int d = 0; d < D + 2; ++d) v[d] = 0.0;
+    // four chunks' sums in flight per thread before they are added;
+    // component-major, so a wavefront's loads of one component are contiguous
+    constexpr int U = 4;
+    for (int c0 = t; c0 < n_items; c0 += U * GG_BLOCK) {
+      double x[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int cc = min(c0 + u * GG_BLOCK, n_items - 1);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          x[u][d] = __hip_atomic_load(a.csum + (int64_t)d * a.n_chunks + F.chunk_off + cc,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c0 + u * GG_BLOCK < n_items)
+#pragma unroll
+          for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + x[u][d];
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (c0 + u * GG_BLOCK < n_items)
-#pragma unroll
-        for (int d = 0; d < D; ++d) v[2 + d] = v[2 + d] + x[u][d];
+    gg_block_totals<D>(v, lds, sum);
   }
-  double sum[D + 2];
-  gg_block_totals<D>(v, lds, sum);
-  // the record may have been written in this launch by another workgroup (the
-  // decision of the systematic resampler's first item): read and write it
-  // through the device-coherent level
-  __shared__ gg_dyn sd;
-  gg_dyn* dy = a.dyn + f;
-  gg_dyn_load_coherent(&sd, dy);
-  __syncthreads();
   if (t == 0) {
     const double n = (double)F.n;
     for (int d = 0; d < D; ++d) sd.mean[d] = sum[2 + d] / n;
@@ -624,7 +678,6 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   __shared__ double lds[8 + GG_BLOCK];
   __shared__ double sx[2];
   __shared__ double sEx[GG_BLOCK + 1];  // exclusive prefix of each thread's tile chunk
-  __shared__ int sItem[4];              // tile b, first output j0, last output j1, K_b
   __shared__ double sTile[3];           // P_b, g_b, m_b of the item's tile
   __shared__ int sM[GG_BLOCK];          // per-thread prefix maxima of the ancestor markers
   __shared__ gg_dyn sd;
@@ -636,9 +689,8 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   const int cl = chunk - F.chunk_off;
   const int n = F.n;
   const int nt = F.n_tiles;
-  double* sT = reinterpret_cast<double*>(smem);
-  int* sKt = reinterpret_cast<int*>(smem + sizeof(double) * (size_t)nt);  // K_b, b in [0, nt]
-  int* sA = reinterpret_cast<int*>(smem);  // ancestors of the outputs, once sT and sKt are read
+  double* sT = reinterpret_cast<double*>(smem);  // tile sums T_b, then CDF offsets P_b
+  int* sA = reinterpret_cast<int*>(smem);        // ancestors of the outputs, once sT is read
   const int t = threadIdx.x;
   const int cur = dy->cur;
   // thread t owns tiles [b0, b1) of the chunked scan
@@ -690,49 +742,66 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
     u0 = (s.u0_mode == 2) ? s.u0_value : dy->u0;
     lwz = dy->lw_zero != 0;
   }
-  // K_b of the thread's tiles (P_b from the combine's chunked scan, the
-  // owner's exclusive prefix plus the owner's running sum as gg_combine forms
-  // them, or from a.Pb) and their item counts
-  int mt = 0;
-  {
+  // CDF offsets P_b of the filter's tiles, in sT in place of the combine's
+  // tile sums T_b: thread t turns its chunk into P_b = excl_t + (T_{tc} + ...
+  // + T_{b-1}), the operations and order of gg_combine's scan (or from a.Pb)
+  if (with_decide) {
     double run = 0.0;
     for (int b = b0; b < b1; ++b) {
-      const double pb = with_decide ? sEx[t] + run : a.Pb[F.tile_off + b];
-      if (with_decide) run = run + sT[b];
-      const int kb = (b == 0) ? 0 : gg_kcount(pb / W, u0, n);
-      sKt[b] = kb;
+      const double tsum = sT[b];
+      sT[b] = sEx[t] + run;
+      run = run + tsum;
     }
-    if (t == 0) sKt[nt] = n;
+  } else {
+    for (int b = t; b < nt; b += GG_BLOCK) sT[b] = a.Pb[F.tile_off + b];
   }
   __syncthreads();
-  for (int b = b0; b < b1; ++b) mt += (sKt[b + 1] - sKt[b] + GG_QCHUNK - 1) / GG_QCHUNK;
-  double mex, mtot;
-  gg_block_scan256((double)mt, lds, mex, mtot);  // exact: integers below 2^53
-  if (t == 0) sItem[0] = -1;
-  __syncthreads();
-  {
-    int start = (int)mex;
-    for (int b = b0; b < b1; ++b) {
-      const int kb = sKt[b], kn = sKt[b + 1];
-      const int m = (kn - kb + GG_QCHUNK - 1) / GG_QCHUNK;
-      if (cl >= start && cl < start + m) {
-        const int j0 = kb + (cl - start) * GG_QCHUNK;
-        sItem[0] = b;
-        sItem[1] = j0;
-        sItem[2] = min(j0 + GG_QCHUNK, kn);
-        sItem[3] = kb;
-      }
-      start += m;
+  // K_b: outputs before tile b (K_0 = 0, K_nt = n)
+  auto Kb = [&](int b) -> int {
+    if (b <= 0) return 0;
+    if (b >= nt) return n;
+    return gg_kcount(sT[b] / W, u0, n);
+  };
+  // Work items: the intervals between consecutive elements of the merged
+  // sequence of interior tile ends A_k = K_{k+1} (k < nt - 1) and interior
+  // grid points G_k = (k + 1) QCHUNK (k < m - 1, m = ceil(n / QCHUNK)), a
+  // tile end first on ties.  Item cl is the interval that follows merged
+  // element cl - 1: it lies in one tile, the number of tile ends among those
+  // cl elements, and holds at most QCHUNK outputs.  nt + m - 1 items cover
+  // [0, n); the filter's last item (cl = nt + m - 1) and equal neighbours
+  // give empty intervals.  The count of tile ends is the largest i in
+  // [max(0, cl - (m - 1)), min(cl, nt - 1)] with i = 0 or A_{i-1} <= G_{cl-i}
+  // (true, then false, as i grows), found by two rounds of 256 probes.
+  const int nA = nt - 1;
+  const int nG = (n + GG_QCHUNK - 1) / GG_QCHUNK - 1;
+  auto grid = [&](int k) -> int { return (k >= nG) ? 0x7fffffff : (k + 1) * GG_QCHUNK; };
+  int tb = -1, j0 = 0, j1 = 0, kb = 0, kn = 0;
+  if (n > 0 && cl <= nA + nG) {  // uniform over the workgroup
+    const int imax = min(cl, nA);
+    int lo = max(0, cl - nG);
+    int span = imax - lo + 1;
+    while (span > 1) {  // uniform; one round per factor of 256 in the span
+      const int stride = (span + GG_BLOCK - 1) / GG_BLOCK;
+      const int i = lo + t * stride;
+      int cand = -1;
+      if (i < lo + span && (i == 0 || Kb(i) <= grid(cl - i))) cand = i;
+      const int best = gg_block_max256_i(cand, ilds);  // >= lo: the probe at lo holds
+      span = min(stride, imax - best + 1);
+      lo = best;
     }
+    if (t < 2) ilds[t] = Kb(lo + t);  // K_b and K_{b+1} of the item's tile
+    __syncthreads();
+    tb = lo;
+    kb = ilds[0];
+    kn = ilds[1];
+    j0 = max(kb, (cl - tb) * GG_QCHUNK);
+    j1 = min(kn, grid(cl - tb));
+    __syncthreads();  // ilds is reused below
   }
-  __syncthreads();
-  const int tb = sItem[0];
-  if (tb < 0) {  // no outputs for this item: count it in with zero sums
+  if (tb < 0 || j0 >= j1) {  // no outputs for this item: count it in with zero sums
     gg_gather_copy<D>(a, f, F, chunk, F.n_chunks, dy, cur, 0, 0, 0, 0, sA, lds, ilds);
     return;
   }
-  const int j0 = sItem[1], j1 = sItem[2], kb = sItem[3];
-  const int kn = sKt[tb + 1];  // read before the __syncthreads that precedes the sA writes
   const int nout = j1 - j0;
   // the tile's offset, scale and maximum, beside its log-weights
   const int i0 = tb * GG_TILE + t * GG_ITEMS;
@@ -740,16 +809,7 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   if (!lwz && i0 < n) lv = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
   if (t == 0) {
     const double* pb = a.part + (int64_t)(F.tile_off + tb) * GG_NPART;
-    double P;
-    if (with_decide) {
-      const int o = tb / c;
-      double run = 0.0;
-      for (int k = o * c; k < tb; ++k) run = run + sT[k];
-      P = sEx[o] + run;
-    } else {
-      P = a.Pb[F.tile_off + tb];
-    }
-    sTile[0] = P;
+    sTile[0] = sT[tb];
     sTile[1] = with_decide ? gg_tile_shift(pb[GG_NPART - 1], pb[GG_NPART - 2] - K)
                            : a.g[F.tile_off + tb];
     sTile[2] = pb[0];
