@@ -393,6 +393,24 @@ __device__ __forceinline__ double gg_block_max256(double v, double* lds) {
   return m;
 }
 
+// Block-wide maxima of two doubles in one pass (each as gg_block_max256).
+__device__ __forceinline__ void gg_block_max256x2(double a, double b, double* lds, double& ma,
+                                                  double& mb) {
+  const int t = threadIdx.x;
+  GG_BUTTERFLY32(f64, a, gg_max_f64);
+  GG_BUTTERFLY32(f64, b, gg_max_f64);
+  if ((t & 31) == 0) {
+    lds[t >> 5] = a;
+    lds[8 + (t >> 5)] = b;
+  }
+  __syncthreads();
+  ma = fmax(fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3])),
+            fmax(fmax(lds[4], lds[5]), fmax(lds[6], lds[7])));
+  mb = fmax(fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11])),
+            fmax(fmax(lds[12], lds[13]), fmax(lds[14], lds[15])));
+  __syncthreads();
+}
+
 // Block-wide max of a float (exact, NaN ignored as fmaxf does).
 __device__ __forceinline__ float gg_block_max256f(float v, double* lds) {
   const int t = threadIdx.x;

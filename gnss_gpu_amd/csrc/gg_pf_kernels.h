@@ -981,6 +981,57 @@ __device__ __forceinline__ void gg_block_totals_n(const double (&v)[NV], double*
   __syncthreads();
 }
 
+// gg_block_scan256 of v (the same operations, so the same excl and total bit
+// for bit) together with gg_block_totals_n of vv (the same operations, totals
+// in thread 0, zeros elsewhere), sharing the scan's barriers.  ltot holds
+// NV * 8 doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void gg_block_scan256_totals(double v, const double (&vv)[NV],
+                                                        double* lds, double* ltot, double& excl,
+                                                        double& total, double (&tot)[NV]) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  double x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x = y + x;
+  }
+  if (lane == 63) lds[w] = x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double y = vv[q];
+    GG_BUTTERFLY32(f64, y, gg_add_f64);
+    if ((lane & 31) == 0) ltot[q * 8 + w * 2 + (lane >> 5)] = y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+  if (t == 0) {
+    const double o1 = lds[0];
+    const double o2 = o1 + lds[1];
+    const double o3 = o2 + lds[2];
+    lds[4] = o1;
+    lds[5] = o2;
+    lds[6] = o3;
+    lds[7] = o3 + lds[3];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double* l = ltot + q * 8;
+      tot[q] = (((l[0] + l[1]) + (l[2] + l[3])) + (l[4] + l[5])) + (l[6] + l[7]);
+    }
+  }
+  __syncthreads();
+  if (w > 0) x = lds[3 + w] + x;
+  lds[8 + t] = x;
+  __syncthreads();
+  excl = (t == 0) ? 0.0 : lds[8 + t - 1];
+  total = lds[7];
+  __syncthreads();
+}
+
 // Each value is reduced on its own (same operations whatever the count).
 template <int D>
 __device__ __forceinline__ void gg_block_totals(const double (&v)[D + 2], double* lds, double* out) {
@@ -1195,7 +1246,7 @@ __device__ __forceinline__ void gg_commit_frame(const gg_step_args& s, gg_dyn* d
 
 // Result of combining a filter's tile partials.
 struct gg_comb {
-  double M, W, W2, g_own, P_own;
+  double M, W, W2;
   double K;     // max tile exponent k_b (g_b = 2^(k_b - K) c_b)
   double excl;  // this thread's exclusive prefix in the chunked tile scan (pass 3)
   double sum[GG_MAX_D];
@@ -1211,15 +1262,16 @@ struct gg_comb {
 //           strided order; g_b = 2^(k_b - K) c_b;
 //   pass 3: W and the CDF tile offsets P_b in the fixed chunked order (thread
 //           t owns tiles [t*c, t*c+c), sums them left to right from 0.0, the
-//           per-thread values go through gg_block_scan256, then
-//           P_b = excl_t + (T_{t*c} + ... + T_{b-1})).
-// ALL writes g_b and P_b of every tile; otherwise only tile `own`'s g and P
-// are returned.  sT needs 8 * n_tiles bytes of LDS, sx two doubles.
+//           per-thread values go through gg_block_scan256's operations, then
+//           P_b = excl_t + (T_{t*c} + ... + T_{b-1})); the ESS and mean totals
+//           share the scan's barriers (gg_block_scan256_totals).
+// ALL writes g_b and P_b of every tile.  sT needs 8 * n_tiles bytes of LDS.
 template <int D, bool FULL, bool ALL>
-__device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, double* lds,
-                           double* sT, double* sx, gg_comb& r) {
+__device__ void gg_combine(const gg_arena& a, const gg_filter& F, double* lds, double* sT,
+                           gg_comb& r) {
 #pragma clang fp contract(off)
   constexpr int NV = FULL ? D + 2 : 1;
+  __shared__ double ltot[NV * 8];
   const int nt = F.n_tiles;
   const int t = threadIdx.x;
   const double* part = a.part + (int64_t)F.tile_off * GG_NPART;
@@ -1230,11 +1282,7 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   // pass 2 for tile b given K (scaled S, S2 and the tile's exponent from p4)
   auto pass2 = [&](int b, double Sc, double S2c, double kb, double K) {
     const double dk = kb - K;
-    if (ALL || b == own) {
-      const double gb = gg_tile_shift(part[(int64_t)b * GG_NPART + GG_NPART - 1], dk);
-      if (ALL) a.g[F.tile_off + b] = gb;
-      if (!ALL) sx[1] = gb;
-    }
+    if (ALL) a.g[F.tile_off + b] = gg_tile_shift(part[(int64_t)b * GG_NPART + GG_NPART - 1], dk);
     sT[b] = gg_tile_shift(Sc, dk);
     vv[0] = vv[0] + gg_tile_shift(S2c, 2.0 * dk);
     if (FULL) {
@@ -1264,8 +1312,7 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
         m = fmax(m, pm[k]);
         km = fmax(km, pk[k]);
       }
-    M = gg_block_max256(m, lds);
-    K = gg_block_max256(km, lds);
+    gg_block_max256x2(m, km, lds, M, K);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int b = t + k * GG_BLOCK;
@@ -1277,8 +1324,7 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
       m = fmax(m, p4[(int64_t)b * 4]);
       km = fmax(km, p4[(int64_t)b * 4 + 3]);
     }
-    M = gg_block_max256(m, lds);
-    K = gg_block_max256(km, lds);
+    gg_block_max256x2(m, km, lds, M, K);
     for (int b = t; b < nt; b += GG_BLOCK) {
       const double* q = p4 + (int64_t)b * 4;
       pass2(b, q[1], q[2], q[3], K);
@@ -1291,7 +1337,8 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
   double T = 0.0;
   for (int b = b0; b < b1; ++b) T = T + sT[b];
   double excl, W;
-  gg_block_scan256(T, lds, excl, W);
+  double tot[NV];
+  gg_block_scan256_totals<NV>(T, vv, lds, ltot, excl, W, tot);
   r.excl = excl;
   r.K = K;
   if (ALL) {
@@ -1300,20 +1347,10 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, int own, doubl
       a.Pb[F.tile_off + b] = excl + run;
       run = run + sT[b];
     }
-  } else if (own >= b0 && own < b1) {
-    double run = 0.0;
-    for (int b = b0; b < own; ++b) run = run + sT[b];
-    sx[0] = excl + run;
   }
-  double tot[NV];
-  gg_block_totals_n<NV>(vv, lds, tot);  // synchronises: sx is visible below
   r.M = M;
   r.W = W;
   r.W2 = tot[0];
-  if (!ALL) {
-    r.P_own = sx[0];
-    r.g_own = sx[1];
-  }
   if (FULL)
     for (int d = 0; d < D; ++d) r.sum[d] = tot[(FULL ? 2 : 0) + d];
 }

@@ -363,7 +363,6 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
   constexpr int D = 4 + NISB;
   extern __shared__ __align__(16) char smem[];
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ double sx[2];
   __shared__ gg_dyn sd;
   __shared__ double svg[3];
   const int f = blockIdx.x;
@@ -376,7 +375,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
   double vg = 0.0;
   if ((mode & 1) && threadIdx.x < 3) vg = gg_guide_ptr(a, s, f, sd.epoch)[threadIdx.x];
   gg_comb r;
-  gg_combine<D, true, true>(a, F, 0, lds, sT, sx, r);
+  gg_combine<D, true, true>(a, F, lds, sT, r);
   if ((mode & 1) && threadIdx.x < 3) svg[threadIdx.x] = vg;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -676,7 +675,6 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   __shared__ int ilds[8];
   __shared__ int sflag;
   __shared__ double lds[8 + GG_BLOCK];
-  __shared__ double sx[2];
   __shared__ double sEx[GG_BLOCK + 1];  // exclusive prefix of each thread's tile chunk
   __shared__ double sTile[3];           // P_b, g_b, m_b of the item's tile
   __shared__ int sM[GG_BLOCK];          // per-thread prefix maxima of the ancestor markers
@@ -706,9 +704,9 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
     if (first && t < 3) vg = gg_guide_ptr(a, s, f, dy->epoch)[t];
     gg_comb r;
     if (first)
-      gg_combine<D, true, false>(a, F, 0, lds, sT, sx, r);
+      gg_combine<D, true, false>(a, F, lds, sT, r);
     else
-      gg_combine<D, false, false>(a, F, 0, lds, sT, sx, r);
+      gg_combine<D, false, false>(a, F, lds, sT, r);
     W = r.W;
     K = r.K;
     sEx[t] = r.excl;
