@@ -679,7 +679,6 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   __shared__ double sTile[3];           // P_b, g_b, m_b of the item's tile
   __shared__ int sM[GG_BLOCK];          // per-thread prefix maxima of the ancestor markers
   __shared__ gg_dyn sd;
-  __shared__ double svg[3];
   const int chunk = blockIdx.x;
   const int f = a.chunk_filter[chunk];
   const gg_filter F = a.filt[f];
@@ -700,8 +699,14 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
   if (with_decide) {
     const bool first = (cl == 0);
     if (first) gg_dyn_copy(&sd, dy);
-    double vg = 0.0;
-    if (first && t < 3) vg = gg_guide_ptr(a, s, f, dy->epoch)[t];
+    // the guide velocity as uniform loads (scalar registers across the combine)
+    double vge[3] = {0.0, 0.0, 0.0};
+    if (first) {
+      const double* vg = gg_guide_ptr(a, s, f, dy->epoch);
+      vge[0] = vg[0];
+      vge[1] = vg[1];
+      vge[2] = vg[2];
+    }
     gg_comb r;
     if (first)
       gg_combine<D, true, false>(a, F, lds, sT, r);
@@ -717,12 +722,11 @@ void gg_k_sysres(gg_arena a, gg_step_args s, int force, int with_decide) {
       const double ess = (r.W2 > 0.0) ? (r.W * r.W) / r.W2 : 0.0;
       sflag = s.auto_flag ? ((ess < s.ess_frac * (double)n) ? 1 : 0) : dy->flag;
     }
-    if (first && t < 3) svg[t] = vg;
     __syncthreads();
     if (first) {
       if (t == 0) {
         gg_frame fr;
-        gg_step_frame<D>(&sd, s, svg, fr);
+        gg_step_frame<D>(&sd, s, vge, fr);
         gg_commit_frame<D>(s, &sd, fr);
         gg_store_decision<D>(s, F, r, u0, &sd);
         gg_make_estimate<D>(&sd);

@@ -226,6 +226,51 @@ def test_systematic_ancestors_bit_exact(gpu, n):
     assert dev.pf_device_ess(st) == pytest.approx(n)
 
 
+@pytest.mark.parametrize("far", [0.0, 5.0e8])
+def test_resampled_estimate_is_the_resampled_mean(gpu, far):
+    """The estimate after a systematic resample is the plain mean of the
+    resampled particles.  The last work item adds the items' sums as 2^-24
+    fixed point; particles 5e8 m from the anchor give sums beyond that range,
+    which must take the binary64 chunk sums instead, to the same mean (the
+    states are re-anchored on their mean, so the two halves go to +far and
+    -far)."""
+    dev = _dev()
+    n = 100_000
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, -3957199.0, 3310205.0, 3737911.0, 3000.0, 40.0, 10.0, SEED)
+    s = dev.pf_device_get_particle_states(st)
+    s[: n // 2, 0] += far
+    s[n // 2:, 0] -= far
+    dev.pf_device_set_particle_states(st, s)
+    lw = np.random.default_rng(3).normal(0.0, 3.0, n)
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_systematic(st, SEED)
+    p = dev.pf_device_get_particles(st)
+    est = dev.pf_device_estimate(st)
+    mean = p.mean(axis=0)
+    tol = 1e-6 + 1e-12 * np.abs(mean)  # the particles' own binary32 offsets, summed exactly on the host
+    assert np.all(np.abs(est - mean) <= tol), (est - mean)
+
+
+def test_resampled_estimate_propagates_nan(gpu):
+    """Non-finite states make the resampled estimate non-finite: the fixed-point
+    sums flag them and the binary64 chunk sums carry the NaN (one NaN state
+    makes the anchor, and so every offset of that axis, NaN)."""
+    dev = _dev()
+    n = 20_000
+    st = dev.pf_device_create(n)
+    dev.pf_device_initialize(st, -3957199.0, 3310205.0, 3737911.0, 3000.0, 40.0, 10.0, SEED)
+    s = dev.pf_device_get_particle_states(st)
+    s[123, 0] = np.nan
+    dev.pf_device_set_particle_states(st, s)
+    lw = np.full(n, -50.0)
+    lw[123] = 0.0
+    dev.pf_device_set_log_weights(st, lw)
+    dev.pf_device_resample_systematic(st, SEED)
+    est = dev.pf_device_estimate(st)
+    assert np.isnan(est[0]) and np.all(np.isfinite(est[3:]))
+
+
 @pytest.mark.parametrize("n", [1000, 4099, 1_000_000])
 @pytest.mark.parametrize("seed", [2_000_000, 2_000_001])
 def test_systematic_counts_on_the_targets(gpu, n, seed):
