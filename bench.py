@@ -881,16 +881,38 @@ def raytrace_leg(args):
                      "rays_per_s": round(n * ns / (ms * 1e-3), 1)})
     cpu = None
     if not args.no_cpu:
-        from oracle import raytrace as oray
+        # the C/OpenMP restatement of the same tree walk (binary64) over the same
+        # particles, satellites and trees, host arrays in
+        from oracle import build_cpu
 
-        k = 2000
-        tris = _box_city(100, 500.0, 1)
-        t0 = time.perf_counter()
-        oray.weight_3d(*(a[:k] for a in p), sat, pr, np.ones(ns), tris, 3.0, 30.0, 20.0)
-        sec = time.perf_counter() - t0
-        cpu = {"kind": "port", "rays_per_s": round(k * ns / sec, 1), "cores": 1,
-               "sample": "NumPy brute-force oracle (oracle/raytrace.py), 2,000 particles x 12 "
-                         "satellites x 1,200 triangles"}
+        lib = build_cpu.load()
+        DP = C.POINTER(C.c_double)
+        hp = [np.ascontiguousarray(a) for a in (*p, sat, pr, np.ones(ns))]
+        lw_h = np.zeros(n)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        per_mesh = []
+        for row, (nb, span) in zip(rows, ((100, 500.0), (1000, 1500.0), (10000, 5000.0))):
+            b = BVHAccelerator(_box_city(nb, span, 1))
+            nd, st = (np.ascontiguousarray(x) for x in (b._nodes_flat, b._sorted_tris))
+            rate = {}
+            for thr in (threads, 1):
+                t0 = time.perf_counter()
+                rc = lib.cpu_weight_3d_bvh(*(x.ctypes.data_as(DP) for x in hp[:4]), n,
+                                           *(x.ctypes.data_as(DP) for x in hp[4:]), ns,
+                                           nd.ctypes.data_as(DP), b.n_nodes,
+                                           st.ctypes.data_as(DP), b.n_triangles, 3.0, 30.0,
+                                           20.0, 1.0, 0.0, lw_h.ctypes.data_as(DP), thr)
+                sec = time.perf_counter() - t0
+                if rc != 0:
+                    raise RuntimeError("cpu_weight_3d_bvh rejected the tree")
+                rate[thr] = round(n * ns / sec, 1)
+            per_mesh.append(rate[threads])
+            row["cpu_rays_per_s"] = rate[threads]
+            row["cpu_rays_per_s_1_thread"] = rate[1]
+        cpu = {"kind": "port", "rays_per_s": per_mesh[0], "cores": threads,
+               "sample": "the full workload (1M particles x 12 satellites) on each mesh, "
+                         f"{threads} OpenMP threads and 1 thread; rays_per_s is the "
+                         "1,200-triangle mesh; C/OpenMP BVH walk (oracle/cpu_raytrace.c)"}
     return {"workload": "raytrace_weight_1Mx12", "particles": n, "satellites": ns,
             "meshes": rows, "cpu_baseline": cpu}
 

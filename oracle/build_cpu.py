@@ -1,6 +1,7 @@
 """Build the CPU baseline library liboracle_cpu_pf.so (gcc, OpenMP): the C
 restatements of the particle-filter step (cpu_pf.c), of the observation updates
-(cpu_updates.c) and of the WLS solvers, single-clock and multi-GNSS (cpu_wls.c).
+(cpu_updates.c), of the WLS solvers, single-clock and multi-GNSS (cpu_wls.c), and
+of the ray-traced NLOS weight over a BVH (cpu_raytrace.c).
 
 Measurement infrastructure only; the output lands in oracle/_build/.
 """
@@ -18,7 +19,7 @@ LIB = os.path.join(OUT_DIR, "liboracle_cpu_pf.so")
 def build() -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = [os.path.join(HERE, "cpu_pf.c"), os.path.join(HERE, "cpu_wls.c"),
-            os.path.join(HERE, "cpu_updates.c")]
+            os.path.join(HERE, "cpu_updates.c"), os.path.join(HERE, "cpu_raytrace.c")]
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11",
            "-D_GNU_SOURCE", *srcs, "-o", tmp, "-lm"]
@@ -49,6 +50,10 @@ def load():
     lib.cpu_wls_mg_batch.restype = None
     lib.cpu_wls_mg_batch.argtypes = [DP, DP, DP, IP, I, I, I, I, D, I, I, I, DP, IP]
     lib.cpu_updates_threads.restype = I
+    lib.cpu_raytrace_threads.restype = I
+    lib.cpu_weight_3d_bvh.restype = I
+    lib.cpu_weight_3d_bvh.argtypes = [DP, DP, DP, DP, I, DP, DP, DP, I, DP, I, DP, I, D, D, D,
+                                      D, D, DP, I]
     for name, args in {
         "cpu_dd_pr_loglik": [I, DP, I, DP, DP, DP, DP, DP, DP, D, D, I, D, DP, I],
         "cpu_carrier_afv_loglik": [I, DP, DP, I, DP, DP, DP, D, D, DP, I],

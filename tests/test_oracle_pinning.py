@@ -1236,6 +1236,79 @@ def test_cpu_multi_gnss_baseline_matches_oracle():
         np.testing.assert_array_equal(it, rit)
 
 
+def _median_tree(tris, leaf=4):
+    """Flat BVH rows [n, 10] (min xyz, max xyz, left, right, start, count; left -1
+    at a leaf) by median splits of the centroids along the widest axis, and the
+    triangles in the tree's order: a tree built apart from the library's."""
+    cen = tris.mean(axis=1)
+    rows, order = [], []
+
+    def rec(idx):
+        i = len(rows)
+        rows.append(None)
+        lo, hi = tris[idx].min(axis=(0, 1)), tris[idx].max(axis=(0, 1))
+        if len(idx) <= leaf:
+            rows[i] = [*lo, *hi, -1, -1, len(order), len(idx)]
+            order.extend(idx.tolist())
+            return i
+        s = idx[np.argsort(cen[idx, int(np.argmax(hi - lo))], kind="stable")]
+        left = rec(s[: len(s) // 2])
+        right = rec(s[len(s) // 2:])
+        rows[i] = [*lo, *hi, left, right, 0, 0]
+        return i
+
+    rec(np.arange(len(tris)))
+    return np.array(rows, dtype=np.float64), np.ascontiguousarray(tris[order])
+
+
+@pytest.mark.parametrize("probs", [(1.0, 0.0), (0.85, 0.05)])
+def test_cpu_raytrace_baseline_matches_oracle(probs):
+    """The C/OpenMP BVH walk (timed CPU baseline of the ray-traced weight) against
+    the brute-force NumPy oracle on a 60-box city (720 triangles): the same
+    log-weights, with rays both blocked and clear; an empty tree blocks nothing."""
+    import ctypes as C
+
+    from gnss_gpu_amd.raytrace import BuildingModel
+    from oracle import build_cpu
+    from oracle import raytrace as oray
+
+    rng = np.random.RandomState(8)
+    tris = np.concatenate([BuildingModel.create_box(
+        [rng.uniform(-200, 200), rng.uniform(-200, 200), rng.uniform(0, 40)],
+        rng.uniform(5, 30), rng.uniform(5, 30), rng.uniform(10, 60)).triangles
+        for _ in range(60)])
+    nodes, stris = _median_tree(tris)
+    n, ns = 160, 10
+    p = [rng.uniform(-200, 200, n), rng.uniform(-200, 200, n), rng.uniform(0, 30, n),
+         rng.uniform(90, 110, n)]
+    th, ph = rng.uniform(0, 2 * np.pi, ns), rng.uniform(np.radians(8), np.radians(60), ns)
+    sat = 2.2e7 * np.stack([np.cos(ph) * np.cos(th), np.cos(ph) * np.sin(th), np.sin(ph)], 1)
+    pr = np.sqrt((sat ** 2).sum(1)) + 100.0 + rng.normal(0, 20, ns)
+    ws = rng.uniform(0.5, 1.5, ns)
+    pos = np.stack(p[:3], 1)
+    frac = np.mean([oray.los_blocked(pos, np.broadcast_to(sat[s], pos.shape), tris, tmin=1e-6,
+                                     recip=True).mean() for s in range(ns)])
+    assert 0.05 < frac < 0.95
+    lib = build_cpu.load()
+    DP = C.POINTER(C.c_double)
+
+    def run(nd, tr):
+        lw = np.zeros(n)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (*p, sat, pr, ws, nd, tr)]
+        rc = lib.cpu_weight_3d_bvh(*(x.ctypes.data_as(DP) for x in a[:4]), n,
+                                   *(x.ctypes.data_as(DP) for x in a[4:7]), ns,
+                                   a[7].ctypes.data_as(DP), len(nd), a[8].ctypes.data_as(DP),
+                                   len(tr), 3.0, 30.0, 20.0, *probs, lw.ctypes.data_as(DP), 0)
+        assert rc == 0
+        return lw
+
+    ref = oray.weight_3d(*p, sat, pr, ws, tris, 3.0, 30.0, 20.0, *probs)
+    np.testing.assert_allclose(run(nodes, stris), ref, rtol=1e-12, atol=1e-12)
+    clear = oray.weight_3d(*p, sat, pr, ws, np.zeros((0, 3, 3)), 3.0, 30.0, 20.0, *probs)
+    np.testing.assert_allclose(run(np.zeros((0, 10)), np.zeros((0, 3, 3))), clear,
+                               rtol=1e-12, atol=1e-12)
+    assert np.abs(ref - clear).max() > 1.0
+
 # ------------------------------------------- DD pseudorange / carrier AFV ----
 def _dd_scenario():
     d = GOLD["dd_closed_forms"]
