@@ -111,6 +111,7 @@ gg_arena arena_of(const gg_pf* h) {
   a.E = h->E;
   a.mw = reinterpret_cast<float*>(h->E);  // Megopolis steps do not use the ends
   a.aos = h->aos;
+  a.aos_step = 0;
   a.csum = h->csum;
   a.fin = h->fin;
   a.acc = h->acc;
@@ -284,13 +285,18 @@ gg_step_args with_derived(const gg_step_args& in) {
   return s;
 }
 
-int launch_weight(gg_pf* h, const gg_step_args& s_in, bool pred, bool with_decide = true) {
+// aos: the launch writes the predicted state to the packed records only, for
+// the Megopolis launches that follow it in the same step (launch_mego with
+// aos_ready), which leave the state buffers complete.
+int launch_weight(gg_pf* h, const gg_step_args& s_in, bool pred, bool with_decide = true,
+                  bool aos = false) {
   const gg_step_args s = with_derived(s_in);
   const bool learned = s.lik == GG_LIK_LEARNED;
   if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
   const bool table = learned && s.learned_mode == 1;
   if (table && ensure_table(h)) return -1;
-  const gg_arena a = arena_of(h);
+  gg_arena a = arena_of(h);
+  a.aos_step = aos ? 1 : 0;
   const size_t smem = sat_smem(h, learned && !table);
   if (table) {
     const dim3 tgrid((GG_MLP_CODES + 1023) / 1024, h->sat_rows);
@@ -394,8 +400,12 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
   return 0;
 }
 
-int launch_mego(gg_pf* h, int n_iter, int force) {
-  const gg_arena a = arena_of(h);
+// aos_ready: the step's likelihood launch, enqueued just before with nothing
+// between that reads or changes the state, wrote the predicted state to the
+// packed records only (a step that does not resample copies it back).
+int launch_mego(gg_pf* h, int n_iter, int force, bool aos_ready = false) {
+  gg_arena a = arena_of(h);
+  a.aos_step = aos_ready ? 1 : 0;
   GG_DISPATCH_ISB(h->n_isb, gg_launch(h, gg_k_mego_w<NI>, dim3(h->n_tiles), dim3(GG_BLOCK), 0, a,
                                        force));
   GG_HIP(hipGetLastError());
@@ -423,9 +433,10 @@ int launch_mego(gg_pf* h, int n_iter, int force) {
 }
 
 // Megopolis part of a step, after the decision kernel.
-int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
+int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
+                                  bool aos_ready = false) {
   if (resampler == 1) {
-    return launch_mego(h, n_iter, 0);
+    return launch_mego(h, n_iter, 0, aos_ready);
   }
   int fb = 0;
   if (n_iter <= 0) return launch_gather(h, s, 0, 2, 0);
@@ -436,12 +447,13 @@ int enqueue_resample_after_decide(gg_pf* h, const gg_step_args& s, int resampler
 // Resampling part of a device-driven step, after the weight launch (the
 // gather's last work item per filter finalises).  Systematic: the CDF kernel takes the decision in every
 // workgroup (and commits the predict's frame), so no decision kernel runs.
-int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
+int enqueue_resample(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
+                     bool aos_ready = false) {
   if (resampler == 0) {
     return launch_sysres(h, s, 0, 1);
   }
   if (launch_decide(h, s, 3)) return -1;
-  return enqueue_resample_after_decide(h, s, resampler, n_iter);
+  return enqueue_resample_after_decide(h, s, resampler, n_iter, aos_ready);
 }
 
 // One device-driven step in two parts (no host synchronisation): part 0 is the
@@ -452,9 +464,14 @@ int enqueue_step_part(gg_pf* h, const gg_step_args& s, int resampler, int n_iter
   return enqueue_resample(h, s, resampler, n_iter);
 }
 
+// A whole step: in a Megopolis step the likelihood launch writes the predicted
+// state as packed records, which the chains read, instead of the state
+// buffers.  (Separate parts keep the plain layout: a caller may read or change
+// the state between them.)
 int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
-  if (enqueue_step_part(h, s, resampler, n_iter, 0)) return -1;
-  return enqueue_step_part(h, s, resampler, n_iter, 1);
+  const bool aos = resampler == 1;
+  if (launch_weight(h, s, true, false, aos)) return -1;
+  return enqueue_resample(h, s, resampler, n_iter, aos);
 }
 
 gg_step_args default_args() {
@@ -1673,7 +1690,8 @@ namespace {
 // index after group g.
 int enqueue_staged_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
                         size_t (&ends)[3]) {
-  int rc = launch_weight(h, s, true, false);
+  const bool aos = resampler == 1;
+  int rc = launch_weight(h, s, true, false, aos);
   ends[0] = h->kt_next;
   if (!rc) {
     if (resampler == 0) {
@@ -1682,7 +1700,7 @@ int enqueue_staged_step(gg_pf* h, const gg_step_args& s, int resampler, int n_it
     } else {
       rc = launch_decide(h, s, 3);
       ends[1] = h->kt_next;
-      if (!rc) rc = enqueue_resample_after_decide(h, s, resampler, n_iter);
+      if (!rc) rc = enqueue_resample_after_decide(h, s, resampler, n_iter, aos);
     }
   }
   ends[2] = h->kt_next;

@@ -104,6 +104,11 @@ struct gg_arena {
   int32_t n_filters;
   int32_t n_tiles;   // tiles of all filters
   int32_t n_chunks;  // resampling work items of all filters
+  // 1 in a Megopolis step: its likelihood launch writes the predicted state to
+  // the packed records (aos) only, the chains read it from there, and the
+  // Megopolis weight launch copies it into the current state buffer when the
+  // step does not resample
+  int32_t aos_step;
 };
 
 __device__ __forceinline__ float* gg_comp(const gg_arena& a, int buf, int comp) {
@@ -1266,7 +1271,10 @@ struct gg_comb {
 //           P_b = excl_t + (T_{t*c} + ... + T_{b-1})); the ESS and mean totals
 //           share the scan's barriers (gg_block_scan256_totals).
 // ALL writes g_b and P_b of every tile.  sT needs 8 * n_tiles bytes of LDS.
-template <int D, bool FULL, bool ALL>
+// Beyond 4 tiles per thread the strided passes load U tiles' records before
+// using them (U round trips in flight instead of one); the order of every
+// accumulation is the same for any U.
+template <int D, bool FULL, bool ALL, int U = 1>
 __device__ void gg_combine(const gg_arena& a, const gg_filter& F, double* lds, double* sT,
                            gg_comb& r) {
 #pragma clang fp contract(off)
@@ -1279,18 +1287,23 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, double* lds, d
   double vv[NV];
 #pragma unroll
   for (int d = 0; d < NV; ++d) vv[d] = 0.0;
-  // pass 2 for tile b given K (scaled S, S2 and the tile's exponent from p4)
-  auto pass2 = [&](int b, double Sc, double S2c, double kb, double K) {
+  // pass 2 for tile b given K (scaled S, S2 and the tile's exponent from p4;
+  // gl and sums[0 .. D) the partial record's last word and state sums)
+  auto pass2v = [&](int b, double Sc, double S2c, double kb, double K, double gl,
+                    const double* sums) {
     const double dk = kb - K;
-    if (ALL) a.g[F.tile_off + b] = gg_tile_shift(part[(int64_t)b * GG_NPART + GG_NPART - 1], dk);
+    if (ALL) a.g[F.tile_off + b] = gg_tile_shift(gl, dk);
     sT[b] = gg_tile_shift(Sc, dk);
     vv[0] = vv[0] + gg_tile_shift(S2c, 2.0 * dk);
     if (FULL) {
-      const double* pb = part + (int64_t)b * GG_NPART;
 #pragma unroll
       for (int d = 0; d < D; ++d)
-        vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + gg_tile_shift(pb[3 + d], dk);
+        vv[(FULL ? 2 : 0) + d] = vv[(FULL ? 2 : 0) + d] + gg_tile_shift(sums[d], dk);
     }
+  };
+  auto pass2 = [&](int b, double Sc, double S2c, double kb, double K) {
+    const double* pb = part + (int64_t)b * GG_NPART;
+    pass2v(b, Sc, S2c, kb, K, ALL ? pb[GG_NPART - 1] : 0.0, pb + 3);
   };
   constexpr int R = 4;  // tiles per thread held in registers across the maxima
   double M, K;
@@ -1318,7 +1331,7 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, double* lds, d
       const int b = t + k * GG_BLOCK;
       if (b < nt) pass2(b, pS[k], pS2[k], pk[k], K);
     }
-  } else {
+  } else if (U == 1) {
     double m = -__builtin_huge_val(), km = -__builtin_huge_val();
     for (int b = t; b < nt; b += GG_BLOCK) {
       m = fmax(m, p4[(int64_t)b * 4]);
@@ -1328,6 +1341,43 @@ __device__ void gg_combine(const gg_arena& a, const gg_filter& F, double* lds, d
     for (int b = t; b < nt; b += GG_BLOCK) {
       const double* q = p4 + (int64_t)b * 4;
       pass2(b, q[1], q[2], q[3], K);
+    }
+  } else {
+    // U tiles per round: their loads first (indices clamped to the last tile;
+    // a repeated tile does not change a maximum and is not used in pass 2)
+    double m = -__builtin_huge_val(), km = -__builtin_huge_val();
+    for (int bb = t; bb < nt; bb += U * GG_BLOCK) {
+      double2 q[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = min(bb + u * GG_BLOCK, nt - 1);
+        q[u] = make_double2(p4[(int64_t)b * 4], p4[(int64_t)b * 4 + 3]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        m = fmax(m, q[u].x);
+        km = fmax(km, q[u].y);
+      }
+    }
+    gg_block_max256x2(m, km, lds, M, K);
+    constexpr int NW = FULL ? D + 1 : 1;
+    for (int bb = t; bb < nt; bb += U * GG_BLOCK) {
+      double4 q[U];
+      double pv[U][NW];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = min(bb + u * GG_BLOCK, nt - 1);
+        q[u] = *reinterpret_cast<const double4*>(p4 + (int64_t)b * 4);
+        const double* pb = part + (int64_t)b * GG_NPART;
+        pv[u][0] = ALL ? pb[GG_NPART - 1] : 0.0;
+        if (FULL)
+#pragma unroll
+          for (int d = 0; d < D; ++d) pv[u][(FULL ? 1 : 0) + d] = pb[3 + d];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (bb + u * GG_BLOCK < nt)
+          pass2v(bb + u * GG_BLOCK, q[u].y, q[u].z, q[u].w, K, pv[u][0], &pv[u][FULL ? 1 : 0]);
     }
   }
   __syncthreads();
@@ -1396,6 +1446,32 @@ __device__ __forceinline__ void gg_load4(const gg_arena& a, int buf, int64_t bas
     p[1][d] = v.y;
     p[2][d] = v.z;
     p[3][d] = v.w;
+  }
+}
+
+// Packed-record slot of particle slot i (filters start on tile boundaries):
+// inside each tile the thread-major order 4t + q becomes q * 256 + t, so the
+// records that a wavefront stores for one item q lie in one 2 KB span.
+__device__ __forceinline__ int64_t gg_aos_slot(int64_t i) {
+  const int64_t j = i & (GG_TILE - 1);
+  return (i - j) + ((j & (GG_ITEMS - 1)) << 8) + (j >> 2);
+}
+static_assert(GG_TILE == 1024 && GG_ITEMS == 4, "gg_aos_slot: 256 threads x 4 items per tile");
+
+// The thread's four particles (slots base .. base + 3) as packed 32-byte
+// records (aos): D components, zero above.
+template <int D>
+__device__ __forceinline__ void gg_store_aos4(const gg_arena& a, int64_t base,
+                                              const float (&p)[GG_ITEMS][D]) {
+  float4* ao = reinterpret_cast<float4*>(a.aos);
+#pragma unroll
+  for (int q = 0; q < GG_ITEMS; ++q) {
+    float r[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) r[d] = (d < D) ? p[q][d] : 0.0f;
+    const int64_t o = 2 * gg_aos_slot(base + q);
+    ao[o] = make_float4(r[0], r[1], r[2], r[3]);
+    ao[o + 1] = make_float4(r[4], r[5], r[6], r[7]);
   }
 }
 

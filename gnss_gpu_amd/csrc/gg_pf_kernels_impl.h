@@ -198,8 +198,12 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     if (PRED) {
       if (!fast) gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, sfr.pl, p);
       gg_apply_shift<D>(sfr, p);
-      gg_store4<D>(a, cur, base, p);
+      // a Megopolis step keeps the predicted state in the packed records only:
+      // the chains read them, and a step that does not resample copies them
+      // back (gg_k_mego_w)
+      if (!a.aos_step) gg_store4<D>(a, cur, base, p);
     }
+    if (a.aos_step) gg_store_aos4<D>(a, base, p);  // uniform
     float lw[GG_ITEMS];
     if constexpr (LEARN == 2) {
       gg_loglik_table<NISB>(sv, F.sat_off, a.mlp_table, p, lw);
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_decide(gg_arena a, gg_step_args
   double vg = 0.0;
   if ((mode & 1) && threadIdx.x < 3) vg = gg_guide_ptr(a, s, f, sd.epoch)[threadIdx.x];
   gg_comb r;
-  gg_combine<D, true, true>(a, F, lds, sT, r);
+  gg_combine<D, true, true, 4>(a, F, lds, sT, r);
   if ((mode & 1) && threadIdx.x < 3) svg[threadIdx.x] = vg;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -568,8 +572,9 @@ __device__ __forceinline__ void gg_gather_copy(const gg_arena& a, int f, const g
       for (int u = 0; u < U; ++u) {
         float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), y = x;
         if (ok[u]) {
-          x = ao[2 * src[u]];
-          y = ao[2 * src[u] + 1];
+          const int64_t o = 2 * gg_aos_slot(src[u]);
+          x = ao[o];
+          y = ao[o + 1];
         }
         const float r[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
 #pragma unroll
@@ -942,7 +947,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, in
 // Megopolis weights w = exp32(lw - M), and the particles' D state components
 // packed 8 floats per particle (aos): a chain's final ancestor is any particle,
 // so the gather that follows reads one 32-byte record per output instead of D
-// scattered words.
+// scattered words.  Inside a Megopolis step (a.aos_step) the likelihood launch
+// has written the records already, and nothing else: a step that does not
+// resample copies them into the current state buffer here.
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   constexpr int D = 4 + NISB;
@@ -950,11 +957,26 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   const int f = a.tile_filter[tile];
   const gg_filter F = a.filt[f];
   const gg_dyn* dy = a.dyn + f;
-  if (!force && !dy->flag) return;
   const int lt = tile - F.tile_off;
+  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
+  if (!force && !dy->flag) {
+    if (a.aos_step) {  // no resampling: the records hold the current state
+      const float4* ao = reinterpret_cast<const float4*>(a.aos);
+      float p[GG_ITEMS][D];
+#pragma unroll
+      for (int q = 0; q < GG_ITEMS; ++q) {
+        const int64_t o = 2 * gg_aos_slot(F.p_off + i0 + q);
+        const float4 x = ao[o], y = ao[o + 1];
+        const float r[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[q][d] = r[d];
+      }
+      gg_store4<D>(a, dy->cur, F.p_off + i0, p);
+    }
+    return;
+  }
   const bool lwz = dy->lw_zero != 0;
   const float M = (float)dy->M;
-  const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
   const float4 lv = *reinterpret_cast<const float4*>(a.lw + F.p_off + i0);
   const float l4[GG_ITEMS] = {lv.x, lv.y, lv.z, lv.w};
   float w[GG_ITEMS];
@@ -962,17 +984,10 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   for (int q = 0; q < GG_ITEMS; ++q)
     w[q] = (i0 + q >= F.n) ? 0.0f : (lwz ? 1.0f : gg_exp32(l4[q] - M));
   *reinterpret_cast<float4*>(a.mw + F.p_off + i0) = make_float4(w[0], w[1], w[2], w[3]);
+  if (a.aos_step) return;  // the step's likelihood launch wrote the records
   float p[GG_ITEMS][D];
   gg_load4<D>(a, dy->cur, F.p_off + i0, p);
-  float4* ao = reinterpret_cast<float4*>(a.aos) + 2 * (F.p_off + i0);
-#pragma unroll
-  for (int q = 0; q < GG_ITEMS; ++q) {
-    float r[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) r[d] = (d < D) ? p[q][d] : 0.0f;
-    ao[2 * q] = make_float4(r[0], r[1], r[2], r[3]);
-    ao[2 * q + 1] = make_float4(r[4], r[5], r[6], r[7]);
-  }
+  gg_store_aos4<D>(a, F.p_off + i0, p);
 }
 
 // DEPTH: iterations whose proposal weights are loaded before the first of
@@ -1072,8 +1087,9 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego(gg_arena a, int B, int for
     x[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     y[q] = x[q];
     if (ib + 64 * q < n) {
-      x[q] = ao[2 * k[q]];
-      y[q] = ao[2 * k[q] + 1];
+      const int64_t o = 2 * gg_aos_slot(k[q]);
+      x[q] = ao[o];
+      y[q] = ao[o + 1];
     }
   }
 #pragma unroll
