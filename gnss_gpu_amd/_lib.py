@@ -97,7 +97,7 @@ class Dyn(C.Structure):
         ("cnt_a", C.c_uint32),
         ("cnt_b", C.c_uint32),
         ("cnt_c", C.c_uint32),
-        ("cnt_d", C.c_uint32),
+        ("aos_only", C.c_int32),
     ]
 
 

@@ -285,25 +285,32 @@ gg_step_args with_derived(const gg_step_args& in) {
   return s;
 }
 
-// aos: the launch writes the predicted state to the packed records only, for
-// the Megopolis launches that follow it in the same step (launch_mego with
-// aos_ready), which leave the state buffers complete.
+// aos (in: requested, out: done; the plain predict + likelihood and code-table
+// kernels do it): the launch writes the predicted state to the packed records
+// (only, or also to the state arrays: gg_dyn.aos_only), for the Megopolis
+// launches that follow it in the same step (launch_mego with aos_ready), which
+// leave the state arrays complete.
 int launch_weight(gg_pf* h, const gg_step_args& s_in, bool pred, bool with_decide = true,
-                  bool aos = false) {
+                  bool* aos = nullptr) {
   const gg_step_args s = with_derived(s_in);
   const bool learned = s.lik == GG_LIK_LEARNED;
   if (learned && !h->mlp_w) return gg_fail(-3, "learned likelihood requested but no gg_mlp attached");
   const bool table = learned && s.learned_mode == 1;
   if (table && ensure_table(h)) return -1;
+  const bool want_aos = aos && *aos && pred && (table || !(learned || h->weight_tpw > 1));
+  if (aos) *aos = want_aos;
   gg_arena a = arena_of(h);
-  a.aos_step = aos ? 1 : 0;
+  a.aos_step = want_aos ? 1 : 0;
   const size_t smem = sat_smem(h, learned && !table);
   if (table) {
     const dim3 tgrid((GG_MLP_CODES + 1023) / 1024, h->sat_rows);
     GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_mlp_table<NI>), tgrid, dim3(GG_BLOCK), 0,
                                                    a, s, pred ? 1 : 0, h->row_filter));
     GG_HIP(hipGetLastError());
-    if (pred) {
+    if (want_aos) {
+      GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_table<NI, true, true>),
+                                           dim3(h->n_tiles), dim3(GG_BLOCK), smem, a, s));
+    } else if (pred) {
       GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_table<NI, true>), dim3(h->n_tiles),
                                                     dim3(GG_BLOCK), smem, a, s));
     } else {
@@ -323,6 +330,9 @@ int launch_weight(gg_pf* h, const gg_step_args& s_in, bool pred, bool with_decid
     const int grid = (h->n_tiles + tpw - 1) / tpw;
     GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight_mt<NI, true>), dim3(grid),
                                                   dim3(GG_BLOCK), smem, a, s, tpw));
+  } else if (want_aos) {
+    GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight<NI, true, true>), dim3(h->n_tiles),
+                                         dim3(GG_BLOCK), smem, a, s));
   } else if (pred) {
     GG_DISPATCH_ISB(h->n_isb, gg_launch(h, (gg_k_weight<NI, true>), dim3(h->n_tiles),
                                                   dim3(GG_BLOCK), smem, a, s));
@@ -402,7 +412,7 @@ int launch_mego_ref(gg_pf* h, int n_iter, int force, uint64_t seed, int use_ovr,
 
 // aos_ready: the step's likelihood launch, enqueued just before with nothing
 // between that reads or changes the state, wrote the predicted state to the
-// packed records only (a step that does not resample copies it back).
+// packed records (and, unless gg_dyn.aos_only, to the state arrays).
 int launch_mego(gg_pf* h, int n_iter, int force, bool aos_ready = false) {
   gg_arena a = arena_of(h);
   a.aos_step = aos_ready ? 1 : 0;
@@ -465,12 +475,12 @@ int enqueue_step_part(gg_pf* h, const gg_step_args& s, int resampler, int n_iter
 }
 
 // A whole step: in a Megopolis step the likelihood launch writes the predicted
-// state as packed records, which the chains read, instead of the state
-// buffers.  (Separate parts keep the plain layout: a caller may read or change
-// the state between them.)
+// state as packed records, which the chains read, and after a resampling step
+// only those.  (Separate parts keep the plain layout: a caller may read or
+// change the state between them.)
 int enqueue_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter) {
-  const bool aos = resampler == 1;
-  if (launch_weight(h, s, true, false, aos)) return -1;
+  bool aos = resampler == 1;
+  if (launch_weight(h, s, true, false, &aos)) return -1;
   return enqueue_resample(h, s, resampler, n_iter, aos);
 }
 
@@ -1329,7 +1339,8 @@ int gg_pf_device_set_particle_states(gg_pf* h, const double* in) {
   d.ess = (double)n;
   d.flag = 0;
   d.est_valid = 0;
-  d.cnt_a = d.cnt_b = d.cnt_c = d.cnt_d = 0;
+  d.cnt_a = d.cnt_b = d.cnt_c = 0;
+  d.aos_only = 0;
   if (write_dyn(h, 0, d)) return -1;
   const size_t bytes = sizeof(double) * (size_t)n * 16;
   if (ensure_io(h, bytes)) return -1;
@@ -1690,8 +1701,8 @@ namespace {
 // index after group g.
 int enqueue_staged_step(gg_pf* h, const gg_step_args& s, int resampler, int n_iter,
                         size_t (&ends)[3]) {
-  const bool aos = resampler == 1;
-  int rc = launch_weight(h, s, true, false, aos);
+  bool aos = resampler == 1;
+  int rc = launch_weight(h, s, true, false, &aos);
   ends[0] = h->kt_next;
   if (!rc) {
     if (resampler == 0) {

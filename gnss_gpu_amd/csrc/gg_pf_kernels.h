@@ -105,9 +105,10 @@ struct gg_arena {
   int32_t n_tiles;   // tiles of all filters
   int32_t n_chunks;  // resampling work items of all filters
   // 1 in a Megopolis step: its likelihood launch writes the predicted state to
-  // the packed records (aos) only, the chains read it from there, and the
-  // Megopolis weight launch copies it into the current state buffer when the
-  // step does not resample
+  // the packed records (aos), which the chains read; after a step that
+  // resampled it writes them only (gg_dyn.aos_only), and the Megopolis weight
+  // launch copies them into the current state buffer if the step does not
+  // resample
   int32_t aos_step;
 };
 

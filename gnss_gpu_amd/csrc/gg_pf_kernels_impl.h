@@ -116,8 +116,9 @@ __global__ void gg_k_commit_predict(gg_arena a, gg_step_args s) {
 // binary64 values do not occupy registers) and the satellite records are
 // staged once per filter met.  PREFETCH issues the loads of tile k + 1 before
 // tile k is computed, so the memory traffic of a multi-tile workgroup
-// overlaps its arithmetic instead of waiting for it.
-template <int NISB, bool PRED, int LEARN, bool PREFETCH>
+// overlaps its arithmetic instead of waiting for it.  AOS (a Megopolis step,
+// with a.aos_step set) also writes the predicted state as packed records.
+template <int NISB, bool PRED, int LEARN, bool PREFETCH, bool AOS = false>
 __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_args& s,
                                                char* smem, int tb, int te) {
   constexpr int D = 4 + NISB;
@@ -150,6 +151,13 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     const int epoch = dy->epoch;
     const int slot = s.slot_from_device ? (epoch % s.n_slots) : 0;
     const uint32_t step = s.step_from_device ? (uint32_t)(epoch + 1) : (uint32_t)s.step;
+    // A Megopolis step writes the predicted state as packed records, which the
+    // chains read.  After a step that resampled it writes them only (the next
+    // is likely to resample too; if it does not, gg_k_mego_w copies them back),
+    // otherwise the state arrays as well.  Every workgroup of the filter
+    // records the same choice.
+    const bool aos_only = AOS && dy->did_resample;
+    if (PRED && AOS && threadIdx.x == 0) a.dyn[f].aos_only = aos_only ? 1 : 0;
     float p[GG_ITEMS][D];
     if (PREFETCH) {
 #pragma unroll
@@ -198,12 +206,9 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
     if (PRED) {
       if (!fast) gg_predict4<NISB>(a, cur, base, F.n, F.seed, i0, step, s, sfr.pl, p);
       gg_apply_shift<D>(sfr, p);
-      // a Megopolis step keeps the predicted state in the packed records only:
-      // the chains read them, and a step that does not resample copies them
-      // back (gg_k_mego_w)
-      if (!a.aos_step) gg_store4<D>(a, cur, base, p);
+      if (!aos_only) gg_store4<D>(a, cur, base, p);
     }
-    if (a.aos_step) gg_store_aos4<D>(a, base, p);  // uniform
+    if (AOS) gg_store_aos4<D>(a, base, p);
     float lw[GG_ITEMS];
     if constexpr (LEARN == 2) {
       gg_loglik_table<NISB>(sv, F.sat_off, a.mlp_table, p, lw);
@@ -234,11 +239,11 @@ __device__ __forceinline__ void gg_weight_body(const gg_arena& a, const gg_step_
 // register budget of the plain likelihood is held to 128 VGPRs (one tile per
 // workgroup).  The multi-tile form (gg_k_weight_mt) runs tpw tiles per
 // workgroup with the next tile's state prefetched.
-template <int NISB, bool PRED>
+template <int NISB, bool PRED, bool AOS = false>
 __global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void gg_k_weight(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, 0, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
+  gg_weight_body<NISB, PRED, 0, false, AOS>(a, s, smem, blockIdx.x, blockIdx.x + 1);
 }
 
 template <int NISB, bool PRED>
@@ -255,11 +260,11 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_weight_learned(gg_arena a, gg_s
   gg_weight_body<NISB, PRED, 1, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
 }
 
-template <int NISB, bool PRED>
+template <int NISB, bool PRED, bool AOS = false>
 __global__ __launch_bounds__(GG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void gg_k_weight_table(gg_arena a, gg_step_args s) {
   extern __shared__ __align__(16) char smem[];
-  gg_weight_body<NISB, PRED, 2, false>(a, s, smem, blockIdx.x, blockIdx.x + 1);
+  gg_weight_body<NISB, PRED, 2, false, AOS>(a, s, smem, blockIdx.x, blockIdx.x + 1);
 }
 
 // Per-satellite code tables of the learned likelihood: for satellite row
@@ -948,8 +953,8 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_ref(gg_arena a, int it, in
 // packed 8 floats per particle (aos): a chain's final ancestor is any particle,
 // so the gather that follows reads one 32-byte record per output instead of D
 // scattered words.  Inside a Megopolis step (a.aos_step) the likelihood launch
-// has written the records already, and nothing else: a step that does not
-// resample copies them into the current state buffer here.
+// has written the records already, and when it wrote them only (aos_only), a
+// step that does not resample copies them into the current state buffer here.
 template <int NISB>
 __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   constexpr int D = 4 + NISB;
@@ -960,7 +965,7 @@ __global__ __launch_bounds__(GG_BLOCK) void gg_k_mego_w(gg_arena a, int force) {
   const int lt = tile - F.tile_off;
   const int i0 = lt * GG_TILE + threadIdx.x * GG_ITEMS;
   if (!force && !dy->flag) {
-    if (a.aos_step) {  // no resampling: the records hold the current state
+    if (a.aos_step && dy->aos_only) {  // no resampling: only the records hold the state
       const float4* ao = reinterpret_cast<const float4*>(a.aos);
       float p[GG_ITEMS][D];
 #pragma unroll

@@ -86,7 +86,8 @@ struct gg_dyn {
   uint32_t cnt_a;       // last-block counters
   uint32_t cnt_b;
   uint32_t cnt_c;
-  uint32_t cnt_d;
+  int32_t aos_only;     // this Megopolis step keeps its predicted state in the packed
+                        // records only (set by the likelihood launch)
 };
 
 // Per-launch scalar arguments shared by every filter of a batch.
