@@ -138,6 +138,47 @@ def test_batched_megopolis_steps_match_oracle_ancestors(gpu, resampler):
     _ = opf
 
 
+@pytest.mark.parametrize("n_isb", [0, 3])
+def test_megopolis_whole_steps_equal_two_part_steps(gpu, n_isb):
+    """A whole Megopolis step keeps the predicted state in packed records (only,
+    after a step that resampled; a step that then does not resample copies them
+    back), the two-part step in the state arrays: the same particles, weights,
+    estimates and records after every step of a sequence that covers each pair
+    of (previous step resampled, this step resamples)."""
+    from gnss_gpu_amd.batched import BatchedParticleFilter
+
+    n = 30_000 + 517
+    fracs = [1.1, 0.0, 0.0, 1.1, 1.1, 0.0, 1.1, 0.0]
+    steps = len(fracs)
+    sat, pr, truth, vel = _scenario(9, 10, steps)
+    sys_ids = (np.arange(10) % 4) if n_isb else None
+    init = np.concatenate([truth[0] - vel, [50.0], np.zeros(n_isb)])[None]
+
+    def make():
+        b = BatchedParticleFilter([n], n_isb=n_isb, sat_cap=10, n_slots=steps)
+        b.load_epochs([[sat]] * steps, [[pr[t]] for t in range(steps)], [[10]] * steps,
+                      sys=None if sys_ids is None else [[sys_ids]] * steps,
+                      vel=np.tile(vel, (steps, 1, 1)))
+        b.initialize(init, [33], 8.0, 8.0, 2.0 if n_isb else 0.0)
+        return b
+
+    whole, parts = make(), make()
+    for t, fr in enumerate(fracs):
+        st = whole.step_args(sigma_pos=1.0, sigma_cb=2.0, sigma_isb=0.1, sigma_pr=5.0,
+                             ess_frac=fr)
+        whole.step(st, resampler="megopolis", n_iter=16)
+        parts.step_part(st, 0, resampler="megopolis", n_iter=16)
+        parts.step_part(st, 1, resampler="megopolis", n_iter=16)
+        dw, dp = whole.read_dyn()[0], parts.read_dyn()[0]
+        assert dw.did_resample == dp.did_resample == (1 if fr > 1.0 else 0), t
+        assert dw.cur == dp.cur, t
+        np.testing.assert_array_equal(whole.get_particles(0), parts.get_particles(0))
+        np.testing.assert_array_equal(whole.get_log_weights_raw(0), parts.get_log_weights_raw(0))
+        np.testing.assert_array_equal(whole.estimates(), parts.estimates())
+    assert np.linalg.norm(whole.estimates()[0, :3] - truth[steps - 1]) < 30.0
+    whole.close()
+    parts.close()
+
 def test_multi_step_graph_matches_eager_steps(gpu):
     """A graph holding several consecutive steps advances the device epoch
     counter inside the graph; its replays equal the eager steps."""
