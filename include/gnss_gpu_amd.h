@@ -171,7 +171,10 @@ int gg_pfb_get_log_weights(gg_pf* h, int32_t f, double* out);
 /* Inspection of the device-driven step (no reference counterpart):
  * gg_pfb_step_part(.., 0) enqueues the step's likelihood launch (predict +
  * weight + tile partials) and (.., 1) the rest (decision, resampling and its
- * finalize); part 0 then part 1 is exactly gg_pfb_step.
+ * finalize); part 0 then part 1 gives exactly gg_pfb_step's results (a whole
+ * Megopolis step also keeps its predicted state as packed records between its
+ * launches; the parts keep the state arrays, which a caller may read between
+ * them).
  * gg_pfb_get_log_weights_raw copies filter f's binary32 log-weight buffer as
  * the last likelihood launch wrote it (no zeroed-weights flag applied);
  * gg_pfb_get_ancestors the ancestor indices of its last resampling. */
